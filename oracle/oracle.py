@@ -1,0 +1,320 @@
+"""ctypes bindings for the CPU oracle (oracle/svs_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline. The
+product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+P = (1 << 61) - 1
+N = 1024
+N1 = 128
+LAMBDA = 3
+EMVP_N, EMVP_S, EMVP_B = 1292, 76, 17
+
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc only)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(variant: str = "") -> C.CDLL:
+    global _LIB
+    name = "liboracle%s.so" % variant
+    path = os.path.join(_HERE, "_build", name)
+    if variant:
+        return _bind(C.CDLL(path))
+    if _LIB is None:
+        if not os.path.exists(path):
+            build()
+        _LIB = _bind(C.CDLL(path))
+    return _LIB
+
+
+def _bind(L: C.CDLL) -> C.CDLL:
+    sz = C.c_size_t
+    L.or_mersenne_reduce.restype = C.c_uint64
+    L.or_mersenne_reduce.argtypes = [C.c_uint64, C.c_uint64]
+    for f in ("or_add_mod", "or_sub_mod", "or_mul_mod"):
+        getattr(L, f).restype = C.c_uint64
+        getattr(L, f).argtypes = [C.c_uint64, C.c_uint64]
+    L.or_chacha20_block.argtypes = [u32p, C.c_uint64, u32p]
+    L.or_domain_separate.argtypes = [u8p, C.c_char_p, sz, u8p]
+    L.or_sparse_threshold.restype = C.c_uint32
+    L.or_quantise_vector.argtypes = [f32p, sz, sz, C.c_uint64, u64p]
+    L.or_fp_to_signed.restype = C.c_int64
+    L.or_fp_to_signed.argtypes = [C.c_uint64]
+    L.or_generate_subspace.argtypes = [u8p, sz, sz, u64p]
+    L.or_regenerate_h_s.restype = sz
+    L.or_regenerate_h_s.argtypes = [u8p, sz, sz, sz, u64p, u32p, u32p, i8p, sz]
+    L.or_encrypt_cluster.argtypes = [u64p, sz, sz, sz, u64p, u64p, u32p, u32p, i8p, u64p, u64p]
+    L.or_encode_query.restype = sz
+    L.or_encode_query.argtypes = [u64p, u64p, sz, sz, u8p, u64p, u64p, u32p, i8p]
+    L.or_matvec.argtypes = [u64p, sz, sz, u64p, u64p]
+    L.or_matvec_mt.argtypes = [u64p, sz, sz, u64p, u64p, C.c_int]
+    L.or_matvec_transpose.argtypes = [u64p, sz, sz, u64p, u64p]
+    L.or_matvec_batch.argtypes = [u64p, sz, sz, u64p, sz, u64p]
+    L.or_dot_product.restype = C.c_uint64
+    L.or_dot_product.argtypes = [u64p, u64p, sz]
+    L.or_verify_response.restype = C.c_int
+    L.or_verify_response.argtypes = [u64p, u64p, sz, sz, u64p, C.c_int, u8p]
+    L.or_decode_scores.argtypes = [u64p, sz, sz, sz, u64p, u64p, u64p, u32p, u32p, i8p,
+                                   u64p, u64p, u64p, u32p, i8p, sz, u64p]
+    L.or_topk.restype = sz
+    L.or_topk.argtypes = [u32p, i64p, sz, sz, u32p, i64p]
+    L.or_l2_f32.restype = C.c_float
+    L.or_l2_f32.argtypes = [f32p, f32p, sz]
+    L.or_probe_route.argtypes = [f32p, sz, sz, f32p, sz, u32p]
+    L.or_emvp_fp_reduce.restype = C.c_uint64
+    L.or_emvp_fp_reduce.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_emvp_compute_products.argtypes = [u64p, sz, u64p, u64p]
+    L.or_emvp_compute_products_batch.argtypes = [u64p, sz, u64p, sz, u64p]
+    L.or_tiptoe_server_answer_u64.argtypes = [u64p, sz, sz, u64p, u64p]
+    L.or_tiptoe_server_answer_u32.argtypes = [u32p, sz, sz, u32p, u32p]
+    L.or_bntm_flat_score.restype = C.c_int
+    L.or_bntm_flat_score.argtypes = [u64p, u64p, u64p, u64p, u32p, u32p, i8p, u64p, sz, u64p,
+                                     u8p, C.c_int, sz, u32p, i64p, C.POINTER(C.c_size_t)]
+    return L
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- field ops
+def mersenne_reduce(x: int) -> int:
+    return lib().or_mersenne_reduce(x & ((1 << 64) - 1), x >> 64)
+
+
+def add_mod(a, b):
+    return lib().or_add_mod(a, b)
+
+
+def sub_mod(a, b):
+    return lib().or_sub_mod(a, b)
+
+
+def mul_mod(a, b):
+    return lib().or_mul_mod(a, b)
+
+
+def chacha20_block(key_words, counter: int) -> np.ndarray:
+    out = np.zeros(16, np.uint32)
+    lib().or_chacha20_block(np.ascontiguousarray(key_words, np.uint32), counter, out)
+    return out
+
+
+def seed32(b) -> np.ndarray:
+    a = np.frombuffer(bytes(b), np.uint8).copy() if not isinstance(b, np.ndarray) else b
+    assert a.size == 32
+    return np.ascontiguousarray(a, np.uint8)
+
+
+def domain_separate(seed, tag: bytes) -> np.ndarray:
+    out = np.zeros(32, np.uint8)
+    lib().or_domain_separate(seed32(seed), tag, len(tag), out)
+    return out
+
+
+def sparse_threshold() -> int:
+    return lib().or_sparse_threshold()
+
+
+def quantise_vector(v, n=N, q=1 << 20) -> np.ndarray:
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.zeros(n, np.uint64)
+    lib().or_quantise_vector(v, v.size, n, q, out)
+    return out
+
+
+def fp_to_signed(s: int) -> int:
+    return lib().or_fp_to_signed(s)
+
+
+def generate_subspace(key_seed, n=N, n1=N1) -> np.ndarray:
+    out = np.zeros(n * n1, np.uint64)
+    lib().or_generate_subspace(seed32(key_seed), n, n1, out)
+    return out
+
+
+def regenerate_h_s(h_seed, m, n=N, n1=N1):
+    h = np.zeros(max(m * n1, 1), np.uint64)
+    rowptr = np.zeros(m + 1, np.uint32)
+    cap = m * n + 1
+    idx = np.zeros(cap, np.uint32)
+    sign = np.zeros(cap, np.int8)
+    nnz = lib().or_regenerate_h_s(seed32(h_seed), m, n, n1, h, rowptr, idx, sign, cap)
+    return h[: m * n1], rowptr, idx[:nnz].copy(), sign[:nnz].copy()
+
+
+def encrypt_cluster(m_plain, m, l, h, rowptr, idx, sign, n=N, n1=N1):
+    m_enc = np.zeros(max(m * n, 1), np.uint64)
+    a_l = np.zeros(max(m * n1, 1), np.uint64)
+    lib().or_encrypt_cluster(_u64(m_plain).reshape(-1) if m else np.zeros(1, np.uint64), m, n,
+                             n1, _u64(l), _u64(h) if m else np.zeros(1, np.uint64), rowptr,
+                             idx if idx.size else np.zeros(1, np.uint32),
+                             sign if sign.size else np.zeros(1, np.int8), m_enc, a_l)
+    return m_enc[: m * n], a_l[: m * n1]
+
+
+def encode_query(v, l, q_seed, n=N, n1=N1):
+    v_enc = np.zeros(n, np.uint64)
+    g = np.zeros(n1, np.uint64)
+    t_idx = np.zeros(n, np.uint32)
+    t_sign = np.zeros(n, np.int8)
+    nnz = lib().or_encode_query(_u64(v), _u64(l), n, n1, seed32(q_seed), v_enc, g, t_idx, t_sign)
+    return v_enc, g, t_idx[:nnz].copy(), t_sign[:nnz].copy()
+
+
+def matvec(mat, rows, cols, vec) -> np.ndarray:
+    out = np.zeros(max(rows, 1), np.uint64)
+    if rows:
+        lib().or_matvec(_u64(mat).reshape(-1), rows, cols, _u64(vec), out)
+    return out[:rows]
+
+
+def matvec_mt(mat, rows, cols, vec, nthreads, variant="") -> np.ndarray:
+    out = np.zeros(max(rows, 1), np.uint64)
+    if rows:
+        lib(variant).or_matvec_mt(_u64(mat).reshape(-1), rows, cols, _u64(vec), out, nthreads)
+    return out[:rows]
+
+
+def matvec_transpose(mat, rows, cols, vec) -> np.ndarray:
+    out = np.zeros(cols, np.uint64)
+    lib().or_matvec_transpose(_u64(mat).reshape(-1), rows, cols, _u64(vec), out)
+    return out
+
+
+def matvec_batch(mat, rows, cols, vecs) -> np.ndarray:
+    vecs = _u64(vecs).reshape(-1, cols)
+    out = np.zeros((vecs.shape[0], max(rows, 1)), np.uint64)
+    lib().or_matvec_batch(_u64(mat).reshape(-1), rows, cols, vecs.reshape(-1), vecs.shape[0],
+                          out.reshape(-1))
+    return out[:, :rows]
+
+
+def dot_product(a, b) -> int:
+    a, b = _u64(a), _u64(b)
+    return lib().or_dot_product(a, b, a.size)
+
+
+def verify_response(r, m_enc, m, v_enc, rng_seed, trials=LAMBDA, n=N) -> int:
+    """Returns -1 (accept) or the first failing trial."""
+    return lib().or_verify_response(_u64(r), _u64(m_enc).reshape(-1), m, n, _u64(v_enc),
+                                    trials, seed32(rng_seed))
+
+
+def decode_scores(r, m, a_l, m_plain, h, rowptr, idx, sign, l, v_enc, g, t_idx, t_sign,
+                  n=N, n1=N1) -> np.ndarray:
+    out = np.zeros(max(m, 1), np.uint64)
+    z64 = np.zeros(1, np.uint64)
+    lib().or_decode_scores(_u64(r) if m else z64, m, n, n1, _u64(a_l) if m else z64,
+                           _u64(m_plain).reshape(-1) if m else z64, _u64(h) if m else z64,
+                           rowptr, idx if idx.size else np.zeros(1, np.uint32),
+                           sign if sign.size else np.zeros(1, np.int8), _u64(l), _u64(v_enc),
+                           _u64(g), t_idx if t_idx.size else np.zeros(1, np.uint32),
+                           t_sign if t_sign.size else np.zeros(1, np.int8), t_idx.size, out)
+    return out[:m]
+
+
+def topk(ids, scores, k):
+    ids = np.ascontiguousarray(ids, np.uint32)
+    scores = np.ascontiguousarray(scores, np.int64)
+    oi = np.zeros(max(k, 1), np.uint32)
+    os_ = np.zeros(max(k, 1), np.int64)
+    c = lib().or_topk(ids, scores, ids.size, k, oi, os_)
+    return oi[:c], os_[:c]
+
+
+def fp_to_signed_vec(mv) -> np.ndarray:
+    mv = _u64(mv)
+    s = mv.astype(np.int64)  # canonical values are < 2^61
+    s[mv >= np.uint64(P // 2)] -= np.int64(P)
+    return s
+
+
+def l2_f32(a, b) -> float:
+    a = np.ascontiguousarray(a, np.float32)
+    return lib().or_l2_f32(a, np.ascontiguousarray(b, np.float32), a.size)
+
+
+def probe_route(centroids, query, nprobe) -> np.ndarray:
+    centroids = np.ascontiguousarray(centroids, np.float32)
+    nc, d = centroids.shape
+    nprobe = min(nprobe, nc)
+    out = np.zeros(max(nprobe, 1), np.uint32)
+    lib().or_probe_route(centroids.reshape(-1), nc, d, np.ascontiguousarray(query, np.float32),
+                         nprobe, out)
+    return out[:nprobe]
+
+
+def emvp_fp_reduce(x: int) -> int:
+    return lib().or_emvp_fp_reduce(x & ((1 << 64) - 1), x >> 64)
+
+
+def emvp_compute_products(m_hat, m, q_hat) -> np.ndarray:
+    out = np.zeros(EMVP_S * max(m, 1), np.uint64)
+    lib().or_emvp_compute_products(_u64(m_hat).reshape(-1), m, _u64(q_hat), out)
+    return out[: EMVP_S * m]
+
+
+def emvp_compute_products_batch(m_hat, m, q_hats) -> np.ndarray:
+    q_hats = _u64(q_hats).reshape(-1, EMVP_N)
+    out = np.zeros(q_hats.shape[0] * EMVP_S * max(m, 1), np.uint64)
+    lib().or_emvp_compute_products_batch(_u64(m_hat).reshape(-1), m, q_hats.reshape(-1),
+                                         q_hats.shape[0], out)
+    return out.reshape(q_hats.shape[0], EMVP_S * max(m, 1))[:, : EMVP_S * m]
+
+
+def tiptoe_answer_u64(m_db, r, c, ct) -> np.ndarray:
+    out = np.zeros(r, np.uint64)
+    lib().or_tiptoe_server_answer_u64(_u64(m_db).reshape(-1), r, c, _u64(ct), out)
+    return out
+
+
+def tiptoe_answer_u32(m_db, r, c, ct) -> np.ndarray:
+    out = np.zeros(r, np.uint32)
+    lib().or_tiptoe_server_answer_u32(np.ascontiguousarray(m_db, np.uint32).reshape(-1), r, c,
+                                      np.ascontiguousarray(ct, np.uint32), out)
+    return out
+
+
+def bntm_flat_score(cl, l, v_q, q_seed, verify, k):
+    """BnTmScorer::score on one cluster dict (keys: m, m_enc, a_l, m_plain, h,
+    s_rowptr, s_idx, s_sign). Returns (status, ids, scores)."""
+    m = cl["m"]
+    oi = np.zeros(max(k, 1), np.uint32)
+    os_ = np.zeros(max(k, 1), np.int64)
+    cnt = C.c_size_t(0)
+    st = lib().or_bntm_flat_score(_u64(cl["m_enc"]), _u64(cl["a_l"]), _u64(cl["m_plain"]),
+                                  _u64(cl["h"]), cl["s_rowptr"],
+                                  cl["s_idx"] if cl["s_idx"].size else np.zeros(1, np.uint32),
+                                  cl["s_sign"] if cl["s_sign"].size else np.zeros(1, np.int8),
+                                  _u64(l), m, _u64(v_q), seed32(q_seed), int(verify), k, oi,
+                                  os_, C.byref(cnt))
+    return st, oi[: cnt.value], os_[: cnt.value]
+
+
+def build_cluster(m_plain_u64, m, l, h_seed):
+    """regenerate_h_s + encrypt_cluster -> dict with the client+server state."""
+    h, rowptr, idx, sign = regenerate_h_s(h_seed, m)
+    m_enc, a_l = encrypt_cluster(m_plain_u64, m, l, h, rowptr, idx, sign)
+    return dict(m=m, m_enc=m_enc, a_l=a_l, m_plain=_u64(m_plain_u64).reshape(-1), h=h,
+                s_rowptr=rowptr, s_idx=idx, s_sign=sign)

@@ -1,0 +1,307 @@
+"""Pins the CPU oracle against published vectors, the reference's own
+known-answer constants, exact big-int closed forms and protocol invariants.
+
+Mirrors the reference's unit tests in scorer-bntm/src/crypto.rs:1070-1560,
+scorer-bntm/src/params.rs:197-285, scorer-emvp/src/crypto.rs:455-543 and
+scorer-tiptoe/src/pir/simplepir.rs:280-501.
+"""
+import hashlib
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+P = O.P
+
+
+def _sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def _field(rng, size):
+    return rng.integers(0, P, size=size, dtype=np.uint64)
+
+
+def _exact_matvec(mat, vec):
+    v = [int(x) for x in vec]
+    return [sum(int(a) * b for a, b in zip(row.tolist(), v)) % P for row in mat]
+
+
+# ------------------------------------------------------------------ ChaCha20
+def test_chacha20_block_rfc7539(golden):
+    for tv in golden["chacha20_rfc7539_a1"]:
+        key = np.frombuffer(bytes.fromhex(tv["key"]), dtype="<u4").astype(np.uint32)
+        words = O.chacha20_block(key, tv["counter"])
+        assert struct.pack("<16I", *[int(w) for w in words]).hex() == tv["keystream"]
+
+
+def test_stream_conventions_cross_implementation(golden):
+    # generate_subspace draws sample_fp from ChaCha20Rng(seed): its first
+    # entries are the stream's first field samples (crypto.rs:235-238).
+    for st in golden["streams"]:
+        seed = bytes.fromhex(st["seed"])
+        l = O.generate_subspace(seed, 8, 1)
+        assert [str(int(x)) for x in l] == st["first_fp"]
+        assert O.domain_separate(seed, b"protocol2").tobytes().hex() == st["protocol2"]
+
+
+# --------------------------------------------------------------------- field
+def test_field_kat(golden):
+    for row in golden["field_kat"]:
+        op = row[0]
+        vals = [int(v) for v in row[1:]]
+        if op == "reduce":
+            assert O.mersenne_reduce(vals[0]) == vals[1], row
+        else:
+            got = {"add": O.add_mod, "sub": O.sub_mod, "mul": O.mul_mod}[op](vals[0], vals[1])
+            assert got == vals[2], row
+
+
+def test_params_and_threshold(golden):
+    p = golden["params"]
+    assert O.sparse_threshold() == p["sparse_threshold"] == (1 << 29) - 1
+    assert p["n1_by_k"]["3"] == O.N1 and p["lambda_prime"] == O.LAMBDA
+
+
+def test_quantise_half_away_from_zero(golden):
+    q = golden["quantise"]
+    out = O.quantise_vector(np.array(q["v"], np.float32), q["n"], q["q"])
+    assert [str(int(x)) for x in out] == q["out"]
+
+
+def test_fp_to_signed(golden):
+    for s, want in golden["fp_to_signed"]:
+        assert O.fp_to_signed(int(s)) == int(want)
+
+
+# ------------------------------------------------------------ server matvec
+def test_matvec_golden(golden):
+    for case in golden["matvec"]:
+        rng = np.random.default_rng(case["seed"])
+        mat = _field(rng, (case["rows"], case["cols"]))
+        vec = _field(rng, case["cols"])
+        assert _sha(mat, vec) == case["sha"], "input generator drifted"
+        got = O.matvec(mat, case["rows"], case["cols"], vec)
+        assert [str(int(x)) for x in got] == case["out"]
+
+
+def test_matvec_max_accumulator(golden):
+    c = golden["matvec_all_pm1"]
+    mat = np.full((c["rows"], c["cols"]), P - 1, np.uint64)
+    vec = np.full(c["cols"], P - 1, np.uint64)
+    assert all(str(int(x)) == c["out"] for x in O.matvec(mat, c["rows"], c["cols"], vec))
+
+
+def test_matvec_one_hot_extracts_column():
+    rng = np.random.default_rng(3)
+    mat = _field(rng, (9, 1024))
+    for col in (0, 1, 511, 1023):
+        v = np.zeros(1024, np.uint64)
+        v[col] = 1
+        assert np.array_equal(O.matvec(mat, 9, 1024, v), mat[:, col])
+
+
+def test_matvec_transpose_golden(golden):
+    c = golden["matvec_transpose"]
+    rng = np.random.default_rng(c["seed"])
+    mat = _field(rng, (c["rows"], c["cols"]))
+    vec = _field(rng, c["rows"])
+    assert _sha(mat, vec) == c["sha"]
+    got = O.matvec_transpose(mat, c["rows"], c["cols"], vec)
+    assert [str(int(x)) for x in got] == c["out"]
+
+
+@pytest.mark.parametrize("big_b", [1, 8, 64])
+def test_batch_matches_per_query(big_b):
+    # crypto.rs:1533-1559
+    rng = np.random.default_rng(57 + big_b)
+    m = 16
+    m_enc = _field(rng, (m, 1024))
+    vs = _field(rng, (big_b, 1024))
+    batch = O.matvec_batch(m_enc, m, 1024, vs)
+    for q in range(big_b):
+        assert np.array_equal(batch[q], O.matvec(m_enc, m, 1024, vs[q]))
+
+
+def test_matvec_mt_matches_scalar():
+    rng = np.random.default_rng(5)
+    mat = _field(rng, (257, 1024))
+    vec = _field(rng, 1024)
+    assert np.array_equal(O.matvec_mt(mat, 257, 1024, vec, 4), O.matvec(mat, 257, 1024, vec))
+
+
+# ------------------------------------------------------ protocol invariants
+def _toy_cluster(seed_byte, m, n1=128):
+    rng = np.random.default_rng(seed_byte)
+    m_plain = _field(rng, (m, 1024))
+    l = O.generate_subspace(bytes([seed_byte + 1] * 32), 1024, n1)
+    h, rowptr, idx, sign = O.regenerate_h_s(bytes([seed_byte + 2] * 32), m, 1024, n1)
+    m_enc, a_l = O.encrypt_cluster(m_plain, m, l, h, rowptr, idx, sign, 1024, n1)
+    return dict(m=m, m_plain=m_plain.reshape(-1), m_enc=m_enc, a_l=a_l, h=h, s_rowptr=rowptr,
+                s_idx=idx, s_sign=sign), l, rng
+
+
+@pytest.mark.parametrize("n1", [512, 256, 128, 64])
+def test_protocol_roundtrip_recovers_mv_exactly(n1):
+    # crypto.rs:1232-1266 and the delta sweep :1306-1343
+    cl, l, rng = _toy_cluster(7 + n1 % 13, 16, n1)
+    v = _field(rng, 1024)
+    v_enc, g, t_idx, t_sign = O.encode_query(v, l, bytes([9] * 32), 1024, n1)
+    r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+    mv = O.decode_scores(r, 16, cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"], cl["s_idx"],
+                         cl["s_sign"], l, v_enc, g, t_idx, t_sign, 1024, n1)
+    want = _exact_matvec(cl["m_plain"].reshape(16, 1024), v)
+    assert [int(x) for x in mv] == want
+
+
+def test_protocol_two_independent_queries():
+    # crypto.rs:1349-1376
+    cl, l, rng = _toy_cluster(11, 16)
+    for q_seed in (bytes([13] * 32), bytes([17] * 32)):
+        v = _field(rng, 1024)
+        v_enc, g, ti, ts = O.encode_query(v, l, q_seed)
+        r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+        mv = O.decode_scores(r, 16, cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"],
+                             cl["s_idx"], cl["s_sign"], l, v_enc, g, ti, ts)
+        assert [int(x) for x in mv] == _exact_matvec(cl["m_plain"].reshape(16, 1024), v)
+
+
+def test_encrypt_cluster_closed_form():
+    # M_enc = M + H L^T + S elementwise, checked in exact big ints on 2 rows.
+    cl, l, _ = _toy_cluster(23, 2)
+    L = l.reshape(1024, 128)
+    H = cl["h"].reshape(2, 128)
+    M = cl["m_plain"].reshape(2, 1024)
+    for i in range(2):
+        s_row = {int(j): int(s) for j, s in zip(cl["s_idx"][cl["s_rowptr"][i]:cl["s_rowptr"][i + 1]],
+                                                cl["s_sign"][cl["s_rowptr"][i]:cl["s_rowptr"][i + 1]])}
+        for j in (0, 5, 777, 1023):
+            hl = sum(int(H[i, k]) * int(L[j, k]) for k in range(128))
+            want = (int(M[i, j]) + hl + s_row.get(j, 0)) % P
+            assert int(cl["m_enc"][i * 1024 + j]) == want
+        for k in (0, 127):
+            assert int(cl["a_l"][i * 128 + k]) == sum(int(M[i, j]) * int(L[j, k])
+                                                      for j in range(1024)) % P
+
+
+def test_h_s_regeneration_deterministic_and_density():
+    # crypto.rs:1379-1409 (mu = 0.125 at Sec128, params.rs:242-244)
+    a = O.regenerate_h_s(bytes([42] * 32), 64)
+    b = O.regenerate_h_s(bytes([42] * 32), 64)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    mean = a[2].size / 64
+    assert abs(mean - 128) < 64
+    # rows are sorted with +-1 signs
+    for i in range(64):
+        row = a[2][a[1][i]:a[1][i + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0)
+    assert set(np.unique(a[3]).tolist()) <= {-1, 1}
+
+
+def test_verification_passes_on_honest_server():
+    # crypto.rs:1415-1439
+    cl, l, rng = _toy_cluster(21, 16)
+    for b in range(16):
+        v = _field(rng, 1024)
+        v_enc, *_ = O.encode_query(v, l, bytes([b] * 32))
+        r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+        assert O.verify_response(r, cl["m_enc"], 16, v_enc, bytes([b ^ 0xAA] * 32)) == -1
+
+
+def test_verification_detects_tampered_response():
+    # crypto.rs:1446-1474: one coordinate + 1 -> rejected at trial 0
+    cl, l, rng = _toy_cluster(31, 16)
+    v = _field(rng, 1024)
+    v_enc, *_ = O.encode_query(v, l, bytes([34] * 32))
+    r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+    r[8] = O.add_mod(int(r[8]), 1)
+    assert O.verify_response(r, cl["m_enc"], 16, v_enc, bytes([35] * 32)) == 0
+
+
+def test_flat_score_topk_equals_plaintext_ranking():
+    # Zero-noise: top-k of decode == top-k of the exact quantised scores.
+    cl, l, rng = _toy_cluster(41, 40)
+    v = _field(rng, 1024)
+    st, ids, scores = O.bntm_flat_score(cl, l, v, bytes([3] * 32), True, 10)
+    assert st == 0
+    exact = _exact_matvec(cl["m_plain"].reshape(40, 1024), v)
+    signed = [x if x < P // 2 else x - P for x in exact]
+    order = sorted(range(40), key=lambda i: (-signed[i], i))[:10]
+    assert ids.tolist() == order
+    assert scores.tolist() == [signed[i] for i in order]
+
+
+def test_k_capped_at_cluster_size():
+    # lib.rs:1069-1083
+    cl, l, rng = _toy_cluster(5, 3)
+    st, ids, _ = O.bntm_flat_score(cl, l, _field(rng, 1024), bytes([8] * 32), False, 100)
+    assert st == 0 and ids.size == 3
+
+
+# --------------------------------------------------------------- top-k/route
+def test_topk_tie_rule(golden):
+    t = golden["topk"]
+    ids, scores = O.topk(t["ids"], t["scores"], t["k"])
+    assert ids.tolist() == t["out_ids"] and scores.tolist() == t["out_scores"]
+
+
+def test_probe_route_golden(golden):
+    g = golden["probe_route"]
+    rng = np.random.default_rng(g["seed"])
+    cents = rng.standard_normal((20, 24)).astype(np.float32)
+    qs = rng.standard_normal((4, 24)).astype(np.float32)
+    assert _sha(cents, qs) == g["sha"]
+    for q, want in zip(qs, g["out"]):
+        assert O.probe_route(cents, q, g["nprobe"]).tolist() == want
+
+
+# --------------------------------------------------------------------- EMVP
+def test_emvp_golden_and_kat(golden):
+    e = golden["emvp"]
+    rng = np.random.default_rng(e["seed"])
+    m_hat = _field(rng, (e["m"], 1292))
+    q_hat = _field(rng, 1292)
+    assert _sha(m_hat, q_hat) == e["sha"]
+    got = O.emvp_compute_products(m_hat, e["m"], q_hat)
+    assert [str(int(x)) for x in got] == e["out"]
+    # scorer-emvp/src/crypto.rs:463-498
+    x = (P - 1) * (P - 1)
+    assert O.emvp_fp_reduce(x) == 1 == int(e["kat_mul_pm1"])
+    assert O.add_mod(P - 1, 1) == 0 and O.add_mod(P - 1, P - 1) == P - 2
+
+
+def test_emvp_batch_matches_per_query():
+    rng = np.random.default_rng(77)
+    m_hat = _field(rng, (5, 1292))
+    qs = _field(rng, (4, 1292))
+    b = O.emvp_compute_products_batch(m_hat, 5, qs)
+    for q in range(4):
+        assert np.array_equal(b[q], O.emvp_compute_products(m_hat, 5, qs[q]))
+
+
+# ------------------------------------------------------------------- Tiptoe
+def test_tiptoe_kat(golden):
+    t = golden["tiptoe_kat"]
+    m = np.array(t["m"], np.uint64)
+    x = np.array(t["x"], np.uint64)
+    for col in range(2):
+        ans = O.tiptoe_answer_u64(m, 2, 3, np.ascontiguousarray(x[:, col]))
+        assert ans.tolist() == [row[col] for row in t["mx"]]
+
+
+def test_tiptoe_golden_wrapping(golden):
+    t = golden["tiptoe"]
+    rng = np.random.default_rng(t["seed"])
+    mdb = rng.integers(0, 32, size=(t["rows"], t["cols"]), dtype=np.uint64)
+    ct = rng.integers(0, 1 << 64, size=t["cols"], dtype=np.uint64)
+    assert _sha(mdb, ct) == t["sha"]
+    assert [str(int(x)) for x in O.tiptoe_answer_u64(mdb, t["rows"], t["cols"], ct)] == t["ans64"]
+    a32 = O.tiptoe_answer_u32(mdb.astype(np.uint32), t["rows"], t["cols"],
+                              (ct & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    assert [str(int(x)) for x in a32] == t["ans32"]
