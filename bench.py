@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""BN-IVF on one MI355X: served-batched vs plain per-query GPU path.
+
+Workload (defaults): N = 1,000,000 synthetic d = 768 unit vectors from a
+seeded 1024-component Gaussian mixture with Zipf(1.1) component weights;
+nlist = 1024 IVF lists (the mixture components); nprobe = 32; k = 10;
+Sec128 BN encryption (n = 1024, n1 = 128, p = 2^61 - 1). A step is one batch
+of queries through the whole device path: encode -> server product ->
+[Freivalds] -> decode -> top-k. Routing and quantisation (client-side CPU
+work in the reference) happen before the timed region; the batch's query
+vectors cross PCIe inside it.
+
+Prints one JSON line (the contract) with the batched throughput as `value`,
+the plain one-query-per-launch throughput beside it, p50/p99 latency, the
+server-product roofline (HIP events on the library's stream) and a CPU
+baseline of the reference algorithm (the C oracle) on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "secure-vector-search_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
+
+
+def run_batches(index, v_q, seeds, probes, verify, k, batch, nbatches, mode, offset=0):
+    """Runs nbatches batches; returns per-batch wall seconds and results."""
+    nprobe = probes.shape[1]
+    lat, res = [], []
+    for b in range(nbatches):
+        s = slice(offset + b * batch, offset + (b + 1) * batch)
+        nq = v_q[s].shape[0]
+        pp = np.arange(0, nq * nprobe + 1, nprobe, dtype=np.uint32)
+        t0 = time.perf_counter()
+        out = index.score_batch(v_q[s], seeds[s], pp, probes[s].reshape(-1), verify[s], k, mode)
+        lat.append(time.perf_counter() - t0)
+        res.append(out)
+    return lat, res
+
+
+def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threads):
+    """The reference CPU path (BnTmIvfScorer::score, ivf.rs:746-822) as the C
+    oracle, one query per thread, on a bounded sample of the workload's
+    queries. Encrypted state comes from the device build (downloaded)."""
+    from oracle import oracle as O
+    import svs
+
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    # per-query cost estimate from one query, then size the sample
+    need = set()
+    probes = svs.probe_route(corpus.centres, qs[: 4 * threads], nprobe)
+    state = {}
+    lock = threading.Lock()
+
+    def load(ci):
+        members = lists[ci]
+        m = members.size
+        me, al, hh = index.download_list(ci)
+        mq = svs.quantise(corpus.vectors[members]).reshape(-1) if m else np.zeros(0, np.uint64)
+        h2, rowptr, sidx, ssign = O.regenerate_h_s(O.derive_h_seed(7, ci), m)
+        cl = dict(m=m, m_enc=me, a_l=al, m_plain=mq, h=hh, s_rowptr=rowptr, s_idx=sidx,
+                  s_sign=ssign)
+        with lock:
+            state[ci] = cl
+
+    def ensure(ci_list):
+        todo = [c for c in ci_list if c not in state]
+        ts = []
+        for i in range(0, len(todo), threads):
+            ts = [threading.Thread(target=load, args=(c,)) for c in todo[i:i + threads]]
+            [t.start() for t in ts]
+            [t.join() for t in ts]
+
+    ensure(sorted(set(probes[0].tolist())))
+    t0 = time.perf_counter()
+    O.ivf_score(state, lists, corpus.centres, l_sub, qs[0], nprobe, bytes(32), False, k)
+    one = time.perf_counter() - t0
+    sample = int(max(threads, min(probes.shape[0], budget_s * threads / max(one, 1e-3))))
+    sample = max(1, min(sample, probes.shape[0]))
+    ensure(sorted(set(probes[:sample].reshape(-1).tolist())))
+    log(f"cpu baseline: {sample} queries on {threads} threads (one query {one * 1e3:.0f} ms)")
+    done = []
+
+    def worker(tid):
+        for qi in range(tid, sample, threads):
+            O.ivf_score(state, lists, corpus.centres, l_sub, qs[qi], nprobe,
+                        bytes([qi % 256]) * 32, False, k)
+            done.append(qi)
+
+    t0 = time.perf_counter()
+    ts = [threading.Thread(target=worker, args=(i,)) for i in range(threads)]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    wall = time.perf_counter() - t0
+    rows = int(sum(sum(lists[c].size for c in probes[q]) for q in range(sample)))
+    return dict(value=round(len(done) / wall, 3), unit="queries/s", cores=threads, kind="port",
+                sample=f"{sample} queries of the same workload (nprobe {nprobe}, {rows} probed "
+                       f"rows), whole query (route, encode, product, decode, top-k), "
+                       f"verification off, C oracle of the Rust CPU path, one query per thread",
+                seconds=round(wall, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nlist", type=int, default=1024)
+    ap.add_argument("--nprobe", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--plain-queries", type=int, default=64)
+    ap.add_argument("--verify-frac", type=float, default=0.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import svs
+    from svs import data
+
+    t_all = time.perf_counter()
+    log(f"corpus N={args.n} nlist={args.nlist}")
+    corpus = data.make_corpus(args.n, 768, components=args.nlist, zipf_s=1.1, seed=42)
+    lists = data.ivf_lists(corpus)
+    sizes = np.array([x.size for x in lists])
+    log(f"lists: max {sizes.max()} min {sizes.min()} mean {sizes.mean():.1f} "
+        f"({time.perf_counter() - t_all:.1f} s)")
+
+    ctx = svs.Context(0)
+    scorer = svs.BnTmIvfScorer(ctx)
+    cfg = svs.BnTmIvfConfig(index=svs.IvfIndex(corpus.centres, lists), nprobe=args.nprobe,
+                            key_seed=bytes([42] * 32), upload_seed=7)
+    t0 = time.perf_counter()
+    handle, _ = scorer.upload_cluster(cfg, corpus.vectors)
+    log(f"device index built in {time.perf_counter() - t0:.1f} s; {ctx.stats()}")
+
+    total_q = (args.warmup + args.steps) * args.batch
+    qs = data.make_queries(corpus, max(total_q, args.plain_queries), seed=4242)
+    probes = svs.probe_route(corpus.centres, qs, args.nprobe)
+    v_q = svs.quantise(qs)
+    rng = np.random.default_rng(7)
+    seeds = rng.integers(0, 256, size=(qs.shape[0], 32), dtype=np.uint8)
+    verify = (rng.random(qs.shape[0]) < args.verify_frac).astype(np.uint8)
+    index = handle.index
+    probed_rows = np.array([sizes[p].sum() for p in probes])
+    log(f"probed rows per query: mean {probed_rows.mean():.0f} p50 {np.median(probed_rows):.0f}")
+
+    # batched (served-batched) path
+    run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.warmup, 1)
+    ctx.profile(True, True)
+    lat, res_b = run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.steps, 1,
+                             offset=args.warmup * args.batch)
+    prof = ctx.profile(False, False)
+    wall = sum(lat)
+    qps = args.steps * args.batch / wall
+    log(f"batched: {qps:.0f} q/s, batch p50 {pct(lat, 50) * 1e3:.2f} ms; stages {prof}")
+
+    # plain path: one query per launch sequence, same queries as batch 0 of
+    # the timed region -> also a bit-exactness check between the two modes
+    npl = min(args.plain_queries, args.batch)
+    off = args.warmup * args.batch
+    run_batches(index, v_q, seeds, probes, verify, args.k, 1, 2, 0, offset=off)
+    lat_p, res_p = run_batches(index, v_q, seeds, probes, verify, args.k, 1, npl, 0, offset=off)
+    qps_plain = npl / sum(lat_p)
+    ids_b, sc_b = res_b[0][0][:npl], res_b[0][1][:npl]
+    exact = all(np.array_equal(res_p[i][0][0], ids_b[i]) and np.array_equal(res_p[i][1][0], sc_b[i])
+                for i in range(npl))
+    log(f"plain: {qps_plain:.0f} q/s, p50 {pct(lat_p, 50) * 1e3:.2f} ms; batched==plain {exact}")
+
+    gemv_ms, gemv_calls = prof["product"]
+    achieved = prof["gemv_bytes"] / (gemv_ms * 1e-3) / 1e9 if gemv_ms else 0.0
+    rows = prof["gemv_rows"]
+    mulmods = rows * 1024
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_gemv_chunks",
+                    algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
+                    avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
+                    mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(index, corpus, lists, handle.l_subspace, qs[off:], args.nprobe, args.k,
+                           args.cpu_seconds, args.cpu_threads)
+
+    out = {
+        "metric": "BN-IVF queries/s + p50/p99 ms, served-batched vs plain GEMV, 1 MI355X, bit-exact",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (F_p, p = 2^61-1)",
+        "data": "synthetic: seeded 1024-component Gaussian mixture (Zipf 1.1 weights), d=768, "
+                "unit-norm; lists = mixture components",
+        "config": {"workload": f"bn-ivf N={args.n} d=768 nlist={args.nlist} nprobe={args.nprobe} "
+                               f"k={args.k} batch={args.batch} verify_frac={args.verify_frac}",
+                   "batch": args.batch, "nprobe": args.nprobe, "mean_probed_rows":
+                       round(float(probed_rows.mean()), 1)},
+        "p50_ms": round(pct(lat, 50) * 1e3, 3),
+        "p99_ms": round(pct(lat, 99) * 1e3, 3),
+        "plain": {"value": round(qps_plain, 2), "unit": "queries/s",
+                  "p50_ms": round(pct(lat_p, 50) * 1e3, 3),
+                  "p99_ms": round(pct(lat_p, 99) * 1e3, 3), "queries": npl},
+        "speedup_vs_plain": round(qps / qps_plain, 2),
+        "bit_exact_batched_vs_plain": bool(exact),
+        "stages_ms_per_batch": {s: round(prof[s][0] / max(prof[s][1], 1), 4)
+                                for s in ("encode", "product", "verify", "decode", "topk")},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "wall_s": round(time.perf_counter() - t_all, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

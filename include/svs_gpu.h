@@ -1,0 +1,221 @@
+/*
+ * svs_gpu.h — C ABI of the MI355X (gfx950) backend for the encrypted-score
+ * matrix-vector path of sacs-epfl/secure-vector-search.
+ *
+ * This library replaces the CUDA path behind the reference's `Scorer` trait:
+ * scorer-gpu-common (ClusterStore, budget, sampler) and the per-scheme GPU
+ * states scorer-bntm/src/gpu/mod.rs (FlatGpuState, IvfGpuState) and
+ * scorer-emvp/src/gpu/mod.rs, plus Tiptoe's server answer
+ * (scorer-tiptoe/src/pir/simplepir.rs:191-204), which has no GPU path in the
+ * reference. A Rust FFI crate binds these symbols.
+ *
+ * Conventions
+ *  - Every field element in and out is canonical: a u64 in [0, p), p = 2^61-1.
+ *  - Ring dimension n = 1024, subspace n1 = 128, lambda' = 3 (Sec128,
+ *    scorer-bntm/src/params.rs:102-119,198-205).
+ *  - Seeds are explicit inputs (the reference draws q_seed / h_seed from OS
+ *    entropy, lib.rs:882-885, ivf.rs:770); the caller injects them.
+ *  - Pointers named `*_host` are host memory; `*_dev` are device memory on the
+ *    context's GPU. No allocation crosses the ABI: outputs are caller-owned.
+ *  - Return value: SVS_OK (0), a non-negative status, or a negative error;
+ *    svs_last_error() gives the thread-local message of the last error.
+ *  - Thread safety: calls on one context are serialised internally.
+ */
+#ifndef SVS_GPU_H
+#define SVS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------
+ * SVS_NONE mirrors `Ok(None)` for an empty / out-of-range list
+ * (scorer-bntm/src/gpu/mod.rs:506-536). SVS_ERR_BUDGET mirrors
+ * GpuError::BudgetTooSmall (gpu/mod.rs:97-144, store.rs:183-211). */
+#define SVS_OK 0
+#define SVS_NONE 1
+#define SVS_ERR_ARG (-1)
+#define SVS_ERR_HIP (-2)
+#define SVS_ERR_BUDGET (-3)
+#define SVS_ERR_STATE (-4)
+#define SVS_ERR_RNG (-5) /* a device u-draw hit the rejected value p (prob 2^-61) */
+
+typedef struct svs_ctx svs_ctx;
+typedef struct svs_index svs_index;
+
+typedef struct {
+    uint64_t budget_bytes;        /* resolved HBM budget for resident state */
+    uint64_t resident_bytes;      /* index bytes resident now */
+    uint64_t peak_resident_bytes; /* max of resident + scratch since create */
+    uint64_t hbm_total_bytes;     /* hipMemGetInfo total */
+    uint64_t hbm_free_bytes;      /* hipMemGetInfo free, sampled at this call */
+    uint64_t uploads;             /* lists uploaded */
+    uint64_t launches;            /* kernels launched */
+} svs_stats;
+
+/* Thread-local message of the last failing call; returns its length. */
+int svs_last_error(char* buf, size_t len);
+
+/* Create a context on `device`. budget_bytes = 0 resolves to 90 % of the
+ * free HBM. Replaces the process-wide `kernel()` OnceLock and
+ * `resolve_budget` (scorer-bntm/src/gpu/mod.rs:169-200,
+ * scorer-gpu-common/src/budget.rs:132-150). */
+int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out);
+int svs_ctx_destroy(svs_ctx* ctx);
+/* peak_vram_bytes / vram_budget_bytes / global_stats
+ * (gpu/mod.rs:336-349, scorer-gpu-common/src/store.rs:74-81). */
+int svs_ctx_stats(svs_ctx* ctx, svs_stats* out);
+
+/* Stage timing of svs_score_batch: HIP events recorded on the context's own
+ * stream around each stage (0 encode, 1 server product, 2 verification,
+ * 3 decode, 4 top-k). Accumulated ms and call counts, plus the algorithmic
+ * bytes of the server product (sum over each batch's distinct probed lists
+ * of m_c * n * 8) and its candidate rows. The reference's per-substep
+ * breakdown (BnTmIvfTiming, ivf.rs:1380-1677) is the host-clock analogue. */
+typedef struct {
+    double ms[8];
+    uint64_t calls[8];
+    uint64_t gemv_bytes;
+    uint64_t gemv_rows;
+} svs_profile;
+int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile* out);
+
+/* ---- resident index -----------------------------------------------------
+ * An index holds n_lists encrypted lists (a flat scorer is one list) and the
+ * shared subspace L (n x n1 row-major, generate_subspace crypto.rs:235-238).
+ * Everything stays resident in HBM; no LRU streaming. Replaces
+ * IvfGpuState::build / FlatGpuState::build (gpu/mod.rs:300-332,443-481) and
+ * ClusterStore::get_or_upload (scorer-gpu-common/src/store.rs:327-380). */
+int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* l_host, svs_index** out);
+int svs_index_destroy(svs_index* idx);
+
+/* Upload one list's server state (M_enc) and client state (AL, H, M_plain,
+ * S as CSR with sorted column indices, signs +-1) as produced by
+ * encrypt_cluster (crypto.rs:405-478), plus the list's global vector ids
+ * (index.clusters[ci][local].0, ivf.rs:815-817). m = 0 uploads an empty
+ * list. Row-major host arrays; m_plain may be NULL when decode is unused. */
+int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m, const uint64_t* m_enc_host,
+                          const uint64_t* a_l_host, const uint64_t* h_host,
+                          const uint64_t* m_plain_host, const uint32_t* s_rowptr_host,
+                          const uint32_t* s_idx_host, const int8_t* s_sign_host,
+                          const uint32_t* ids_host);
+
+/* Build one list on the device from its quantised plaintext rows: H and S
+ * are regenerated from h_seed (regenerate_h_s, crypto.rs:258-270) on the
+ * host, then M_enc = M + H L^T + S and AL = M L are computed on the GPU
+ * (encrypt_cluster, crypto.rs:405-478). Bit-identical to uploading the
+ * host-encrypted list. */
+int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m, const uint64_t* m_plain_host,
+                         const uint8_t h_seed[32], const uint32_t* ids_host);
+
+/* Copy a list's M_enc, AL and H back to the host (row-major; NULL skips an
+ * array): lets tests compare the device build with the host encryption and
+ * lets a CPU baseline reuse the device-built state. */
+int svs_index_download_list(svs_index* idx, uint32_t list, uint64_t* m_enc_host,
+                            uint64_t* a_l_host, uint64_t* h_host);
+
+/* Rows of a list (0 for empty/out-of-range). */
+uint32_t svs_index_list_rows(const svs_index* idx, uint32_t list);
+
+/* ---- plain per-list drop-ins (reference GPU state API) ------------------ */
+/* r = M_enc . v_enc (mod p) for one list; mersenne_matvec semantics
+ * (crypto.rs:606-626). Replaces IvfGpuState::compute_products(ci, v_enc)
+ * (gpu/mod.rs:506-536) / FlatGpuState::compute_products (:362-379) and the
+ * bntm_compute_products kernel (kernels.cu:100-178). Returns SVS_NONE for an
+ * empty or out-of-range list. */
+int svs_compute_products(svs_index* idx, uint32_t list, const uint64_t* v_enc_host,
+                         uint64_t* r_host);
+
+/* Same product with device pointers, asynchronous on the context stream:
+ * the bench's resident-input path. */
+int svs_compute_products_dev(svs_index* idx, uint32_t list, const uint64_t* v_enc_dev,
+                             uint64_t* r_dev);
+
+/* AL . g + H . lt_venc (mod p): decode_dense_terms (gpu/mod.rs:390-408,
+ * 541-577) with the reference's stale N1 = 512 kernel replaced by n1 = 128. */
+int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint64_t* g_host,
+                           const uint64_t* lt_venc_host, uint64_t* out_host);
+
+/* Protocol-2 iterated Freivalds check of a claimed r (verify_response,
+ * crypto.rs:1013-1046). `rng_seed` seeds ChaCha20Rng for the u draws; the
+ * scorers pass domain_separate(q_seed, "protocol2") (lib.rs:420-431).
+ * *failed_trial = -1 on accept, else the first failing trial. */
+int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint64_t* v_enc_host,
+               const uint8_t rng_seed[32], uint32_t trials, int32_t* failed_trial);
+
+/* Full client decode (decode_scores, crypto.rs:901-940): Mv from r, the
+ * per-query mask (g, sparse T) and v_enc. */
+int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint64_t* v_enc_host,
+               const uint64_t* g_host, const uint32_t* t_idx_host, const int8_t* t_sign_host,
+               uint32_t t_nnz, uint64_t* mv_host);
+
+/* ---- whole-query pipeline (encode -> product -> verify -> decode -> top-k)
+ * For each query q: v_q (quantised, n u64), q_seed (32 B), its probe lists
+ * (probe_ptr[q]..probe_ptr[q+1] into probe_lists, in probe order), a verify
+ * flag and k. Encoding uses q_seed exactly as encode_query
+ * (crypto.rs:508-553); verification re-seeds ChaCha20Rng with
+ * domain_separate(q_seed, "protocol2") per probed list (ivf.rs:790-804).
+ * Outputs per query: up to k (id, signed score) hits, descending, ties in
+ * probe-order position (the reference leaves ties unspecified, lib.rs:458);
+ * status[q] = 0, or 1 + trial + 4 * list for the first probed list (in
+ * probe order) whose verification failed (then no hits are written).
+ *
+ * mode SVS_MODE_PLAIN launches one gathered product per query (the
+ * one-query-per-launch drop-in); SVS_MODE_BATCHED runs the whole batch
+ * through one gathered launch per stage. Results are identical. */
+#define SVS_MODE_PLAIN 0
+#define SVS_MODE_BATCHED 1
+
+typedef struct {
+    uint32_t n_queries;
+    const uint64_t* v_q;          /* n_queries x 1024 */
+    const uint8_t* q_seeds;       /* n_queries x 32 */
+    const uint32_t* probe_ptr;    /* n_queries + 1 */
+    const uint32_t* probe_lists;  /* probe_ptr[n_queries] list ids */
+    const uint8_t* verify;        /* n_queries flags (NULL = none) */
+    uint32_t k;                   /* hits per query (<= 128) */
+    uint32_t* out_ids;            /* n_queries x k */
+    int64_t* out_scores;          /* n_queries x k */
+    uint32_t* out_count;          /* n_queries */
+    int32_t* out_status;          /* n_queries */
+} svs_query_batch;
+
+int svs_score_batch(svs_index* idx, const svs_query_batch* batch_host, int mode);
+
+/* ---- test hook -------------------------------------------------------------
+ * After every server product of svs_score_batch, add `delta` to r[row] of
+ * each item probing `list` (list < 0 turns it off): a malicious or faulty
+ * server. No counterpart in the reference's API; it models the reference's
+ * CPU-forced fault (crypto.rs:1446-1474), which the tests use to show that
+ * verification rejects and reports (list, trial). */
+int svs_debug_inject_fault(svs_ctx* ctx, int32_t list, uint32_t row, uint64_t delta);
+
+/* ---- host-side client functions (no GPU) ---------------------------------
+ * The client half of the protocol that the Rust scorers run on the host,
+ * kept native so callers need nothing else. */
+
+/* generate_subspace (crypto.rs:235-238): L, n x n1 row-major. */
+int svs_keygen_subspace(const uint8_t key_seed[32], uint64_t* l_out);
+/* derive_h_seed (ivf.rs:258-264). */
+int svs_keygen_derive_h_seed(uint64_t upload_seed, uint64_t cluster, uint8_t out[32]);
+/* domain_separate (crypto.rs:142-150). */
+int svs_keygen_domain_separate(const uint8_t seed[32], const uint8_t* tag, size_t tag_len,
+                               uint8_t out[32]);
+/* regenerate_h_s (crypto.rs:258-270): H (m x n1) and S as CSR. */
+int svs_keygen_regenerate_h_s(const uint8_t h_seed[32], uint32_t m, uint64_t* h_out,
+                              uint32_t* s_rowptr_out, uint32_t* s_idx_out, int8_t* s_sign_out,
+                              size_t s_cap, size_t* nnz_out);
+/* quantise_vector (crypto.rs:105-122) for rows x d floats, padded to n. */
+int svs_quantise(const float* v, uint32_t rows, uint32_t d, uint64_t q, uint64_t* out);
+/* probe_route (ivf-index/src/ivf.rs:150-164): nprobe nearest centroids by
+ * f32 L2 (distance.rs:30-37), per query, in ascending distance order. */
+int svs_probe_route(const float* centroids, uint32_t n_centroids, uint32_t d,
+                    const float* queries, uint32_t n_queries, uint32_t nprobe, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVS_GPU_H */

@@ -318,3 +318,53 @@ def build_cluster(m_plain_u64, m, l, h_seed):
     m_enc, a_l = encrypt_cluster(m_plain_u64, m, l, h, rowptr, idx, sign)
     return dict(m=m, m_enc=m_enc, a_l=a_l, m_plain=_u64(m_plain_u64).reshape(-1), h=h,
                 s_rowptr=rowptr, s_idx=idx, s_sign=sign)
+
+
+def derive_h_seed(upload_seed: int, cluster: int) -> np.ndarray:
+    """derive_h_seed (scorer-bntm/src/ivf.rs:258-264) from the oracle's
+    ChaCha20 block: first 32 stream words, one byte each."""
+    seed = int(upload_seed).to_bytes(8, "little") + int(cluster).to_bytes(8, "little") + bytes(16)
+    key = np.frombuffer(seed, "<u4").astype(np.uint32)
+    words = np.concatenate([chacha20_block(key, 0), chacha20_block(key, 1)])
+    return (words & 0xFF).astype(np.uint8)
+
+
+def u_draws(rng_seed, count: int) -> list:
+    """The first `count` sample_fp draws of ChaCha20Rng(rng_seed) (the
+    Freivalds u vectors, crypto.rs:1028), via generate_subspace's stream."""
+    n1 = 1
+    out = np.zeros(count, np.uint64)
+    lib().or_generate_subspace(seed32(rng_seed), count, n1, out)
+    return [int(x) for x in out]
+
+
+def ivf_score(clusters, ids_per_list, centroids, l, query_f32, nprobe, q_seed, verify, k):
+    """BnTmIvfScorer::score (scorer-bntm/src/ivf.rs:746-822) on the oracle:
+    route, encode, then per probed non-empty list compute / verify / decode,
+    concatenate (global id, signed score) in probe order, top-k.
+    Returns (hits_ids, hits_scores) or raises VerificationFailedOracle."""
+    probes = probe_route(centroids, query_f32, nprobe)
+    v_q = quantise_vector(query_f32)
+    v_enc, g, t_idx, t_sign = encode_query(v_q, l, q_seed)
+    all_ids, all_scores = [], []
+    for ci in probes.tolist():
+        cl = clusters[ci]
+        m = cl["m"]
+        if m == 0:
+            continue
+        r = matvec(cl["m_enc"], m, N, v_enc)
+        if verify:
+            t = verify_response(r, cl["m_enc"], m, v_enc, domain_separate(q_seed, b"protocol2"))
+            if t >= 0:
+                raise VerificationFailedOracle(ci, t)
+        mv = decode_scores(r, m, cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"], cl["s_idx"],
+                           cl["s_sign"], l, v_enc, g, t_idx, t_sign)
+        all_ids.extend(ids_per_list[ci].tolist())
+        all_scores.extend(fp_to_signed_vec(mv).tolist())
+    return topk(np.array(all_ids, np.uint32), np.array(all_scores, np.int64), k)
+
+
+class VerificationFailedOracle(Exception):
+    def __init__(self, cluster, trial):
+        super().__init__(f"cluster {cluster} trial {trial}")
+        self.cluster, self.trial = cluster, trial

@@ -1,0 +1,1020 @@
+// svs_api.hip — C ABI of the gfx950 BN scoring backend (include/svs_gpu.h):
+// context, resident index, device index build, per-list drop-ins and the
+// gathered whole-query pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/svs_gpu.h"
+#include "svs_internal.h"
+#include "svs_kernels.h"
+
+using namespace svs;
+
+// ===================================================================== errors
+static thread_local std::string g_last_error;
+
+int svs::fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" int svs_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        size_t n = std::min(len - 1, g_last_error.size());
+        memcpy(buf, g_last_error.data(), n);
+        buf[n] = 0;
+    }
+    return (int)g_last_error.size();
+}
+
+// ==================================================================== context
+int DevBuf::ensure(svs_ctx* ctx, size_t bytes) {
+    if (bytes <= cap) return SVS_OK;
+    if (ptr) {
+        hipFree(ptr);
+        ctx->scratch_bytes -= cap;
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t want = std::max(bytes, (size_t)4096);
+    want = (want + 4095) & ~(size_t)4095;
+    HIP_TRY(hipMalloc(&ptr, want));
+    cap = want;
+    ctx->scratch_bytes += cap;
+    ctx->note_peak();
+    return SVS_OK;
+}
+
+void DevBuf::release(svs_ctx* ctx) {
+    if (ptr) {
+        hipFree(ptr);
+        if (ctx) ctx->scratch_bytes -= cap;
+    }
+    ptr = nullptr;
+    cap = 0;
+}
+
+void svs_ctx::note_peak() {
+    uint64_t now = resident_bytes + scratch_bytes;
+    if (now > peak_bytes) peak_bytes = now;
+}
+
+int svs::dev_alloc(svs_ctx* ctx, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return SVS_OK;
+    if (ctx->resident_bytes + bytes > ctx->budget_bytes)
+        return fail(SVS_ERR_BUDGET,
+                    "list payload %zu bytes exceeds budget %llu bytes (resident %llu)", bytes,
+                    (unsigned long long)ctx->budget_bytes,
+                    (unsigned long long)ctx->resident_bytes);
+    HIP_TRY(hipMalloc(p, bytes));
+    ctx->resident_bytes += bytes;
+    ctx->note_peak();
+    return SVS_OK;
+}
+
+void svs::dev_free(svs_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    hipFree(p);
+    ctx->resident_bytes -= bytes;
+}
+
+extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) {
+    if (!out) return fail(SVS_ERR_ARG, "out is null");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SVS_ERR_ARG, "device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SVS_ERR_STATE, "device %d is %s; this library is built for gfx950", device,
+                    prop.gcnArchName);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    auto* ctx = new svs_ctx();
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
+    ctx->hbm_total = total_b;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(SVS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return SVS_OK;
+}
+
+extern "C" int svs_ctx_destroy(svs_ctx* ctx) {
+    if (!ctx) return SVS_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : ctx->all_scratch()) b->release(ctx);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SVS_OK;
+}
+
+extern "C" int svs_ctx_stats(svs_ctx* ctx, svs_stats* out) {
+    if (!ctx || !out) return fail(SVS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    out->budget_bytes = ctx->budget_bytes;
+    out->resident_bytes = ctx->resident_bytes;
+    out->peak_resident_bytes = ctx->peak_bytes;
+    out->hbm_total_bytes = total_b;
+    out->hbm_free_bytes = free_b;
+    out->uploads = ctx->uploads;
+    out->launches = ctx->launches;
+    return SVS_OK;
+}
+
+// ====================================================================== index
+extern "C" int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* l_host,
+                                svs_index** out) {
+    if (!ctx || !out || !l_host) return fail(SVS_ERR_ARG, "null argument");
+    if (n_lists == 0) return fail(SVS_ERR_ARG, "n_lists must be > 0");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (size_t i = 0; i < (size_t)N * N1; ++i)
+        if (l_host[i] >= P) return fail(SVS_ERR_ARG, "L[%zu] is not canonical", i);
+    auto* idx = new svs_index();
+    idx->ctx = ctx;
+    idx->lists.resize(n_lists);
+    int rc = dev_alloc(ctx, (void**)&idx->l_dev, sizeof(uint64_t) * N * N1);
+    if (rc == SVS_OK) rc = dev_alloc(ctx, (void**)&idx->lt_dev, sizeof(uint64_t) * N * N1);
+    if (rc == SVS_OK) rc = dev_alloc(ctx, (void**)&idx->lists_dev, sizeof(ListDev) * n_lists);
+    if (rc != SVS_OK) {
+        dev_free(ctx, idx->l_dev, sizeof(uint64_t) * N * N1);
+        dev_free(ctx, idx->lt_dev, sizeof(uint64_t) * N * N1);
+        delete idx;
+        return rc;
+    }
+    std::vector<uint64_t> lt((size_t)N * N1);
+    for (int j = 0; j < N; ++j)
+        for (int k = 0; k < N1; ++k) lt[(size_t)k * N + j] = l_host[(size_t)j * N1 + k];
+    HIP_TRY(hipMemcpy(idx->l_dev, l_host, sizeof(uint64_t) * N * N1, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(idx->lt_dev, lt.data(), sizeof(uint64_t) * N * N1, hipMemcpyHostToDevice));
+    std::vector<ListDev> zero(n_lists);
+    memset(zero.data(), 0, sizeof(ListDev) * n_lists);
+    HIP_TRY(hipMemcpy(idx->lists_dev, zero.data(), sizeof(ListDev) * n_lists,
+                      hipMemcpyHostToDevice));
+    *out = idx;
+    return SVS_OK;
+}
+
+static void free_list(svs_ctx* ctx, ListHost& L) {
+    for (auto& a : L.allocs) dev_free(ctx, a.first, a.second);
+    L = ListHost();
+}
+
+extern "C" int svs_index_destroy(svs_index* idx) {
+    if (!idx) return SVS_OK;
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& L : idx->lists) free_list(ctx, L);
+    dev_free(ctx, idx->l_dev, sizeof(uint64_t) * N * N1);
+    dev_free(ctx, idx->lt_dev, sizeof(uint64_t) * N * N1);
+    dev_free(ctx, idx->lists_dev, sizeof(ListDev) * idx->lists.size());
+    delete idx;
+    return SVS_OK;
+}
+
+extern "C" uint32_t svs_index_list_rows(const svs_index* idx, uint32_t list) {
+    if (!idx || list >= idx->lists.size()) return 0;
+    return idx->lists[list].m;
+}
+
+// Sliced-ELL S from CSR: per 64-row slice, width = max row nnz; entry
+// (e, lane) at slice_off + e * 64 + lane holds idx | 0x8000 for sign -1.
+static void build_s_ell(uint32_t m, const uint32_t* rowptr, const uint32_t* sidx,
+                        const int8_t* ssign, std::vector<uint32_t>& slice_off,
+                        std::vector<uint16_t>& ell) {
+    const uint32_t nslices = (m + SLICE - 1) / SLICE;
+    slice_off.assign(nslices + 1, 0);
+    for (uint32_t s = 0; s < nslices; ++s) {
+        uint32_t w = 0;
+        for (uint32_t r = s * SLICE; r < std::min(m, (s + 1) * SLICE); ++r)
+            w = std::max(w, rowptr[r + 1] - rowptr[r]);
+        slice_off[s + 1] = slice_off[s] + w * SLICE;
+    }
+    ell.assign(slice_off[nslices], S_PAD);
+    for (uint32_t r = 0; r < m; ++r) {
+        const uint32_t s = r / SLICE, lane = r % SLICE;
+        for (uint32_t e = rowptr[r]; e < rowptr[r + 1]; ++e)
+            ell[slice_off[s] + (size_t)(e - rowptr[r]) * SLICE + lane] =
+                (uint16_t)((sidx[e] & 0x3FF) | (ssign[e] < 0 ? 0x8000 : 0));
+    }
+}
+
+// Allocates and fills the list's device state except menc / alh_t content.
+static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool want_mp) {
+    svs_ctx* ctx = idx->ctx;
+    ListHost& L = idx->lists[list];
+    free_list(ctx, L);
+    L.m = m;
+    if (m == 0) return SVS_OK;
+    auto take = [&](void** p, size_t bytes) -> int {
+        int rc = dev_alloc(ctx, p, bytes);
+        if (rc == SVS_OK && *p) L.allocs.push_back({*p, bytes});
+        return rc;
+    };
+    int rc = take((void**)&L.d.menc, sizeof(uint64_t) * (size_t)m * N);
+    if (rc == SVS_OK) rc = take((void**)&L.d.alh_t, sizeof(uint64_t) * (size_t)m * 2 * N1);
+    if (rc == SVS_OK && want_mp) {
+        if (mp32) rc = take((void**)&L.d.mp_t32, sizeof(int32_t) * (size_t)m * N);
+        else rc = take((void**)&L.d.mp_t64, sizeof(uint64_t) * (size_t)m * N);
+    }
+    if (rc == SVS_OK) rc = take((void**)&L.d.ids, sizeof(uint32_t) * m);
+    if (rc != SVS_OK) free_list(ctx, L);
+    L.d.m = m;
+    return rc;
+}
+
+static int list_upload_s_ids(svs_index* idx, uint32_t list, const uint32_t* rowptr,
+                             const uint32_t* sidx, const int8_t* ssign, const uint32_t* ids) {
+    svs_ctx* ctx = idx->ctx;
+    ListHost& L = idx->lists[list];
+    const uint32_t m = L.m;
+    std::vector<uint32_t> soff;
+    std::vector<uint16_t> ell;
+    build_s_ell(m, rowptr, sidx, ssign, soff, ell);
+    void* p = nullptr;
+    int rc = dev_alloc(ctx, &p, soff.size() * sizeof(uint32_t));
+    if (rc != SVS_OK) return rc;
+    L.allocs.push_back({p, soff.size() * sizeof(uint32_t)});
+    L.d.s_slice = (const uint32_t*)p;
+    HIP_TRY(hipMemcpy(p, soff.data(), soff.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!ell.empty()) {
+        rc = dev_alloc(ctx, &p, ell.size() * sizeof(uint16_t));
+        if (rc != SVS_OK) return rc;
+        L.allocs.push_back({p, ell.size() * sizeof(uint16_t)});
+        L.d.s_ell = (const uint16_t*)p;
+        HIP_TRY(hipMemcpy(p, ell.data(), ell.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
+    std::vector<uint32_t> idv(m);
+    for (uint32_t i = 0; i < m; ++i) idv[i] = ids ? ids[i] : i;
+    HIP_TRY(hipMemcpy((void*)L.d.ids, idv.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    return SVS_OK;
+}
+
+static bool fits_i32(const uint64_t* mp, size_t count) {
+    for (size_t i = 0; i < count; ++i) {
+        const uint64_t x = mp[i];
+        if (!(x <= 0x7FFFFFFFull || x >= P - 0x80000000ull)) return false;
+    }
+    return true;
+}
+
+static int publish_list(svs_index* idx, uint32_t list) {
+    ListDev d = idx->lists[list].d;
+    d.m = idx->lists[list].m;
+    HIP_TRY(hipMemcpy(idx->lists_dev + list, &d, sizeof(ListDev), hipMemcpyHostToDevice));
+    idx->ctx->uploads++;
+    idx->ctx->cached_plan_list = -1;
+    return SVS_OK;
+}
+
+// Transposes a host row-major rows x cols u64 matrix (via a temp device
+// buffer) into dst (cols x dst_ld).
+static int upload_transposed(svs_ctx* ctx, const uint64_t* host, uint32_t rows, uint32_t cols,
+                             uint64_t* dst, uint32_t dst_ld) {
+    DevBuf& tmp = ctx->tmp_a;
+    SVS_TRY(tmp.ensure(ctx, sizeof(uint64_t) * (size_t)rows * cols));
+    HIP_TRY(hipMemcpy(tmp.ptr, host, sizeof(uint64_t) * (size_t)rows * cols,
+                      hipMemcpyHostToDevice));
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    if (grid.y > 65535) return fail(SVS_ERR_ARG, "list of %u rows too large to transpose", rows);
+    hipLaunchKernelGGL(k_transpose_u64, grid, dim3(32, 8), 0, ctx->stream,
+                       (const uint64_t*)tmp.ptr, rows, cols, dst, dst_ld);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+static int upload_mplain(svs_ctx* ctx, ListHost& L, const uint64_t* mp) {
+    const uint32_t m = L.m;
+    if (L.d.mp_t32) {
+        DevBuf& tmp = ctx->tmp_a;
+        SVS_TRY(tmp.ensure(ctx, sizeof(uint64_t) * (size_t)m * N));
+        HIP_TRY(hipMemcpy(tmp.ptr, mp, sizeof(uint64_t) * (size_t)m * N, hipMemcpyHostToDevice));
+        dim3 grid(N / 32, (m + 31) / 32);
+        if (grid.y > 65535) return fail(SVS_ERR_ARG, "list of %u rows too large", m);
+        hipLaunchKernelGGL(k_transpose_mp32, grid, dim3(32, 8), 0, ctx->stream,
+                           (const uint64_t*)tmp.ptr, m, (int32_t*)L.d.mp_t32);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return SVS_OK;
+    }
+    if (L.d.mp_t64) return upload_transposed(ctx, mp, m, N, (uint64_t*)L.d.mp_t64, m);
+    return SVS_OK;
+}
+
+static int check_canonical(const uint64_t* a, size_t count, const char* what) {
+    for (size_t i = 0; i < count; ++i)
+        if (a[i] >= P) return fail(SVS_ERR_ARG, "%s[%zu] is not canonical", what, i);
+    return SVS_OK;
+}
+
+extern "C" int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m,
+                                     const uint64_t* m_enc, const uint64_t* a_l,
+                                     const uint64_t* h, const uint64_t* m_plain,
+                                     const uint32_t* s_rowptr, const uint32_t* s_idx,
+                                     const int8_t* s_sign, const uint32_t* ids) {
+    if (!idx) return fail(SVS_ERR_ARG, "null index");
+    if (list >= idx->lists.size()) return fail(SVS_ERR_ARG, "list %u out of range", list);
+    if (m > 0 && (!m_enc || !a_l || !h || !s_rowptr))
+        return fail(SVS_ERR_ARG, "m_enc, a_l, h and s_rowptr are required");
+    if (m > 0 && s_rowptr[m] > 0 && (!s_idx || !s_sign))
+        return fail(SVS_ERR_ARG, "s_idx and s_sign are required");
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(check_canonical(m_enc, (size_t)m * N, "m_enc"));
+    SVS_TRY(check_canonical(a_l, (size_t)m * N1, "a_l"));
+    SVS_TRY(check_canonical(h, (size_t)m * N1, "h"));
+    if (m_plain) SVS_TRY(check_canonical(m_plain, (size_t)m * N, "m_plain"));
+    for (uint32_t i = 0; i < m; ++i)
+        for (uint32_t e = s_rowptr[i]; e < s_rowptr[i + 1]; ++e)
+            if (s_idx[e] >= (uint32_t)N || (s_sign[e] != 1 && s_sign[e] != -1) ||
+                (e > s_rowptr[i] && s_idx[e] <= s_idx[e - 1]))
+                return fail(SVS_ERR_ARG, "S row %u is not a sorted +-1 row", i);
+    const bool mp32 = m_plain && fits_i32(m_plain, (size_t)m * N);
+    SVS_TRY(list_alloc(idx, list, m, mp32, m_plain != nullptr));
+    ListHost& L = idx->lists[list];
+    if (m > 0) {
+        HIP_TRY(hipMemcpy((void*)L.d.menc, m_enc, sizeof(uint64_t) * (size_t)m * N,
+                          hipMemcpyHostToDevice));
+        SVS_TRY(upload_transposed(ctx, a_l, m, N1, (uint64_t*)L.d.alh_t, m));
+        SVS_TRY(upload_transposed(ctx, h, m, N1, (uint64_t*)L.d.alh_t + (size_t)N1 * m, m));
+        if (m_plain) SVS_TRY(upload_mplain(ctx, L, m_plain));
+        SVS_TRY(list_upload_s_ids(idx, list, s_rowptr, s_idx, s_sign, ids));
+    }
+    return publish_list(idx, list);
+}
+
+extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
+                                    const uint64_t* m_plain, const uint8_t h_seed[32],
+                                    const uint32_t* ids) {
+    if (!idx || !h_seed) return fail(SVS_ERR_ARG, "null argument");
+    if (list >= idx->lists.size()) return fail(SVS_ERR_ARG, "list %u out of range", list);
+    if (m > 0 && !m_plain) return fail(SVS_ERR_ARG, "m_plain is required");
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(check_canonical(m_plain, (size_t)m * N, "m_plain"));
+    // H and S from h_seed on the host (regenerate_h_s, crypto.rs:258-270).
+    std::vector<uint64_t> h;
+    std::vector<uint32_t> rowptr, sidx;
+    std::vector<int8_t> ssign;
+    regenerate_h_s_host(h_seed, m, h, rowptr, sidx, ssign);
+    const bool mp32 = fits_i32(m_plain, (size_t)m * N);
+    SVS_TRY(list_alloc(idx, list, m, mp32, true));
+    ListHost& L = idx->lists[list];
+    if (m > 0) {
+        DevBuf& tm = ctx->tmp_a;
+        DevBuf& th = ctx->tmp_b;
+        DevBuf& ts = ctx->tmp_c;
+        const size_t nnz = sidx.size();
+        SVS_TRY(tm.ensure(ctx, sizeof(uint64_t) * (size_t)m * N));
+        SVS_TRY(th.ensure(ctx, sizeof(uint64_t) * (size_t)m * N1));
+        SVS_TRY(ts.ensure(ctx, sizeof(uint32_t) * (m + 1 + nnz) + nnz + 16));
+        HIP_TRY(hipMemcpy(tm.ptr, m_plain, sizeof(uint64_t) * (size_t)m * N,
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(th.ptr, h.data(), sizeof(uint64_t) * (size_t)m * N1,
+                          hipMemcpyHostToDevice));
+        uint32_t* d_rowptr = (uint32_t*)ts.ptr;
+        uint32_t* d_sidx = d_rowptr + m + 1;
+        int8_t* d_ssign = (int8_t*)(d_sidx + nnz);
+        HIP_TRY(hipMemcpy(d_rowptr, rowptr.data(), sizeof(uint32_t) * (m + 1),
+                          hipMemcpyHostToDevice));
+        if (nnz) {
+            HIP_TRY(hipMemcpy(d_sidx, sidx.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d_ssign, ssign.data(), nnz, hipMemcpyHostToDevice));
+        }
+        const uint64_t* d_m = (const uint64_t*)tm.ptr;
+        hipLaunchKernelGGL(k_encrypt_menc, dim3(m * (N / 256)), dim3(256), 0, ctx->stream, d_m,
+                           (const uint64_t*)th.ptr, (const uint64_t*)idx->lt_dev, d_rowptr,
+                           (const uint32_t*)d_sidx, (const int8_t*)d_ssign, m,
+                           (uint64_t*)L.d.menc);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_encrypt_al, dim3((m + 7) / 8), dim3(128), 0, ctx->stream, d_m,
+                           (const uint64_t*)idx->l_dev, m, (uint64_t*)L.d.alh_t);
+        HIP_TRY(hipGetLastError());
+        dim3 tg((N1 + 31) / 32, (m + 31) / 32);
+        if (tg.y > 65535) return fail(SVS_ERR_ARG, "list of %u rows too large", m);
+        hipLaunchKernelGGL(k_transpose_u64, tg, dim3(32, 8), 0, ctx->stream,
+                           (const uint64_t*)th.ptr, m, (uint32_t)N1,
+                           (uint64_t*)L.d.alh_t + (size_t)N1 * m, m);
+        HIP_TRY(hipGetLastError());
+        if (L.d.mp_t32) {
+            dim3 mg(N / 32, (m + 31) / 32);
+            hipLaunchKernelGGL(k_transpose_mp32, mg, dim3(32, 8), 0, ctx->stream, d_m, m,
+                               (int32_t*)L.d.mp_t32);
+        } else {
+            dim3 mg(N / 32, (m + 31) / 32);
+            hipLaunchKernelGGL(k_transpose_u64, mg, dim3(32, 8), 0, ctx->stream, d_m, m,
+                               (uint32_t)N, (uint64_t*)L.d.mp_t64, m);
+        }
+        HIP_TRY(hipGetLastError());
+        ctx->launches += 4;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        SVS_TRY(list_upload_s_ids(idx, list, rowptr.data(), sidx.data(), ssign.data(), ids));
+    }
+    return publish_list(idx, list);
+}
+
+extern "C" int svs_index_download_list(svs_index* idx, uint32_t list, uint64_t* m_enc,
+                                       uint64_t* a_l, uint64_t* h) {
+    if (!idx) return fail(SVS_ERR_ARG, "null argument");
+    if (list >= idx->lists.size()) return fail(SVS_ERR_ARG, "list %u out of range", list);
+    std::lock_guard<std::mutex> g(idx->ctx->mu);
+    HIP_TRY(hipSetDevice(idx->ctx->device));
+    const ListHost& L = idx->lists[list];
+    const size_t m = L.m;
+    if (!m) return SVS_OK;
+    if (m_enc)
+        HIP_TRY(hipMemcpy(m_enc, L.d.menc, sizeof(uint64_t) * m * N, hipMemcpyDeviceToHost));
+    if (a_l || h) {
+        std::vector<uint64_t> t(m * 2 * N1);
+        HIP_TRY(hipMemcpy(t.data(), L.d.alh_t, sizeof(uint64_t) * t.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < m; ++i)
+            for (int k = 0; k < N1; ++k) {
+                if (a_l) a_l[i * N1 + k] = t[(size_t)k * m + i];
+                if (h) h[i * N1 + k] = t[(size_t)(N1 + k) * m + i];
+            }
+    }
+    return SVS_OK;
+}
+
+// ================================================================== pipeline
+// Device-side work description of a batch: items, prefixes, flags.
+struct BatchPlan {
+    uint32_t n_queries = 0, n_items = 0, n_chunks = 0, n_cands = 0;
+    uint64_t distinct_bytes = 0;  // sum of m_c * n * 8 over distinct lists
+    bool any_verify = false;
+    std::vector<Item> items;
+    std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
+    std::vector<uint8_t> item_verify;
+};
+
+static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
+                     const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl) {
+    pl = BatchPlan();
+    pl.n_queries = nq;
+    if (probe_ptr[0] != 0) return fail(SVS_ERR_ARG, "probe_ptr[0] must be 0");
+    pl.item_ptr.resize(nq + 1);
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (probe_ptr[q + 1] < probe_ptr[q]) return fail(SVS_ERR_ARG, "probe_ptr not monotone");
+        pl.item_ptr[q] = (uint32_t)pl.items.size();
+        for (uint32_t p = probe_ptr[q]; p < probe_ptr[q + 1]; ++p) {
+            const uint32_t c = probe_lists[p];
+            const uint32_t m = c < idx->lists.size() ? idx->lists[c].m : 0;
+            pl.items.push_back(Item{q, c < idx->lists.size() ? c : 0, p - probe_ptr[q], m});
+            const bool v = verify && verify[q];
+            pl.item_verify.push_back(v ? 1 : 0);
+            pl.any_verify |= v && m > 0;
+        }
+    }
+    pl.item_ptr[nq] = (uint32_t)pl.items.size();
+    pl.n_items = (uint32_t)pl.items.size();
+    pl.chunk_ptr.resize(pl.n_items + 1);
+    pl.cand_ptr.resize(pl.n_items + 1);
+    uint64_t ch = 0, cd = 0;
+    for (uint32_t i = 0; i < pl.n_items; ++i) {
+        pl.chunk_ptr[i] = (uint32_t)ch;
+        pl.cand_ptr[i] = (uint32_t)cd;
+        ch += (pl.items[i].m + CHUNK_ROWS - 1) / CHUNK_ROWS;
+        cd += pl.items[i].m;
+    }
+    if (cd >= 0xFFFFFFFFull) return fail(SVS_ERR_ARG, "batch has too many candidate rows");
+    pl.chunk_ptr[pl.n_items] = (uint32_t)ch;
+    pl.cand_ptr[pl.n_items] = (uint32_t)cd;
+    pl.n_chunks = (uint32_t)ch;
+    pl.n_cands = (uint32_t)cd;
+    std::vector<uint32_t> seen;
+    seen.reserve(pl.n_items);
+    for (const Item& it : pl.items) seen.push_back(it.list);
+    std::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+    for (uint32_t c : seen)
+        if (c < idx->lists.size()) pl.distinct_bytes += (uint64_t)idx->lists[c].m * N * 8;
+    return SVS_OK;
+}
+
+// Per query: status = 0, or 1 + trial + 4 * list of the first failing item.
+__global__ void k_query_status(const Item* items, const uint32_t* item_ptr, uint32_t nq,
+                               const int32_t* item_verdict, int32_t* q_status, int any_verify) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int32_t st = 0;
+    if (any_verify)
+        for (uint32_t it = item_ptr[q]; it < item_ptr[q + 1]; ++it)
+            if (item_verdict[it] >= 0) {
+                st = 1 + item_verdict[it] + 4 * (int32_t)items[it].list;
+                break;
+            }
+    q_status[q] = st;
+}
+
+static uint32_t row_grid(svs_ctx* ctx, uint64_t waves_needed) {
+    const uint64_t blocks = (waves_needed + 3) / 4;
+    const uint64_t cap = (uint64_t)ctx->n_cu * 16;
+    return (uint32_t)std::max<uint64_t>(1, std::min(blocks, cap));
+}
+
+// ------------------------------------------------------------ stage timing
+// HIP events recorded on the context stream around each pipeline stage when
+// profiling is on; elapsed times are accumulated after the batch syncs.
+static int prof_mark(svs_ctx* ctx, int stage, bool begin) {
+    if (!ctx->prof.on) return SVS_OK;
+    HIP_TRY(hipEventRecord(ctx->prof.ev[2 * stage + (begin ? 0 : 1)], ctx->stream));
+    if (!begin) ctx->prof.pending |= 1u << stage;
+    return SVS_OK;
+}
+
+static int prof_collect(svs_ctx* ctx) {
+    if (!ctx->prof.on) return SVS_OK;
+    for (int s = 0; s < SVS_PROF_STAGES; ++s) {
+        if (!(ctx->prof.pending & (1u << s))) continue;
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->prof.ev[2 * s], ctx->prof.ev[2 * s + 1]));
+        ctx->prof.ms[s] += ms;
+        ctx->prof.calls[s] += 1;
+    }
+    ctx->prof.pending = 0;
+    return SVS_OK;
+}
+
+extern "C" int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile* out) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (enable && !ctx->prof.created) {
+        for (auto& e : ctx->prof.ev) HIP_TRY(hipEventCreate(&e));
+        ctx->prof.created = true;
+    }
+    if (out) {
+        for (int s = 0; s < SVS_PROF_STAGES; ++s) {
+            out->ms[s] = ctx->prof.ms[s];
+            out->calls[s] = ctx->prof.calls[s];
+        }
+        out->gemv_bytes = ctx->prof.gemv_bytes;
+        out->gemv_rows = ctx->prof.gemv_rows;
+    }
+    if (reset) {
+        for (int s = 0; s < SVS_PROF_STAGES; ++s) ctx->prof.ms[s] = 0, ctx->prof.calls[s] = 0;
+        ctx->prof.gemv_bytes = ctx->prof.gemv_rows = 0;
+    }
+    ctx->prof.on = enable != 0;
+    return SVS_OK;
+}
+
+// Runs encode (+ product, verify, decode, top-k) for one sub-batch whose
+// query inputs are already on the device (ctx->q_*). Asynchronous.
+static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool product_only) {
+    svs_ctx* ctx = idx->ctx;
+    hipStream_t st = ctx->stream;
+    const uint32_t nq = pl.n_queries;
+    // plan arrays
+    SVS_TRY(ctx->items.ensure(ctx, sizeof(Item) * std::max<uint32_t>(pl.n_items, 1)));
+    SVS_TRY(ctx->prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + nq + 1)));
+    SVS_TRY(ctx->flags.ensure(ctx, std::max<uint32_t>(pl.n_items, 1) + 64));
+    uint32_t* d_chunk_ptr = (uint32_t*)ctx->prefixes.ptr;
+    uint32_t* d_cand_ptr = d_chunk_ptr + pl.n_items + 1;
+    uint32_t* d_item_ptr = d_cand_ptr + pl.n_items + 1;
+    HIP_TRY(hipMemcpyAsync(ctx->items.ptr, pl.items.data(), sizeof(Item) * pl.n_items,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_chunk_ptr, pl.chunk_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_cand_ptr, pl.cand_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_item_ptr, pl.item_ptr.data(), sizeof(uint32_t) * (nq + 1),
+                           hipMemcpyHostToDevice, st));
+    if (pl.n_items)
+        HIP_TRY(hipMemcpyAsync(ctx->flags.ptr, pl.item_verify.data(), pl.n_items,
+                               hipMemcpyHostToDevice, st));
+    const Item* d_items = (const Item*)ctx->items.ptr;
+    const ListDev* d_lists = idx->lists_dev;
+    // per-query encode
+    uint64_t* d_venc = (uint64_t*)ctx->q_venc.ptr;
+    uint64_t* d_gl = (uint64_t*)ctx->q_gl.ptr;
+    int8_t* d_tdense = (int8_t*)ctx->q_tdense.ptr;
+    uint16_t* d_tlist = (uint16_t*)ctx->q_tlist.ptr;
+    uint32_t* d_tcnt = (uint32_t*)ctx->q_tcnt.ptr;
+    uint32_t* d_vseed = (uint32_t*)ctx->q_vseed.ptr;
+    SVS_TRY(prof_mark(ctx, 0, true));
+    hipLaunchKernelGGL(k_encode_stream, dim3((nq + 63) / 64), dim3(64), 0, st,
+                       (const uint8_t*)ctx->q_seeds.ptr, nq, d_gl, d_tdense, d_tlist, d_tcnt,
+                       d_vseed);
+    hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
+                       (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
+                       (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
+    hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1), 0, st, (const uint64_t*)idx->l_dev,
+                       (const uint64_t*)d_venc, d_gl);
+    ctx->launches += 3;
+    HIP_TRY(hipGetLastError());
+    SVS_TRY(prof_mark(ctx, 0, false));
+    // server product
+    SVS_TRY(prof_mark(ctx, 1, true));
+    SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+    if (pl.n_chunks) {
+        hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
+                           st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
+                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+                           (uint64_t*)ctx->r.ptr);
+        ctx->launches++;
+        HIP_TRY(hipGetLastError());
+    }
+    SVS_TRY(prof_mark(ctx, 1, false));
+    ctx->prof.gemv_bytes += pl.distinct_bytes;
+    ctx->prof.gemv_rows += pl.n_cands;
+    if (ctx->fault_list >= 0 && pl.n_items) {
+        hipLaunchKernelGGL(k_inject_fault, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
+                           d_items, pl.n_items, (const uint32_t*)d_cand_ptr,
+                           (uint32_t)ctx->fault_list, ctx->fault_row, ctx->fault_delta,
+                           (uint64_t*)ctx->r.ptr);
+        HIP_TRY(hipGetLastError());
+    }
+    if (product_only) return SVS_OK;
+    // verification
+    SVS_TRY(ctx->verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
+    int32_t* d_item_verdict = (int32_t*)ctx->verdict.ptr;
+    int32_t* d_qstatus = d_item_verdict + std::max<uint32_t>(pl.n_items, 1);
+    uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_rng_flag, 1u, 0u);
+    SVS_TRY(prof_mark(ctx, 2, true));
+    if (pl.any_verify) {
+        SVS_TRY(ctx->partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
+        hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS),
+                           0, st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
+                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+                           (const uint64_t*)ctx->r.ptr, (const uint32_t*)d_vseed,
+                           (const uint8_t*)ctx->flags.ptr, (uint64_t*)ctx->partials.ptr,
+                           d_rng_flag);
+        hipLaunchKernelGGL(k_verify_finalize, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
+                           d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
+                           (const uint8_t*)ctx->flags.ptr, (const uint64_t*)ctx->partials.ptr,
+                           d_item_verdict);
+        ctx->launches += 2;
+        HIP_TRY(hipGetLastError());
+    }
+    SVS_TRY(prof_mark(ctx, 2, false));
+    hipLaunchKernelGGL(k_query_status, dim3((nq + 255) / 256), dim3(256), 0, st, d_items,
+                       (const uint32_t*)d_item_ptr, nq, (const int32_t*)d_item_verdict,
+                       d_qstatus, pl.any_verify ? 1 : 0);
+    // decode
+    SVS_TRY(ctx->scores.ensure(ctx, sizeof(int64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+    SVS_TRY(prof_mark(ctx, 3, true));
+    if (pl.n_cands) {
+        const uint32_t blocks =
+            (uint32_t)std::min<uint64_t>((pl.n_cands + 255) / 256, (uint64_t)ctx->n_cu * 32);
+        hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
+                           pl.n_items, (const uint32_t*)d_cand_ptr, (const uint64_t*)ctx->r.ptr,
+                           (const uint64_t*)d_gl, (const uint64_t*)d_venc,
+                           (const uint16_t*)d_tlist, (const uint32_t*)d_tcnt,
+                           (int64_t*)ctx->scores.ptr, (uint64_t*)nullptr);
+        ctx->launches++;
+    }
+    SVS_TRY(prof_mark(ctx, 3, false));
+    // top-k
+    SVS_TRY(prof_mark(ctx, 4, true));
+    SVS_TRY(ctx->out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
+                                     sizeof(uint32_t) * nq + 64));
+    uint32_t* d_oids = (uint32_t*)ctx->out.ptr;
+    int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
+    uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
+    if (k <= 16)
+        hipLaunchKernelGGL((k_topk<16, 256>), dim3(nq), dim3(256), 0, st, d_items,
+                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
+                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
+                           d_oscores, d_ocount);
+    else
+        hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items,
+                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
+                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
+                           d_oscores, d_ocount);
+    ctx->launches += 2;
+    HIP_TRY(hipGetLastError());
+    SVS_TRY(prof_mark(ctx, 4, false));
+    return SVS_OK;
+}
+
+static int stage_queries(svs_ctx* ctx, uint32_t nq, const uint64_t* v_q, const uint8_t* seeds) {
+    SVS_TRY(ctx->q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(ctx->q_seeds.ensure(ctx, (size_t)nq * 32));
+    SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(ctx->q_gl.ensure(ctx, sizeof(uint64_t) * (size_t)nq * 2 * N1));
+    SVS_TRY(ctx->q_tdense.ensure(ctx, (size_t)nq * N));
+    SVS_TRY(ctx->q_tlist.ensure(ctx, sizeof(uint16_t) * (size_t)nq * N));
+    SVS_TRY(ctx->q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
+    SVS_TRY(ctx->q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
+    HIP_TRY(hipMemcpyAsync(ctx->q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
+                           hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyHostToDevice,
+                           ctx->stream));
+    return SVS_OK;
+}
+
+static int collect(svs_ctx* ctx, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,
+                   int64_t* scores, uint32_t* count, int32_t* status) {
+    hipStream_t st = ctx->stream;
+    uint32_t* d_oids = (uint32_t*)ctx->out.ptr;
+    int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
+    uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
+    int32_t* d_qstatus = (int32_t*)ctx->verdict.ptr + std::max<uint32_t>(n_items, 1);
+    uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
+    uint32_t rng_flag = 0;
+    HIP_TRY(hipMemcpyAsync(ids, d_oids, sizeof(uint32_t) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(scores, d_oscores, sizeof(int64_t) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(count, d_ocount, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(status, d_qstatus, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&rng_flag, d_rng_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    SVS_TRY(prof_collect(ctx));
+    if (rng_flag)
+        return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p (probability 2^-61 per draw)");
+    return SVS_OK;
+}
+
+extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mode) {
+    if (!idx || !b) return fail(SVS_ERR_ARG, "null argument");
+    if (b->n_queries == 0) return SVS_OK;
+    if (!b->v_q || !b->q_seeds || !b->probe_ptr || !b->probe_lists || !b->out_ids ||
+        !b->out_scores || !b->out_count || !b->out_status)
+        return fail(SVS_ERR_ARG, "missing batch array");
+    if (b->k == 0 || b->k > 128) return fail(SVS_ERR_ARG, "k = %u must be in 1..128", b->k);
+    if (mode != SVS_MODE_PLAIN && mode != SVS_MODE_BATCHED)
+        return fail(SVS_ERR_ARG, "unknown mode %d", mode);
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(check_canonical(b->v_q, (size_t)b->n_queries * N, "v_q"));
+    const uint32_t nq_total = b->n_queries;
+    const uint32_t step = mode == SVS_MODE_PLAIN ? 1 : nq_total;
+    for (uint32_t q0 = 0; q0 < nq_total; q0 += step) {
+        const uint32_t nq = std::min(step, nq_total - q0);
+        std::vector<uint32_t> pptr(nq + 1);
+        for (uint32_t q = 0; q <= nq; ++q) pptr[q] = b->probe_ptr[q0 + q] - b->probe_ptr[q0];
+        BatchPlan pl;
+        SVS_TRY(make_plan(idx, nq, pptr.data(), b->probe_lists + b->probe_ptr[q0],
+                          b->verify ? b->verify + q0 : nullptr, pl));
+        for (const Item& it : pl.items)
+            if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
+                return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
+        SVS_TRY(stage_queries(ctx, nq, b->v_q + (size_t)q0 * N, b->q_seeds + (size_t)q0 * 32));
+        SVS_TRY(run_plan(idx, pl, b->k, false));
+        SVS_TRY(collect(ctx, nq, b->k, pl.n_items, b->out_ids + (size_t)q0 * b->k,
+                        b->out_scores + (size_t)q0 * b->k, b->out_count + q0,
+                        b->out_status + q0));
+    }
+    return SVS_OK;
+}
+
+// ============================================================ per-list drop-ins
+static int one_item_plan(svs_index* idx, uint32_t list, BatchPlan& pl, bool verify) {
+    uint32_t pptr[2] = {0, 1};
+    uint8_t v = verify ? 1 : 0;
+    return make_plan(idx, 1, pptr, &list, &v, pl);
+}
+
+static int list_ok(svs_index* idx, uint32_t list) {
+    if (!idx) return fail(SVS_ERR_ARG, "null index");
+    if (list >= idx->lists.size() || idx->lists[list].m == 0) return SVS_NONE;
+    return SVS_OK;
+}
+
+extern "C" int svs_compute_products_dev(svs_index* idx, uint32_t list, const uint64_t* v_enc_dev,
+                                        uint64_t* r_dev) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // Single-item plan, cached per list so repeated calls launch without
+    // host-side planning or plan uploads.
+    if (ctx->cached_plan_list != (int64_t)list || ctx->cached_plan_index != idx) {
+        BatchPlan pl;
+        SVS_TRY(one_item_plan(idx, list, pl, false));
+        SVS_TRY(ctx->items1.ensure(ctx, sizeof(Item)));
+        SVS_TRY(ctx->prefixes1.ensure(ctx, sizeof(uint32_t) * 4));
+        uint32_t pre[4] = {pl.chunk_ptr[0], pl.chunk_ptr[1], pl.cand_ptr[0], pl.cand_ptr[1]};
+        HIP_TRY(hipMemcpy(ctx->items1.ptr, pl.items.data(), sizeof(Item), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctx->prefixes1.ptr, pre, sizeof(pre), hipMemcpyHostToDevice));
+        ctx->cached_plan_list = list;
+        ctx->cached_plan_index = idx;
+        ctx->cached_plan_chunks = pl.n_chunks;
+    }
+    const uint32_t* pre = (const uint32_t*)ctx->prefixes1.ptr;
+    hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, ctx->cached_plan_chunks)),
+                       dim3(ROW_THREADS), 0, ctx->stream, (const ListDev*)idx->lists_dev,
+                       (const Item*)ctx->items1.ptr, 1u, pre, pre + 2, v_enc_dev, r_dev);
+    ctx->launches++;
+    HIP_TRY(hipGetLastError());
+    return SVS_OK;
+}
+
+extern "C" int svs_compute_products(svs_index* idx, uint32_t list, const uint64_t* v_enc,
+                                    uint64_t* r) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    if (!v_enc || !r) return fail(SVS_ERR_ARG, "null argument");
+    SVS_TRY(check_canonical(v_enc, N, "v_enc"));
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t m = idx->lists[list].m;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        HIP_TRY(hipSetDevice(ctx->device));
+        SVS_TRY(ctx->io_a.ensure(ctx, sizeof(uint64_t) * N));
+        SVS_TRY(ctx->io_b.ensure(ctx, sizeof(uint64_t) * m));
+        HIP_TRY(hipMemcpyAsync(ctx->io_a.ptr, v_enc, sizeof(uint64_t) * N, hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
+    SVS_TRY(svs_compute_products_dev(idx, list, (const uint64_t*)ctx->io_a.ptr,
+                                     (uint64_t*)ctx->io_b.ptr));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipMemcpyAsync(r, ctx->io_b.ptr, sizeof(uint64_t) * m, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+// Stages a one-item plan with an explicit v_enc and claimed r on the device.
+static int stage_single(svs_index* idx, uint32_t list, const uint64_t* v_enc, bool verify,
+                        BatchPlan& pl) {
+    svs_ctx* ctx = idx->ctx;
+    SVS_TRY(one_item_plan(idx, list, pl, verify));
+    SVS_TRY(ctx->items.ensure(ctx, sizeof(Item)));
+    SVS_TRY(ctx->prefixes.ensure(ctx, sizeof(uint32_t) * 8));
+    SVS_TRY(ctx->flags.ensure(ctx, 64));
+    SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * N));
+    uint32_t pre[6] = {pl.chunk_ptr[0], pl.chunk_ptr[1], pl.cand_ptr[0], pl.cand_ptr[1], 0, 1};
+    HIP_TRY(hipMemcpyAsync(ctx->items.ptr, pl.items.data(), sizeof(Item), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->prefixes.ptr, pre, sizeof(pre), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->flags.ptr, pl.item_verify.data(), 1, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->q_venc.ptr, v_enc, sizeof(uint64_t) * N, hipMemcpyHostToDevice,
+                           ctx->stream));
+    return SVS_OK;
+}
+
+extern "C" int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r,
+                          const uint64_t* v_enc, const uint8_t rng_seed[32], uint32_t trials,
+                          int32_t* failed_trial) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    if (!r || !v_enc || !rng_seed || !failed_trial) return fail(SVS_ERR_ARG, "null argument");
+    if (trials != (uint32_t)LAMBDA)
+        return fail(SVS_ERR_ARG, "trials = %u; Sec128 uses lambda' = %d", trials, LAMBDA);
+    SVS_TRY(check_canonical(v_enc, N, "v_enc"));
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t m = idx->lists[list].m;
+    BatchPlan pl;
+    SVS_TRY(stage_single(idx, list, v_enc, true, pl));
+    SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * m));
+    SVS_TRY(ctx->q_vseed.ensure(ctx, 32));
+    SVS_TRY(ctx->partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * pl.n_chunks));
+    SVS_TRY(ctx->verdict.ensure(ctx, 64));
+    uint32_t key[8];
+    seed_to_key(rng_seed, key);
+    HIP_TRY(hipMemcpyAsync(ctx->q_vseed.ptr, key, 32, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->r.ptr, r, sizeof(uint64_t) * m, hipMemcpyHostToDevice, ctx->stream));
+    int32_t* d_verdict = (int32_t*)ctx->verdict.ptr;
+    uint32_t* d_flag = (uint32_t*)(d_verdict + 2);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, d_flag, 1u, 0u);
+    const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
+    hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
+                       ctx->stream, (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr,
+                       1u, pre, pre + 2, (const uint64_t*)ctx->q_venc.ptr,
+                       (const uint64_t*)ctx->r.ptr, (const uint32_t*)ctx->q_vseed.ptr,
+                       (const uint8_t*)ctx->flags.ptr, (uint64_t*)ctx->partials.ptr, d_flag);
+    hipLaunchKernelGGL(k_verify_finalize, dim3(1), dim3(64), 0, ctx->stream,
+                       (const Item*)ctx->items.ptr, 1u, pre, (const uint8_t*)ctx->flags.ptr,
+                       (const uint64_t*)ctx->partials.ptr, d_verdict);
+    ctx->launches += 2;
+    HIP_TRY(hipGetLastError());
+    int32_t host[3];
+    HIP_TRY(hipMemcpyAsync(host, d_verdict, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (host[2]) return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p");
+    *failed_trial = host[0];
+    return SVS_OK;
+}
+
+extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, const uint64_t* v_enc,
+                          const uint64_t* g, const uint32_t* t_idx, const int8_t* t_sign,
+                          uint32_t t_nnz, uint64_t* mv) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    if (!r || !v_enc || !g || !mv || (t_nnz && (!t_idx || !t_sign)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (t_nnz > (uint32_t)N) return fail(SVS_ERR_ARG, "t_nnz > n");
+    SVS_TRY(check_canonical(v_enc, N, "v_enc"));
+    SVS_TRY(check_canonical(g, N1, "g"));
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t m = idx->lists[list].m;
+    if (!idx->lists[list].d.mp_t32 && !idx->lists[list].d.mp_t64)
+        return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", list);
+    BatchPlan pl;
+    SVS_TRY(stage_single(idx, list, v_enc, false, pl));
+    SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * m));
+    SVS_TRY(ctx->scores.ensure(ctx, sizeof(uint64_t) * m));
+    SVS_TRY(ctx->q_gl.ensure(ctx, sizeof(uint64_t) * 2 * N1));
+    SVS_TRY(ctx->q_tlist.ensure(ctx, sizeof(uint16_t) * N));
+    SVS_TRY(ctx->q_tcnt.ensure(ctx, sizeof(uint32_t)));
+    std::vector<uint16_t> tl(t_nnz);
+    for (uint32_t e = 0; e < t_nnz; ++e) {
+        if (t_idx[e] >= (uint32_t)N || (t_sign[e] != 1 && t_sign[e] != -1))
+            return fail(SVS_ERR_ARG, "T entry %u invalid", e);
+        tl[e] = (uint16_t)(t_idx[e] | (t_sign[e] < 0 ? 0x8000 : 0));
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->r.ptr, r, sizeof(uint64_t) * m, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->q_gl.ptr, g, sizeof(uint64_t) * N1, hipMemcpyHostToDevice,
+                           ctx->stream));
+    if (t_nnz)
+        HIP_TRY(hipMemcpyAsync(ctx->q_tlist.ptr, tl.data(), sizeof(uint16_t) * t_nnz,
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->q_tcnt.ptr, &t_nnz, sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ctx->stream));
+    hipLaunchKernelGGL(k_lt, dim3(1), dim3(N1), 0, ctx->stream, (const uint64_t*)idx->l_dev,
+                       (const uint64_t*)ctx->q_venc.ptr, (uint64_t*)ctx->q_gl.ptr);
+    const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
+    hipLaunchKernelGGL(k_decode_cands, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr, 1u, pre + 2,
+                       (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_gl.ptr,
+                       (const uint64_t*)ctx->q_venc.ptr, (const uint16_t*)ctx->q_tlist.ptr,
+                       (const uint32_t*)ctx->q_tcnt.ptr, (int64_t*)nullptr,
+                       (uint64_t*)ctx->scores.ptr);
+    ctx->launches += 2;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(mv, ctx->scores.ptr, sizeof(uint64_t) * m, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+// decode_dense_terms: AL g + H lt for one list (the reference's GPU decode
+// split, gpu/mod.rs:390-408). Runs the decode kernel with T = 0, S = 0
+// contributions removed by subtraction: out = r - decode(r) with r = 0.
+extern "C" int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint64_t* g,
+                                      const uint64_t* lt_venc, uint64_t* out) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    if (!g || !lt_venc || !out) return fail(SVS_ERR_ARG, "null argument");
+    SVS_TRY(check_canonical(g, N1, "g"));
+    SVS_TRY(check_canonical(lt_venc, N1, "lt_venc"));
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t m = idx->lists[list].m;
+    SVS_TRY(ctx->q_gl.ensure(ctx, sizeof(uint64_t) * 2 * N1));
+    SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * m));
+    HIP_TRY(hipMemcpyAsync(ctx->q_gl.ptr, g, sizeof(uint64_t) * N1, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync((uint64_t*)ctx->q_gl.ptr + N1, lt_venc, sizeof(uint64_t) * N1,
+                           hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_dense_terms, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)idx->lists[list].d.alh_t, m,
+                       (const uint64_t*)ctx->q_gl.ptr, (uint64_t*)ctx->r.ptr);
+    ctx->launches++;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, ctx->r.ptr, sizeof(uint64_t) * m, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+extern "C" int svs_debug_inject_fault(svs_ctx* ctx, int32_t list, uint32_t row, uint64_t delta) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    if (delta >= P) return fail(SVS_ERR_ARG, "delta must be canonical");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->fault_list = list;
+    ctx->fault_row = row;
+    ctx->fault_delta = delta;
+    return SVS_OK;
+}

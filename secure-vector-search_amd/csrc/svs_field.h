@@ -1,0 +1,135 @@
+// svs_field.h — F_p (p = 2^61 - 1) arithmetic and the ChaCha20 keystream,
+// shared by host and gfx950 device code.
+//
+// Semantics follow the reference scorer:
+//   field ops            scorer-bntm/src/crypto.rs:47-89
+//   sample_fp            scorer-bntm/src/crypto.rs:155-162
+//   ChaCha20Rng stream   rand_chacha 0.10 (key = seed, 64-bit block counter,
+//                        zero stream id; next_u64 = two consecutive words,
+//                        low word first)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SVS_HD __host__ __device__ __forceinline__
+
+namespace svs {
+
+constexpr uint64_t P = 0x1FFFFFFFFFFFFFFFull;  // 2^61 - 1
+constexpr int N = 1024;                        // ring dimension (Sec128)
+constexpr int N1 = 128;                        // subspace dimension (Sec128)
+constexpr int LAMBDA = 3;                      // Freivalds trials (params.rs:270-277)
+constexpr uint32_t SPARSE_THRESHOLD = 536870911u;  // (mu * 2^32) as u64, mu = 1024^(0.7-1.0)
+
+// ---------------------------------------------------------------- field ops
+SVS_HD uint64_t add_mod(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    return s >= P ? s - P : s;
+}
+
+SVS_HD uint64_t sub_mod(uint64_t a, uint64_t b) { return a >= b ? a - b : a + P - b; }
+
+// Canonical reduction of any 128-bit value hi:lo. Uses 2^61 = 1 (mod p), so
+// 2^64 = 8: x = lo + 8*hi; the high word is split at bit 58 so that no term
+// overflows. Result in [0, p).
+SVS_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
+    uint64_t s = (lo & P) + (lo >> 61) + ((hi << 3) & P) + (hi >> 58);  // < 2^62 + 72
+    uint64_t r = (s & P) + (s >> 61);                                   // <= p + 1
+    return r >= P ? r - P : r;
+}
+
+// Lazy 128-bit accumulator: acc += a*b without reduction. Callers bound the
+// number of accumulated products (each < 2^122) so the sum stays < 2^128.
+struct Acc128 {
+    uint64_t lo, hi;
+    SVS_HD void zero() { lo = 0; hi = 0; }
+    SVS_HD void mac(uint64_t a, uint64_t b) {
+        unsigned __int128 p = (unsigned __int128)a * b;
+        unsigned __int128 s = ((unsigned __int128)hi << 64 | lo) + p;
+        lo = (uint64_t)s;
+        hi = (uint64_t)(s >> 64);
+    }
+    SVS_HD uint64_t reduce() const { return reduce128(lo, hi); }
+};
+
+SVS_HD uint64_t mul_mod(uint64_t a, uint64_t b) {
+    unsigned __int128 p = (unsigned __int128)a * b;
+    return reduce128((uint64_t)p, (uint64_t)(p >> 64));
+}
+
+// x * 2^s mod p for x < p, 0 <= s < 61: a 61-bit rotation (2^61 = 1).
+SVS_HD uint64_t mul_pow2(uint64_t x, int s) {
+    if (s == 0) return x;
+    uint64_t r = ((x << s) & P) | (x >> (61 - s));
+    return r == P ? 0 : r;
+}
+
+// ------------------------------------------------------------------ ChaCha20
+SVS_HD uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+
+#define SVS_QR(a, b, c, d)                     \
+    a += b; d ^= a; d = svs::rotl32(d, 16);    \
+    c += d; b ^= c; b = svs::rotl32(b, 12);    \
+    a += b; d ^= a; d = svs::rotl32(d, 8);     \
+    c += d; b ^= c; b = svs::rotl32(b, 7);
+
+// One 64-byte ChaCha20 block: key = 8 LE words of the 32-byte seed, block
+// counter in words 12-13, stream id (words 14-15) = 0.
+SVS_HD void chacha20_block(const uint32_t key[8], uint64_t counter, uint32_t out[16]) {
+    uint32_t x0 = 0x61707865u, x1 = 0x3320646eu, x2 = 0x79622d32u, x3 = 0x6b206574u;
+    uint32_t x4 = key[0], x5 = key[1], x6 = key[2], x7 = key[3];
+    uint32_t x8 = key[4], x9 = key[5], x10 = key[6], x11 = key[7];
+    uint32_t x12 = (uint32_t)counter, x13 = (uint32_t)(counter >> 32), x14 = 0, x15 = 0;
+    for (int i = 0; i < 10; ++i) {
+        SVS_QR(x0, x4, x8, x12) SVS_QR(x1, x5, x9, x13)
+        SVS_QR(x2, x6, x10, x14) SVS_QR(x3, x7, x11, x15)
+        SVS_QR(x0, x5, x10, x15) SVS_QR(x1, x6, x11, x12)
+        SVS_QR(x2, x7, x8, x13) SVS_QR(x3, x4, x9, x14)
+    }
+    out[0] = x0 + 0x61707865u; out[1] = x1 + 0x3320646eu;
+    out[2] = x2 + 0x79622d32u; out[3] = x3 + 0x6b206574u;
+    out[4] = x4 + key[0]; out[5] = x5 + key[1]; out[6] = x6 + key[2]; out[7] = x7 + key[3];
+    out[8] = x8 + key[4]; out[9] = x9 + key[5]; out[10] = x10 + key[6]; out[11] = x11 + key[7];
+    out[12] = x12 + (uint32_t)counter; out[13] = x13 + (uint32_t)(counter >> 32);
+    out[14] = x14; out[15] = x15;
+}
+
+SVS_HD void seed_to_key(const uint8_t seed[32], uint32_t key[8]) {
+    for (int i = 0; i < 8; ++i)
+        key[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) |
+                 ((uint32_t)seed[4 * i + 2] << 16) | ((uint32_t)seed[4 * i + 3] << 24);
+}
+
+// Sequential word stream (ChaCha20Rng::next_u32 / next_u64).
+struct Stream {
+    uint32_t key[8];
+    uint64_t block;
+    uint32_t buf[16];
+    int idx;
+    SVS_HD void init(const uint32_t k[8]) {
+        for (int i = 0; i < 8; ++i) key[i] = k[i];
+        block = 0;
+        idx = 16;
+    }
+    SVS_HD uint32_t u32() {
+        if (idx >= 16) {
+            chacha20_block(key, block++, buf);
+            idx = 0;
+        }
+        return buf[idx++];
+    }
+    SVS_HD uint64_t u64() {
+        uint64_t lo = u32();
+        return lo | ((uint64_t)u32() << 32);
+    }
+    // sample_fp: 61-bit mask, reject the single value p.
+    SVS_HD uint64_t fp() {
+        for (;;) {
+            uint64_t x = u64() & P;
+            if (x < P) return x;
+        }
+    }
+};
+
+}  // namespace svs

@@ -1,0 +1,108 @@
+// svs_internal.h — host-side state behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/svs_gpu.h"
+#include "svs_types.h"
+
+struct svs_ctx;
+
+namespace svs {
+
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::svs::fail(SVS_ERR_HIP, "%s failed: %s (%s:%d)", #expr,            \
+                               hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+
+#define SVS_TRY(expr)              \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != SVS_OK) return rc_; \
+    } while (0)
+
+// Growable device scratch buffer owned by a context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    int ensure(svs_ctx* ctx, size_t bytes);
+    void release(svs_ctx* ctx);
+};
+
+// Resident allocation against the context budget.
+int dev_alloc(svs_ctx* ctx, void** p, size_t bytes);
+void dev_free(svs_ctx* ctx, void* p, size_t bytes);
+
+// regenerate_h_s (crypto.rs:258-270) on the host: H (m x n1 row-major) then
+// S as CSR, from ChaCha20Rng(h_seed). Keystream blocks are generated in
+// parallel; consumption is sequential and exact (rejections included).
+void regenerate_h_s_host(const uint8_t h_seed[32], uint32_t m, std::vector<uint64_t>& h,
+                         std::vector<uint32_t>& rowptr, std::vector<uint32_t>& sidx,
+                         std::vector<int8_t>& ssign);
+
+struct ListHost {
+    uint32_t m = 0;
+    ListDev d{};
+    std::vector<std::pair<void*, size_t>> allocs;
+};
+
+}  // namespace svs
+
+constexpr int SVS_PROF_STAGES = 5;
+
+struct svs_prof {
+    bool on = false, created = false;
+    hipEvent_t ev[2 * SVS_PROF_STAGES] = {};
+    double ms[SVS_PROF_STAGES] = {};
+    uint64_t calls[SVS_PROF_STAGES] = {};
+    uint64_t gemv_bytes = 0, gemv_rows = 0;
+    uint32_t pending = 0;
+};
+
+struct svs_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    uint64_t budget_bytes = 0, hbm_total = 0;
+    uint64_t resident_bytes = 0, scratch_bytes = 0, peak_bytes = 0;
+    uint64_t uploads = 0, launches = 0;
+    std::mutex mu;
+    // scratch
+    svs::DevBuf tmp_a, tmp_b, tmp_c;
+    svs::DevBuf items, prefixes, flags, items1, prefixes1;
+    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed;
+    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b;
+    // cached single-item plan of svs_compute_products_dev
+    int64_t cached_plan_list = -1;
+    const void* cached_plan_index = nullptr;
+    uint32_t cached_plan_chunks = 0;
+    // test hook: server-side fault injected after the product (list < 0: off)
+    int32_t fault_list = -1;
+    uint32_t fault_row = 0;
+    uint64_t fault_delta = 0;
+    svs_prof prof;
+    void note_peak();
+    std::vector<svs::DevBuf*> all_scratch() {
+        return {&tmp_a, &tmp_b, &tmp_c, &items, &prefixes, &flags, &items1, &prefixes1,
+                &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed,
+                &r, &scores, &partials, &verdict, &out, &io_a, &io_b};
+    }
+};
+
+struct svs_index {
+    svs_ctx* ctx = nullptr;
+    uint64_t* l_dev = nullptr;   // L, n x n1 row-major
+    uint64_t* lt_dev = nullptr;  // L^T, n1 x n
+    svs::ListDev* lists_dev = nullptr;
+    std::vector<svs::ListHost> lists;
+};

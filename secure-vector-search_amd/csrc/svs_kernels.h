@@ -1,0 +1,581 @@
+// svs_kernels.h — gfx950 kernels for the BN encrypted-score path.
+//
+// Work is described as ITEMS: one (query q, probed list c, probe position)
+// per item. Items of one query are contiguous and in probe order. Each item
+// owns m_c consecutive CANDIDATE slots (one per list row) in the per-batch
+// r / score arrays; cand_ptr is the prefix of m_c over items. Row kernels
+// walk CHUNKS of up to CHUNK_ROWS rows of one item; chunk_ptr is the prefix
+// of ceil(m_c / CHUNK_ROWS). Every kernel bounds-checks its chunk, item and
+// row indices against these prefixes.
+#pragma once
+
+#include "svs_types.h"
+
+namespace svs {
+
+// ----------------------------------------------------------- small helpers
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int mask) {
+    uint32_t lo = __shfl_xor((uint32_t)x, mask, WAVE);
+    uint32_t hi = __shfl_xor((uint32_t)(x >> 32), mask, WAVE);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_mod(uint64_t x) {
+#pragma unroll
+    for (int off = WAVE / 2; off >= 1; off >>= 1) x = add_mod(x, shfl_xor_u64(x, off));
+    return x;
+}
+
+// Largest i in [0, n) with ptr[i] <= v (ptr non-decreasing, ptr[0] = 0).
+__device__ __forceinline__ uint32_t find_seg(const uint32_t* ptr, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;  // invariant: ptr[lo] <= v < ptr[hi] (ptr[n] > v)
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (ptr[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int64_t fp_to_signed(uint64_t s) {
+    return s < P / 2 ? (int64_t)s : (int64_t)s - (int64_t)P;
+}
+
+// ================================================================ products
+// Server product over item chunks: r[cand_ptr[it] + row] = M_enc[row] . v_enc[q].
+// One wave per chunk; lane l owns columns 128 j + 2 l + {0,1}, j = 0..7, so a
+// row is streamed as eight fully coalesced 1 KiB loads. v_enc lives in
+// registers for the chunk. Each lane accumulates 16 products (< 2^126) in a
+// lazy 128-bit accumulator, reduces once, and the wave sums mod p.
+__global__ void __launch_bounds__(ROW_THREADS)
+k_gemv_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ items, uint32_t n_items,
+              const uint32_t* __restrict__ chunk_ptr, const uint32_t* __restrict__ cand_ptr,
+              const uint64_t* __restrict__ venc, uint64_t* __restrict__ r_out) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t waves_per_block = blockDim.x / WAVE;
+    const uint32_t n_chunks = chunk_ptr[n_items];
+    for (uint32_t ch = blockIdx.x * waves_per_block + threadIdx.x / WAVE; ch < n_chunks;
+         ch += gridDim.x * waves_per_block) {
+        const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+        const Item item = items[it];
+        const ListDev L = lists[item.list];
+        const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
+        const uint32_t rows = min((uint32_t)CHUNK_ROWS, item.m - row0);
+        const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(venc + (size_t)item.q * N);
+        ulonglong2 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = vq[j * WAVE + lane];
+        uint64_t* r = r_out + cand_ptr[it] + row0;
+        for (uint32_t i = 0; i < rows; ++i) {
+            const ulonglong2* row =
+                reinterpret_cast<const ulonglong2*>(L.menc + (size_t)(row0 + i) * N);
+            Acc128 acc;
+            acc.zero();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ulonglong2 a = row[j * WAVE + lane];
+                acc.mac(a.x, v[j].x);
+                acc.mac(a.y, v[j].y);
+            }
+            uint64_t s = wave_sum_mod(acc.reduce());
+            if (lane == 0) r[i] = s;
+        }
+    }
+}
+
+// ============================================================ verification
+// Protocol-2 Freivalds (crypto.rs:1013-1046) per item. The reference checks
+// u_t . r == (u_t^T M_enc) . v_enc. Both sides are exact field values; this
+// kernel evaluates the right side in row order, sum_i u_t[i] (M_enc[i] . v),
+// which is the same element of F_p. u_t[i] is draw t*m + i of the
+// ChaCha20Rng seeded with the item's verification seed (the stream restarts
+// for every list, ivf.rs:790-804): words 2(t m + i), 2(t m + i) + 1. A draw
+// equal to p (rejected by sample_fp) would shift the stream; the kernel flags
+// it (probability 2^-61 per draw) so the host fails loudly.
+// Output: per chunk, 2*LAMBDA canonical partial sums (ur_t, zv_t).
+__global__ void __launch_bounds__(ROW_THREADS)
+k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ items,
+                uint32_t n_items, const uint32_t* __restrict__ chunk_ptr,
+                const uint32_t* __restrict__ cand_ptr, const uint64_t* __restrict__ venc,
+                const uint64_t* __restrict__ r_claim, const uint32_t* __restrict__ vseed_words,
+                const uint8_t* __restrict__ item_verify, uint64_t* __restrict__ partials,
+                uint32_t* __restrict__ rng_flag) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t wave = threadIdx.x / WAVE;
+    const uint32_t waves_per_block = blockDim.x / WAVE;
+    // words for up to 3 blocks per trial: [trial][3 blocks][16 words]
+    __shared__ uint32_t words[ROW_THREADS / WAVE][LAMBDA][48];
+    const uint32_t n_chunks = chunk_ptr[n_items];
+    for (uint32_t ch = blockIdx.x * waves_per_block + wave; ch < n_chunks;
+         ch += gridDim.x * waves_per_block) {
+        const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+        const Item item = items[it];
+        if (!item_verify[it]) {
+            if (lane < 2 * LAMBDA) partials[(size_t)ch * 2 * LAMBDA + lane] = 0;
+            continue;
+        }
+        const ListDev L = lists[item.list];
+        const uint32_t m = item.m;
+        const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
+        const uint32_t rows = min((uint32_t)CHUNK_ROWS, m - row0);
+        // Keystream blocks covering draws t*m + row0 .. t*m + row0 + 15.
+        if (lane < 3 * LAMBDA) {
+            const int t = lane / 3, b = lane % 3;
+            const uint64_t fb = (2ull * ((uint64_t)t * m + row0)) / 16;
+            uint32_t key[8];
+#pragma unroll
+            for (int w = 0; w < 8; ++w) key[w] = vseed_words[(size_t)item.q * 8 + w];
+            uint32_t out[16];
+            chacha20_block(key, fb + b, out);
+#pragma unroll
+            for (int w = 0; w < 16; ++w) words[wave][t][b * 16 + w] = out[w];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(venc + (size_t)item.q * N);
+        ulonglong2 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = vq[j * WAVE + lane];
+        const uint64_t* rc = r_claim + cand_ptr[it] + row0;
+        // lane t accumulates trial t's sums (lanes >= LAMBDA idle in this part)
+        Acc128 ur, zv;
+        ur.zero();
+        zv.zero();
+        bool rejected = false;
+        for (uint32_t i = 0; i < rows; ++i) {
+            const ulonglong2* row =
+                reinterpret_cast<const ulonglong2*>(L.menc + (size_t)(row0 + i) * N);
+            Acc128 acc;
+            acc.zero();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ulonglong2 a = row[j * WAVE + lane];
+                acc.mac(a.x, v[j].x);
+                acc.mac(a.y, v[j].y);
+            }
+            const uint64_t d = wave_sum_mod(acc.reduce());
+            if (lane < LAMBDA) {
+                const uint64_t fb = (2ull * ((uint64_t)lane * m + row0)) / 16;
+                const uint64_t wpos = 2ull * ((uint64_t)lane * m + row0 + i) - fb * 16;
+                const uint64_t raw = (uint64_t)words[wave][lane][wpos] |
+                                     ((uint64_t)words[wave][lane][wpos + 1] << 32);
+                const uint64_t u = raw & P;
+                rejected |= (u == P);
+                ur.mac(u, rc[i]);
+                zv.mac(u, d);
+            }
+        }
+        if (lane < LAMBDA) {
+            partials[(size_t)ch * 2 * LAMBDA + lane] = ur.reduce();
+            partials[(size_t)ch * 2 * LAMBDA + LAMBDA + lane] = zv.reduce();
+            if (rejected) atomicOr(rng_flag, 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Per item: sum chunk partials, verdict = first trial whose sums differ.
+__global__ void k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
+                                  const uint32_t* __restrict__ chunk_ptr,
+                                  const uint8_t* __restrict__ item_verify,
+                                  const uint64_t* __restrict__ partials,
+                                  int32_t* __restrict__ item_verdict) {
+    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n_items) return;
+    if (!item_verify[it] || items[it].m == 0) {
+        item_verdict[it] = -1;
+        return;
+    }
+    uint64_t s[2 * LAMBDA] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t c = chunk_ptr[it]; c < chunk_ptr[it + 1]; ++c)
+        for (int k = 0; k < 2 * LAMBDA; ++k) s[k] = add_mod(s[k], partials[(size_t)c * 2 * LAMBDA + k]);
+    int32_t verdict = -1;
+    for (int t = LAMBDA - 1; t >= 0; --t)
+        if (s[t] != s[LAMBDA + t]) verdict = t;
+    item_verdict[it] = verdict;
+}
+
+// ================================================================== decode
+// Client decode (crypto.rs:901-940), one thread per candidate row:
+//   Mv = r - (AL g + M T) - (H lt + S v_enc),  lt = L^T v_enc.
+// Column-major client state keeps every access coalesced across a wave:
+// AL^T / H^T rows, the gathered columns M_plain^T[t_e], and S sliced-ELL.
+// Writes the signed score (fp_to_signed, crypto.rs:128-135).
+__global__ void __launch_bounds__(256)
+k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items,
+               uint32_t n_items, const uint32_t* __restrict__ cand_ptr,
+               const uint64_t* __restrict__ r, const uint64_t* __restrict__ gl /* q x 256 */,
+               const uint64_t* __restrict__ venc, const uint16_t* __restrict__ t_list,
+               const uint32_t* __restrict__ t_cnt, int64_t* __restrict__ scores,
+               uint64_t* __restrict__ mv_out) {
+    const uint32_t total = cand_ptr[n_items];
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += gridDim.x * blockDim.x) {
+        const uint32_t it = find_seg(cand_ptr, n_items, g);
+        const Item item = items[it];
+        const ListDev L = lists[item.list];
+        const uint32_t i = g - cand_ptr[it];
+        const uint32_t m = item.m;
+        // AL g + H lt as one 256-long dot product against [g ; lt].
+        const uint64_t* glq = gl + (size_t)item.q * 2 * N1;
+        Acc128 acc;
+        acc.zero();
+        uint64_t dense = 0;
+        for (int k = 0; k < 2 * N1; ++k) {
+            acc.mac(L.alh_t[(size_t)k * m + i], glq[k]);
+            if ((k & 31) == 31) {
+                dense = add_mod(dense, acc.reduce());
+                acc.zero();
+            }
+        }
+        // M T: gathered columns of M_plain (sparse +-1 mask T of the query).
+        uint64_t mt = 0;
+        const uint16_t* tq = t_list + (size_t)item.q * N;
+        const uint32_t tn = t_cnt[item.q];
+        for (uint32_t e = 0; e < tn; ++e) {
+            const uint16_t te = tq[e];
+            const uint32_t col = te & 0x3FF;
+            uint64_t x;
+            if (L.mp_t32) {
+                const int32_t s = L.mp_t32[(size_t)col * m + i];
+                x = s >= 0 ? (uint64_t)s : P - (uint64_t)(-(int64_t)s);
+            } else {
+                x = L.mp_t64[(size_t)col * m + i];
+            }
+            mt = (te & 0x8000) ? sub_mod(mt, x) : add_mod(mt, x);
+        }
+        // S v_enc from the row's sliced-ELL entries.
+        const uint64_t* vq = venc + (size_t)item.q * N;
+        const uint32_t slice = i / SLICE, lane = i % SLICE;
+        const uint32_t s0 = L.s_slice[slice], s1 = L.s_slice[slice + 1];
+        uint64_t sv = 0;
+        for (uint32_t e = s0 + lane; e < s1; e += SLICE) {
+            const uint16_t se = L.s_ell[e];
+            if (se == S_PAD) break;
+            const uint64_t x = vq[se & 0x3FF];
+            sv = (se & 0x8000) ? sub_mod(sv, x) : add_mod(sv, x);
+        }
+        const uint64_t corr = add_mod(add_mod(dense, mt), sv);
+        const uint64_t mv = sub_mod(r[g], corr);
+        if (scores) scores[g] = fp_to_signed(mv);
+        if (mv_out) mv_out[g] = mv;
+    }
+}
+
+// ================================================================== top-k
+// Per query: the k best (score desc, candidate position asc) over the
+// query's candidates, which are contiguous and in probe order. Each thread
+// keeps a sorted list in LDS over a strided subset, then the block merges
+// by repeated arg-max over list heads.
+struct Cand {
+    int64_t score;
+    uint32_t pos;
+    uint32_t pad;
+};
+
+__device__ __forceinline__ bool cand_better(const Cand& a, const Cand& b) {
+    return a.score > b.score || (a.score == b.score && a.pos < b.pos);
+}
+
+template <int KT, int THREADS>
+__global__ void __launch_bounds__(THREADS)
+k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
+       const uint32_t* __restrict__ cand_ptr, const ListDev* __restrict__ lists,
+       const int64_t* __restrict__ scores, const int32_t* __restrict__ q_status, uint32_t k,
+       uint32_t* __restrict__ out_ids, int64_t* __restrict__ out_scores,
+       uint32_t* __restrict__ out_count) {
+    __shared__ Cand lst[THREADS][KT];
+    __shared__ int cnt[THREADS];
+    __shared__ int head[THREADS];
+    __shared__ int red_idx[THREADS];
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (q_status[q] != 0) {
+        if (tid == 0) out_count[q] = 0;
+        return;
+    }
+    const uint32_t i0 = item_ptr[q], i1 = item_ptr[q + 1];
+    const uint32_t c0 = cand_ptr[i0], c1 = cand_ptr[i1];
+    int n = 0;
+    for (uint32_t g = c0 + tid; g < c1; g += THREADS) {
+        Cand c{scores[g], g - c0, 0};
+        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
+        int pos = n < (int)k ? n : (int)k - 1;
+        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
+            lst[tid][pos] = lst[tid][pos - 1];
+            --pos;
+        }
+        lst[tid][pos] = c;
+        if (n < (int)k) ++n;
+    }
+    cnt[tid] = n;
+    head[tid] = 0;
+    __syncthreads();
+    const uint32_t total = c1 - c0;
+    const uint32_t kk = min(k, total);
+    for (uint32_t outi = 0; outi < kk; ++outi) {
+        red_idx[tid] = head[tid] < cnt[tid] ? tid : -1;
+        __syncthreads();
+        for (int s = THREADS / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                int a = red_idx[tid], b = red_idx[tid + s];
+                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                    red_idx[tid] = b;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int w = red_idx[0];
+            const Cand c = lst[w][head[w]];
+            head[w]++;
+            // map candidate position back to (item, row) -> global id
+            const uint32_t g = c0 + c.pos;
+            uint32_t lo = i0, hi = i1;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (cand_ptr[mid] <= g) lo = mid; else hi = mid;
+            }
+            const Item it = items[lo];
+            out_ids[(size_t)q * k + outi] = lists[it.list].ids[g - cand_ptr[lo]];
+            out_scores[(size_t)q * k + outi] = c.score;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out_count[q] = kk;
+}
+
+// ================================================================== encode
+// encode_query (crypto.rs:508-553) stream part, one thread per query:
+// G = 128 sample_fp draws, then T = sample_sparse_pm1 over n = 1024
+// (crypto.rs:207-223): a word below the threshold marks a non-zero and the
+// next word's low bit picks its sign (0 -> -1). Also derives the
+// verification seed domain_separate(q_seed, "protocol2") (crypto.rs:142-150).
+__global__ void k_encode_stream(const uint8_t* __restrict__ q_seeds, uint32_t n_queries,
+                                uint64_t* __restrict__ g_out /* q x 256 (first 128) */,
+                                int8_t* __restrict__ t_dense /* q x 1024 */,
+                                uint16_t* __restrict__ t_list /* q x 1024 */,
+                                uint32_t* __restrict__ t_cnt, uint32_t* __restrict__ vseed_words) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_queries) return;
+    uint8_t seed[32];
+    for (int i = 0; i < 32; ++i) seed[i] = q_seeds[(size_t)q * 32 + i];
+    uint32_t key[8];
+    seed_to_key(seed, key);
+    Stream s;
+    s.init(key);
+    for (int k = 0; k < N1; ++k) g_out[(size_t)q * 2 * N1 + k] = s.fp();
+    uint32_t cnt = 0;
+    for (int j = 0; j < N; ++j) {
+        int8_t v = 0;
+        if (s.u32() < SPARSE_THRESHOLD) {
+            v = (s.u32() & 1u) == 0 ? -1 : 1;
+            t_list[(size_t)q * N + cnt++] = (uint16_t)(j | (v < 0 ? 0x8000 : 0));
+        }
+        t_dense[(size_t)q * N + j] = v;
+    }
+    t_cnt[q] = cnt;
+    // domain_separate(q_seed, "protocol2")
+    const char tag[9] = {'p', 'r', 'o', 't', 'o', 'c', 'o', 'l', '2'};
+    for (int i = 0; i < 9; ++i) seed[i] ^= (uint8_t)tag[i];
+    seed_to_key(seed, key);
+    Stream d;
+    d.init(key);
+    uint8_t out[32];
+    for (int i = 0; i < 32; ++i) out[i] = (uint8_t)d.u32();
+    seed_to_key(out, key);
+    for (int w = 0; w < 8; ++w) vseed_words[(size_t)q * 8 + w] = key[w];
+}
+
+// v_enc = L G + T + v (crypto.rs:533-551). Block = (query, 256 columns);
+// L^T (n1 x n) keeps the column reads coalesced; G is broadcast from LDS.
+__global__ void __launch_bounds__(256)
+k_encode_venc(const uint64_t* __restrict__ lt_mat /* 128 x 1024 */,
+              const uint64_t* __restrict__ g /* q x 256 */, const int8_t* __restrict__ t_dense,
+              const uint64_t* __restrict__ v_q, uint64_t* __restrict__ venc) {
+    __shared__ uint64_t gs[N1];
+    const uint32_t q = blockIdx.y;
+    if (threadIdx.x < N1) gs[threadIdx.x] = g[(size_t)q * 2 * N1 + threadIdx.x];
+    __syncthreads();
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    Acc128 acc;
+    acc.zero();
+    uint64_t lg = 0;
+    for (int k = 0; k < N1; ++k) {
+        acc.mac(lt_mat[(size_t)k * N + j], gs[k]);
+        if ((k & 31) == 31) {
+            lg = add_mod(lg, acc.reduce());
+            acc.zero();
+        }
+    }
+    const int8_t t = t_dense[(size_t)q * N + j];
+    if (t == 1) lg = add_mod(lg, 1);
+    else if (t == -1) lg = sub_mod(lg, 1);
+    venc[(size_t)q * N + j] = add_mod(lg, v_q[(size_t)q * N + j]);
+}
+
+// lt = L^T v_enc (l_transpose_times_vec, crypto.rs:846-848): thread per
+// (query, k), L row-major (n x n1) read coalesced across k. Written into
+// the second half of the query's [g ; lt] vector.
+__global__ void __launch_bounds__(128)
+k_lt(const uint64_t* __restrict__ l_mat, const uint64_t* __restrict__ venc,
+     uint64_t* __restrict__ gl) {
+    const uint32_t q = blockIdx.x;
+    const uint32_t k = threadIdx.x;
+    const uint64_t* v = venc + (size_t)q * N;
+    Acc128 acc;
+    acc.zero();
+    uint64_t s = 0;
+    for (int j = 0; j < N; ++j) {
+        acc.mac(v[j], l_mat[(size_t)j * N1 + k]);
+        if ((j & 31) == 31) {
+            s = add_mod(s, acc.reduce());
+            acc.zero();
+        }
+    }
+    gl[(size_t)q * 2 * N1 + N1 + k] = s;
+}
+
+// ============================================================= index build
+// encrypt_cluster (crypto.rs:405-478): M_enc[i][j] = M[i][j] + (H L^T)[i][j]
+// + S[i][j]. Block = (row i, 256 columns); H row in LDS, L^T columns
+// coalesced; the row's S entries are scattered into a dense LDS mask.
+__global__ void __launch_bounds__(256)
+k_encrypt_menc(const uint64_t* __restrict__ mplain, const uint64_t* __restrict__ h,
+               const uint64_t* __restrict__ lt_mat, const uint32_t* __restrict__ s_rowptr,
+               const uint32_t* __restrict__ s_idx, const int8_t* __restrict__ s_sign, uint32_t m,
+               uint64_t* __restrict__ menc) {
+    __shared__ uint64_t hs[N1];
+    __shared__ int8_t sm[N];
+    const uint32_t i = blockIdx.x / (N / 256);
+    if (i >= m) return;
+    const uint32_t tid = threadIdx.x;
+    if (tid < N1) hs[tid] = h[(size_t)i * N1 + tid];
+    for (int j = tid; j < N; j += blockDim.x) sm[j] = 0;
+    __syncthreads();
+    for (uint32_t e = s_rowptr[i] + tid; e < s_rowptr[i + 1]; e += blockDim.x)
+        sm[s_idx[e] & 0x3FF] = s_sign[e];
+    __syncthreads();
+    const uint32_t j = (blockIdx.x % (N / 256)) * 256 + tid;
+    Acc128 acc;
+    acc.zero();
+    uint64_t hl = 0;
+    for (int k = 0; k < N1; ++k) {
+        acc.mac(hs[k], lt_mat[(size_t)k * N + j]);
+        if ((k & 31) == 31) {
+            hl = add_mod(hl, acc.reduce());
+            acc.zero();
+        }
+    }
+    const int8_t sv = sm[j];
+    const uint64_t mp = sv == 1 ? add_mod(hl, 1) : sv == -1 ? sub_mod(hl, 1) : hl;
+    menc[(size_t)i * N + j] = add_mod(mplain[(size_t)i * N + j], mp);
+}
+
+// AL = M L (crypto.rs:457-462), written transposed into alh_t rows 0..127:
+// block = 8 rows x 128 k; M rows in LDS, L read coalesced across k.
+__global__ void __launch_bounds__(128)
+k_encrypt_al(const uint64_t* __restrict__ mplain, const uint64_t* __restrict__ l_mat, uint32_t m,
+             uint64_t* __restrict__ alh_t) {
+    constexpr int RB = 8;
+    __shared__ uint64_t rows[RB][N];
+    const uint32_t i0 = blockIdx.x * RB;
+    const uint32_t k = threadIdx.x;
+    for (int r = 0; r < RB; ++r)
+        for (int j = k; j < N; j += 128)
+            rows[r][j] = (i0 + r < m) ? mplain[(size_t)(i0 + r) * N + j] : 0;
+    __syncthreads();
+    Acc128 acc[RB];
+    uint64_t s[RB];
+    for (int r = 0; r < RB; ++r) {
+        acc[r].zero();
+        s[r] = 0;
+    }
+    for (int j = 0; j < N; ++j) {
+        const uint64_t lv = l_mat[(size_t)j * N1 + k];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r].mac(rows[r][j], lv);
+        if ((j & 31) == 31) {
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                s[r] = add_mod(s[r], acc[r].reduce());
+                acc[r].zero();
+            }
+        }
+    }
+    for (int r = 0; r < RB; ++r)
+        if (i0 + r < m) alh_t[(size_t)k * m + i0 + r] = s[r];
+}
+
+// Generic tiled transpose of a rows x cols u64 matrix into cols x rows,
+// writing at dst + c * dst_ld + r (dst_ld >= rows).
+__global__ void k_transpose_u64(const uint64_t* __restrict__ src, uint32_t rows, uint32_t cols,
+                                uint64_t* __restrict__ dst, uint32_t dst_ld) {
+    __shared__ uint64_t tile[32][33];
+    const uint32_t c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const uint32_t r = r0 + y, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[y][threadIdx.x] = src[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const uint32_t c = c0 + y, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) dst[(size_t)c * dst_ld + r] = tile[threadIdx.x][y];
+    }
+}
+
+// M_plain (rows x 1024 canonical) -> 1024 x rows signed 32-bit lift.
+__global__ void k_transpose_mp32(const uint64_t* __restrict__ src, uint32_t rows,
+                                 int32_t* __restrict__ dst) {
+    __shared__ int32_t tile[32][33];
+    const uint32_t c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const uint32_t r = r0 + y, c = c0 + threadIdx.x;
+        if (r < rows && c < N) {
+            const uint64_t x = src[(size_t)r * N + c];
+            tile[y][threadIdx.x] = x < P / 2 ? (int32_t)x : (int32_t)((int64_t)x - (int64_t)P);
+        }
+    }
+    __syncthreads();
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const uint32_t c = c0 + y, r = r0 + threadIdx.x;
+        if (r < rows && c < N) dst[(size_t)c * rows + r] = tile[threadIdx.x][y];
+    }
+}
+
+// AL g + H lt for one list (decode_dense_terms): thread per row over the
+// column-major [AL^T ; H^T] block.
+__global__ void __launch_bounds__(256)
+k_dense_terms(const uint64_t* __restrict__ alh_t, uint32_t m, const uint64_t* __restrict__ gl,
+              uint64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    Acc128 acc;
+    acc.zero();
+    uint64_t s = 0;
+    for (int k = 0; k < 2 * N1; ++k) {
+        acc.mac(alh_t[(size_t)k * m + i], gl[k]);
+        if ((k & 31) == 31) {
+            s = add_mod(s, acc.reduce());
+            acc.zero();
+        }
+    }
+    out[i] = s;
+}
+
+// Test hook: r[row] += delta for every item of `list` (a server that
+// returns a wrong product, as the reference's tamper test, crypto.rs:1457).
+__global__ void k_inject_fault(const Item* __restrict__ items, uint32_t n_items,
+                               const uint32_t* __restrict__ cand_ptr, uint32_t list, uint32_t row,
+                               uint64_t delta, uint64_t* __restrict__ r) {
+    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n_items) return;
+    if (items[it].list == list && row < items[it].m)
+        r[cand_ptr[it] + row] = add_mod(r[cand_ptr[it] + row], delta);
+}
+
+// Fill kernel (used instead of hipMemsetAsync so a captured graph stays valid).
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace svs

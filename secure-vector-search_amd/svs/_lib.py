@@ -1,0 +1,145 @@
+"""ctypes binding of libsvs_gpu.so (include/svs_gpu.h).
+
+The library must have been built in-tree (secure-vector-search_amd/build.py);
+there is no fallback: a missing library raises at import of the first call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvs_gpu.so")
+
+SVS_OK, SVS_NONE = 0, 1
+SVS_ERR_ARG, SVS_ERR_HIP, SVS_ERR_BUDGET, SVS_ERR_STATE, SVS_ERR_RNG = -1, -2, -3, -4, -5
+MODE_PLAIN, MODE_BATCHED = 0, 1
+N, N1, LAMBDA = 1024, 128, 3
+P = (1 << 61) - 1
+
+EXPORTS = [
+    "svs_last_error", "svs_ctx_create", "svs_ctx_destroy", "svs_ctx_stats", "svs_index_create",
+    "svs_index_destroy", "svs_index_upload_list", "svs_index_build_list",
+    "svs_index_download_list", "svs_index_list_rows", "svs_compute_products",
+    "svs_compute_products_dev", "svs_decode_dense_terms", "svs_verify", "svs_decode",
+    "svs_score_batch", "svs_keygen_subspace", "svs_keygen_derive_h_seed",
+    "svs_keygen_domain_separate", "svs_keygen_regenerate_h_s", "svs_quantise", "svs_probe_route",
+    "svs_debug_inject_fault", "svs_ctx_profile",
+]
+
+
+class GpuError(RuntimeError):
+    """Device / argument error from the backend (reference: GpuError,
+    scorer-bntm/src/gpu/mod.rs:97-144)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class BudgetTooSmall(GpuError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("budget_bytes", "resident_bytes", "peak_resident_bytes",
+                                          "hbm_total_bytes", "hbm_free_bytes", "uploads",
+                                          "launches")]
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("calls", C.c_uint64 * 8), ("gemv_bytes", C.c_uint64),
+                ("gemv_rows", C.c_uint64)]
+
+
+STAGES = ("encode", "product", "verify", "decode", "topk")
+
+
+class QueryBatch(C.Structure):
+    _fields_ = [
+        ("n_queries", C.c_uint32),
+        ("v_q", C.c_void_p), ("q_seeds", C.c_void_p), ("probe_ptr", C.c_void_p),
+        ("probe_lists", C.c_void_p), ("verify", C.c_void_p), ("k", C.c_uint32),
+        ("out_ids", C.c_void_p), ("out_scores", C.c_void_p), ("out_count", C.c_void_p),
+        ("out_status", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with secure-vector-search_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, sz, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_size_t, C.c_int
+        sig = {
+            "svs_last_error": ([C.c_char_p, sz], i32),
+            "svs_ctx_create": ([i32, u64, C.POINTER(vp)], i32),
+            "svs_ctx_destroy": ([vp], i32),
+            "svs_ctx_stats": ([vp, C.POINTER(Stats)], i32),
+            "svs_index_create": ([vp, u32, vp, C.POINTER(vp)], i32),
+            "svs_index_destroy": ([vp], i32),
+            "svs_index_upload_list": ([vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+            "svs_index_build_list": ([vp, u32, u32, vp, vp, vp], i32),
+            "svs_index_download_list": ([vp, u32, vp], i32),
+            "svs_index_list_rows": ([vp, u32], u32),
+            "svs_compute_products": ([vp, u32, vp, vp], i32),
+            "svs_compute_products_dev": ([vp, u32, vp, vp], i32),
+            "svs_decode_dense_terms": ([vp, u32, vp, vp, vp], i32),
+            "svs_verify": ([vp, u32, vp, vp, vp, u32, C.POINTER(C.c_int32)], i32),
+            "svs_decode": ([vp, u32, vp, vp, vp, vp, vp, u32, vp], i32),
+            "svs_score_batch": ([vp, C.POINTER(QueryBatch), i32], i32),
+            "svs_keygen_subspace": ([vp, vp], i32),
+            "svs_keygen_derive_h_seed": ([u64, u64, vp], i32),
+            "svs_keygen_domain_separate": ([vp, vp, sz, vp], i32),
+            "svs_keygen_regenerate_h_s": ([vp, u32, vp, vp, vp, vp, sz, C.POINTER(sz)], i32),
+            "svs_quantise": ([vp, u32, u32, u64, vp], i32),
+            "svs_probe_route": ([vp, u32, u32, vp, u32, u32, vp], i32),
+            "svs_debug_inject_fault": ([vp, C.c_int32, u32, u64], i32),
+            "svs_ctx_profile": ([vp, i32, i32, C.POINTER(Profile)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes, f.restype = args, res
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().svs_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        msg = last_error()
+        if rc == SVS_ERR_BUDGET:
+            raise BudgetTooSmall(rc, msg)
+        raise GpuError(rc, msg)
+    return rc
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def u64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def seed_array(seed) -> np.ndarray:
+    a = np.frombuffer(bytes(seed), dtype=np.uint8).copy() if not isinstance(seed, np.ndarray) \
+        else np.ascontiguousarray(seed, dtype=np.uint8)
+    if a.size != 32:
+        raise ValueError("seeds are 32 bytes")
+    return a

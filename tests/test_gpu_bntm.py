@@ -1,0 +1,345 @@
+"""GPU parity of the BN path against the CPU oracle (bit-exact).
+
+Mirrors the reference's GPU tests (scorer-bntm/src/lib.rs:1191-1570,
+ivf.rs:2345-2371) and the protocol tests of crypto.rs:1232-1559, called
+through the C ABI.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import svs
+    return svs.Context(0)
+
+
+def _field(rng, size):
+    return rng.integers(0, P, size=size, dtype=np.uint64)
+
+
+def _random_cluster(rng, m):
+    """A list with random field state (shape-only checks of the product)."""
+    rowptr = np.zeros(m + 1, np.uint32)
+    return dict(m=m, m_enc=_field(rng, m * 1024), a_l=_field(rng, m * 128), h=_field(rng, m * 128),
+                m_plain=None, s_rowptr=rowptr, s_idx=np.zeros(0, np.uint32),
+                s_sign=np.zeros(0, np.int8))
+
+
+@pytest.fixture(scope="module")
+def l_sub():
+    return O.generate_subspace(bytes([1] * 32))
+
+
+def test_compute_products_matches_oracle(ctx, l_sub):
+    import svs
+    sizes = [1, 7, 32, 63, 64, 65, 255, 256, 257, 300, 511, 1024, 1500]
+    idx = svs.GpuIndex(ctx, len(sizes), l_sub)
+    rng = np.random.default_rng(0xC0DE)
+    cls = []
+    for i, m in enumerate(sizes):
+        cl = _random_cluster(rng, m)
+        idx.upload_list(i, cl)
+        cls.append(cl)
+    for i, m in enumerate(sizes):
+        say("compute_products m =", m)
+        v = _field(rng, 1024)
+        got = idx.compute_products(i, v)
+        want = O.matvec(cls[i]["m_enc"], m, 1024, v)
+        assert np.array_equal(got, want), f"m={m}"
+
+
+def test_compute_products_extremes(ctx, l_sub):
+    import svs
+    idx = svs.GpuIndex(ctx, 2, l_sub)
+    m = 70
+    allmax = dict(_random_cluster(np.random.default_rng(1), m))
+    allmax["m_enc"] = np.full(m * 1024, P - 1, np.uint64)
+    idx.upload_list(0, allmax)
+    say("compute_products all p-1")
+    r = idx.compute_products(0, np.full(1024, P - 1, np.uint64))
+    assert np.all(r == np.uint64((1024 * (P - 1) ** 2) % P))
+    rnd = _random_cluster(np.random.default_rng(2), m)
+    idx.upload_list(1, rnd)
+    for col in (0, 1, 63, 64, 1023):
+        say("one-hot column", col)
+        v = np.zeros(1024, np.uint64)
+        v[col] = 1
+        assert np.array_equal(idx.compute_products(1, v), rnd["m_enc"].reshape(m, 1024)[:, col])
+
+
+def test_empty_and_out_of_range_lists(ctx, l_sub):
+    import svs
+    idx = svs.GpuIndex(ctx, 2, l_sub)
+    idx.upload_list(0, _random_cluster(np.random.default_rng(3), 0))
+    say("empty list")
+    assert idx.compute_products(0, np.zeros(1024, np.uint64)) is None
+    assert idx.compute_products(1, np.zeros(1024, np.uint64)) is None  # never uploaded
+    assert idx.compute_products(7, np.zeros(1024, np.uint64)) is None  # out of range
+
+
+def test_non_canonical_input_rejected(ctx, l_sub):
+    import svs
+    idx = svs.GpuIndex(ctx, 1, l_sub)
+    idx.upload_list(0, _random_cluster(np.random.default_rng(4), 8))
+    v = np.zeros(1024, np.uint64)
+    v[3] = P
+    with pytest.raises(svs.GpuError):
+        idx.compute_products(0, v)
+
+
+def _real_cluster(m, seed, l):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((m, 768)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    mq = np.stack([O.quantise_vector(r) for r in x]) if m else np.zeros((0, 1024), np.uint64)
+    h_seed = bytes([seed % 256] * 32)
+    return O.build_cluster(mq.reshape(-1), m, l, h_seed), x, h_seed
+
+
+def test_device_build_matches_host_encryption(ctx, l_sub):
+    import svs
+    idx = svs.GpuIndex(ctx, 3, l_sub)
+    for i, m in enumerate([1, 37, 130]):
+        say("build_list m =", m)
+        cl, _, h_seed = _real_cluster(m, 10 + i, l_sub)
+        idx.build_list(i, cl["m_plain"], h_seed)
+        assert np.array_equal(idx.download_menc(i), cl["m_enc"])
+    # random field rows (M_plain not representable in 32 bits)
+    rng = np.random.default_rng(5)
+    mp = _field(rng, 20 * 1024)
+    cl = O.build_cluster(mp, 20, l_sub, bytes([9] * 32))
+    idx.build_list(0, mp, bytes([9] * 32))
+    assert np.array_equal(idx.download_menc(0), cl["m_enc"])
+
+
+def test_verify_honest_and_tampered(ctx, l_sub):
+    import svs
+    cl, _, _ = _real_cluster(16, 21, l_sub)
+    idx = svs.GpuIndex(ctx, 1, l_sub)
+    idx.upload_list(0, cl)
+    rng = np.random.default_rng(21)
+    for b in range(6):
+        say("verify honest", b)
+        v_enc = _field(rng, 1024)
+        r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+        seed = bytes([b ^ 0xAA] * 32)
+        assert idx.verify(0, r, v_enc, seed) == -1 == O.verify_response(r, cl["m_enc"], 16, v_enc,
+                                                                         seed)
+    # one coordinate + 1: rejected at trial 0 (crypto.rs:1446-1474)
+    v_enc = _field(rng, 1024)
+    r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+    r[8] = (int(r[8]) + 1) % P
+    seed = bytes([35] * 32)
+    assert idx.verify(0, r, v_enc, seed) == 0 == O.verify_response(r, cl["m_enc"], 16, v_enc, seed)
+    # an error invisible to trial 0 (u_0 . e = 0) must be caught by trial 1
+    r = O.matvec(cl["m_enc"], 16, 1024, v_enc)
+    u0 = O.u_draws(seed, 16)
+    ea = 5
+    eb = (-ea * u0[3] * pow(u0[11], P - 2, P)) % P
+    r[3] = (int(r[3]) + ea) % P
+    r[11] = (int(r[11]) + eb) % P
+    want = O.verify_response(r, cl["m_enc"], 16, v_enc, seed)
+    say("verify trial-1 tamper, oracle says", want)
+    assert want == 1
+    assert idx.verify(0, r, v_enc, seed) == 1
+
+
+def test_verify_chunk_boundaries(ctx, l_sub):
+    """u draws straddle keystream blocks when t*m is not a multiple of 8."""
+    import svs
+    rng = np.random.default_rng(31)
+    for m in (1, 5, 17, 33, 100):
+        say("verify m =", m)
+        cl = _random_cluster(rng, m)
+        idx = svs.GpuIndex(ctx, 1, l_sub)
+        idx.upload_list(0, cl)
+        v = _field(rng, 1024)
+        r = O.matvec(cl["m_enc"], m, 1024, v)
+        seed = bytes([m] * 32)
+        assert idx.verify(0, r, v, seed) == -1
+        r[m - 1] = (int(r[m - 1]) + 3) % P
+        assert idx.verify(0, r, v, seed) == O.verify_response(r, cl["m_enc"], m, v, seed) == 0
+
+
+def test_decode_matches_oracle(ctx, l_sub):
+    import svs
+    for m, seed in ((1, 40), (50, 41), (129, 42)):
+        say("decode m =", m)
+        cl, _, _ = _real_cluster(m, seed, l_sub)
+        idx = svs.GpuIndex(ctx, 1, l_sub)
+        idx.upload_list(0, cl)
+        v = O.quantise_vector(np.random.default_rng(seed).standard_normal(768).astype(np.float32)
+                              / 30)
+        v_enc, g, t_idx, t_sign = O.encode_query(v, l_sub, bytes([seed] * 32))
+        r = O.matvec(cl["m_enc"], m, 1024, v_enc)
+        want = O.decode_scores(r, m, cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"],
+                               cl["s_idx"], cl["s_sign"], l_sub, v_enc, g, t_idx, t_sign)
+        got = idx.decode(0, r, v_enc, g, t_idx, t_sign)
+        assert np.array_equal(got, want)
+        # zero-noise: decode == M_plain . v exactly
+        assert np.array_equal(got, O.matvec(cl["m_plain"], m, 1024, v))
+        lt = O.matvec_transpose(l_sub, 1024, 128, v_enc)
+        dense = idx.decode_dense_terms(0, g, lt)
+        want_dense = [(int(a) + int(b)) % P for a, b in
+                      zip(O.matvec(cl["a_l"], m, 128, g), O.matvec(cl["h"], m, 128, lt))]
+        assert [int(x) for x in dense] == want_dense
+
+
+def test_decode_with_wide_plaintext_rows(ctx, l_sub):
+    """M_plain outside the 32-bit signed range uses the 64-bit column store."""
+    import svs
+    rng = np.random.default_rng(77)
+    mp = _field(rng, 9 * 1024)
+    cl = O.build_cluster(mp, 9, l_sub, bytes([77] * 32))
+    idx = svs.GpuIndex(ctx, 1, l_sub)
+    idx.upload_list(0, cl)
+    v = _field(rng, 1024)
+    v_enc, g, t_idx, t_sign = O.encode_query(v, l_sub, bytes([78] * 32))
+    r = O.matvec(cl["m_enc"], 9, 1024, v_enc)
+    say("decode wide rows")
+    assert np.array_equal(idx.decode(0, r, v_enc, g, t_idx, t_sign), O.matvec(mp, 9, 1024, v))
+
+
+def test_flat_scorer_matches_oracle(ctx):
+    import svs
+    from svs import data
+    corpus = data.make_corpus(300, components=8, seed=3)
+    qs = data.make_queries(corpus, 6, seed=4)
+    cfg = svs.BnTmConfig(key_seed=bytes([42] * 32), verification_enabled=True)
+    scorer = svs.BnTmScorer(ctx)
+    h_seed = bytes([5] * 32)
+    handle, _ = scorer.upload_cluster(cfg, corpus.vectors, h_seed=h_seed)
+    l = O.generate_subspace(bytes([42] * 32))
+    mq = np.stack([O.quantise_vector(v) for v in corpus.vectors])
+    cl = O.build_cluster(mq.reshape(-1), 300, l, h_seed)
+    seeds = [bytes([100 + i] * 32) for i in range(6)]
+    for mode in (0, 1):
+        say("flat scorer mode", mode)
+        got = scorer.score_batch(handle, qs, 10, q_seeds=np.frombuffer(b"".join(seeds), np.uint8),
+                                 mode=mode)
+        for i, q in enumerate(qs):
+            st, ids, scores = O.bntm_flat_score(cl, l, O.quantise_vector(q), seeds[i], True, 10)
+            assert st == 0
+            assert [h.id for h in got[i]] == ids.tolist()
+            assert [h.score for h in got[i]] == [float(np.float32(s)) for s in scores]
+    # k > m is capped (lib.rs:1069-1083); k > 16 uses the wide top-k kernel
+    assert len(scorer.score(handle, qs[0], 1000, q_seed=seeds[0])) == 300
+    wide = scorer.score(handle, qs[0], 40, q_seed=seeds[0])
+    st, ids, _ = O.bntm_flat_score(cl, l, O.quantise_vector(qs[0]), seeds[0], False, 40)
+    assert [h.id for h in wide] == ids.tolist()
+
+
+@pytest.fixture(scope="module")
+def ivf_setup(ctx):
+    import svs
+    from svs import data
+    corpus = data.make_corpus(2500, components=16, zipf_s=1.1, seed=11)
+    lists = data.ivf_lists(corpus)
+    lists[5] = np.zeros(0, np.uint32)  # an empty list (ivf.rs skips m == 0)
+    ivf = svs.IvfIndex(corpus.centres, lists)
+    cfg = svs.BnTmIvfConfig(index=ivf, nprobe=4, key_seed=bytes([42] * 32), upload_seed=7,
+                            verification_enabled=True)
+    scorer = svs.BnTmIvfScorer(ctx)
+    handle, _ = scorer.upload_cluster(cfg, corpus.vectors)
+    l = O.generate_subspace(bytes([42] * 32))
+    clusters = []
+    for ci, members in enumerate(lists):
+        mq = np.stack([O.quantise_vector(corpus.vectors[i]) for i in members]) if members.size \
+            else np.zeros((0, 1024), np.uint64)
+        clusters.append(O.build_cluster(mq.reshape(-1), members.size, l, O.derive_h_seed(7, ci)))
+    qs = data.make_queries(corpus, 12, seed=12)
+    return dict(scorer=scorer, handle=handle, clusters=clusters, lists=lists, l=l,
+                centres=corpus.centres, qs=qs)
+
+
+def test_ivf_matches_oracle(ivf_setup):
+    s = ivf_setup
+    seeds = np.frombuffer(bytes(range(12 * 32 % 256)) * 2, np.uint8)[: 12 * 32]
+    seeds = np.arange(12 * 32, dtype=np.uint64).astype(np.uint8)
+    for mode in (0, 1):
+        say("ivf mode", mode)
+        got = s["scorer"].score_batch(s["handle"], s["qs"], 10, q_seeds=seeds, mode=mode)
+        for i, q in enumerate(s["qs"]):
+            ids, scores = O.ivf_score(s["clusters"], s["lists"], s["centres"], s["l"], q, 4,
+                                      seeds[i * 32:(i + 1) * 32], True, 10)
+            assert [h.id for h in got[i]] == ids.tolist(), f"query {i}"
+            assert [h.score for h in got[i]] == [float(np.float32(x)) for x in scores]
+
+
+def test_ivf_batch_equals_sequential(ivf_setup):
+    s = ivf_setup
+    seeds = np.arange(12 * 32, dtype=np.uint64).astype(np.uint8) ^ np.uint8(0x5A)
+    batch = s["scorer"].score_batch(s["handle"], s["qs"], 10, q_seeds=seeds)
+    for i, q in enumerate(s["qs"]):
+        one = s["scorer"].score(s["handle"], q, 10, q_seed=seeds[i * 32:(i + 1) * 32])
+        assert one == batch[i]
+
+
+def test_ivf_tampered_server_rejected_with_cluster(ctx, ivf_setup):
+    import svs
+    s = ivf_setup
+    h = s["handle"]
+    probes = svs.probe_route(s["centres"], s["qs"], 4)
+    target = int(probes[0][1]) if len(s["lists"][int(probes[0][1])]) else int(probes[0][0])
+    v_q = svs.quantise(s["qs"])
+    seeds = np.zeros((12, 32), np.uint8)
+    pp = np.arange(0, 12 * 4 + 1, 4, dtype=np.uint32)
+    ctx.inject_fault(target, 0, 1)
+    try:
+        say("tamper list", target)
+        _, _, count, status = h.index.score_batch(v_q, seeds, pp, probes.reshape(-1),
+                                                  np.ones(12, np.uint8), 10)
+    finally:
+        ctx.inject_fault(-1)
+    for q in range(12):
+        if target in probes[q].tolist():
+            assert status[q] == 1 + 0 + 4 * target and count[q] == 0
+        else:
+            assert status[q] == 0 and count[q] == 10
+    with pytest.raises(svs.VerificationFailed):
+        ctx.inject_fault(target, 0, 1)
+        try:
+            s["scorer"].score(h, s["qs"][0], 10, q_seed=bytes(32))
+        finally:
+            ctx.inject_fault(-1)
+
+
+def test_single_list_ivf_equals_flat(ctx):
+    """ivf.rs:2045-2078: one list probed with nprobe 1 == flat scorer."""
+    import svs
+    from svs import data
+    corpus = data.make_corpus(200, components=4, seed=21)
+    qs = data.make_queries(corpus, 3, seed=22)
+    ivf = svs.IvfIndex(corpus.centres[:1], [np.arange(200, dtype=np.uint32)])
+    cfg = svs.BnTmIvfConfig(index=ivf, nprobe=1, key_seed=bytes([3] * 32), upload_seed=0)
+    hi, _ = svs.BnTmIvfScorer(ctx).upload_cluster(cfg, corpus.vectors)
+    hf, _ = svs.BnTmScorer(ctx).upload_cluster(svs.BnTmConfig(key_seed=bytes([3] * 32)),
+                                               corpus.vectors, h_seed=svs.derive_h_seed(0, 0))
+    seeds = np.full(3 * 32, 9, np.uint8)
+    a = svs.BnTmIvfScorer(ctx).score_batch(hi, qs, 10, q_seeds=seeds)
+    b = svs.BnTmScorer(ctx).score_batch(hf, qs, 10, q_seeds=seeds)
+    assert a == b
+
+
+def test_realised_cost_counts_probed_rows(ivf_setup):
+    import svs
+    s = ivf_setup
+    hits, cost = s["scorer"].score_with_realised_cost(s["handle"], s["qs"][0], 10, q_seed=bytes(32))
+    probes = svs.probe_route(s["centres"], s["qs"][:1], 4)[0]
+    rows = sum(len(s["lists"][c]) for c in probes)
+    assert cost.response_bytes == rows * 8
+    assert cost.effective_bytes_per_query == rows * 1024 * 8
+    assert len(hits) == 10

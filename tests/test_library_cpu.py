@@ -1,0 +1,111 @@
+"""CPU-only checks of the product library: it loads, exports every symbol the
+C ABI header declares, and its host-side client functions (keygen,
+quantisation, routing) agree with the oracle. No GPU compute is called."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if name.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", name)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            syms |= set(re.findall(r"\b(svs_[a-z0-9_]+)\s*\(", text))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    import svs
+    lib = svs.lib()
+    declared = _declared_symbols()
+    assert declared, "no declarations found"
+    for s in declared:
+        assert hasattr(lib, s), f"{s} declared in include/ but not exported"
+    assert declared == set(svs.EXPORTS)
+
+
+def test_library_is_gfx950_code_object():
+    import svs
+    data = open(svs.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_keygen_matches_oracle():
+    import svs
+    for seed in (bytes([42] * 32), bytes(range(32))):
+        assert np.array_equal(svs.generate_subspace(seed), O.generate_subspace(seed))
+        assert np.array_equal(svs.domain_separate(seed, b"protocol2"),
+                              O.domain_separate(seed, b"protocol2"))
+    for m in (0, 1, 65, 300):
+        got = svs.regenerate_h_s(bytes([m % 256] * 32), m)
+        want = O.regenerate_h_s(bytes([m % 256] * 32), m)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+
+
+def test_derive_h_seed_is_domain_separated_and_deterministic():
+    import svs
+    a = svs.derive_h_seed(7, 0)
+    assert np.array_equal(a, svs.derive_h_seed(7, 0))
+    assert not np.array_equal(a, svs.derive_h_seed(7, 1))
+    assert not np.array_equal(a, svs.derive_h_seed(8, 0))
+    # ivf.rs:258-264: the derived seed is the first 32 stream bytes (one per
+    # u32 word) of ChaCha20Rng(upload_seed LE || cluster LE || 0...).
+    seed = (7).to_bytes(8, "little") + (3).to_bytes(8, "little") + bytes(16)
+    key = np.frombuffer(seed, "<u4").astype(np.uint32)
+    words = np.concatenate([O.chacha20_block(key, 0), O.chacha20_block(key, 1)])
+    assert np.array_equal(svs.derive_h_seed(7, 3), (words & 0xFF).astype(np.uint8))
+
+
+def test_quantise_matches_oracle(golden):
+    import svs
+    q = golden["quantise"]
+    v = np.array(q["v"], np.float32)
+    assert [str(int(x)) for x in svs.quantise(v)[0][: q["n"]]] == q["out"]
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((50, 768)).astype(np.float32)
+    got = svs.quantise(x)
+    for i in range(50):
+        assert np.array_equal(got[i], O.quantise_vector(x[i]))
+
+
+def test_quantise_rejects_oversized_dimension():
+    import svs
+    with pytest.raises(svs.DimensionTooLarge):
+        svs.quantise(np.zeros((1, 1025), np.float32))
+
+
+def test_probe_route_matches_oracle(golden):
+    import svs
+    g = golden["probe_route"]
+    rng = np.random.default_rng(g["seed"])
+    cents = rng.standard_normal((20, 24)).astype(np.float32)
+    qs = rng.standard_normal((4, 24)).astype(np.float32)
+    assert svs.probe_route(cents, qs, g["nprobe"]).tolist() == g["out"]
+    from svs import data
+    c = data.make_corpus(3000, components=64, zipf_s=1.1, seed=5)
+    qs = data.make_queries(c, 40)
+    got = svs.probe_route(c.centres, qs, 16)
+    for i in range(40):
+        assert got[i].tolist() == O.probe_route(c.centres, qs[i], 16).tolist()
+
+
+def test_synthetic_lists_partition_the_corpus():
+    from svs import data
+    c = data.make_corpus(5000, components=32, zipf_s=1.1, seed=9)
+    lists = data.ivf_lists(c)
+    allids = np.concatenate(lists)
+    assert np.array_equal(np.sort(allids), np.arange(5000))
+    sizes = np.array([len(x) for x in lists])
+    assert sizes[0] > sizes[-1]  # Zipf-skewed
+    for ci, members in enumerate(lists):
+        assert np.all(np.diff(members.astype(np.int64)) > 0)
+        assert np.all(c.component[members] == ci)
