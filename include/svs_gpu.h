@@ -134,6 +134,19 @@ int svs_compute_products(svs_index* idx, uint32_t list, const uint64_t* v_enc_ho
 int svs_compute_products_dev(svs_index* idx, uint32_t list, const uint64_t* v_enc_dev,
                              uint64_t* r_dev);
 
+/* Gathered server products for many (query, list) pairs in one launch:
+ * compute_products_batch (crypto.rs:782-828) generalised to uneven IVF probe
+ * sets. Query q's v_enc (host, n u64) is multiplied with every list
+ * probe_lists[probe_ptr[q] .. probe_ptr[q+1]); r receives, item after item
+ * in that order, m_c values per item (nothing for empty lists). `kernel`
+ * picks the VALU row-streaming kernel or the exact int8 MFMA limb tiles;
+ * both are bit-identical to mersenne_matvec. */
+#define SVS_KERNEL_VALU 0
+#define SVS_KERNEL_MFMA 1
+int svs_compute_products_batch(svs_index* idx, uint32_t n_queries, const uint64_t* v_enc_host,
+                               const uint32_t* probe_ptr_host, const uint32_t* probe_lists_host,
+                               int kernel, uint64_t* r_host);
+
 /* AL . g + H . lt_venc (mod p): decode_dense_terms (gpu/mod.rs:390-408,
  * 541-577) with the reference's stale N1 = 512 kernel replaced by n1 = 128. */
 int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint64_t* g_host,
@@ -167,7 +180,8 @@ int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint
  * one-query-per-launch drop-in); SVS_MODE_BATCHED runs the whole batch
  * through one gathered launch per stage. Results are identical. */
 #define SVS_MODE_PLAIN 0
-#define SVS_MODE_BATCHED 1
+#define SVS_MODE_BATCHED 1      /* server product on the int8 MFMA limb tiles */
+#define SVS_MODE_BATCHED_VALU 2 /* batched, server product on the VALU kernel */
 
 typedef struct {
     uint32_t n_queries;

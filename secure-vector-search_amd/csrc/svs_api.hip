@@ -121,13 +121,31 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     return SVS_OK;
 }
 
-extern "C" int svs_ctx_destroy(svs_ctx* ctx) {
-    if (!ctx) return SVS_OK;
+// A context outlives its indexes: destroying it while indexes exist only
+// marks it, and the last svs_index_destroy frees it (callers with garbage-
+// collected handles cannot order the two calls).
+static std::mutex g_ctx_refs;
+
+static void ctx_free(svs_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : ctx->all_scratch()) b->release(ctx);
+    if (ctx->prof.created)
+        for (auto& e : ctx->prof.ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+extern "C" int svs_ctx_destroy(svs_ctx* ctx) {
+    if (!ctx) return SVS_OK;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_refs);
+        if (ctx->index_refs > 0) {
+            ctx->destroy_pending = true;
+            return SVS_OK;
+        }
+    }
+    ctx_free(ctx);
     return SVS_OK;
 }
 
@@ -177,6 +195,10 @@ extern "C" int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* 
     memset(zero.data(), 0, sizeof(ListDev) * n_lists);
     HIP_TRY(hipMemcpy(idx->lists_dev, zero.data(), sizeof(ListDev) * n_lists,
                       hipMemcpyHostToDevice));
+    {
+        std::lock_guard<std::mutex> r(g_ctx_refs);
+        ctx->index_refs++;
+    }
     *out = idx;
     return SVS_OK;
 }
@@ -189,14 +211,23 @@ static void free_list(svs_ctx* ctx, ListHost& L) {
 extern "C" int svs_index_destroy(svs_index* idx) {
     if (!idx) return SVS_OK;
     svs_ctx* ctx = idx->ctx;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    for (auto& L : idx->lists) free_list(ctx, L);
-    dev_free(ctx, idx->l_dev, sizeof(uint64_t) * N * N1);
-    dev_free(ctx, idx->lt_dev, sizeof(uint64_t) * N * N1);
-    dev_free(ctx, idx->lists_dev, sizeof(ListDev) * idx->lists.size());
-    delete idx;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        for (auto& L : idx->lists) free_list(ctx, L);
+        dev_free(ctx, idx->l_dev, sizeof(uint64_t) * N * N1);
+        dev_free(ctx, idx->lt_dev, sizeof(uint64_t) * N * N1);
+        dev_free(ctx, idx->lists_dev, sizeof(ListDev) * idx->lists.size());
+        if (ctx->cached_plan_index == idx) ctx->cached_plan_index = nullptr;
+        delete idx;
+    }
+    bool free_ctx;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_refs);
+        free_ctx = --ctx->index_refs == 0 && ctx->destroy_pending;
+    }
+    if (free_ctx) ctx_free(ctx);
     return SVS_OK;
 }
 
@@ -246,6 +277,7 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
         else rc = take((void**)&L.d.mp_t64, sizeof(uint64_t) * (size_t)m * N);
     }
     if (rc == SVS_OK) rc = take((void**)&L.d.ids, sizeof(uint32_t) * m);
+    if (rc == SVS_OK) rc = take((void**)&L.d.rowcorr, sizeof(uint64_t) * m);
     if (rc != SVS_OK) free_list(ctx, L);
     L.d.m = m;
     return rc;
@@ -284,6 +316,17 @@ static bool fits_i32(const uint64_t* mp, size_t count) {
         if (!(x <= 0x7FFFFFFFull || x >= P - 0x80000000ull)) return false;
     }
     return true;
+}
+
+// Row term of the MFMA product identity: rowcorr = c * sum_k M_enc[row][k].
+static int compute_rowcorr(svs_ctx* ctx, ListHost& L) {
+    if (!L.m) return SVS_OK;
+    const uint32_t blocks = std::min<uint32_t>((L.m + 3) / 4, (uint32_t)ctx->n_cu * 32);
+    hipLaunchKernelGGL(k_row_corr, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)L.d.menc, L.m, (uint64_t*)L.d.rowcorr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
 }
 
 static int publish_list(svs_index* idx, uint32_t list) {
@@ -365,6 +408,7 @@ extern "C" int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m,
     if (m > 0) {
         HIP_TRY(hipMemcpy((void*)L.d.menc, m_enc, sizeof(uint64_t) * (size_t)m * N,
                           hipMemcpyHostToDevice));
+        SVS_TRY(compute_rowcorr(ctx, L));
         SVS_TRY(upload_transposed(ctx, a_l, m, N1, (uint64_t*)L.d.alh_t, m));
         SVS_TRY(upload_transposed(ctx, h, m, N1, (uint64_t*)L.d.alh_t + (size_t)N1 * m, m));
         if (m_plain) SVS_TRY(upload_mplain(ctx, L, m_plain));
@@ -418,6 +462,9 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
                            (const uint32_t*)d_sidx, (const int8_t*)d_ssign, m,
                            (uint64_t*)L.d.menc);
         HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_row_corr, dim3(std::min<uint32_t>((m + 3) / 4, (uint32_t)ctx->n_cu * 32)),
+                           dim3(256), 0, ctx->stream, (const uint64_t*)L.d.menc, m,
+                           (uint64_t*)L.d.rowcorr);
         hipLaunchKernelGGL(k_encrypt_al, dim3((m + 7) / 8), dim3(128), 0, ctx->stream, d_m,
                            (const uint64_t*)idx->l_dev, m, (uint64_t*)L.d.alh_t);
         HIP_TRY(hipGetLastError());
@@ -476,6 +523,10 @@ struct BatchPlan {
     std::vector<Item> items;
     std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
     std::vector<uint8_t> item_verify;
+    // MFMA product: items grouped by list (probe order kept within a list)
+    // and work tiles of MT_ROWS rows x up to MT_QG of those items.
+    std::vector<uint32_t> gitems;
+    std::vector<MTile> tiles;
 };
 
 static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
@@ -519,6 +570,25 @@ static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_pt
     seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
     for (uint32_t c : seen)
         if (c < idx->lists.size()) pl.distinct_bytes += (uint64_t)idx->lists[c].m * N * 8;
+    // MFMA tiles: list-major, row block, then query group innermost so that
+    // consecutive tiles re-read the same rows while they are in cache.
+    // (items of empty or out-of-range lists have no rows and no tiles)
+    for (uint32_t i = 0; i < pl.n_items; ++i)
+        if (pl.items[i].m) pl.gitems.push_back(i);
+    std::stable_sort(pl.gitems.begin(), pl.gitems.end(), [&](uint32_t x, uint32_t y) {
+        return pl.items[x].list < pl.items[y].list;
+    });
+    const uint32_t ng = (uint32_t)pl.gitems.size();
+    for (uint32_t g = 0; g < ng;) {
+        uint32_t e = g;
+        while (e < ng && pl.items[pl.gitems[e]].list == pl.items[pl.gitems[g]].list) ++e;
+        const uint32_t m = pl.items[pl.gitems[g]].m;
+        for (uint32_t r0 = 0; r0 < m; r0 += MT_ROWS)
+            for (uint32_t q0 = g; q0 < e; q0 += MT_QG)
+                pl.tiles.push_back(MTile{pl.items[pl.gitems[g]].list, r0, q0,
+                                         std::min<uint32_t>(MT_QG, e - q0)});
+        g = e;
+    }
     return SVS_OK;
 }
 
@@ -592,7 +662,12 @@ extern "C" int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile*
 
 // Runs encode (+ product, verify, decode, top-k) for one sub-batch whose
 // query inputs are already on the device (ctx->q_*). Asynchronous.
-static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool product_only) {
+// Product kernels: VALU row streaming (one query per row pass) or the int8
+// MFMA limb-split tiles (many queries per row pass).
+enum { KERNEL_VALU = 0, KERNEL_MFMA = 1 };
+
+static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool product_only,
+                    int kernel, bool encode) {
     svs_ctx* ctx = idx->ctx;
     hipStream_t st = ctx->stream;
     const uint32_t nq = pl.n_queries;
@@ -624,26 +699,48 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     uint32_t* d_tcnt = (uint32_t*)ctx->q_tcnt.ptr;
     uint32_t* d_vseed = (uint32_t*)ctx->q_vseed.ptr;
     SVS_TRY(prof_mark(ctx, 0, true));
-    hipLaunchKernelGGL(k_encode_stream, dim3((nq + 63) / 64), dim3(64), 0, st,
-                       (const uint8_t*)ctx->q_seeds.ptr, nq, d_gl, d_tdense, d_tlist, d_tcnt,
-                       d_vseed);
-    hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
-                       (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
-                       (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
-    hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1), 0, st, (const uint64_t*)idx->l_dev,
-                       (const uint64_t*)d_venc, d_gl);
-    ctx->launches += 3;
-    HIP_TRY(hipGetLastError());
+    if (encode) {
+        hipLaunchKernelGGL(k_encode_stream, dim3((nq + 63) / 64), dim3(64), 0, st,
+                           (const uint8_t*)ctx->q_seeds.ptr, nq, d_gl, d_tdense, d_tlist, d_tcnt,
+                           d_vseed);
+        hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
+                           (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
+                           (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
+        hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1), 0, st, (const uint64_t*)idx->l_dev,
+                           (const uint64_t*)d_venc, d_gl);
+        ctx->launches += 3;
+        HIP_TRY(hipGetLastError());
+    }
     SVS_TRY(prof_mark(ctx, 0, false));
     // server product
     SVS_TRY(prof_mark(ctx, 1, true));
     SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
-    if (pl.n_chunks) {
+    if (pl.n_chunks && kernel == KERNEL_VALU) {
         hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
                            st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
                            (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
                            (uint64_t*)ctx->r.ptr);
         ctx->launches++;
+        HIP_TRY(hipGetLastError());
+    } else if (pl.n_chunks) {
+        const uint32_t nt = (uint32_t)pl.tiles.size();
+        SVS_TRY(ctx->mtiles.ensure(ctx, sizeof(MTile) * nt + sizeof(uint32_t) * pl.gitems.size()));
+        SVS_TRY(ctx->q_corr.ensure(ctx, sizeof(uint64_t) * nq));
+        MTile* d_tiles = (MTile*)ctx->mtiles.ptr;
+        uint32_t* d_gitems = (uint32_t*)(d_tiles + nt);
+        HIP_TRY(hipMemcpyAsync(d_tiles, pl.tiles.data(), sizeof(MTile) * nt,
+                               hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size(),
+                               hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_query_corr, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
+                           (uint64_t*)ctx->q_corr.ptr);
+        const uint32_t blocks =
+            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nt + 3) / 4, (uint64_t)ctx->n_cu * 8));
+        hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists,
+                           (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
+                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+                           (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+        ctx->launches += 2;
         HIP_TRY(hipGetLastError());
     }
     SVS_TRY(prof_mark(ctx, 1, false));
@@ -764,7 +861,7 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
         !b->out_scores || !b->out_count || !b->out_status)
         return fail(SVS_ERR_ARG, "missing batch array");
     if (b->k == 0 || b->k > 128) return fail(SVS_ERR_ARG, "k = %u must be in 1..128", b->k);
-    if (mode != SVS_MODE_PLAIN && mode != SVS_MODE_BATCHED)
+    if (mode != SVS_MODE_PLAIN && mode != SVS_MODE_BATCHED && mode != SVS_MODE_BATCHED_VALU)
         return fail(SVS_ERR_ARG, "unknown mode %d", mode);
     svs_ctx* ctx = idx->ctx;
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -783,7 +880,8 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
             if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
                 return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
         SVS_TRY(stage_queries(ctx, nq, b->v_q + (size_t)q0 * N, b->q_seeds + (size_t)q0 * 32));
-        SVS_TRY(run_plan(idx, pl, b->k, false));
+        SVS_TRY(run_plan(idx, pl, b->k, false,
+                         mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU, true));
         SVS_TRY(collect(ctx, nq, b->k, pl.n_items, b->out_ids + (size_t)q0 * b->k,
                         b->out_scores + (size_t)q0 * b->k, b->out_count + q0,
                         b->out_status + q0));
@@ -1016,5 +1114,32 @@ extern "C" int svs_debug_inject_fault(svs_ctx* ctx, int32_t list, uint32_t row, 
     ctx->fault_list = list;
     ctx->fault_row = row;
     ctx->fault_delta = delta;
+    return SVS_OK;
+}
+
+extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
+                                          const uint64_t* v_enc, const uint32_t* probe_ptr,
+                                          const uint32_t* probe_lists, int kernel, uint64_t* r) {
+    if (!idx || (n_queries && (!v_enc || !probe_ptr || !probe_lists || !r)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (kernel != SVS_KERNEL_VALU && kernel != SVS_KERNEL_MFMA)
+        return fail(SVS_ERR_ARG, "unknown kernel %d", kernel);
+    if (n_queries == 0) return SVS_OK;
+    SVS_TRY(check_canonical(v_enc, (size_t)n_queries * N, "v_enc"));
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    BatchPlan pl;
+    SVS_TRY(make_plan(idx, n_queries, probe_ptr, probe_lists, nullptr, pl));
+    SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)n_queries * N));
+    HIP_TRY(hipMemcpyAsync(ctx->q_venc.ptr, v_enc, sizeof(uint64_t) * (size_t)n_queries * N,
+                           hipMemcpyHostToDevice, ctx->stream));
+    SVS_TRY(run_plan(idx, pl, 1, true, kernel == SVS_KERNEL_MFMA ? KERNEL_MFMA : KERNEL_VALU,
+                     false));
+    if (pl.n_cands)
+        HIP_TRY(hipMemcpyAsync(r, ctx->r.ptr, sizeof(uint64_t) * pl.n_cands,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    SVS_TRY(prof_collect(ctx));
     return SVS_OK;
 }

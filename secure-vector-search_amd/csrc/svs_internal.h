@@ -81,13 +81,16 @@ struct svs_ctx {
     svs::DevBuf tmp_a, tmp_b, tmp_c;
     svs::DevBuf items, prefixes, flags, items1, prefixes1;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed;
-    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b;
+    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr;
     // cached single-item plan of svs_compute_products_dev
     int64_t cached_plan_list = -1;
     const void* cached_plan_index = nullptr;
     uint32_t cached_plan_chunks = 0;
     // test hook: server-side fault injected after the product (list < 0: off)
     int32_t fault_list = -1;
+    // live indexes on this context; destroy is deferred while any remain
+    int index_refs = 0;
+    bool destroy_pending = false;
     uint32_t fault_row = 0;
     uint64_t fault_delta = 0;
     svs_prof prof;
@@ -95,7 +98,7 @@ struct svs_ctx {
     std::vector<svs::DevBuf*> all_scratch() {
         return {&tmp_a, &tmp_b, &tmp_c, &items, &prefixes, &flags, &items1, &prefixes1,
                 &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed,
-                &r, &scores, &partials, &verdict, &out, &io_a, &io_b};
+                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr};
     }
 };
 

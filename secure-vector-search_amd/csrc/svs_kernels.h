@@ -561,6 +561,165 @@ k_dense_terms(const uint64_t* __restrict__ alh_t, uint32_t m, const uint64_t* __
     out[i] = s;
 }
 
+// ====================================================== batched product (MFMA)
+// Exact F_p products for many queries per list on the int8 matrix cores.
+//
+// Write each u64 as signed byte limbs: x = sum_a s_a 256^a + c with
+// s_a = byte_a - 128 and c = 128 * sum_a 256^a. For a row M and query V,
+//   sum_k M_k V_k = sum_k S_k T_k + c sum_k M_k + c sum_k V_k - n c^2   (*)
+// where S_k, T_k are the signed-limb values. sum_k S_k T_k = sum_d 256^d D_d
+// with D_d = sum_k sum_{a+b=d} s_a[k] t_b[k], |D_d| <= 8 * 1024 * 2^14 = 2^27.
+//
+// One v_mfma_i32_16x16x64_i8 contracts K' = 64 bytes. With K' = (element k,
+// limb a) in memory order, the A operand (16 rows) is the raw M_enc bytes of
+// 8 consecutive elements XOR 0x80 — a plain 16-byte load per lane, no
+// transposition. Output column d (0..14, 15 is padding) takes B[(k, a)][d] =
+// t_{d-a}[k], i.e. the 8-byte window of the byte-reversed query limbs
+// x_k = bswap(T_k) starting at byte 7 - d. The accumulator of column d is
+// then exactly D_d, and the epilogue folds sum_d 256^d D_d plus the row and
+// query terms of (*), all mod p.
+//
+// Lane layout of the 16x16x64 i8 MFMA (lane l: A row l%16, B column l%16,
+// 16 K'-bytes from K'-group l/16; result rows 4*(l/16)+i, column l%16): A
+// and B are loaded with the same K' assignment, so the contraction pairs the
+// right bytes whatever order the unit sums them in.
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4i load16(const uint64_t* p) {
+    return *reinterpret_cast<const v4i*>(p);
+}
+
+__global__ void __launch_bounds__(256)
+k_gemv_mfma(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles, uint32_t n_tiles,
+            const uint32_t* __restrict__ gitems, const Item* __restrict__ items,
+            const uint32_t* __restrict__ cand_ptr, const uint64_t* __restrict__ venc,
+            const uint64_t* __restrict__ qcorr, uint64_t* __restrict__ r_out) {
+    __shared__ int32_t dsm[ROW_THREADS / WAVE][4][16][17];
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int grp = lane / 16;     // K'-group of this lane
+    const int col = lane % 16;     // A row / B column / output column of this lane
+    // window shifts of output column `col` (d): x >> 8(7-d) or x << 8(d-7)
+    const int shr = col <= 7 ? 8 * (7 - col) : 0;
+    const int shl = (col >= 8 && col <= 14) ? 8 * (col - 7) : 0;
+    const uint32_t waves_per_block = blockDim.x / WAVE;
+    for (uint32_t t = blockIdx.x * waves_per_block + wave; t < n_tiles;
+         t += gridDim.x * waves_per_block) {
+        const MTile tile = tiles[t];
+        const ListDev L = lists[tile.list];
+        const uint32_t m = L.m;
+        const int gn = (int)tile.gn;
+        // A rows of this lane (clamped into the list; extra rows discarded)
+        const uint64_t* arow[MT_RT];
+#pragma unroll
+        for (int rt = 0; rt < MT_RT; ++rt) {
+            const uint32_t row = min(tile.row0 + 16 * rt + col, m - 1);
+            arow[rt] = L.menc + (size_t)row * N + 2 * grp;
+        }
+        const uint64_t* vq[MT_QG];
+#pragma unroll
+        for (int j = 0; j < MT_QG; ++j) {
+            const uint32_t it = gitems[tile.g0 + min(j, gn - 1)];
+            vq[j] = venc + (size_t)items[it].q * N + 2 * grp;
+        }
+        v4i acc[MT_RT][MT_QG];
+#pragma unroll
+        for (int rt = 0; rt < MT_RT; ++rt)
+#pragma unroll
+            for (int j = 0; j < MT_QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
+        for (int s = 0; s < N / 8; ++s) {  // 128 steps of 8 elements (64 K'-bytes)
+            v4i a[MT_RT];
+#pragma unroll
+            for (int rt = 0; rt < MT_RT; ++rt) {
+                a[rt] = load16(arow[rt] + 8 * s);
+                a[rt] ^= (int)0x80808080;
+            }
+#pragma unroll
+            for (int j = 0; j < MT_QG; ++j) {
+                if (j < gn) {
+                    const v4i v = load16(vq[j] + 8 * s);
+                    uint64_t x0 = __builtin_bswap64(((uint64_t)(uint32_t)v.y << 32 | (uint32_t)v.x) ^
+                                                    0x8080808080808080ull);
+                    uint64_t x1 = __builtin_bswap64(((uint64_t)(uint32_t)v.w << 32 | (uint32_t)v.z) ^
+                                                    0x8080808080808080ull);
+                    uint64_t w0 = col <= 7 ? (x0 >> shr) : (col <= 14 ? (x0 << shl) : 0);
+                    uint64_t w1 = col <= 7 ? (x1 >> shr) : (col <= 14 ? (x1 << shl) : 0);
+                    const v4i b = v4i{(int)(uint32_t)w0, (int)(uint32_t)(w0 >> 32),
+                                      (int)(uint32_t)w1, (int)(uint32_t)(w1 >> 32)};
+#pragma unroll
+                    for (int rt = 0; rt < MT_RT; ++rt)
+                        acc[rt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt], b, acc[rt][j], 0,
+                                                                             0, 0);
+                }
+            }
+        }
+        // Epilogue, four (row tile, query) pairs per round: the D tiles go to
+        // LDS, then lane l folds row l%16 of pair l/16.
+#pragma unroll
+        for (int rnd = 0; rnd < MT_RT * MT_QG / 4; ++rnd) {
+#pragma unroll
+            for (int pp = 0; pp < 4; ++pp) {
+                const int pair = rnd * 4 + pp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    dsm[wave][pp][4 * grp + i][col] = acc[pair / MT_QG][pair % MT_QG][i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int pp = lane / 16, row = lane % 16;
+            const int pair = rnd * 4 + pp;
+            const int rt = pair / MT_QG, j = pair % MT_QG;
+            const uint32_t grow = tile.row0 + 16 * rt + row;
+            if (j < gn && grow < m) {
+                __int128 s = 0;
+#pragma unroll
+                for (int d = 0; d < 15; ++d)
+                    s += (__int128)dsm[wave][pp][row][d] << ((8 * d) % 61);
+                // |s| < 2^87: shift into [0, 2^89) by adding p * 2^27, then reduce
+                const unsigned __int128 u = (unsigned __int128)(s + ((__int128)P << 27));
+                uint64_t v = reduce128((uint64_t)u, (uint64_t)(u >> 64));
+                const uint32_t it = gitems[tile.g0 + j];
+                v = add_mod(v, L.rowcorr[grow]);
+                v = add_mod(v, qcorr[items[it].q]);
+                r_out[cand_ptr[it] + grow] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+// qcorr[q] = c * sum_k v_enc[q][k] - n c^2 (mod p): the query term of (*).
+__global__ void __launch_bounds__(WAVE)
+k_query_corr(const uint64_t* __restrict__ venc, uint64_t* __restrict__ qcorr) {
+    const uint32_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    uint64_t s = 0;
+    for (int k = lane; k < N; k += WAVE) s = add_mod(s, venc[(size_t)q * N + k]);
+    s = wave_sum_mod(s);
+    if (lane == 0) {
+        const uint64_t c = LIMB_OFFSET_C;
+        const uint64_t nc2 = mul_mod(mul_mod(c, c), (uint64_t)N);
+        qcorr[q] = sub_mod(mul_mod(c, s), nc2);
+    }
+}
+
+// rowcorr[i] = c * sum_k M_enc[i][k] (mod p): the row term of (*), one wave
+// per row.
+__global__ void __launch_bounds__(256)
+k_row_corr(const uint64_t* __restrict__ menc, uint32_t m, uint64_t* __restrict__ rowcorr) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < m;
+         i += gridDim.x * (blockDim.x / WAVE)) {
+        uint64_t s = 0;
+        for (int k = lane; k < N; k += WAVE) s = add_mod(s, menc[(size_t)i * N + k]);
+        s = wave_sum_mod(s);
+        if (lane == 0) rowcorr[i] = mul_mod(LIMB_OFFSET_C, s);
+    }
+}
+
 // Test hook: r[row] += delta for every item of `list` (a server that
 // returns a wrong product, as the reference's tamper test, crypto.rs:1457).
 __global__ void k_inject_fault(const Item* __restrict__ items, uint32_t n_items,

@@ -20,9 +20,26 @@ struct ListDev {
     const uint32_t* s_slice; // ceil(m/64)+1 offsets (in entries) into s_ell
     const uint16_t* s_ell;   // per slice: width x 64 entries, idx | 0x8000 if sign < 0
     const uint32_t* ids;     // m global vector ids
+    const uint64_t* rowcorr; // m: c * sum_k M_enc[row][k] mod p, c = 128 * 0x0101010101010101
     uint32_t m;
     uint32_t pad;
 };
+
+// One MFMA work tile: MT_ROWS rows of a list times up to MT_QG items
+// (queries) that probe it; the items are gitems[g0 .. g0 + gn).
+constexpr int MT_RT = 4;                 // 16-row MFMA tiles per wave
+constexpr int MT_ROWS = 16 * MT_RT;      // rows per work tile
+constexpr int MT_QG = 4;                 // queries per work tile
+struct MTile {
+    uint32_t list;
+    uint32_t row0;
+    uint32_t g0;
+    uint32_t gn;
+};
+
+// c = 128 * sum_{a<8} 2^(8a) reduced mod p: the offset between a u64 and its
+// signed-byte-limb value (each byte b read as b - 128).
+constexpr uint64_t LIMB_OFFSET_C = 0x8080808080808080ull % 0x1FFFFFFFFFFFFFFFull;
 
 struct Item {
     uint32_t q;          // query slot in the batch

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libsvs_gpu.so")
 
 SVS_OK, SVS_NONE = 0, 1
 SVS_ERR_ARG, SVS_ERR_HIP, SVS_ERR_BUDGET, SVS_ERR_STATE, SVS_ERR_RNG = -1, -2, -3, -4, -5
-MODE_PLAIN, MODE_BATCHED = 0, 1
+MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_VALU = 0, 1, 2
+KERNEL_VALU, KERNEL_MFMA = 0, 1
 N, N1, LAMBDA = 1024, 128, 3
 P = (1 << 61) - 1
 
@@ -26,7 +27,7 @@ EXPORTS = [
     "svs_compute_products_dev", "svs_decode_dense_terms", "svs_verify", "svs_decode",
     "svs_score_batch", "svs_keygen_subspace", "svs_keygen_derive_h_seed",
     "svs_keygen_domain_separate", "svs_keygen_regenerate_h_s", "svs_quantise", "svs_probe_route",
-    "svs_debug_inject_fault", "svs_ctx_profile",
+    "svs_debug_inject_fault", "svs_ctx_profile", "svs_compute_products_batch",
 ]
 
 
@@ -103,6 +104,7 @@ def lib() -> C.CDLL:
             "svs_probe_route": ([vp, u32, u32, vp, u32, u32, vp], i32),
             "svs_debug_inject_fault": ([vp, C.c_int32, u32, u64], i32),
             "svs_ctx_profile": ([vp, i32, i32, C.POINTER(Profile)], i32),
+            "svs_compute_products_batch": ([vp, u32, vp, vp, vp, i32, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

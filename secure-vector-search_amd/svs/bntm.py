@@ -15,9 +15,11 @@ injectable here so results can be checked; they default to os.urandom.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import time
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -200,6 +202,17 @@ class Context:
 
 
 _default_ctx: Context | None = None
+_live_indexes: "weakref.WeakSet[GpuIndex]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    """Release indexes before contexts at interpreter exit (finalisers run in
+    no particular order then)."""
+    for idx in list(_live_indexes):
+        idx.close()
+    if _default_ctx is not None:
+        _default_ctx.close()
 
 
 def default_context() -> Context:
@@ -219,6 +232,7 @@ class GpuIndex:
         check(L.lib().svs_index_create(ctx.h, n_lists, ptr(self.l), C.byref(h)))
         self.h = h
         self.n_lists = n_lists
+        _live_indexes.add(self)
 
     def upload_list(self, list_id, cl: dict, ids=None):
         m = int(cl["m"])
@@ -261,6 +275,23 @@ class GpuIndex:
         r = np.zeros(max(m, 1), np.uint64)
         rc = check(L.lib().svs_compute_products(self.h, list_id, ptr(u64(v_enc)), ptr(r)))
         return None if rc == L.SVS_NONE else r[:m]
+
+    def compute_products_batch(self, v_enc, probe_ptr, probe_lists, kernel=L.KERNEL_MFMA):
+        """Gathered products: list of r vectors, one per (query, probed list)
+        item in order (None for empty lists)."""
+        v_enc = u64(v_enc).reshape(-1, N)
+        pp = np.ascontiguousarray(probe_ptr, np.uint32)
+        pl = np.ascontiguousarray(probe_lists, np.uint32)
+        sizes = [self.rows(int(c)) for c in pl]
+        r = np.zeros(max(sum(sizes), 1), np.uint64)
+        check(L.lib().svs_compute_products_batch(self.h, v_enc.shape[0], ptr(v_enc), ptr(pp),
+                                                 ptr(pl if pl.size else np.zeros(1, np.uint32)),
+                                                 kernel, ptr(r)))
+        out, off = [], 0
+        for s in sizes:
+            out.append(r[off:off + s] if s else None)
+            off += s
+        return out
 
     def decode_dense_terms(self, list_id, g, lt_venc):
         m = self.rows(list_id)
@@ -327,6 +358,8 @@ def _hits(ids, scores, count) -> list[Hit]:
 def _seeds(n, q_seeds):
     if q_seeds is None:
         return np.frombuffer(os.urandom(32 * n), np.uint8).reshape(n, 32).copy()
+    if isinstance(q_seeds, (list, tuple)):
+        return np.stack([seed_array(s) for s in q_seeds]).reshape(n, 32)
     return np.ascontiguousarray(q_seeds, np.uint8).reshape(n, 32)
 
 

@@ -61,6 +61,65 @@ def test_compute_products_matches_oracle(ctx, l_sub):
         assert np.array_equal(got, want), f"m={m}"
 
 
+def test_batched_products_mfma_and_valu_match_oracle(ctx, l_sub):
+    """compute_products_batch (crypto.rs:782-828) generalised to uneven probe
+    sets; both product kernels against the oracle, bit-exact."""
+    import svs
+    sizes = [1, 15, 16, 17, 63, 64, 65, 130, 300, 0]
+    idx = svs.GpuIndex(ctx, len(sizes), l_sub)
+    rng = np.random.default_rng(0xBA7C)
+    cls = []
+    for i, m in enumerate(sizes):
+        cl = _random_cluster(rng, m)
+        if i == 2:  # extremes: all p-1 rows
+            cl["m_enc"] = np.full(m * 1024, P - 1, np.uint64)
+        if i == 3:  # zero rows
+            cl["m_enc"] = np.zeros(m * 1024, np.uint64)
+        idx.upload_list(i, cl)
+        cls.append(cl)
+    nq = 11
+    v = _field(rng, (nq, 1024))
+    v[0] = P - 1
+    v[1] = 0
+    v[2, ::2] = P - 1
+    probes = [rng.choice(len(sizes), size=int(rng.integers(1, 8)), replace=False) for _ in range(nq)]
+    probes[3] = np.arange(len(sizes))  # one query probes every list
+    pp = np.concatenate([[0], np.cumsum([len(x) for x in probes])]).astype(np.uint32)
+    pl = np.concatenate(probes).astype(np.uint32)
+    for kernel in (svs._lib.KERNEL_MFMA, svs._lib.KERNEL_VALU):
+        say("batched products kernel", kernel)
+        got = idx.compute_products_batch(v, pp, pl, kernel)
+        item = 0
+        for q in range(nq):
+            for c in probes[q]:
+                m = sizes[c]
+                if m == 0:
+                    assert got[item] is None
+                else:
+                    want = O.matvec(cls[c]["m_enc"], m, 1024, v[q])
+                    assert np.array_equal(got[item], want), f"kernel {kernel} q {q} list {c}"
+                item += 1
+
+
+def test_mfma_many_queries_per_list(ctx, l_sub):
+    """A hot list probed by many queries (several MFMA query groups)."""
+    import svs
+    rng = np.random.default_rng(91)
+    m = 200
+    cl = _random_cluster(rng, m)
+    idx = svs.GpuIndex(ctx, 1, l_sub)
+    idx.upload_list(0, cl)
+    nq = 37
+    v = _field(rng, (nq, 1024))
+    pp = np.arange(nq + 1, dtype=np.uint32)
+    pl = np.zeros(nq, np.uint32)
+    say("mfma hot list, queries", nq)
+    got = idx.compute_products_batch(v, pp, pl, svs._lib.KERNEL_MFMA)
+    want = O.matvec_batch(cl["m_enc"], m, 1024, v)
+    for q in range(nq):
+        assert np.array_equal(got[q], want[q]), f"query {q}"
+
+
 def test_compute_products_extremes(ctx, l_sub):
     import svs
     idx = svs.GpuIndex(ctx, 2, l_sub)
@@ -226,7 +285,7 @@ def test_flat_scorer_matches_oracle(ctx):
     mq = np.stack([O.quantise_vector(v) for v in corpus.vectors])
     cl = O.build_cluster(mq.reshape(-1), 300, l, h_seed)
     seeds = [bytes([100 + i] * 32) for i in range(6)]
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         say("flat scorer mode", mode)
         got = scorer.score_batch(handle, qs, 10, q_seeds=np.frombuffer(b"".join(seeds), np.uint8),
                                  mode=mode)
@@ -269,7 +328,7 @@ def test_ivf_matches_oracle(ivf_setup):
     s = ivf_setup
     seeds = np.frombuffer(bytes(range(12 * 32 % 256)) * 2, np.uint8)[: 12 * 32]
     seeds = np.arange(12 * 32, dtype=np.uint64).astype(np.uint8)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         say("ivf mode", mode)
         got = s["scorer"].score_batch(s["handle"], s["qs"], 10, q_seeds=seeds, mode=mode)
         for i, q in enumerate(s["qs"]):
