@@ -190,7 +190,7 @@ typedef struct {
     const uint32_t* probe_ptr;    /* n_queries + 1 */
     const uint32_t* probe_lists;  /* probe_ptr[n_queries] list ids */
     const uint8_t* verify;        /* n_queries flags (NULL = none) */
-    uint32_t k;                   /* hits per query (<= 128) */
+    uint32_t k;                   /* hits per query (1..4096) */
     uint32_t* out_ids;            /* n_queries x k */
     int64_t* out_scores;          /* n_queries x k */
     uint32_t* out_count;          /* n_queries */
@@ -198,6 +198,48 @@ typedef struct {
 } svs_query_batch;
 
 int svs_score_batch(svs_index* idx, const svs_query_batch* batch_host, int mode);
+
+/* ---- EMVP score GEMV (same kernel family) ---------------------------------
+ * Resident EMVP cluster M_hat (m x 1292 canonical u64, SEC128 n = 1292,
+ * s = 76, b = 17; scorer-emvp/src/params.rs). Replaces EMVP's
+ * FlatGpuState/IvfGpuState::build + compute_products
+ * (scorer-emvp/src/gpu/mod.rs:217-380) and emvp_compute_products
+ * (scorer-emvp/src/gpu/kernels.cu:90-110). */
+typedef struct svs_emvp svs_emvp;
+int svs_emvp_create(svs_ctx* ctx, uint32_t m, const uint64_t* m_hat_host, svs_emvp** out);
+int svs_emvp_destroy(svs_emvp* e);
+/* compute_products / compute_products_batch (scorer-emvp/src/crypto.rs:200-222,
+ * 369-415): for each query q, partials[q][si * m + i] (s x m, block-major)
+ * = sum_{l<17} M_hat[i][17 si + l] q_hat[q][17 si + l] mod p. SVS_NONE for
+ * an empty cluster. */
+int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const uint64_t* q_hat_host,
+                              uint64_t* partials_host);
+
+/* ---- Tiptoe SimplePIR answer (same kernel family) --------------------------
+ * Resident database M_db (rows x cols, row-major) of u32 (elem_bytes 4) or
+ * u64 (elem_bytes 8) entries. A u32 database whose entries are all < 128 is
+ * stored one byte per entry (the int8 MFMA operand). server_answer
+ * (scorer-tiptoe/src/pir/simplepir.rs:191-204) has no GPU path in the
+ * reference. */
+typedef struct svs_tiptoe svs_tiptoe;
+int svs_tiptoe_create(svs_ctx* ctx, uint32_t rows, uint32_t cols, int elem_bytes,
+                      const void* m_db_host, svs_tiptoe** out);
+int svs_tiptoe_destroy(svs_tiptoe* t);
+/* Bytes stored on the device and the stored entry width (1, 4 or 8). */
+int svs_tiptoe_stored_bytes(const svs_tiptoe* t, uint64_t* bytes, int* elem_bytes);
+/* ans[q][i] = sum_j M_db[i][j] ct[q][j] mod 2^32 (a u32 database): the
+ * reference's wrapping u64 answer truncated to 32 bits, since 2^32 divides
+ * 2^64. kernel: SVS_KERNEL_VALU, SVS_KERNEL_MFMA (byte-stored database,
+ * cols % 64 == 0) or -1 (MFMA for batches when possible). */
+int svs_tiptoe_answer_u32(svs_tiptoe* t, uint32_t n_queries, const uint32_t* ct_host,
+                          uint32_t* ans_host, int kernel);
+/* Same with device pointers, asynchronous on the context stream. */
+int svs_tiptoe_answer_u32_dev(svs_tiptoe* t, uint32_t n_queries, const uint32_t* ct_dev,
+                              uint32_t* ans_dev, int kernel);
+/* ans[q][i] = sum_j M_db[i][j] ct[q][j] mod 2^64 (a u64 database), exactly
+ * server_answer. */
+int svs_tiptoe_answer_u64(svs_tiptoe* t, uint32_t n_queries, const uint64_t* ct_host,
+                          uint64_t* ans_host);
 
 /* ---- test hook -------------------------------------------------------------
  * After every server product of svs_score_batch, add `delta` to r[row] of

@@ -136,6 +136,20 @@ static void ctx_free(svs_ctx* ctx) {
     delete ctx;
 }
 
+void svs::ctx_retain(svs_ctx* ctx) {
+    std::lock_guard<std::mutex> g(g_ctx_refs);
+    ctx->index_refs++;
+}
+
+void svs::ctx_release(svs_ctx* ctx) {
+    bool free_ctx;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_refs);
+        free_ctx = --ctx->index_refs == 0 && ctx->destroy_pending;
+    }
+    if (free_ctx) ctx_free(ctx);
+}
+
 extern "C" int svs_ctx_destroy(svs_ctx* ctx) {
     if (!ctx) return SVS_OK;
     {
@@ -195,10 +209,7 @@ extern "C" int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* 
     memset(zero.data(), 0, sizeof(ListDev) * n_lists);
     HIP_TRY(hipMemcpy(idx->lists_dev, zero.data(), sizeof(ListDev) * n_lists,
                       hipMemcpyHostToDevice));
-    {
-        std::lock_guard<std::mutex> r(g_ctx_refs);
-        ctx->index_refs++;
-    }
+    ctx_retain(ctx);
     *out = idx;
     return SVS_OK;
 }
@@ -222,12 +233,7 @@ extern "C" int svs_index_destroy(svs_index* idx) {
         if (ctx->cached_plan_index == idx) ctx->cached_plan_index = nullptr;
         delete idx;
     }
-    bool free_ctx;
-    {
-        std::lock_guard<std::mutex> g(g_ctx_refs);
-        free_ctx = --ctx->index_refs == 0 && ctx->destroy_pending;
-    }
-    if (free_ctx) ctx_free(ctx);
+    ctx_release(ctx);
     return SVS_OK;
 }
 
@@ -806,8 +812,13 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
                            (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
                            d_oscores, d_ocount);
-    else
+    else if (k <= 128)
         hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items,
+                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
+                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
+                           d_oscores, d_ocount);
+    else
+        hipLaunchKernelGGL(k_topk_large, dim3(nq), dim3(TOPK_LARGE_THREADS), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
                            (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
                            d_oscores, d_ocount);
@@ -860,7 +871,8 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
     if (!b->v_q || !b->q_seeds || !b->probe_ptr || !b->probe_lists || !b->out_ids ||
         !b->out_scores || !b->out_count || !b->out_status)
         return fail(SVS_ERR_ARG, "missing batch array");
-    if (b->k == 0 || b->k > 128) return fail(SVS_ERR_ARG, "k = %u must be in 1..128", b->k);
+    if (b->k == 0 || b->k > (uint32_t)TOPK_LARGE_MAX)
+        return fail(SVS_ERR_ARG, "k = %u must be in 1..%d", b->k, TOPK_LARGE_MAX);
     if (mode != SVS_MODE_PLAIN && mode != SVS_MODE_BATCHED && mode != SVS_MODE_BATCHED_VALU)
         return fail(SVS_ERR_ARG, "unknown mode %d", mode);
     svs_ctx* ctx = idx->ctx;

@@ -39,6 +39,11 @@ struct DevBuf {
     void release(svs_ctx* ctx);
 };
 
+// Objects that live on a context (indexes, EMVP clusters, Tiptoe databases)
+// retain it; svs_ctx_destroy is deferred until the last one is released.
+void ctx_retain(svs_ctx* ctx);
+void ctx_release(svs_ctx* ctx);
+
 // Resident allocation against the context budget.
 int dev_alloc(svs_ctx* ctx, void** p, size_t bytes);
 void dev_free(svs_ctx* ctx, void* p, size_t bytes);

@@ -343,6 +343,128 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
     if (tid == 0) out_count[q] = kk;
 }
 
+// Large k (up to TOPK_LARGE_MAX): block per query. Binary search on the
+// score for the k-th best value T, then collect every candidate above T and
+// the first (in position order) of those equal to T, then a bitonic sort of
+// the k survivors by (score desc, position asc).
+constexpr int TOPK_LARGE_MAX = 4096;
+constexpr int TOPK_LARGE_THREADS = 1024;
+
+__global__ void __launch_bounds__(TOPK_LARGE_THREADS)
+k_topk_large(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
+             const uint32_t* __restrict__ cand_ptr, const ListDev* __restrict__ lists,
+             const int64_t* __restrict__ scores, const int32_t* __restrict__ q_status, uint32_t k,
+             uint32_t* __restrict__ out_ids, int64_t* __restrict__ out_scores,
+             uint32_t* __restrict__ out_count) {
+    constexpr int T = TOPK_LARGE_THREADS;
+    __shared__ int64_t ks[TOPK_LARGE_MAX];
+    __shared__ uint32_t kp[TOPK_LARGE_MAX];
+    __shared__ uint32_t sa[T], sb[T];
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (q_status[q] != 0) {
+        if (tid == 0) out_count[q] = 0;
+        return;
+    }
+    const uint32_t i0 = item_ptr[q], i1 = item_ptr[q + 1];
+    const uint32_t c0 = cand_ptr[i0], c1 = cand_ptr[i1];
+    const uint32_t n = c1 - c0;
+    const uint32_t kk = min(min(k, n), (uint32_t)TOPK_LARGE_MAX);
+    if (kk == 0) {
+        if (tid == 0) out_count[q] = 0;
+        return;
+    }
+    // block-wide count of scores >= v
+    auto count_ge = [&](int64_t v) -> uint32_t {
+        uint32_t c = 0;
+        for (uint32_t g = c0 + tid; g < c1; g += T) c += scores[g] >= v ? 1u : 0u;
+        sa[tid] = c;
+        __syncthreads();
+        for (int s = T / 2; s > 0; s >>= 1) {
+            if (tid < s) sa[tid] += sa[tid + s];
+            __syncthreads();
+        }
+        const uint32_t r = sa[0];
+        __syncthreads();
+        return r;
+    };
+    int64_t lo = -(int64_t(1) << 62), hi = int64_t(1) << 62;  // count(>=lo) >= kk > count(>=hi)
+    while (hi - lo > 1) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (count_ge(mid) >= kk) lo = mid; else hi = mid;
+    }
+    const int64_t thr = lo;
+    // Contiguous position segments per thread keep the position order.
+    const uint32_t seg = (n + T - 1) / T;
+    const uint32_t a = c0 + min(n, seg * tid), b = c0 + min(n, seg * (tid + 1));
+    uint32_t ngt = 0, neq = 0;
+    for (uint32_t g = a; g < b; ++g) {
+        ngt += scores[g] > thr ? 1u : 0u;
+        neq += scores[g] == thr ? 1u : 0u;
+    }
+    sa[tid] = ngt;
+    sb[tid] = neq;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {  // inclusive scans
+        const uint32_t xa = tid >= off ? sa[tid - off] : 0, xb = tid >= off ? sb[tid - off] : 0;
+        __syncthreads();
+        sa[tid] += xa;
+        sb[tid] += xb;
+        __syncthreads();
+    }
+    const uint32_t total_gt = sa[T - 1];
+    uint32_t slot_gt = sa[tid] - ngt, rank_eq = sb[tid] - neq;
+    for (uint32_t g = a; g < b; ++g) {
+        const int64_t s = scores[g];
+        if (s > thr) {
+            ks[slot_gt] = s;
+            kp[slot_gt] = g - c0;
+            ++slot_gt;
+        } else if (s == thr) {
+            if (total_gt + rank_eq < kk) {
+                ks[total_gt + rank_eq] = s;
+                kp[total_gt + rank_eq] = g - c0;
+            }
+            ++rank_eq;
+        }
+    }
+    uint32_t p2 = 1;
+    while (p2 < kk) p2 <<= 1;
+    for (uint32_t i = kk + tid; i < p2; i += T) {
+        ks[i] = INT64_MIN;
+        kp[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // bitonic sort: "before" = higher score, then lower position
+    for (uint32_t size = 2; size <= p2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = tid; i < p2; i += T) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const bool asc = (i & size) == 0;
+                    const bool i_before_j = ks[i] > ks[j] || (ks[i] == ks[j] && kp[i] < kp[j]);
+                    if (asc != i_before_j) {
+                        const int64_t ts = ks[i]; ks[i] = ks[j]; ks[j] = ts;
+                        const uint32_t tp = kp[i]; kp[i] = kp[j]; kp[j] = tp;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t o = tid; o < kk; o += T) {
+        const uint32_t g = c0 + kp[o];
+        uint32_t lo_i = i0, hi_i = i1;
+        while (hi_i - lo_i > 1) {
+            const uint32_t mid = (lo_i + hi_i) >> 1;
+            if (cand_ptr[mid] <= g) lo_i = mid; else hi_i = mid;
+        }
+        out_ids[(size_t)q * k + o] = lists[items[lo_i].list].ids[g - cand_ptr[lo_i]];
+        out_scores[(size_t)q * k + o] = ks[o];
+    }
+    if (tid == 0) out_count[q] = kk;
+}
+
 // ================================================================== encode
 // encode_query (crypto.rs:508-553) stream part, one thread per query:
 // G = 128 sample_fp draws, then T = sample_sparse_pm1 over n = 1024

@@ -1,0 +1,96 @@
+"""EMVP's score GEMV and Tiptoe's SimplePIR answer on the gfx950 backend.
+
+EmvpCluster mirrors the EMVP GPU state (scorer-emvp/src/gpu/mod.rs:217-380):
+a resident M_hat and compute_products / compute_products_batch
+(scorer-emvp/src/crypto.rs:200-222, 369-415) returning the s x m block-major
+partials. TiptoeDatabase mirrors server_answer
+(scorer-tiptoe/src/pir/simplepir.rs:191-204) over a resident database, in the
+mod-2^32 regime (u32 database) and the reference's mod-2^64 regime.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, ptr, u64
+from .bntm import Context, _live_indexes, default_context
+
+EMVP_N, EMVP_S, EMVP_B = 1292, 76, 17
+
+
+class EmvpCluster:
+    def __init__(self, m_hat, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        a = u64(m_hat).reshape(-1, EMVP_N)
+        self.m = a.shape[0]
+        h = C.c_void_p()
+        check(L.lib().svs_emvp_create(self.ctx.h, self.m, ptr(a) if self.m else None, C.byref(h)))
+        self.h = h
+        _live_indexes.add(self)
+
+    def compute_products(self, q_hat):
+        """s x m partials (block-major); None for an empty cluster."""
+        out = self.compute_products_batch(u64(q_hat).reshape(1, EMVP_N))
+        return None if out is None else out[0]
+
+    def compute_products_batch(self, q_hats):
+        q = u64(q_hats).reshape(-1, EMVP_N)
+        out = np.zeros((q.shape[0], EMVP_S * max(self.m, 1)), np.uint64)
+        rc = check(L.lib().svs_emvp_compute_products(self.h, q.shape[0], ptr(q), ptr(out)))
+        return None if rc == L.SVS_NONE else out[:, : EMVP_S * self.m]
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().svs_emvp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TiptoeDatabase:
+    def __init__(self, m_db, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        a = np.ascontiguousarray(m_db)
+        if a.dtype not in (np.uint32, np.uint64):
+            raise TypeError("database entries must be uint32 or uint64")
+        self.rows, self.cols = a.shape
+        self.elem = a.dtype.itemsize
+        h = C.c_void_p()
+        check(L.lib().svs_tiptoe_create(self.ctx.h, self.rows, self.cols, self.elem, ptr(a),
+                                        C.byref(h)))
+        self.h = h
+        _live_indexes.add(self)
+
+    def stored(self):
+        b, e = C.c_uint64(0), C.c_int(0)
+        check(L.lib().svs_tiptoe_stored_bytes(self.h, C.byref(b), C.byref(e)))
+        return b.value, e.value
+
+    def server_answer(self, ct, kernel: int = -1):
+        """ans = M_db . ct, wrapping at 2^32 (u32 database) or 2^64 (u64)."""
+        if self.elem == 8:
+            c = u64(ct).reshape(-1, self.cols)
+            ans = np.zeros((c.shape[0], self.rows), np.uint64)
+            check(L.lib().svs_tiptoe_answer_u64(self.h, c.shape[0], ptr(c), ptr(ans)))
+        else:
+            c = np.ascontiguousarray(ct, np.uint32).reshape(-1, self.cols)
+            ans = np.zeros((c.shape[0], self.rows), np.uint32)
+            check(L.lib().svs_tiptoe_answer_u32(self.h, c.shape[0], ptr(c), ptr(ans), kernel))
+        return ans[0] if np.ndim(ct) == 1 else ans
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().svs_tiptoe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
