@@ -388,7 +388,8 @@ k_topk_large(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
         __syncthreads();
         return r;
     };
-    int64_t lo = -(int64_t(1) << 62), hi = int64_t(1) << 62;  // count(>=lo) >= kk > count(>=hi)
+    // Signed scores lie in (-2^60, 2^60); the bracket keeps hi - lo < 2^63.
+    int64_t lo = -(int64_t(1) << 61), hi = int64_t(1) << 61;  // count(>=lo) >= kk > count(>=hi)
     while (hi - lo > 1) {
         const int64_t mid = lo + (hi - lo) / 2;
         if (count_ge(mid) >= kk) lo = mid; else hi = mid;
@@ -417,8 +418,10 @@ k_topk_large(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
     for (uint32_t g = a; g < b; ++g) {
         const int64_t s = scores[g];
         if (s > thr) {
-            ks[slot_gt] = s;
-            kp[slot_gt] = g - c0;
+            if (slot_gt < kk) {  // always true when thr is the k-th best value
+                ks[slot_gt] = s;
+                kp[slot_gt] = g - c0;
+            }
             ++slot_gt;
         } else if (s == thr) {
             if (total_gt + rank_eq < kk) {

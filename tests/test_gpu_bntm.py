@@ -296,9 +296,11 @@ def test_flat_scorer_matches_oracle(ctx):
             assert [h.score for h in got[i]] == [float(np.float32(s)) for s in scores]
     # k > m is capped (lib.rs:1069-1083); k > 16 uses the wide top-k kernel
     assert len(scorer.score(handle, qs[0], 1000, q_seed=seeds[0])) == 300
-    wide = scorer.score(handle, qs[0], 40, q_seed=seeds[0])
-    st, ids, _ = O.bntm_flat_score(cl, l, O.quantise_vector(qs[0]), seeds[0], False, 40)
-    assert [h.id for h in wide] == ids.tolist()
+    for k in (40, 257):
+        say("flat scorer k =", k)
+        wide = scorer.score(handle, qs[0], k, q_seed=seeds[0])
+        st, ids, _ = O.bntm_flat_score(cl, l, O.quantise_vector(qs[0]), seeds[0], False, k)
+        assert [h.id for h in wide] == ids.tolist()
 
 
 @pytest.fixture(scope="module")
