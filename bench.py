@@ -65,9 +65,8 @@ def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threa
     import svs
 
     threads = max(1, min(max_threads, os.cpu_count() or 1))
-    # per-query cost estimate from one query, then size the sample
-    need = set()
-    probes = svs.probe_route(corpus.centres, qs[: 4 * threads], nprobe)
+    pool = min(qs.shape[0], 64 * threads)
+    probes = svs.probe_route(corpus.centres, qs[:pool], nprobe)
     state = {}
     lock = threading.Lock()
 
@@ -90,27 +89,27 @@ def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threa
             [t.start() for t in ts]
             [t.join() for t in ts]
 
-    ensure(sorted(set(probes[0].tolist())))
-    t0 = time.perf_counter()
-    O.ivf_score(state, lists, corpus.centres, l_sub, qs[0], nprobe, bytes(32), False, k)
-    one = time.perf_counter() - t0
-    sample = int(max(threads, min(probes.shape[0], budget_s * threads / max(one, 1e-3))))
-    sample = max(1, min(sample, probes.shape[0]))
-    ensure(sorted(set(probes[:sample].reshape(-1).tolist())))
-    log(f"cpu baseline: {sample} queries on {threads} threads (one query {one * 1e3:.0f} ms)")
+    ensure(sorted(set(probes.reshape(-1).tolist())))
     done = []
 
-    def worker(tid):
-        for qi in range(tid, sample, threads):
+    def worker(tid, n):
+        for qi in range(tid, n, threads):
             O.ivf_score(state, lists, corpus.centres, l_sub, qs[qi], nprobe,
                         bytes([qi % 256]) * 32, False, k)
             done.append(qi)
 
-    t0 = time.perf_counter()
-    ts = [threading.Thread(target=worker, args=(i,)) for i in range(threads)]
-    [t.start() for t in ts]
-    [t.join() for t in ts]
-    wall = time.perf_counter() - t0
+    def run(n):
+        done.clear()
+        t0 = time.perf_counter()
+        ts = [threading.Thread(target=worker, args=(i, n)) for i in range(threads)]
+        [t.start() for t in ts]
+        [t.join() for t in ts]
+        return time.perf_counter() - t0
+
+    pilot = run(min(pool, threads))  # one query per thread: sizes the sample
+    sample = int(min(pool, max(threads, budget_s / max(pilot, 1e-3) * threads)))
+    log(f"cpu baseline: pilot {threads} queries in {pilot:.2f} s -> sample {sample} queries")
+    wall = run(sample)
     rows = int(sum(sum(lists[c].size for c in probes[q]) for q in range(sample)))
     return dict(value=round(len(done) / wall, 3), unit="queries/s", cores=threads, kind="port",
                 sample=f"{sample} queries of the same workload (nprobe {nprobe}, {rows} probed "
@@ -134,6 +133,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="size of the plain-copy calibration dispatch (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -198,10 +199,19 @@ def main():
     mulmods = rows * 1024
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_gemv_chunks",
+                    kernel="k_gemv_mfma",
                     algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
                     avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
                     mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0)
+
+    copy = None
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib << 20
+        ctx.copy_probe(nb)
+        ms = min(ctx.copy_probe(nb) for _ in range(3))
+        copy = dict(bytes=nb, ms=round(ms, 4), gbs=round(2 * nb / (ms * 1e-3) / 1e9, 1),
+                    note="k_copy_probe reads and writes `bytes`; gbs counts both")
+        log(f"copy probe: {copy}")
 
     cpu = None
     if not args.no_cpu:
@@ -236,6 +246,7 @@ def main():
         "stages_ms_per_batch": {s: round(prof[s][0] / max(prof[s][1], 1), 4)
                                 for s in ("encode", "product", "verify", "decode", "topk")},
         "roofline": roofline,
+        "copy_probe": copy,
         "cpu_baseline": cpu,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }

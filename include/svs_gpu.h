@@ -83,6 +83,12 @@ typedef struct {
 } svs_profile;
 int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile* out);
 
+/* Copies `bytes` (a multiple of 16) between two device buffers with a plain
+ * 16-byte-vector kernel (k_copy_probe) and returns its HIP-event time: the
+ * reference dispatch for calibrating HBM byte counters (FETCH_SIZE /
+ * WRITE_SIZE) and the measured copy bandwidth. */
+int svs_ctx_copy_probe(svs_ctx* ctx, uint64_t bytes, double* ms);
+
 /* ---- resident index -----------------------------------------------------
  * An index holds n_lists encrypted lists (a flat scorer is one list) and the
  * shared subspace L (n x n1 row-major, generate_subspace crypto.rs:235-238).

@@ -1155,3 +1155,26 @@ extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
     SVS_TRY(prof_collect(ctx));
     return SVS_OK;
 }
+
+extern "C" int svs_ctx_copy_probe(svs_ctx* ctx, uint64_t bytes, double* ms) {
+    if (!ctx || bytes == 0 || bytes % 16) return fail(SVS_ERR_ARG, "bytes must be a multiple of 16");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(ctx->tmp_a.ensure(ctx, bytes));
+    SVS_TRY(ctx->tmp_b.ensure(ctx, bytes));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(ctx->n_cu * 16), dim3(256), 0, ctx->stream,
+                       (const uint4*)ctx->tmp_a.ptr, (uint4*)ctx->tmp_b.ptr, bytes / 16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return SVS_OK;
+}

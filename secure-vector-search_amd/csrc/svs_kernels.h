@@ -856,6 +856,14 @@ __global__ void k_inject_fault(const Item* __restrict__ items, uint32_t n_items,
         r[cand_ptr[it] + row] = add_mod(r[cand_ptr[it] + row], delta);
 }
 
+// Plain 16-byte-vector copy: the calibration reference for HBM counters.
+__global__ void __launch_bounds__(256)
+k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Fill kernel (used instead of hipMemsetAsync so a captured graph stays valid).
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -30,7 +30,7 @@ EXPORTS = [
     "svs_debug_inject_fault", "svs_ctx_profile", "svs_compute_products_batch",
     "svs_emvp_create", "svs_emvp_destroy", "svs_emvp_compute_products", "svs_tiptoe_create",
     "svs_tiptoe_destroy", "svs_tiptoe_stored_bytes", "svs_tiptoe_answer_u32",
-    "svs_tiptoe_answer_u32_dev", "svs_tiptoe_answer_u64",
+    "svs_tiptoe_answer_u32_dev", "svs_tiptoe_answer_u64", "svs_ctx_copy_probe",
 ]
 
 
@@ -117,6 +117,7 @@ def lib() -> C.CDLL:
             "svs_tiptoe_answer_u32": ([vp, u32, vp, vp, i32], i32),
             "svs_tiptoe_answer_u32_dev": ([vp, u32, vp, vp, i32], i32),
             "svs_tiptoe_answer_u64": ([vp, u32, vp, vp], i32),
+            "svs_ctx_copy_probe": ([vp, u64, C.POINTER(C.c_double)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

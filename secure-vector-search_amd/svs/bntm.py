@@ -184,6 +184,12 @@ class Context:
         out["gemv_rows"] = pr.gemv_rows
         return out
 
+    def copy_probe(self, nbytes: int) -> float:
+        """ms for a plain device copy of nbytes (HBM-counter calibration)."""
+        ms = C.c_double(0)
+        check(L.lib().svs_ctx_copy_probe(self.h, nbytes, C.byref(ms)))
+        return ms.value
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""
