@@ -199,7 +199,7 @@ def main():
     mulmods = rows * 1024
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_gemv_mfma",
+                    kernel="k_fp_gemm_tiles<1024,0>",
                     algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
                     avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
                     mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0)

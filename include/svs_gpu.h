@@ -145,10 +145,13 @@ int svs_compute_products_dev(svs_index* idx, uint32_t list, const uint64_t* v_en
  * sets. Query q's v_enc (host, n u64) is multiplied with every list
  * probe_lists[probe_ptr[q] .. probe_ptr[q+1]); r receives, item after item
  * in that order, m_c values per item (nothing for empty lists). `kernel`
- * picks the VALU row-streaming kernel or the exact int8 MFMA limb tiles;
- * both are bit-identical to mersenne_matvec. */
+ * picks the VALU row-streaming kernel, the exact int8 MFMA limb tiles with
+ * 256-row x 8-query block tiles (SVS_KERNEL_MFMA) or the same arithmetic on
+ * 64-row x 4-query single-wave tiles (SVS_KERNEL_MFMA_WAVE); all are
+ * bit-identical to mersenne_matvec. */
 #define SVS_KERNEL_VALU 0
 #define SVS_KERNEL_MFMA 1
+#define SVS_KERNEL_MFMA_WAVE 2
 int svs_compute_products_batch(svs_index* idx, uint32_t n_queries, const uint64_t* v_enc_host,
                                const uint32_t* probe_ptr_host, const uint32_t* probe_lists_host,
                                int kernel, uint64_t* r_host);

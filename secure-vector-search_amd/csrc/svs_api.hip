@@ -265,7 +265,8 @@ static void build_s_ell(uint32_t m, const uint32_t* rowptr, const uint32_t* sidx
 }
 
 // Allocates and fills the list's device state except menc / alh_t content.
-static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool want_mp) {
+static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool want_mp,
+                      bool planes) {
     svs_ctx* ctx = idx->ctx;
     ListHost& L = idx->lists[list];
     free_list(ctx, L);
@@ -284,6 +285,11 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
     }
     if (rc == SVS_OK) rc = take((void**)&L.d.ids, sizeof(uint32_t) * m);
     if (rc == SVS_OK) rc = take((void**)&L.d.rowcorr, sizeof(uint64_t) * m);
+    if (rc == SVS_OK) rc = take((void**)&L.d.alh, sizeof(uint64_t) * (size_t)m * 2 * N1);
+    if (rc == SVS_OK) rc = take((void**)&L.d.alh_rowcorr, sizeof(uint64_t) * m);
+    if (rc == SVS_OK && planes) rc = take((void**)&L.d.mp8, (size_t)m * 3 * N);
+    if (rc == SVS_OK) rc = take((void**)&L.d.s8, (size_t)m * N);
+    if (rc == SVS_OK) rc = take((void**)&L.d.s_rowterm, sizeof(uint64_t) * m);
     if (rc != SVS_OK) free_list(ctx, L);
     L.d.m = m;
     return rc;
@@ -313,7 +319,25 @@ static int list_upload_s_ids(svs_index* idx, uint32_t list, const uint32_t* rowp
     std::vector<uint32_t> idv(m);
     for (uint32_t i = 0; i < m; ++i) idv[i] = ids ? ids[i] : i;
     HIP_TRY(hipMemcpy((void*)L.d.ids, idv.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    // dense int8 S and its row term for the MFMA decode
+    const uint64_t words = (uint64_t)m * N / 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint32_t*)L.d.s8, (uint32_t)words, 0u);
+    hipLaunchKernelGGL(k_make_s8, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, L.d.s_slice,
+                       L.d.s_ell, m, (int8_t*)L.d.s8, (uint64_t*)L.d.s_rowterm);
+    HIP_TRY(hipGetLastError());
+    ctx->launches += 2;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SVS_OK;
+}
+
+// Signed lifts within (-2^23, 2^23): the rows get the MFMA decode planes.
+static bool fits_i24(const uint64_t* mp, size_t count) {
+    for (size_t i = 0; i < count; ++i) {
+        const uint64_t x = mp[i];
+        if (!(x < (uint64_t)MP8_LIMIT || x > P - (uint64_t)MP8_LIMIT)) return false;
+    }
+    return true;
 }
 
 static bool fits_i32(const uint64_t* mp, size_t count) {
@@ -324,13 +348,23 @@ static bool fits_i32(const uint64_t* mp, size_t count) {
     return true;
 }
 
-// Row term of the MFMA product identity: rowcorr = c * sum_k M_enc[row][k].
-static int compute_rowcorr(svs_ctx* ctx, ListHost& L) {
+// Derived device state shared by both upload paths, once menc and alh_t are
+// in place: row-major [AL | H] for the MFMA decode, and the row terms of the
+// MFMA identity for M_enc and for [AL | H].
+static int finish_list(svs_ctx* ctx, ListHost& L) {
     if (!L.m) return SVS_OK;
-    const uint32_t blocks = std::min<uint32_t>((L.m + 3) / 4, (uint32_t)ctx->n_cu * 32);
-    hipLaunchKernelGGL(k_row_corr, dim3(blocks), dim3(256), 0, ctx->stream,
-                       (const uint64_t*)L.d.menc, L.m, (uint64_t*)L.d.rowcorr);
+    const uint32_t m = L.m;
+    dim3 tg((m + 31) / 32, (2 * N1) / 32);
+    hipLaunchKernelGGL(k_transpose_u64, tg, dim3(32, 8), 0, ctx->stream,
+                       (const uint64_t*)L.d.alh_t, (uint32_t)(2 * N1), m, (uint64_t*)L.d.alh,
+                       (uint32_t)(2 * N1));
+    const uint32_t blocks = std::min<uint32_t>((m + 3) / 4, (uint32_t)ctx->n_cu * 32);
+    hipLaunchKernelGGL(k_row_corr<N>, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)L.d.menc, m, (uint64_t*)L.d.rowcorr);
+    hipLaunchKernelGGL(k_row_corr<2 * N1>, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)L.d.alh, m, (uint64_t*)L.d.alh_rowcorr);
     HIP_TRY(hipGetLastError());
+    ctx->launches += 3;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SVS_OK;
 }
@@ -371,6 +405,10 @@ static int upload_mplain(svs_ctx* ctx, ListHost& L, const uint64_t* mp) {
         if (grid.y > 65535) return fail(SVS_ERR_ARG, "list of %u rows too large", m);
         hipLaunchKernelGGL(k_transpose_mp32, grid, dim3(32, 8), 0, ctx->stream,
                            (const uint64_t*)tmp.ptr, m, (int32_t*)L.d.mp_t32);
+        if (L.d.mp8)
+            hipLaunchKernelGGL(k_make_mp8, dim3(std::min<uint32_t>(m * 4, 65536)),
+                               dim3(256), 0, ctx->stream, (const uint64_t*)tmp.ptr, m,
+                               (int8_t*)L.d.mp8);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         return SVS_OK;
@@ -409,14 +447,15 @@ extern "C" int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m,
                 (e > s_rowptr[i] && s_idx[e] <= s_idx[e - 1]))
                 return fail(SVS_ERR_ARG, "S row %u is not a sorted +-1 row", i);
     const bool mp32 = m_plain && fits_i32(m_plain, (size_t)m * N);
-    SVS_TRY(list_alloc(idx, list, m, mp32, m_plain != nullptr));
+    const bool planes = mp32 && fits_i24(m_plain, (size_t)m * N);
+    SVS_TRY(list_alloc(idx, list, m, mp32, m_plain != nullptr, planes));
     ListHost& L = idx->lists[list];
     if (m > 0) {
         HIP_TRY(hipMemcpy((void*)L.d.menc, m_enc, sizeof(uint64_t) * (size_t)m * N,
                           hipMemcpyHostToDevice));
-        SVS_TRY(compute_rowcorr(ctx, L));
         SVS_TRY(upload_transposed(ctx, a_l, m, N1, (uint64_t*)L.d.alh_t, m));
         SVS_TRY(upload_transposed(ctx, h, m, N1, (uint64_t*)L.d.alh_t + (size_t)N1 * m, m));
+        SVS_TRY(finish_list(ctx, L));
         if (m_plain) SVS_TRY(upload_mplain(ctx, L, m_plain));
         SVS_TRY(list_upload_s_ids(idx, list, s_rowptr, s_idx, s_sign, ids));
     }
@@ -439,7 +478,8 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
     std::vector<int8_t> ssign;
     regenerate_h_s_host(h_seed, m, h, rowptr, sidx, ssign);
     const bool mp32 = fits_i32(m_plain, (size_t)m * N);
-    SVS_TRY(list_alloc(idx, list, m, mp32, true));
+    const bool planes = mp32 && fits_i24(m_plain, (size_t)m * N);
+    SVS_TRY(list_alloc(idx, list, m, mp32, true, planes));
     ListHost& L = idx->lists[list];
     if (m > 0) {
         DevBuf& tm = ctx->tmp_a;
@@ -468,9 +508,6 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
                            (const uint32_t*)d_sidx, (const int8_t*)d_ssign, m,
                            (uint64_t*)L.d.menc);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_row_corr, dim3(std::min<uint32_t>((m + 3) / 4, (uint32_t)ctx->n_cu * 32)),
-                           dim3(256), 0, ctx->stream, (const uint64_t*)L.d.menc, m,
-                           (uint64_t*)L.d.rowcorr);
         hipLaunchKernelGGL(k_encrypt_al, dim3((m + 7) / 8), dim3(128), 0, ctx->stream, d_m,
                            (const uint64_t*)idx->l_dev, m, (uint64_t*)L.d.alh_t);
         HIP_TRY(hipGetLastError());
@@ -484,6 +521,9 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
             dim3 mg(N / 32, (m + 31) / 32);
             hipLaunchKernelGGL(k_transpose_mp32, mg, dim3(32, 8), 0, ctx->stream, d_m, m,
                                (int32_t*)L.d.mp_t32);
+            if (L.d.mp8)
+                hipLaunchKernelGGL(k_make_mp8, dim3(std::min<uint32_t>(m * 4, 65536)), dim3(256),
+                                   0, ctx->stream, d_m, m, (int8_t*)L.d.mp8);
         } else {
             dim3 mg(N / 32, (m + 31) / 32);
             hipLaunchKernelGGL(k_transpose_u64, mg, dim3(32, 8), 0, ctx->stream, d_m, m,
@@ -492,6 +532,7 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
         HIP_TRY(hipGetLastError());
         ctx->launches += 4;
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        SVS_TRY(finish_list(ctx, L));
         SVS_TRY(list_upload_s_ids(idx, list, rowptr.data(), sidx.data(), ssign.data(), ids));
     }
     return publish_list(idx, list);
@@ -526,6 +567,7 @@ struct BatchPlan {
     uint32_t n_queries = 0, n_items = 0, n_chunks = 0, n_cands = 0;
     uint64_t distinct_bytes = 0;  // sum of m_c * n * 8 over distinct lists
     bool any_verify = false;
+    bool planes_ok = true;  // every probed list has the MFMA decode planes
     std::vector<Item> items;
     std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
     std::vector<uint8_t> item_verify;
@@ -536,7 +578,8 @@ struct BatchPlan {
 };
 
 static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
-                     const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl) {
+                     const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
+                     uint32_t tile_rows = PB_ROWS, uint32_t tile_qg = PB_QG) {
     pl = BatchPlan();
     pl.n_queries = nq;
     if (probe_ptr[0] != 0) return fail(SVS_ERR_ARG, "probe_ptr[0] must be 0");
@@ -551,6 +594,7 @@ static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_pt
             const bool v = verify && verify[q];
             pl.item_verify.push_back(v ? 1 : 0);
             pl.any_verify |= v && m > 0;
+            if (m && !idx->lists[c].d.mp8) pl.planes_ok = false;
         }
     }
     pl.item_ptr[nq] = (uint32_t)pl.items.size();
@@ -589,10 +633,10 @@ static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_pt
         uint32_t e = g;
         while (e < ng && pl.items[pl.gitems[e]].list == pl.items[pl.gitems[g]].list) ++e;
         const uint32_t m = pl.items[pl.gitems[g]].m;
-        for (uint32_t r0 = 0; r0 < m; r0 += MT_ROWS)
-            for (uint32_t q0 = g; q0 < e; q0 += MT_QG)
+        for (uint32_t r0 = 0; r0 < m; r0 += tile_rows)
+            for (uint32_t q0 = g; q0 < e; q0 += tile_qg)
                 pl.tiles.push_back(MTile{pl.items[pl.gitems[g]].list, r0, q0,
-                                         std::min<uint32_t>(MT_QG, e - q0)});
+                                         std::min<uint32_t>(tile_qg, e - q0)});
         g = e;
     }
     return SVS_OK;
@@ -670,7 +714,13 @@ extern "C" int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile*
 // query inputs are already on the device (ctx->q_*). Asynchronous.
 // Product kernels: VALU row streaming (one query per row pass) or the int8
 // MFMA limb-split tiles (many queries per row pass).
-enum { KERNEL_VALU = 0, KERNEL_MFMA = 1 };
+enum { KERNEL_VALU = 0, KERNEL_MFMA = 1, KERNEL_MFMA_WAVE = 2 };
+
+// Tile shape a product kernel's plan is built with.
+static void tile_shape(int kernel, uint32_t& rows, uint32_t& qg) {
+    if (kernel == KERNEL_MFMA_WAVE) rows = MT_ROWS, qg = MT_QG;
+    else rows = PB_ROWS, qg = PB_QG;
+}
 
 static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool product_only,
                     int kernel, bool encode) {
@@ -704,11 +754,17 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     uint16_t* d_tlist = (uint16_t*)ctx->q_tlist.ptr;
     uint32_t* d_tcnt = (uint32_t*)ctx->q_tcnt.ptr;
     uint32_t* d_vseed = (uint32_t*)ctx->q_vseed.ptr;
+    // verdicts, per-query status and the RNG flag (set by encode or verify)
+    SVS_TRY(ctx->verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
+    int32_t* d_item_verdict = (int32_t*)ctx->verdict.ptr;
+    int32_t* d_qstatus = d_item_verdict + std::max<uint32_t>(pl.n_items, 1);
+    uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_rng_flag, 1u, 0u);
     SVS_TRY(prof_mark(ctx, 0, true));
     if (encode) {
-        hipLaunchKernelGGL(k_encode_stream, dim3((nq + 63) / 64), dim3(64), 0, st,
-                           (const uint8_t*)ctx->q_seeds.ptr, nq, d_gl, d_tdense, d_tlist, d_tcnt,
-                           d_vseed);
+        hipLaunchKernelGGL(k_encode_keystream, dim3(nq), dim3(256), 0, st,
+                           (const uint8_t*)ctx->q_seeds.ptr, d_gl, d_tdense, d_tlist, d_tcnt,
+                           d_vseed, d_rng_flag);
         hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
                            (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
                            (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
@@ -731,21 +787,30 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     } else if (pl.n_chunks) {
         const uint32_t nt = (uint32_t)pl.tiles.size();
         SVS_TRY(ctx->mtiles.ensure(ctx, sizeof(MTile) * nt + sizeof(uint32_t) * pl.gitems.size()));
-        SVS_TRY(ctx->q_corr.ensure(ctx, sizeof(uint64_t) * nq));
+        SVS_TRY(ctx->q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
         MTile* d_tiles = (MTile*)ctx->mtiles.ptr;
         uint32_t* d_gitems = (uint32_t*)(d_tiles + nt);
         HIP_TRY(hipMemcpyAsync(d_tiles, pl.tiles.data(), sizeof(MTile) * nt,
                                hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size(),
                                hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_query_corr, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
+        hipLaunchKernelGGL(k_query_corr<N>, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
                            (uint64_t*)ctx->q_corr.ptr);
-        const uint32_t blocks =
-            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nt + 3) / 4, (uint64_t)ctx->n_cu * 8));
-        hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists,
-                           (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
-                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+        if (kernel == KERNEL_MFMA_WAVE) {
+            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>((nt + 3) / 4, (uint64_t)ctx->n_cu * 8));
+            hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists,
+                               (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
+                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+                               (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+        } else {
+            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
+            hipLaunchKernelGGL((k_fp_gemm_tiles<N, 0>), dim3(blocks), dim3(256), 0, st, d_lists,
+                               (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
+                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+                               (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+        }
         ctx->launches += 2;
         HIP_TRY(hipGetLastError());
     }
@@ -761,11 +826,6 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     }
     if (product_only) return SVS_OK;
     // verification
-    SVS_TRY(ctx->verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
-    int32_t* d_item_verdict = (int32_t*)ctx->verdict.ptr;
-    int32_t* d_qstatus = d_item_verdict + std::max<uint32_t>(pl.n_items, 1);
-    uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_rng_flag, 1u, 0u);
     SVS_TRY(prof_mark(ctx, 2, true));
     if (pl.any_verify) {
         SVS_TRY(ctx->partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
@@ -790,14 +850,52 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     SVS_TRY(ctx->scores.ensure(ctx, sizeof(int64_t) * std::max<uint32_t>(pl.n_cands, 1)));
     SVS_TRY(prof_mark(ctx, 3, true));
     if (pl.n_cands) {
-        const uint32_t blocks =
-            (uint32_t)std::min<uint64_t>((pl.n_cands + 255) / 256, (uint64_t)ctx->n_cu * 32);
-        hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
-                           pl.n_items, (const uint32_t*)d_cand_ptr, (const uint64_t*)ctx->r.ptr,
-                           (const uint64_t*)d_gl, (const uint64_t*)d_venc,
-                           (const uint16_t*)d_tlist, (const uint32_t*)d_tcnt,
-                           (int64_t*)ctx->scores.ptr, (uint64_t*)nullptr);
-        ctx->launches++;
+        // The dense decode terms AL G + H L^T v_enc: the same MFMA tiles over
+        // the row-major [AL | H] when the batch runs on block tiles, else
+        // inside the per-candidate decode kernel.
+        const uint64_t* dense = nullptr;
+        const bool tiles = kernel == KERNEL_MFMA && pl.n_chunks;
+        const uint32_t nt = (uint32_t)pl.tiles.size();
+        const MTile* d_tiles = (const MTile*)ctx->mtiles.ptr;
+        const uint32_t* d_gitems = (const uint32_t*)(d_tiles + nt);
+        const uint32_t tblocks =
+            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
+        if (tiles) {
+            SVS_TRY(ctx->dense.ensure(ctx, sizeof(uint64_t) * pl.n_cands));
+            uint64_t* qc_gl = (uint64_t*)ctx->q_corr.ptr + nq;
+            hipLaunchKernelGGL(k_query_corr<2 * N1>, dim3(nq), dim3(WAVE), 0, st,
+                               (const uint64_t*)d_gl, qc_gl);
+            hipLaunchKernelGGL((k_fp_gemm_tiles<2 * N1, 1>), dim3(tblocks), dim3(256), 0, st,
+                               d_lists, d_tiles, nt, d_gitems, d_items,
+                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_gl,
+                               (const uint64_t*)qc_gl, (uint64_t*)ctx->dense.ptr);
+            ctx->launches += 2;
+            dense = (const uint64_t*)ctx->dense.ptr;
+        }
+        if (tiles && pl.planes_ok) {
+            // M T and S v_enc on the same tiles; the decode ends there.
+            SVS_TRY(ctx->q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+            uint64_t* mt_q = (uint64_t*)ctx->q_corr.ptr + 2 * nq;
+            hipLaunchKernelGGL(k_decode_prep, dim3(nq), dim3(256), 0, st, (const uint64_t*)d_venc,
+                               (const int8_t*)d_tdense, (int8_t*)ctx->q_vplanes.ptr, mt_q);
+            hipLaunchKernelGGL(k_decode_tiles, dim3(tblocks), dim3(256), 0, st, d_lists, d_tiles,
+                               nt, d_gitems, d_items, (const uint32_t*)d_cand_ptr,
+                               (const int8_t*)d_tdense, (const int8_t*)ctx->q_vplanes.ptr,
+                               (const uint64_t*)mt_q, (const uint64_t*)ctx->r.ptr, dense,
+                               (int64_t*)ctx->scores.ptr);
+            ctx->launches += 2;
+        } else {
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((pl.n_cands + 255) / 256,
+                                                                 (uint64_t)ctx->n_cu * 32);
+            hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
+                               pl.n_items, (const uint32_t*)d_cand_ptr,
+                               (const uint64_t*)ctx->r.ptr, (const uint64_t*)d_gl,
+                               (const uint64_t*)d_venc, (const uint16_t*)d_tlist,
+                               (const uint32_t*)d_tcnt, dense, (int64_t*)ctx->scores.ptr,
+                               (uint64_t*)nullptr);
+            ctx->launches++;
+        }
+        HIP_TRY(hipGetLastError());
     }
     SVS_TRY(prof_mark(ctx, 3, false));
     // top-k
@@ -860,8 +958,10 @@ static int collect(svs_ctx* ctx, uint32_t nq, uint32_t k, uint32_t n_items, uint
     HIP_TRY(hipMemcpyAsync(&rng_flag, d_rng_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     SVS_TRY(prof_collect(ctx));
-    if (rng_flag)
+    if (rng_flag & 1u)
         return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p (probability 2^-61 per draw)");
+    if (rng_flag & 2u)
+        return fail(SVS_ERR_RNG, "a query's encode keystream ran past %d words", ENC_WORDS);
     return SVS_OK;
 }
 
@@ -885,15 +985,17 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
         const uint32_t nq = std::min(step, nq_total - q0);
         std::vector<uint32_t> pptr(nq + 1);
         for (uint32_t q = 0; q <= nq; ++q) pptr[q] = b->probe_ptr[q0 + q] - b->probe_ptr[q0];
+        const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
+        uint32_t trows, tqg;
+        tile_shape(kernel, trows, tqg);
         BatchPlan pl;
         SVS_TRY(make_plan(idx, nq, pptr.data(), b->probe_lists + b->probe_ptr[q0],
-                          b->verify ? b->verify + q0 : nullptr, pl));
+                          b->verify ? b->verify + q0 : nullptr, pl, trows, tqg));
         for (const Item& it : pl.items)
             if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
                 return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
         SVS_TRY(stage_queries(ctx, nq, b->v_q + (size_t)q0 * N, b->q_seeds + (size_t)q0 * 32));
-        SVS_TRY(run_plan(idx, pl, b->k, false,
-                         mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU, true));
+        SVS_TRY(run_plan(idx, pl, b->k, false, kernel, true));
         SVS_TRY(collect(ctx, nq, b->k, pl.n_items, b->out_ids + (size_t)q0 * b->k,
                         b->out_scores + (size_t)q0 * b->k, b->out_count + q0,
                         b->out_status + q0));
@@ -1078,8 +1180,8 @@ extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, cons
                        (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr, 1u, pre + 2,
                        (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_gl.ptr,
                        (const uint64_t*)ctx->q_venc.ptr, (const uint16_t*)ctx->q_tlist.ptr,
-                       (const uint32_t*)ctx->q_tcnt.ptr, (int64_t*)nullptr,
-                       (uint64_t*)ctx->scores.ptr);
+                       (const uint32_t*)ctx->q_tcnt.ptr, (const uint64_t*)nullptr,
+                       (int64_t*)nullptr, (uint64_t*)ctx->scores.ptr);
     ctx->launches += 2;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(mv, ctx->scores.ptr, sizeof(uint64_t) * m, hipMemcpyDeviceToHost,
@@ -1134,20 +1236,23 @@ extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
                                           const uint32_t* probe_lists, int kernel, uint64_t* r) {
     if (!idx || (n_queries && (!v_enc || !probe_ptr || !probe_lists || !r)))
         return fail(SVS_ERR_ARG, "null argument");
-    if (kernel != SVS_KERNEL_VALU && kernel != SVS_KERNEL_MFMA)
+    if (kernel != SVS_KERNEL_VALU && kernel != SVS_KERNEL_MFMA && kernel != SVS_KERNEL_MFMA_WAVE)
         return fail(SVS_ERR_ARG, "unknown kernel %d", kernel);
+    const int kern = kernel == SVS_KERNEL_MFMA ? KERNEL_MFMA
+                     : kernel == SVS_KERNEL_MFMA_WAVE ? KERNEL_MFMA_WAVE : KERNEL_VALU;
+    uint32_t trows, tqg;
+    tile_shape(kern, trows, tqg);
     if (n_queries == 0) return SVS_OK;
     SVS_TRY(check_canonical(v_enc, (size_t)n_queries * N, "v_enc"));
     svs_ctx* ctx = idx->ctx;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     BatchPlan pl;
-    SVS_TRY(make_plan(idx, n_queries, probe_ptr, probe_lists, nullptr, pl));
+    SVS_TRY(make_plan(idx, n_queries, probe_ptr, probe_lists, nullptr, pl, trows, tqg));
     SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)n_queries * N));
     HIP_TRY(hipMemcpyAsync(ctx->q_venc.ptr, v_enc, sizeof(uint64_t) * (size_t)n_queries * N,
                            hipMemcpyHostToDevice, ctx->stream));
-    SVS_TRY(run_plan(idx, pl, 1, true, kernel == SVS_KERNEL_MFMA ? KERNEL_MFMA : KERNEL_VALU,
-                     false));
+    SVS_TRY(run_plan(idx, pl, 1, true, kern, false));
     if (pl.n_cands)
         HIP_TRY(hipMemcpyAsync(r, ctx->r.ptr, sizeof(uint64_t) * pl.n_cands,
                                hipMemcpyDeviceToHost, ctx->stream));

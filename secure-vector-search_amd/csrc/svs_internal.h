@@ -85,8 +85,8 @@ struct svs_ctx {
     // scratch
     svs::DevBuf tmp_a, tmp_b, tmp_c;
     svs::DevBuf items, prefixes, flags, items1, prefixes1;
-    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed;
-    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr;
+    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
+    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr, dense;
     // cached single-item plan of svs_compute_products_dev
     int64_t cached_plan_list = -1;
     const void* cached_plan_index = nullptr;
@@ -102,8 +102,8 @@ struct svs_ctx {
     void note_peak();
     std::vector<svs::DevBuf*> all_scratch() {
         return {&tmp_a, &tmp_b, &tmp_c, &items, &prefixes, &flags, &items1, &prefixes1,
-                &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed,
-                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr};
+                &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed, &q_vplanes,
+                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr, &dense};
     }
 };
 

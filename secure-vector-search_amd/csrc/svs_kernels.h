@@ -205,8 +205,8 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
                uint32_t n_items, const uint32_t* __restrict__ cand_ptr,
                const uint64_t* __restrict__ r, const uint64_t* __restrict__ gl /* q x 256 */,
                const uint64_t* __restrict__ venc, const uint16_t* __restrict__ t_list,
-               const uint32_t* __restrict__ t_cnt, int64_t* __restrict__ scores,
-               uint64_t* __restrict__ mv_out) {
+               const uint32_t* __restrict__ t_cnt, const uint64_t* __restrict__ dense_in,
+               int64_t* __restrict__ scores, uint64_t* __restrict__ mv_out) {
     const uint32_t total = cand_ptr[n_items];
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += gridDim.x * blockDim.x) {
@@ -215,16 +215,21 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
         const ListDev L = lists[item.list];
         const uint32_t i = g - cand_ptr[it];
         const uint32_t m = item.m;
-        // AL g + H lt as one 256-long dot product against [g ; lt].
-        const uint64_t* glq = gl + (size_t)item.q * 2 * N1;
-        Acc128 acc;
-        acc.zero();
+        // AL g + H lt as one 256-long dot product against [g ; lt], unless
+        // the batch already computed it (dense_in, indexed like r).
         uint64_t dense = 0;
-        for (int k = 0; k < 2 * N1; ++k) {
-            acc.mac(L.alh_t[(size_t)k * m + i], glq[k]);
-            if ((k & 31) == 31) {
-                dense = add_mod(dense, acc.reduce());
-                acc.zero();
+        if (dense_in) {
+            dense = dense_in[g];
+        } else {
+            const uint64_t* glq = gl + (size_t)item.q * 2 * N1;
+            Acc128 acc;
+            acc.zero();
+            for (int k = 0; k < 2 * N1; ++k) {
+                acc.mac(L.alh_t[(size_t)k * m + i], glq[k]);
+                if ((k & 31) == 31) {
+                    dense = add_mod(dense, acc.reduce());
+                    acc.zero();
+                }
             }
         }
         // M T: gathered columns of M_plain (sparse +-1 mask T of the query).
@@ -469,45 +474,142 @@ k_topk_large(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
 }
 
 // ================================================================== encode
-// encode_query (crypto.rs:508-553) stream part, one thread per query:
-// G = 128 sample_fp draws, then T = sample_sparse_pm1 over n = 1024
+// encode_query (crypto.rs:508-553) stream part, one 256-thread block per
+// query: G = 128 sample_fp draws, then T = sample_sparse_pm1 over n = 1024
 // (crypto.rs:207-223): a word below the threshold marks a non-zero and the
 // next word's low bit picks its sign (0 -> -1). Also derives the
 // verification seed domain_separate(q_seed, "protocol2") (crypto.rs:142-150).
-__global__ void k_encode_stream(const uint8_t* __restrict__ q_seeds, uint32_t n_queries,
-                                uint64_t* __restrict__ g_out /* q x 256 (first 128) */,
-                                int8_t* __restrict__ t_dense /* q x 1024 */,
-                                uint16_t* __restrict__ t_list /* q x 1024 */,
-                                uint32_t* __restrict__ t_cnt, uint32_t* __restrict__ vseed_words) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_queries) return;
-    uint8_t seed[32];
-    for (int i = 0; i < 32; ++i) seed[i] = q_seeds[(size_t)q * 32 + i];
-    uint32_t key[8];
-    seed_to_key(seed, key);
-    Stream s;
-    s.init(key);
-    for (int k = 0; k < N1; ++k) g_out[(size_t)q * 2 * N1 + k] = s.fp();
-    uint32_t cnt = 0;
-    for (int j = 0; j < N; ++j) {
+//
+// ChaCha blocks are independent, so the query's keystream (ENC_WORDS words,
+// enough for G plus a T that selects every entry) is generated in parallel
+// into LDS. G's draws sit at words 2k, 2k+1 unless a draw is rejected
+// (probability 2^-61 each; then one thread walks G sequentially). T's walk
+// (each visited word advances 1, or 2 when it is below the threshold) runs
+// on wave 0 over 64-word windows: a ballot gives the window's "below" mask
+// and the scalar walk records which words it visits; all threads then place
+// entry j = (visited words before) at once.
+constexpr int ENC_BLOCKS = 160;
+constexpr int ENC_WORDS = ENC_BLOCKS * 16;  // 2560 >= 2 * 128 + 2 * 1024 + 2
+constexpr int ENC_WIN = ENC_WORDS / 64;
+
+__global__ void __launch_bounds__(256)
+k_encode_keystream(const uint8_t* __restrict__ q_seeds, uint64_t* __restrict__ g_out /* q x 256 (first 128) */,
+                   int8_t* __restrict__ t_dense /* q x 1024 */,
+                   uint16_t* __restrict__ t_list /* q x 1024 */, uint32_t* __restrict__ t_cnt,
+                   uint32_t* __restrict__ vseed_words, uint32_t* __restrict__ rng_flag) {
+    __shared__ uint32_t ks[ENC_WORDS];
+    __shared__ uint64_t vis[ENC_WIN], selv[ENC_WIN];
+    __shared__ uint32_t vis_base[ENC_WIN], sel_base[ENC_WIN];
+    __shared__ uint32_t key[8];
+    __shared__ uint32_t sh_tpos, sh_wend, sh_nsel;
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid % WAVE;
+    const uint8_t* seed = q_seeds + (size_t)q * 32;
+    if (tid < 8)
+        key[tid] = (uint32_t)seed[4 * tid] | ((uint32_t)seed[4 * tid + 1] << 8) |
+                   ((uint32_t)seed[4 * tid + 2] << 16) | ((uint32_t)seed[4 * tid + 3] << 24);
+    if (tid < ENC_WIN) vis[tid] = selv[tid] = 0;
+    __syncthreads();
+    if (tid < ENC_BLOCKS) {
+        uint32_t k[8], out[16];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) k[i] = key[i];
+        chacha20_block(k, (uint64_t)tid, out);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ks[tid * 16 + i] = out[i];
+    } else if (tid == 255) {
+        // domain_separate(q_seed, "protocol2"): first 32 next_u32 of the
+        // tagged seed's stream, truncated to bytes, are the new seed.
+        const char tag[9] = {'p', 'r', 'o', 't', 'o', 'c', 'o', 'l', '2'};
+        uint8_t sd[32];
+        for (int i = 0; i < 32; ++i) sd[i] = seed[i] ^ (i < 9 ? (uint8_t)tag[i] : 0);
+        uint32_t k[8], w[32];
+        seed_to_key(sd, k);
+        chacha20_block(k, 0, w);
+        chacha20_block(k, 1, w + 16);
+        for (int i = 0; i < 8; ++i)
+            vseed_words[(size_t)q * 8 + i] = (w[4 * i] & 0xFF) | ((w[4 * i + 1] & 0xFF) << 8) |
+                                             ((w[4 * i + 2] & 0xFF) << 16) | ((w[4 * i + 3] & 0xFF) << 24);
+    }
+    __syncthreads();
+    // G
+    bool rej = false;
+    if (tid < N1) {
+        const uint64_t x = ((uint64_t)ks[2 * tid] | ((uint64_t)ks[2 * tid + 1] << 32)) & P;
+        rej = x == P;
+        g_out[(size_t)q * 2 * N1 + tid] = x;
+    }
+    const bool any_rej = __syncthreads_or(rej);
+    if (tid == 0) {
+        uint32_t pos = 2 * N1;
+        if (any_rej) {
+            pos = 0;
+            for (int k = 0; k < N1; ++k) {
+                uint64_t x = P;
+                while (x == P && pos + 1 < (uint32_t)ENC_WORDS) {
+                    x = ((uint64_t)ks[pos] | ((uint64_t)ks[pos + 1] << 32)) & P;
+                    pos += 2;
+                }
+                if (x == P) atomicOr(rng_flag, 2u);
+                g_out[(size_t)q * 2 * N1 + k] = x == P ? 0 : x;
+            }
+        }
+        sh_tpos = pos;
+    }
+    __syncthreads();
+    // T walk on wave 0
+    if (tid < WAVE) {
+        const uint32_t tpos = sh_tpos;
+        uint32_t w = tpos / 64, x = tpos % 64, j = 0, nsel = 0;
+        bool overflow = false;
+        while (j < (uint32_t)N) {
+            if (w >= (uint32_t)ENC_WIN) {
+                overflow = true;
+                break;
+            }
+            const uint64_t below = __ballot(ks[64 * w + lane] < SPARSE_THRESHOLD);
+            uint64_t v = 0;
+            const uint32_t jb = j;
+            while (x < 64 && j < (uint32_t)N) {
+                v |= 1ull << x;
+                ++j;
+                x += 1 + (uint32_t)((below >> x) & 1);
+            }
+            // a selected word needs its sign word inside the buffer
+            if (64 * w + x > (uint32_t)ENC_WORDS) overflow = true;
+            if (lane == 0) {
+                vis[w] = v;
+                selv[w] = v & below;
+                vis_base[w] = jb;
+                sel_base[w] = nsel;
+            }
+            nsel += (uint32_t)__popcll(v & below);
+            x -= 64;
+            ++w;
+        }
+        if (lane == 0) {
+            sh_wend = w;
+            sh_nsel = nsel;
+            if (overflow) atomicOr(rng_flag, 2u);
+        }
+    }
+    __syncthreads();
+    // place T entries
+    const uint32_t w0 = sh_tpos / 64, w1 = sh_wend;
+    for (uint32_t p = 64 * w0 + tid; p < 64 * w1; p += blockDim.x) {
+        const uint32_t w = p / 64, b = p % 64;
+        const uint64_t below_mask = (1ull << b) - 1;
+        if (!((vis[w] >> b) & 1)) continue;
+        const uint32_t j = vis_base[w] + (uint32_t)__popcll(vis[w] & below_mask);
         int8_t v = 0;
-        if (s.u32() < SPARSE_THRESHOLD) {
-            v = (s.u32() & 1u) == 0 ? -1 : 1;
-            t_list[(size_t)q * N + cnt++] = (uint16_t)(j | (v < 0 ? 0x8000 : 0));
+        if ((selv[w] >> b) & 1) {
+            v = p + 1 < (uint32_t)ENC_WORDS && (ks[p + 1] & 1u) == 0 ? -1 : 1;
+            const uint32_t e = sel_base[w] + (uint32_t)__popcll(selv[w] & below_mask);
+            t_list[(size_t)q * N + e] = (uint16_t)(j | (v < 0 ? 0x8000 : 0));
         }
         t_dense[(size_t)q * N + j] = v;
     }
-    t_cnt[q] = cnt;
-    // domain_separate(q_seed, "protocol2")
-    const char tag[9] = {'p', 'r', 'o', 't', 'o', 'c', 'o', 'l', '2'};
-    for (int i = 0; i < 9; ++i) seed[i] ^= (uint8_t)tag[i];
-    seed_to_key(seed, key);
-    Stream d;
-    d.init(key);
-    uint8_t out[32];
-    for (int i = 0; i < 32; ++i) out[i] = (uint8_t)d.u32();
-    seed_to_key(out, key);
-    for (int w = 0; w < 8; ++w) vseed_words[(size_t)q * 8 + w] = key[w];
+    if (tid == 0) t_cnt[q] = sh_nsel;
 }
 
 // v_enc = L G + T + v (crypto.rs:533-551). Block = (query, 256 columns);
@@ -710,8 +812,17 @@ k_dense_terms(const uint64_t* __restrict__ alh_t, uint32_t m, const uint64_t* __
 // right bytes whatever order the unit sums them in.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// 16-byte load through the global address space. Pointers read out of a
+// ListDev are generic, and a generic (flat) load also counts against the LDS
+// wait counter, so every LDS wait or barrier would drain the loads still in
+// flight; a global load keeps the prefetch ring in the air.
+typedef __attribute__((address_space(1))) const v4i gv4i;
+typedef __attribute__((address_space(1))) const uint64_t gu64;
 __device__ __forceinline__ v4i load16(const uint64_t* p) {
-    return *reinterpret_cast<const v4i*>(p);
+    return *(gv4i*)(p);
+}
+__device__ __forceinline__ uint64_t gload_u64(const uint64_t* p) {
+    return *(gu64*)(p);
 }
 
 __global__ void __launch_bounds__(256)
@@ -816,32 +927,387 @@ k_gemv_mfma(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles, 
     }
 }
 
-// qcorr[q] = c * sum_k v_enc[q][k] - n c^2 (mod p): the query term of (*).
+// qcorr[q] = c * sum_k vec[q][k] - KE c^2 (mod p): the query term of (*) for
+// KE-long vectors (v_enc: 1024; [G ; L^T v_enc]: 256).
+template <int KE>
 __global__ void __launch_bounds__(WAVE)
-k_query_corr(const uint64_t* __restrict__ venc, uint64_t* __restrict__ qcorr) {
+k_query_corr(const uint64_t* __restrict__ vecs, uint64_t* __restrict__ qcorr) {
     const uint32_t q = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     uint64_t s = 0;
-    for (int k = lane; k < N; k += WAVE) s = add_mod(s, venc[(size_t)q * N + k]);
+    for (int k = lane; k < KE; k += WAVE) s = add_mod(s, vecs[(size_t)q * KE + k]);
     s = wave_sum_mod(s);
     if (lane == 0) {
         const uint64_t c = LIMB_OFFSET_C;
-        const uint64_t nc2 = mul_mod(mul_mod(c, c), (uint64_t)N);
+        const uint64_t nc2 = mul_mod(mul_mod(c, c), (uint64_t)KE);
         qcorr[q] = sub_mod(mul_mod(c, s), nc2);
     }
 }
 
-// rowcorr[i] = c * sum_k M_enc[i][k] (mod p): the row term of (*), one wave
-// per row.
+// rowcorr[i] = c * sum_k mat[i][k] (mod p): the row term of (*) for a
+// row-major m x KE matrix, one wave per row.
+template <int KE>
 __global__ void __launch_bounds__(256)
-k_row_corr(const uint64_t* __restrict__ menc, uint32_t m, uint64_t* __restrict__ rowcorr) {
+k_row_corr(const uint64_t* __restrict__ mat, uint32_t m, uint64_t* __restrict__ rowcorr) {
     const uint32_t lane = threadIdx.x % WAVE;
     for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < m;
          i += gridDim.x * (blockDim.x / WAVE)) {
         uint64_t s = 0;
-        for (int k = lane; k < N; k += WAVE) s = add_mod(s, menc[(size_t)i * N + k]);
+        for (int k = lane; k < KE; k += WAVE) s = add_mod(s, mat[(size_t)i * KE + k]);
         s = wave_sum_mod(s);
         if (lane == 0) rowcorr[i] = mul_mod(LIMB_OFFSET_C, s);
+    }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// With a grid that is a multiple of 8, worker w = (b % 8) * (G / 8) + b / 8
+// gives each XCD a contiguous range of workers, so neighbouring tiles (the
+// query groups of one row block, the row blocks of one list) share an L2.
+__device__ __forceinline__ uint32_t xcd_worker() {
+    const uint32_t b = blockIdx.x, g = gridDim.x;
+    return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Block-cooperative F_p GEMM tiles on int8 MFMA (same identity (*) as
+// k_gemv_mfma). A work tile is PB_ROWS rows of one list's row-major KE-wide
+// matrix (SRC 0: M_enc, KE = 1024; SRC 1: [AL | H], KE = 256) times up to
+// PB_QG query vectors. The B operand windows of a K'-step (query x output
+// column x element) are built once per block into double-buffered LDS and
+// read by all four waves; each wave owns PB_RT 16-row tiles. The next step's
+// A rows and query words are loaded before the current step's MFMAs, so HBM
+// latency overlaps the matrix-core work.
+template <int KE, int SRC>
+__global__ void __launch_bounds__(256)
+k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+                uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+                const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+                const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
+                uint64_t* __restrict__ out) {
+    constexpr int STEPS = KE / 8;  // 64 K'-bytes (8 elements) per step
+    __shared__ uint64_t bw[2][PB_QG][16][8];  // [buf][query][column][element]
+    __shared__ int32_t dsm[PB_WAVES][4][16][17];
+    const int tid = threadIdx.x;
+    const int lane = tid % WAVE, wave = tid / WAVE;
+    const int grp = lane / 16, col = lane % 16;
+    // producer role: query pj, element pe, columns pd0 .. pd0 + 3
+    const int pj = tid >> 5, pe = (tid >> 2) & 7, pd0 = (tid & 3) * 4;
+    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+        const MTile tile = tiles[t];
+        const ListDev L = lists[tile.list];
+        const uint32_t m = L.m;
+        const int gn = (int)tile.gn;
+        const uint64_t* base = SRC == 0 ? L.menc : L.alh;
+        const uint64_t* arow[PB_RT];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt) {
+            const uint32_t row = min(tile.row0 + 64 * wave + 16 * rt + col, m - 1);
+            arow[rt] = base + (size_t)row * KE + 2 * grp;
+        }
+        const bool producer = pj < gn;
+        const uint64_t* pv =
+            vecs + (size_t)items[gitems[tile.g0 + (producer ? pj : 0)]].q * KE + pe;
+        auto produce = [&](int buf, uint64_t v) {
+            const uint64_t x = __builtin_bswap64(v ^ 0x8080808080808080ull);
+#pragma unroll
+            for (int dd = 0; dd < 4; ++dd) {
+                const int d = pd0 + dd;
+                uint64_t w = 0;
+                if (d <= 7) w = x >> (8 * (7 - d));
+                else if (d <= 14) w = x << (8 * (d - 7));
+                bw[buf][pj][d][pe] = w;
+            }
+        };
+        // PB_PF-deep register ring: ar[d] / vr[d] hold step s with s % PB_PF == d.
+        v4i ar[PB_PF][PB_RT];
+        uint64_t vr[PB_PF];
+#pragma unroll
+        for (int d = 0; d < PB_PF; ++d) {
+#pragma unroll
+            for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * d);
+            vr[d] = gload_u64(pv + 8 * d);
+        }
+        if (producer) produce(0, vr[0]);
+        __syncthreads();
+        v4i acc[PB_RT][PB_QG];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt)
+#pragma unroll
+            for (int j = 0; j < PB_QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
+        static_assert(STEPS % PB_PF == 0 && PB_PF % 2 == 0, "ring must tile the steps");
+        for (int s0 = 0; s0 < STEPS; s0 += PB_PF) {
+#pragma unroll
+            for (int d = 0; d < PB_PF; ++d) {
+                // Loads are unconditional (the last PB_PF steps re-load the
+                // final step, in bounds) so the wait counts stay exact.
+                const int nxt = min(s0 + d + PB_PF, STEPS - 1);
+                v4i a[PB_RT];
+#pragma unroll
+                for (int rt = 0; rt < PB_RT; ++rt) a[rt] = ar[d][rt] ^ (int)0x80808080;
+#pragma unroll
+                for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * nxt);
+#pragma unroll
+                for (int j = 0; j < PB_QG; ++j) {
+                    if (j < gn) {
+                        const v4i b = *reinterpret_cast<const v4i*>(&bw[d & 1][j][col][2 * grp]);
+#pragma unroll
+                        for (int rt = 0; rt < PB_RT; ++rt)
+                            acc[rt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt], b,
+                                                                                 acc[rt][j], 0, 0, 0);
+                    }
+                }
+                // (after the last step this fills the idle buffer with a
+                // repeat of the final step; nothing reads it)
+                if (producer) produce((d + 1) & 1, vr[(d + 1) % PB_PF]);
+                vr[d] = gload_u64(pv + 8 * nxt);
+                __syncthreads();
+            }
+        }
+        // Epilogue, four (row tile, query) pairs per round (see k_gemv_mfma).
+#pragma unroll
+        for (int rnd = 0; rnd < PB_RT * PB_QG / 4; ++rnd) {
+#pragma unroll
+            for (int pp = 0; pp < 4; ++pp) {
+                const int pair = rnd * 4 + pp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    dsm[wave][pp][4 * grp + i][col] = acc[pair / PB_QG][pair % PB_QG][i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int pp = lane / 16, row = lane % 16;
+            const int pair = rnd * 4 + pp;
+            const int rt = pair / PB_QG, j = pair % PB_QG;
+            const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;
+            if (j < gn && grow < m) {
+                __int128 sacc = 0;
+#pragma unroll
+                for (int d = 0; d < 15; ++d)
+                    sacc += (__int128)dsm[wave][pp][row][d] << ((8 * d) % 61);
+                const unsigned __int128 u = (unsigned __int128)(sacc + ((__int128)P << 27));
+                uint64_t v = reduce128((uint64_t)u, (uint64_t)(u >> 64));
+                const uint32_t it = gitems[tile.g0 + j];
+                v = add_mod(v, (SRC == 0 ? L.rowcorr : L.alh_rowcorr)[grow]);
+                v = add_mod(v, qcorr[items[it].q]);
+                out[cand_ptr[it] + grow] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ MFMA decode
+// Signed integers to F_p.
+__device__ __forceinline__ uint64_t i64_to_fp(int64_t x) {
+    return x >= 0 ? (uint64_t)x % P : sub_mod(0, (uint64_t)(-x) % P);
+}
+__device__ __forceinline__ uint64_t i128_to_fp(__int128 x) {
+    // |x| < 2^75 for every caller; adding p * 2^14 (> 2^75) keeps it positive.
+    const unsigned __int128 u = (unsigned __int128)(x + ((__int128)P << 14));
+    return reduce128((uint64_t)u, (uint64_t)(u >> 64));
+}
+
+// M_plain rows (canonical, row-major m x 1024) -> the 24-bit byte planes of
+// mp8. The caller has checked that every signed lift is in (-2^23, 2^23).
+__global__ void k_make_mp8(const uint64_t* __restrict__ mp, uint32_t m, int8_t* __restrict__ mp8) {
+    const uint64_t total = (uint64_t)m * N;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = mp[e];
+        const int64_t sl = x <= P / 2 ? (int64_t)x : -(int64_t)(P - x);
+        const uint32_t y = (uint32_t)(sl + MP8_LIMIT);
+        int8_t* row = mp8 + (e / N) * 3 * N + (e % N);
+        row[0] = (int8_t)((y & 0xFF) ^ 0x80);
+        row[N] = (int8_t)(((y >> 8) & 0xFF) ^ 0x80);
+        row[2 * N] = (int8_t)(((y >> 16) & 0xFF) ^ 0x80);
+    }
+}
+
+// S (sliced ELL) -> dense int8 rows (pre-zeroed) and the row term
+// c * sum_k S[row][k] of S v_enc's byte-offset identity.
+__global__ void k_make_s8(const uint32_t* __restrict__ s_slice, const uint16_t* __restrict__ s_ell,
+                          uint32_t m, int8_t* __restrict__ s8, uint64_t* __restrict__ rowterm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int8_t* row = s8 + (size_t)i * N;
+    const uint32_t slice = i / SLICE, lane = i % SLICE;
+    int64_t sum = 0;
+    for (uint32_t e = s_slice[slice] + lane; e < s_slice[slice + 1]; e += SLICE) {
+        const uint16_t se = s_ell[e];
+        if (se == S_PAD) break;
+        const int v = (se & 0x8000) ? -1 : 1;
+        row[se & 0x3FF] = (int8_t)v;
+        sum += v;
+    }
+    rowterm[i] = mul_mod(LIMB_OFFSET_C, i64_to_fp(sum));
+}
+
+// Per query: the signed byte planes of v_enc (q x 8 x 1024; plane b holds
+// byte b of each element, ^ 0x80) and MP8_OFFSET * sum_k T[k] (mod p).
+__global__ void __launch_bounds__(256)
+k_decode_prep(const uint64_t* __restrict__ venc, const int8_t* __restrict__ t_dense,
+              int8_t* __restrict__ vplanes, uint64_t* __restrict__ mt_qterm) {
+    __shared__ int red[256];
+    const uint32_t q = blockIdx.x;
+    int tsum = 0;
+    for (int k = threadIdx.x; k < N; k += 256) {
+        const uint64_t v = venc[(size_t)q * N + k];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            vplanes[((size_t)q * 8 + b) * N + k] = (int8_t)(((v >> (8 * b)) & 0xFF) ^ 0x80);
+        tsum += t_dense[(size_t)q * N + k];
+    }
+    red[threadIdx.x] = tsum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mt_qterm[q] = mul_mod((uint64_t)MP8_OFFSET, i64_to_fp(red[0]));
+}
+
+__device__ __forceinline__ v4i load16b(const int8_t* p) {
+    return *(gv4i*)(p);
+}
+
+// Block-cooperative MFMA decode over the product's tiles (crypto.rs:901-940):
+//   Mv = r - (AL g + H lt) - M T - S v_enc (mod p), score = fp_to_signed(Mv).
+// The dense term comes in precomputed (indexed like r). M T and S v_enc are
+// exact int8 products over K = 1024:
+//   M T_j   = sum_a 256^a (s_a . T_j) + MP8_OFFSET sum T_j,
+//   S v_j   = sum_b 256^b (S . v'_{j,b}) + c sum S,
+// with A = the row's three M planes and S, B = T_j (one column per query,
+// columns 8..15 zero) and the eight v_enc byte planes of query j (two
+// queries per 16-column tile). Each wave owns PB_RT 16-row tiles.
+__global__ void __launch_bounds__(256)
+k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+               uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+               const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+               const int8_t* __restrict__ t_dense, const int8_t* __restrict__ vplanes,
+               const uint64_t* __restrict__ mt_qterm, const uint64_t* __restrict__ r,
+               const uint64_t* __restrict__ dense, int64_t* __restrict__ scores) {
+    constexpr int KC = N / 64;  // 16 K-steps of 64 bytes
+    constexpr int NCT = PB_QG / 2;  // S v column tiles
+    __shared__ int32_t dsm[PB_WAVES][3 + NCT][16][17];
+    const int tid = threadIdx.x;
+    const int lane = tid % WAVE, wave = tid / WAVE;
+    const int grp = lane / 16, col = lane % 16;
+    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+        const MTile tile = tiles[t];
+        const ListDev L = lists[tile.list];
+        const uint32_t m = L.m;
+        const int gn = (int)tile.gn;
+        const int8_t* amp[PB_RT];
+        const int8_t* as8[PB_RT];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt) {
+            const uint32_t row = min(tile.row0 + 64 * wave + 16 * rt + col, m - 1);
+            amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
+            as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
+        }
+        // B sources; a column without a query reads query 0's data and is
+        // masked to zero (loads stay unconditional).
+        const uint32_t q0 = items[gitems[tile.g0]].q;
+        const bool t_ok = col < gn;
+        const int8_t* bt =
+            t_dense + (size_t)(t_ok ? items[gitems[tile.g0 + col]].q : q0) * N + 16 * grp;
+        const int tmask = t_ok ? -1 : 0;
+        const int8_t* bv[NCT];
+        int vmask[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) {
+            const int j = 2 * ct + col / 8;
+            const bool ok = j < gn;
+            const uint32_t qj = ok ? items[gitems[tile.g0 + j]].q : q0;
+            bv[ct] = vplanes + ((size_t)qj * 8 + col % 8) * N + 16 * grp;
+            vmask[ct] = ok ? -1 : 0;
+        }
+        v4i amt[PB_RT][3], asv[PB_RT][NCT];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) amt[rt][a] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
+        }
+        // two-deep register ring over the K-steps
+        v4i ra[2][PB_RT][4], rb[2][1 + NCT];
+        auto issue = [&](int slot, int kc) {
+#pragma unroll
+            for (int rt = 0; rt < PB_RT; ++rt) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kc);
+                ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
+            }
+            rb[slot][0] = load16b(bt + 64 * kc);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) rb[slot][1 + ct] = load16b(bv[ct] + 64 * kc);
+        };
+        issue(0, 0);
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            const int slot = kc & 1;
+            if (kc + 1 < KC) issue(slot ^ 1, kc + 1);
+            const v4i b_t = rb[slot][0] & tmask;
+#pragma unroll
+            for (int rt = 0; rt < PB_RT; ++rt)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][a], b_t,
+                                                                        amt[rt][a], 0, 0, 0);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const v4i b_v = rb[slot][1 + ct] & vmask[ct];
+#pragma unroll
+                for (int rt = 0; rt < PB_RT; ++rt)
+                    asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][3], b_v,
+                                                                         asv[rt][ct], 0, 0, 0);
+            }
+        }
+        // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i, column
+        // col), then each lane finishes (row, query) pairs lane and lane + 64.
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dsm[wave][a][4 * grp + i][col] = amt[rt][a][i];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) dsm[wave][3 + ct][4 * grp + i][col] = asv[rt][ct][i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int pr = lane + 64 * h;
+                const int row = pr % 16, j = pr / 16;
+                const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;
+                if (j < gn && grow < m) {
+                    const uint32_t it = gitems[tile.g0 + j];
+                    const int64_t mt = (int64_t)dsm[wave][0][row][j] +
+                                       (int64_t)dsm[wave][1][row][j] * 256 +
+                                       (int64_t)dsm[wave][2][row][j] * 65536;
+                    __int128 sv = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        sv += (__int128)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
+                              (__int128)(1ull << (8 * b));
+                    const uint64_t g = cand_ptr[it] + grow;
+                    uint64_t corr = add_mod(i64_to_fp(mt), mt_qterm[items[it].q]);
+                    corr = add_mod(corr, add_mod(i128_to_fp(sv), L.s_rowterm[grow]));
+                    corr = add_mod(corr, dense[g]);
+                    scores[g] = fp_to_signed(sub_mod(r[g], corr));
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
     }
 }
 

@@ -365,6 +365,8 @@ extern "C" int svs_tiptoe_answer_u32_dev(svs_tiptoe* t, uint32_t n_queries, cons
                                          uint32_t* ans_dev, int kernel) {
     if (!t || !ct_dev || !ans_dev) return fail(SVS_ERR_ARG, "null argument");
     if (t->elem == 8) return fail(SVS_ERR_STATE, "database was uploaded as u64");
+    if (kernel != -1 && kernel != SVS_KERNEL_VALU && kernel != SVS_KERNEL_MFMA)
+        return fail(SVS_ERR_ARG, "unknown kernel %d", kernel);
     if (n_queries == 0) return SVS_OK;
     svs_ctx* ctx = t->ctx;
     std::lock_guard<std::mutex> g(ctx->mu);

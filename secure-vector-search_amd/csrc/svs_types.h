@@ -21,9 +21,20 @@ struct ListDev {
     const uint16_t* s_ell;   // per slice: width x 64 entries, idx | 0x8000 if sign < 0
     const uint32_t* ids;     // m global vector ids
     const uint64_t* rowcorr; // m: c * sum_k M_enc[row][k] mod p, c = 128 * 0x0101010101010101
+    const uint64_t* alh;     // m x 256, row-major [AL | H] (MFMA decode)
+    const uint64_t* alh_rowcorr;  // m: c * sum_k alh[row][k] mod p
+    // MFMA decode (null when M_plain does not fit the 24-bit planes):
+    const int8_t* mp8;       // m x 3 x 1024: byte planes s_a of M_plain + 2^23, each ^ 0x80
+    const int8_t* s8;        // m x 1024: S as dense int8 (+-1 / 0)
+    const uint64_t* s_rowterm;  // m: 128 * 0x0101010101010101 * sum_k S[row][k] mod p
     uint32_t m;
     uint32_t pad;
 };
+
+// M_plain = sum_a 256^a s_a + MP8_OFFSET for the planes of mp8 (|M| < 2^23):
+// 128 * (1 + 256 + 65536) - 2^23.
+constexpr int64_t MP8_OFFSET = 32896;
+constexpr int64_t MP8_LIMIT = 1 << 23;
 
 // One MFMA work tile: MT_ROWS rows of a list times up to MT_QG items
 // (queries) that probe it; the items are gitems[g0 .. g0 + gn).
@@ -36,6 +47,14 @@ struct MTile {
     uint32_t g0;
     uint32_t gn;
 };
+
+// Block-cooperative MFMA tiles: PB_WAVES waves x PB_RT 16-row tiles each
+// (PB_ROWS rows) times up to PB_QG queries of one list.
+constexpr int PB_WAVES = 4;
+constexpr int PB_RT = 4;
+constexpr int PB_ROWS = 16 * PB_RT * PB_WAVES;  // 256
+constexpr int PB_QG = 8;
+constexpr int PB_PF = 8;   // K'-steps of A rows and query words loaded ahead
 
 // c = 128 * sum_{a<8} 2^(8a) reduced mod p: the offset between a u64 and its
 // signed-byte-limb value (each byte b read as b - 128).

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libsvs_gpu.so")
 SVS_OK, SVS_NONE = 0, 1
 SVS_ERR_ARG, SVS_ERR_HIP, SVS_ERR_BUDGET, SVS_ERR_STATE, SVS_ERR_RNG = -1, -2, -3, -4, -5
 MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_VALU = 0, 1, 2
-KERNEL_VALU, KERNEL_MFMA = 0, 1
+KERNEL_VALU, KERNEL_MFMA, KERNEL_MFMA_WAVE = 0, 1, 2
 N, N1, LAMBDA = 1024, 128, 3
 P = (1 << 61) - 1
 

@@ -86,7 +86,7 @@ def test_batched_products_mfma_and_valu_match_oracle(ctx, l_sub):
     probes[3] = np.arange(len(sizes))  # one query probes every list
     pp = np.concatenate([[0], np.cumsum([len(x) for x in probes])]).astype(np.uint32)
     pl = np.concatenate(probes).astype(np.uint32)
-    for kernel in (svs._lib.KERNEL_MFMA, svs._lib.KERNEL_VALU):
+    for kernel in (svs._lib.KERNEL_MFMA, svs._lib.KERNEL_MFMA_WAVE, svs._lib.KERNEL_VALU):
         say("batched products kernel", kernel)
         got = idx.compute_products_batch(v, pp, pl, kernel)
         item = 0
@@ -105,19 +105,20 @@ def test_mfma_many_queries_per_list(ctx, l_sub):
     """A hot list probed by many queries (several MFMA query groups)."""
     import svs
     rng = np.random.default_rng(91)
-    m = 200
+    m = 300  # two 256-row block tiles, the second ragged
     cl = _random_cluster(rng, m)
     idx = svs.GpuIndex(ctx, 1, l_sub)
     idx.upload_list(0, cl)
-    nq = 37
+    nq = 37  # 5 groups of 8 (block tiles) or 10 of 4 (wave tiles), last ragged
     v = _field(rng, (nq, 1024))
     pp = np.arange(nq + 1, dtype=np.uint32)
     pl = np.zeros(nq, np.uint32)
-    say("mfma hot list, queries", nq)
-    got = idx.compute_products_batch(v, pp, pl, svs._lib.KERNEL_MFMA)
     want = O.matvec_batch(cl["m_enc"], m, 1024, v)
-    for q in range(nq):
-        assert np.array_equal(got[q], want[q]), f"query {q}"
+    for kernel in (svs._lib.KERNEL_MFMA, svs._lib.KERNEL_MFMA_WAVE):
+        say("mfma hot list, queries", nq, "kernel", kernel)
+        got = idx.compute_products_batch(v, pp, pl, kernel)
+        for q in range(nq):
+            assert np.array_equal(got[q], want[q]), f"kernel {kernel} query {q}"
 
 
 def test_compute_products_extremes(ctx, l_sub):
@@ -404,3 +405,57 @@ def test_realised_cost_counts_probed_rows(ivf_setup):
     assert cost.response_bytes == rows * 8
     assert cost.effective_bytes_per_query == rows * 1024 * 8
     assert len(hits) == 10
+
+
+def _oracle_probe_scores(clusters, probes, v_q, seed, l, k):
+    """ivf_score's per-list steps (ivf.rs:746-822) for an explicit probe set."""
+    v_enc, g, t_idx, t_sign = O.encode_query(v_q, l, seed)
+    ids, scores = [], []
+    for c in probes:
+        cl = clusters[c]
+        r = O.matvec(cl["m_enc"], cl["m"], 1024, v_enc)
+        mv = O.decode_scores(r, cl["m"], cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"],
+                             cl["s_idx"], cl["s_sign"], l, v_enc, g, t_idx, t_sign)
+        ids.extend((1000 * c + np.arange(cl["m"])).tolist())
+        scores.extend(O.fp_to_signed_vec(mv).tolist())
+    return O.topk(np.array(ids, np.uint32), np.array(scores, np.int64), k)
+
+
+def test_score_batch_decode_planes_and_fallback(ctx, l_sub):
+    """Lists whose plaintext fits the 24-bit decode planes use the MFMA
+    decode; a batch touching any other list (|M| >= 2^23 in 32 bits, or a
+    full-width field element) decodes per candidate. Both bit-exact."""
+    import svs
+    rng = np.random.default_rng(2024)
+    lim = 1 << 23
+    m = 70
+
+    def lift(v):
+        v = np.asarray(v, np.int64)
+        return np.array([int(x) % P for x in v], np.uint64)
+
+    small = rng.integers(-lim + 1, lim, size=m * 1024)
+    small[:4] = [lim - 1, -(lim - 1), 0, -1]
+    edge = rng.integers(-1000, 1000, size=m * 1024)
+    edge[7] = lim  # one entry just outside the planes
+    wide32 = rng.integers(-(1 << 30), 1 << 30, size=m * 1024)
+    mps = [lift(small), lift(edge), lift(wide32), _field(rng, m * 1024)]
+    clusters = [O.build_cluster(mp, m, l_sub, bytes([60 + i] * 32)) for i, mp in enumerate(mps)]
+    idx = svs.GpuIndex(ctx, len(clusters), l_sub)
+    for i, cl in enumerate(clusters):
+        idx.upload_list(i, cl, ids=1000 * i + np.arange(m, dtype=np.uint32))
+    nq, k = 5, 12
+    v_q = lift(rng.integers(-(1 << 20), 1 << 20, size=(nq, 1024)).reshape(-1)).reshape(nq, 1024)
+    seeds = rng.integers(0, 256, size=(nq, 32), dtype=np.uint8)
+    for probes in ([[0]] * nq, [[0, 1], [2], [3, 0], [1, 2, 3], [0]]):
+        pp = np.concatenate([[0], np.cumsum([len(p) for p in probes])]).astype(np.uint32)
+        pl = np.concatenate(probes).astype(np.uint32)
+        want = [_oracle_probe_scores(clusters, probes[q], v_q[q], bytes(seeds[q]), l_sub, k)
+                for q in range(nq)]
+        for mode in (0, 1, 2):
+            say("plane fallback probes", probes, "mode", mode)
+            ids, scores, count, status = idx.score_batch(v_q, seeds, pp, pl, None, k, mode=mode)
+            for q in range(nq):
+                assert status[q] == 0 and count[q] == k
+                assert ids[q].tolist() == want[q][0].tolist(), f"mode {mode} query {q}"
+                assert scores[q].tolist() == want[q][1].tolist(), f"mode {mode} query {q}"
