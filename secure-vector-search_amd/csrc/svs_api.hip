@@ -570,6 +570,8 @@ struct BatchPlan {
     bool planes_ok = true;  // every probed list has the MFMA decode planes
     std::vector<Item> items;
     std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
+    std::vector<uint32_t> qchunk_ptr;  // per query: its top-k wave chunks
+    uint32_t n_tchunks = 0;
     std::vector<uint8_t> item_verify;
     // MFMA product: items grouped by list (probe order kept within a list)
     // and work tiles of MT_ROWS rows x up to MT_QG of those items.
@@ -613,6 +615,15 @@ static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_pt
     pl.cand_ptr[pl.n_items] = (uint32_t)cd;
     pl.n_chunks = (uint32_t)ch;
     pl.n_cands = (uint32_t)cd;
+    pl.qchunk_ptr.resize(nq + 1);
+    uint32_t tch = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        pl.qchunk_ptr[q] = tch;
+        const uint32_t nc = pl.cand_ptr[pl.item_ptr[q + 1]] - pl.cand_ptr[pl.item_ptr[q]];
+        tch += (nc + TOPK_WCHUNK - 1) / TOPK_WCHUNK;
+    }
+    pl.qchunk_ptr[nq] = tch;
+    pl.n_tchunks = tch;
     std::vector<uint32_t> seen;
     seen.reserve(pl.n_items);
     for (const Item& it : pl.items) seen.push_back(it.list);
@@ -729,11 +740,12 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     const uint32_t nq = pl.n_queries;
     // plan arrays
     SVS_TRY(ctx->items.ensure(ctx, sizeof(Item) * std::max<uint32_t>(pl.n_items, 1)));
-    SVS_TRY(ctx->prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + nq + 1)));
+    SVS_TRY(ctx->prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + 2 * (nq + 1))));
     SVS_TRY(ctx->flags.ensure(ctx, std::max<uint32_t>(pl.n_items, 1) + 64));
     uint32_t* d_chunk_ptr = (uint32_t*)ctx->prefixes.ptr;
     uint32_t* d_cand_ptr = d_chunk_ptr + pl.n_items + 1;
     uint32_t* d_item_ptr = d_cand_ptr + pl.n_items + 1;
+    uint32_t* d_qchunk_ptr = d_item_ptr + nq + 1;
     HIP_TRY(hipMemcpyAsync(ctx->items.ptr, pl.items.data(), sizeof(Item) * pl.n_items,
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_chunk_ptr, pl.chunk_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
@@ -741,6 +753,8 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     HIP_TRY(hipMemcpyAsync(d_cand_ptr, pl.cand_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_item_ptr, pl.item_ptr.data(), sizeof(uint32_t) * (nq + 1),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_qchunk_ptr, pl.qchunk_ptr.data(), sizeof(uint32_t) * (nq + 1),
                            hipMemcpyHostToDevice, st));
     if (pl.n_items)
         HIP_TRY(hipMemcpyAsync(ctx->flags.ptr, pl.item_verify.data(), pl.n_items,
@@ -768,7 +782,7 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
         hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
                            (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
                            (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
-        hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1), 0, st, (const uint64_t*)idx->l_dev,
+        hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1 * LT_PARTS), 0, st, (const uint64_t*)idx->l_dev,
                            (const uint64_t*)d_venc, d_gl);
         ctx->launches += 3;
         HIP_TRY(hipGetLastError());
@@ -905,12 +919,19 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     uint32_t* d_oids = (uint32_t*)ctx->out.ptr;
     int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
     uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
-    if (k <= 16)
-        hipLaunchKernelGGL((k_topk<16, 256>), dim3(nq), dim3(256), 0, st, d_items,
+    if (k <= (uint32_t)TOPK_LANE) {
+        SVS_TRY(ctx->topk_part.ensure(ctx, sizeof(Cand) * (size_t)std::max<uint32_t>(pl.n_tchunks, 1) * k));
+        if (pl.n_tchunks)
+            hipLaunchKernelGGL(k_topk_chunks, dim3((pl.n_tchunks + 3) / 4), dim3(256), 0, st,
+                               (const uint32_t*)d_qchunk_ptr, nq, pl.n_tchunks,
+                               (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr,
+                               (const int64_t*)ctx->scores.ptr, k, (Cand*)ctx->topk_part.ptr);
+        hipLaunchKernelGGL((k_topk_final<256>), dim3(nq), dim3(256), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
-                           d_oscores, d_ocount);
-    else if (k <= 128)
+                           (const uint32_t*)d_qchunk_ptr, (const Cand*)ctx->topk_part.ptr,
+                           (const int32_t*)d_qstatus, k, d_oids, d_oscores, d_ocount);
+        ctx->launches++;
+    } else if (k <= 128)
         hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
                            (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
@@ -1173,7 +1194,7 @@ extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, cons
                                hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->q_tcnt.ptr, &t_nnz, sizeof(uint32_t), hipMemcpyHostToDevice,
                            ctx->stream));
-    hipLaunchKernelGGL(k_lt, dim3(1), dim3(N1), 0, ctx->stream, (const uint64_t*)idx->l_dev,
+    hipLaunchKernelGGL(k_lt, dim3(1), dim3(N1 * LT_PARTS), 0, ctx->stream, (const uint64_t*)idx->l_dev,
                        (const uint64_t*)ctx->q_venc.ptr, (uint64_t*)ctx->q_gl.ptr);
     const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
     hipLaunchKernelGGL(k_decode_cands, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,

@@ -86,7 +86,7 @@ struct svs_ctx {
     svs::DevBuf tmp_a, tmp_b, tmp_c;
     svs::DevBuf items, prefixes, flags, items1, prefixes1;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
-    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr, dense;
+    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr, dense, topk_part;
     // cached single-item plan of svs_compute_products_dev
     int64_t cached_plan_list = -1;
     const void* cached_plan_index = nullptr;
@@ -103,7 +103,7 @@ struct svs_ctx {
     std::vector<svs::DevBuf*> all_scratch() {
         return {&tmp_a, &tmp_b, &tmp_c, &items, &prefixes, &flags, &items1, &prefixes1,
                 &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed, &q_vplanes,
-                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr, &dense};
+                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr, &dense, &topk_part};
     }
 };
 

@@ -348,6 +348,132 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
     if (tid == 0) out_count[q] = kk;
 }
 
+// Two-phase top-k for k <= TOPK_LANE (the served k = 10 case).
+// Phase 1: every 1024-candidate chunk of a query (one wave, 16 candidates
+// per lane held in registers) yields its k best by k rounds of a wave
+// arg-max; the winning lane retires its candidate. A query's chunks are
+// numbered qchunk_ptr[q] .. qchunk_ptr[q+1]. Output per chunk: k (score,
+// position) pairs best first, position 0xFFFFFFFF once the chunk runs out.
+constexpr int TOPK_LANE = 16;
+constexpr int TOPK_WCHUNK = WAVE * TOPK_LANE;
+
+__device__ __forceinline__ void wave_best(int64_t& score, uint32_t& pos) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int64_t os = __shfl_xor(score, off);
+        const uint32_t op = __shfl_xor(pos, off);
+        if (os > score || (os == score && op < pos)) {
+            score = os;
+            pos = op;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_topk_chunks(const uint32_t* __restrict__ qchunk_ptr, uint32_t nq, uint32_t n_chunks,
+              const uint32_t* __restrict__ item_ptr, const uint32_t* __restrict__ cand_ptr,
+              const int64_t* __restrict__ scores, uint32_t k, Cand* __restrict__ part) {
+    const uint32_t c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (c >= n_chunks) return;  // uniform per wave
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t q = find_seg(qchunk_ptr, nq, c);
+    const uint32_t c0 = cand_ptr[item_ptr[q]], c1 = cand_ptr[item_ptr[q + 1]];
+    const uint32_t lo = c0 + (c - qchunk_ptr[q]) * TOPK_WCHUNK;
+    int64_t v[TOPK_LANE];
+#pragma unroll
+    for (int i = 0; i < TOPK_LANE; ++i) {
+        const uint32_t g = lo + i * WAVE + lane;
+        v[i] = g < c1 ? scores[g] : INT64_MIN;  // scores are in (-p/2, p/2)
+    }
+    uint32_t used = 0;
+    for (uint32_t r = 0; r < k; ++r) {
+        int64_t best = INT64_MIN;
+        int bi = -1;
+#pragma unroll
+        for (int i = 0; i < TOPK_LANE; ++i)
+            if (!((used >> i) & 1) && v[i] > best) {
+                best = v[i];
+                bi = i;
+            }
+        const uint32_t mine = bi >= 0 ? lo - c0 + bi * WAVE + lane : 0xFFFFFFFFu;
+        uint32_t pos = mine;
+        wave_best(best, pos);
+        if (lane == 0) part[(size_t)c * k + r] = Cand{best, pos, 0};
+        if (bi >= 0 && pos == mine) used |= 1u << bi;
+    }
+}
+
+// Phase 2: per query, the k best of its chunks' lists (each thread keeps a
+// sorted list in LDS over a strided subset, then k rounds of block arg-max),
+// then positions mapped back to (item, row) and global ids.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS)
+k_topk_final(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
+             const uint32_t* __restrict__ cand_ptr, const ListDev* __restrict__ lists,
+             const uint32_t* __restrict__ qchunk_ptr, const Cand* __restrict__ part,
+             const int32_t* __restrict__ q_status, uint32_t k, uint32_t* __restrict__ out_ids,
+             int64_t* __restrict__ out_scores, uint32_t* __restrict__ out_count) {
+    constexpr int KT = TOPK_LANE;
+    __shared__ Cand lst[THREADS][KT];
+    __shared__ int cnt[THREADS];
+    __shared__ int head[THREADS];
+    __shared__ int red_idx[THREADS];
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (q_status[q] != 0) {
+        if (tid == 0) out_count[q] = 0;
+        return;
+    }
+    const uint32_t i0 = item_ptr[q], i1 = item_ptr[q + 1];
+    const uint32_t c0 = cand_ptr[i0];
+    const uint32_t e0 = qchunk_ptr[q] * k, e1 = qchunk_ptr[q + 1] * k;
+    int n = 0;
+    for (uint32_t e = e0 + tid; e < e1; e += THREADS) {
+        const Cand c = part[e];
+        if (c.pos == 0xFFFFFFFFu) continue;
+        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
+        int pos = n < (int)k ? n : (int)k - 1;
+        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
+            lst[tid][pos] = lst[tid][pos - 1];
+            --pos;
+        }
+        lst[tid][pos] = c;
+        if (n < (int)k) ++n;
+    }
+    cnt[tid] = n;
+    head[tid] = 0;
+    __syncthreads();
+    const uint32_t kk = min(k, cand_ptr[i1] - c0);
+    for (uint32_t outi = 0; outi < kk; ++outi) {
+        red_idx[tid] = head[tid] < cnt[tid] ? tid : -1;
+        __syncthreads();
+        for (int s = THREADS / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                int a = red_idx[tid], b = red_idx[tid + s];
+                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                    red_idx[tid] = b;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int w = red_idx[0];
+            const Cand c = lst[w][head[w]];
+            head[w]++;
+            const uint32_t g = c0 + c.pos;
+            uint32_t lo = i0, hi = i1;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (cand_ptr[mid] <= g) lo = mid; else hi = mid;
+            }
+            const Item it = items[lo];
+            out_ids[(size_t)q * k + outi] = lists[it.list].ids[g - cand_ptr[lo]];
+            out_scores[(size_t)q * k + outi] = c.score;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out_count[q] = kk;
+}
+
 // Large k (up to TOPK_LARGE_MAX): block per query. Binary search on the
 // score for the k-th best value T, then collect every candidate above T and
 // the first (in position order) of those equal to T, then a bitonic sort of
@@ -639,26 +765,41 @@ k_encode_venc(const uint64_t* __restrict__ lt_mat /* 128 x 1024 */,
     venc[(size_t)q * N + j] = add_mod(lg, v_q[(size_t)q * N + j]);
 }
 
-// lt = L^T v_enc (l_transpose_times_vec, crypto.rs:846-848): thread per
-// (query, k), L row-major (n x n1) read coalesced across k. Written into
-// the second half of the query's [g ; lt] vector.
-__global__ void __launch_bounds__(128)
+// lt = L^T v_enc (l_transpose_times_vec, crypto.rs:846-848): block per
+// query; thread (k, part) sums rows part*128 .. part*128+127 of L (row-major
+// n x n1, read coalesced across k) against v_enc staged in LDS, and the
+// eight partial sums are added in LDS. Written into the second half of the
+// query's [g ; lt] vector.
+constexpr int LT_PARTS = 8;
+__global__ void __launch_bounds__(N1 * LT_PARTS)
 k_lt(const uint64_t* __restrict__ l_mat, const uint64_t* __restrict__ venc,
      uint64_t* __restrict__ gl) {
+    __shared__ uint64_t vs[N];
+    __shared__ uint64_t part[LT_PARTS][N1];
     const uint32_t q = blockIdx.x;
-    const uint32_t k = threadIdx.x;
-    const uint64_t* v = venc + (size_t)q * N;
+    const uint32_t k = threadIdx.x % N1, pt = threadIdx.x / N1;
+    for (int j = threadIdx.x; j < N; j += N1 * LT_PARTS) vs[j] = venc[(size_t)q * N + j];
+    __syncthreads();
+    constexpr int ROWS = N / LT_PARTS;  // 128 products: fold after 32 as elsewhere
     Acc128 acc;
     acc.zero();
     uint64_t s = 0;
-    for (int j = 0; j < N; ++j) {
-        acc.mac(v[j], l_mat[(size_t)j * N1 + k]);
-        if ((j & 31) == 31) {
+    for (int jj = 0; jj < ROWS; ++jj) {
+        const int j = pt * ROWS + jj;
+        acc.mac(vs[j], l_mat[(size_t)j * N1 + k]);
+        if ((jj & 31) == 31) {
             s = add_mod(s, acc.reduce());
             acc.zero();
         }
     }
-    gl[(size_t)q * 2 * N1 + N1 + k] = s;
+    part[pt][k] = s;
+    __syncthreads();
+    if (pt == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int p = 0; p < LT_PARTS; ++p) t = add_mod(t, part[p][k]);
+        gl[(size_t)q * 2 * N1 + N1 + k] = t;
+    }
 }
 
 // ============================================================= index build
@@ -985,7 +1126,10 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
                 uint64_t* __restrict__ out) {
     constexpr int STEPS = KE / 8;  // 64 K'-bytes (8 elements) per step
-    __shared__ uint64_t bw[2][PB_QG][16][8];  // [buf][query][column][element]
+    // [buf][query][column][element], each column row padded from 8 to 10
+    // words so that the 16 columns of a ds_read_b128 pass start on distinct
+    // bank groups
+    __shared__ uint64_t bw[2][PB_QG][16][10];
     __shared__ int32_t dsm[PB_WAVES][4][16][17];
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;

@@ -459,3 +459,31 @@ def test_score_batch_decode_planes_and_fallback(ctx, l_sub):
                 assert status[q] == 0 and count[q] == k
                 assert ids[q].tolist() == want[q][0].tolist(), f"mode {mode} query {q}"
                 assert scores[q].tolist() == want[q][1].tolist(), f"mode {mode} query {q}"
+
+
+def test_topk_ties_across_chunks(ctx):
+    """Duplicate rows give equal scores; the top-k keeps the earliest
+    positions (score desc, position asc) across 1024-candidate chunks, for
+    k = 1, 10, 16 (two-phase kernel) and 17, 40 (block kernel)."""
+    import svs
+    from svs import data
+    base = data.make_corpus(7, components=2, seed=5).vectors
+    rng = np.random.default_rng(6)
+    rows = base[rng.integers(0, 7, size=3000)]
+    qs = data.make_queries(data.make_corpus(50, components=2, seed=5), 3, seed=8)
+    cfg = svs.BnTmConfig(key_seed=bytes([42] * 32), verification_enabled=False)
+    scorer = svs.BnTmScorer(ctx)
+    h_seed = bytes([9] * 32)
+    handle, _ = scorer.upload_cluster(cfg, rows, h_seed=h_seed)
+    l = O.generate_subspace(bytes([42] * 32))
+    mq = np.stack([O.quantise_vector(v) for v in rows])
+    cl = O.build_cluster(mq.reshape(-1), rows.shape[0], l, h_seed)
+    seeds = np.full(3 * 32, 3, np.uint8)
+    for k in (1, 10, 16, 17, 40):
+        say("ties k =", k)
+        got = scorer.score_batch(handle, qs, k, q_seeds=seeds)
+        for i, q in enumerate(qs):
+            st, ids, scores = O.bntm_flat_score(cl, l, O.quantise_vector(q), bytes(seeds[:32]),
+                                                False, k)
+            assert [h.id for h in got[i]] == ids.tolist(), f"k {k} query {i}"
+            assert len(set(scores.tolist())) < k or k == 1
