@@ -133,6 +133,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain-copy calibration dispatch (0 = skip)")
     args = ap.parse_args()
@@ -171,6 +172,17 @@ def main():
     probed_rows = np.array([sizes[p].sum() for p in probes])
     log(f"probed rows per query: mean {probed_rows.mean():.0f} p50 {np.median(probed_rows):.0f}")
 
+    # block-tile variant for this batch size: timed on the first warm-up batch
+    tune = None
+    if not args.no_autotune:
+        s0 = slice(0, args.batch)
+        pp0 = np.arange(0, v_q[s0].shape[0] * args.nprobe + 1, args.nprobe, dtype=np.uint32)
+        best, tms = index.autotune(v_q[s0], seeds[s0], pp0, probes[s0].reshape(-1), verify[s0],
+                                   args.k, reps=2)
+        tune = dict(best=best, ms_per_variant=[round(x, 3) for x in tms])
+        log(f"autotune: {tune}")
+    variant = ctx.tile_variant(args.batch)
+
     # batched (served-batched) path
     run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.warmup, 1)
     ctx.profile(True, True)
@@ -199,7 +211,8 @@ def main():
     mulmods = rows * 1024
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_fp_gemm_tiles<1024,0>",
+                    kernel=f"k_fp_gemm_tiles<1024,0,{variant['query_group']},"
+                           f"{variant['ring_depth']},{variant['waves_per_simd']}>",
                     algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
                     avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
                     mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0)
@@ -245,6 +258,8 @@ def main():
         "bit_exact_batched_vs_plain": bool(exact),
         "stages_ms_per_batch": {s: round(prof[s][0] / max(prof[s][1], 1), 4)
                                 for s in ("encode", "product", "verify", "decode", "topk")},
+        "tile_variant": variant,
+        "autotune": tune,
         "roofline": roofline,
         "copy_probe": copy,
         "cpu_baseline": cpu,

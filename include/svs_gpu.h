@@ -208,6 +208,26 @@ typedef struct {
 
 int svs_score_batch(svs_index* idx, const svs_query_batch* batch_host, int mode);
 
+/* ---- block-tile variants and the autotuner ---------------------------------
+ * No reference counterpart (the reference has one CUDA kernel shape): the
+ * MFMA batch path (SVS_MODE_BATCHED, SVS_KERNEL_MFMA) runs on 256-row block
+ * tiles whose query group, prefetch depth and waves per SIMD come from a
+ * small table of variants, chosen per batch-size class (n_queries <= 16,
+ * <= 64, <= 256, larger). Every variant returns identical results. */
+#define SVS_TILE_VARIANTS 5
+/* n_queries == 0 sets every class. */
+int svs_ctx_set_tile_variant(svs_ctx* ctx, uint32_t n_queries, int variant);
+int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* variant,
+                             uint32_t* query_group, uint32_t* ring_depth,
+                             uint32_t* waves_per_simd);
+/* Runs the batch once per variant (one warm-up plus `reps` timed runs,
+ * SVS_MODE_BATCHED), fails with SVS_ERR_STATE if any variant's results
+ * differ from variant 0's, and keeps the fastest for this batch size.
+ * ms_per_variant (SVS_TILE_VARIANTS entries, may be NULL) receives the mean
+ * wall time per run; the batch's outputs hold the last variant's results. */
+int svs_autotune(svs_index* idx, const svs_query_batch* batch_host, uint32_t reps, int* best,
+                 double* ms_per_variant);
+
 /* ---- EMVP score GEMV (same kernel family) ---------------------------------
  * Resident EMVP cluster M_hat (m x 1292 canonical u64, SEC128 n = 1292,
  * s = 76, b = 17; scorer-emvp/src/params.rs). Replaces EMVP's

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -677,23 +678,23 @@ static uint32_t row_grid(svs_ctx* ctx, uint64_t waves_needed) {
 // ------------------------------------------------------------ stage timing
 // HIP events recorded on the context stream around each pipeline stage when
 // profiling is on; elapsed times are accumulated after the batch syncs.
-static int prof_mark(svs_ctx* ctx, int stage, bool begin) {
-    if (!ctx->prof.on) return SVS_OK;
-    HIP_TRY(hipEventRecord(ctx->prof.ev[2 * stage + (begin ? 0 : 1)], ctx->stream));
-    if (!begin) ctx->prof.pending |= 1u << stage;
+static int prof_mark(svs_slot& sl, int stage, bool begin) {
+    if (!sl.prof.on) return SVS_OK;
+    HIP_TRY(hipEventRecord(sl.prof.ev[2 * stage + (begin ? 0 : 1)], sl.stream));
+    if (!begin) sl.prof.pending |= 1u << stage;
     return SVS_OK;
 }
 
-static int prof_collect(svs_ctx* ctx) {
-    if (!ctx->prof.on) return SVS_OK;
+static int prof_collect(svs_slot& sl) {
+    if (!sl.prof.on) return SVS_OK;
     for (int s = 0; s < SVS_PROF_STAGES; ++s) {
-        if (!(ctx->prof.pending & (1u << s))) continue;
+        if (!(sl.prof.pending & (1u << s))) continue;
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx->prof.ev[2 * s], ctx->prof.ev[2 * s + 1]));
-        ctx->prof.ms[s] += ms;
-        ctx->prof.calls[s] += 1;
+        HIP_TRY(hipEventElapsedTime(&ms, sl.prof.ev[2 * s], sl.prof.ev[2 * s + 1]));
+        sl.prof.ms[s] += ms;
+        sl.prof.calls[s] += 1;
     }
-    ctx->prof.pending = 0;
+    sl.prof.pending = 0;
     return SVS_OK;
 }
 
@@ -727,26 +728,117 @@ extern "C" int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile*
 // MFMA limb-split tiles (many queries per row pass).
 enum { KERNEL_VALU = 0, KERNEL_MFMA = 1, KERNEL_MFMA_WAVE = 2 };
 
-// Tile shape a product kernel's plan is built with.
-static void tile_shape(int kernel, uint32_t& rows, uint32_t& qg) {
-    if (kernel == KERNEL_MFMA_WAVE) rows = MT_ROWS, qg = MT_QG;
-    else rows = PB_ROWS, qg = PB_QG;
+// Block-tile variants of the MFMA path (product, dense decode and decode
+// tiles share the plan's tiles): query group, prefetch ring depth, waves per
+// SIMD. All give identical results; svs_autotune times them.
+struct TileVariant {
+    uint32_t qg, pf, occ;
+};
+static constexpr TileVariant kTileVariants[SVS_TILE_VARIANTS] = {
+    {8, 8, 1}, {8, 2, 2}, {16, 4, 1}, {4, 4, 2}, {16, 2, 1}};
+
+// batch-size class of a tile-variant choice
+static int nq_class(uint32_t nq) { return nq <= 16 ? 0 : nq <= 64 ? 1 : nq <= 256 ? 2 : 3; }
+static int ctx_variant_for(const svs_ctx* ctx, uint32_t nq) { return ctx->tile_variant[nq_class(nq)]; }
+
+extern "C" int svs_ctx_set_tile_variant(svs_ctx* ctx, uint32_t n_queries, int variant) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    if (variant < 0 || variant >= SVS_TILE_VARIANTS)
+        return fail(SVS_ERR_ARG, "tile variant %d out of range", variant);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (n_queries == 0)
+        for (int& v : ctx->tile_variant) v = variant;
+    else
+        ctx->tile_variant[nq_class(n_queries)] = variant;
+    return SVS_OK;
 }
 
-static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool product_only,
-                    int kernel, bool encode) {
+extern "C" int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* variant,
+                                        uint32_t* query_group, uint32_t* ring_depth,
+                                        uint32_t* waves_per_simd) {
+    if (!ctx || !variant) return fail(SVS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int v = ctx_variant_for(ctx, std::max<uint32_t>(n_queries, 1));
+    *variant = v;
+    if (query_group) *query_group = kTileVariants[v].qg;
+    if (ring_depth) *ring_depth = kTileVariants[v].pf;
+    if (waves_per_simd) *waves_per_simd = kTileVariants[v].occ;
+    return SVS_OK;
+}
+
+struct TileArgs {
+    const ListDev* lists;
+    const MTile* tiles;
+    uint32_t nt;
+    const uint32_t* gitems;
+    const Item* items;
+    const uint32_t* cand_ptr;
+};
+
+template <int KE, int SRC, int QG, int PF, int OCC>
+static void launch_gemm_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
+                          const uint64_t* vecs, const uint64_t* qc, uint64_t* out) {
+    hipLaunchKernelGGL((k_fp_gemm_tiles<KE, SRC, QG, PF, OCC>), dim3(blocks), dim3(256), 0, st,
+                       a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
+}
+
+// The product (SRC 0, KE = n) or dense decode (SRC 1, KE = 2 n1) tiles.
+template <int KE, int SRC>
+static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& a,
+                        const uint64_t* vecs, const uint64_t* qc, uint64_t* out) {
+    switch (v) {
+        case 1: launch_gemm_v<KE, SRC, 8, 2, 2>(st, blocks, a, vecs, qc, out); break;
+        case 2: launch_gemm_v<KE, SRC, 16, 4, 1>(st, blocks, a, vecs, qc, out); break;
+        case 4: launch_gemm_v<KE, SRC, 16, 2, 1>(st, blocks, a, vecs, qc, out); break;
+        case 3: launch_gemm_v<KE, SRC, 4, 4, 2>(st, blocks, a, vecs, qc, out); break;
+        default: launch_gemm_v<KE, SRC, 8, 8, 1>(st, blocks, a, vecs, qc, out); break;
+    }
+}
+
+static void launch_decode_tiles(int v, hipStream_t st, uint32_t blocks, const TileArgs& a,
+                                const int8_t* t_dense, const int8_t* vplanes,
+                                const uint64_t* mt_q, const uint64_t* r, const uint64_t* dense,
+                                int64_t* scores) {
+    switch (kTileVariants[v].qg) {
+        case 16:
+            hipLaunchKernelGGL(k_decode_tiles<16>, dim3(blocks), dim3(256), 0, st, a.lists,
+                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
+                               mt_q, r, dense, scores);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_decode_tiles<4>, dim3(blocks), dim3(256), 0, st, a.lists,
+                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
+                               mt_q, r, dense, scores);
+            break;
+        default:
+            hipLaunchKernelGGL(k_decode_tiles<8>, dim3(blocks), dim3(256), 0, st, a.lists,
+                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
+                               mt_q, r, dense, scores);
+            break;
+    }
+}
+
+// Tile shape a product kernel's plan is built with.
+static void tile_shape(int kernel, int variant, uint32_t& rows, uint32_t& qg) {
+    if (kernel == KERNEL_MFMA_WAVE) rows = MT_ROWS, qg = MT_QG;
+    else rows = PB_ROWS, qg = kTileVariants[variant].qg;
+}
+
+static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
+                    bool product_only,
+                    int kernel, bool encode, int variant) {
     svs_ctx* ctx = idx->ctx;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = sl.stream;
     const uint32_t nq = pl.n_queries;
     // plan arrays
-    SVS_TRY(ctx->items.ensure(ctx, sizeof(Item) * std::max<uint32_t>(pl.n_items, 1)));
-    SVS_TRY(ctx->prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + 2 * (nq + 1))));
-    SVS_TRY(ctx->flags.ensure(ctx, std::max<uint32_t>(pl.n_items, 1) + 64));
-    uint32_t* d_chunk_ptr = (uint32_t*)ctx->prefixes.ptr;
+    SVS_TRY(sl.items.ensure(ctx, sizeof(Item) * std::max<uint32_t>(pl.n_items, 1)));
+    SVS_TRY(sl.prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + 2 * (nq + 1))));
+    SVS_TRY(sl.flags.ensure(ctx, std::max<uint32_t>(pl.n_items, 1) + 64));
+    uint32_t* d_chunk_ptr = (uint32_t*)sl.prefixes.ptr;
     uint32_t* d_cand_ptr = d_chunk_ptr + pl.n_items + 1;
     uint32_t* d_item_ptr = d_cand_ptr + pl.n_items + 1;
     uint32_t* d_qchunk_ptr = d_item_ptr + nq + 1;
-    HIP_TRY(hipMemcpyAsync(ctx->items.ptr, pl.items.data(), sizeof(Item) * pl.n_items,
+    HIP_TRY(hipMemcpyAsync(sl.items.ptr, pl.items.data(), sizeof(Item) * pl.n_items,
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_chunk_ptr, pl.chunk_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
                            hipMemcpyHostToDevice, st));
@@ -757,112 +849,111 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
     HIP_TRY(hipMemcpyAsync(d_qchunk_ptr, pl.qchunk_ptr.data(), sizeof(uint32_t) * (nq + 1),
                            hipMemcpyHostToDevice, st));
     if (pl.n_items)
-        HIP_TRY(hipMemcpyAsync(ctx->flags.ptr, pl.item_verify.data(), pl.n_items,
+        HIP_TRY(hipMemcpyAsync(sl.flags.ptr, pl.item_verify.data(), pl.n_items,
                                hipMemcpyHostToDevice, st));
-    const Item* d_items = (const Item*)ctx->items.ptr;
+    const Item* d_items = (const Item*)sl.items.ptr;
     const ListDev* d_lists = idx->lists_dev;
     // per-query encode
-    uint64_t* d_venc = (uint64_t*)ctx->q_venc.ptr;
-    uint64_t* d_gl = (uint64_t*)ctx->q_gl.ptr;
-    int8_t* d_tdense = (int8_t*)ctx->q_tdense.ptr;
-    uint16_t* d_tlist = (uint16_t*)ctx->q_tlist.ptr;
-    uint32_t* d_tcnt = (uint32_t*)ctx->q_tcnt.ptr;
-    uint32_t* d_vseed = (uint32_t*)ctx->q_vseed.ptr;
+    uint64_t* d_venc = (uint64_t*)sl.q_venc.ptr;
+    uint64_t* d_gl = (uint64_t*)sl.q_gl.ptr;
+    int8_t* d_tdense = (int8_t*)sl.q_tdense.ptr;
+    uint16_t* d_tlist = (uint16_t*)sl.q_tlist.ptr;
+    uint32_t* d_tcnt = (uint32_t*)sl.q_tcnt.ptr;
+    uint32_t* d_vseed = (uint32_t*)sl.q_vseed.ptr;
     // verdicts, per-query status and the RNG flag (set by encode or verify)
-    SVS_TRY(ctx->verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
-    int32_t* d_item_verdict = (int32_t*)ctx->verdict.ptr;
+    SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
+    int32_t* d_item_verdict = (int32_t*)sl.verdict.ptr;
     int32_t* d_qstatus = d_item_verdict + std::max<uint32_t>(pl.n_items, 1);
     uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_rng_flag, 1u, 0u);
-    SVS_TRY(prof_mark(ctx, 0, true));
+    SVS_TRY(prof_mark(sl, 0, true));
     if (encode) {
         hipLaunchKernelGGL(k_encode_keystream, dim3(nq), dim3(256), 0, st,
-                           (const uint8_t*)ctx->q_seeds.ptr, d_gl, d_tdense, d_tlist, d_tcnt,
+                           (const uint8_t*)sl.q_seeds.ptr, d_gl, d_tdense, d_tlist, d_tcnt,
                            d_vseed, d_rng_flag);
         hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
                            (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
-                           (const int8_t*)d_tdense, (const uint64_t*)ctx->q_vq.ptr, d_venc);
+                           (const int8_t*)d_tdense, (const uint64_t*)sl.q_vq.ptr, d_venc);
         hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1 * LT_PARTS), 0, st, (const uint64_t*)idx->l_dev,
                            (const uint64_t*)d_venc, d_gl);
         ctx->launches += 3;
         HIP_TRY(hipGetLastError());
     }
-    SVS_TRY(prof_mark(ctx, 0, false));
+    SVS_TRY(prof_mark(sl, 0, false));
     // server product
-    SVS_TRY(prof_mark(ctx, 1, true));
-    SVS_TRY(ctx->r.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+    SVS_TRY(prof_mark(sl, 1, true));
+    SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
     if (pl.n_chunks && kernel == KERNEL_VALU) {
         hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
                            st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
                            (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (uint64_t*)ctx->r.ptr);
+                           (uint64_t*)sl.r.ptr);
         ctx->launches++;
         HIP_TRY(hipGetLastError());
     } else if (pl.n_chunks) {
         const uint32_t nt = (uint32_t)pl.tiles.size();
-        SVS_TRY(ctx->mtiles.ensure(ctx, sizeof(MTile) * nt + sizeof(uint32_t) * pl.gitems.size()));
-        SVS_TRY(ctx->q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
-        MTile* d_tiles = (MTile*)ctx->mtiles.ptr;
+        SVS_TRY(sl.mtiles.ensure(ctx, sizeof(MTile) * nt + sizeof(uint32_t) * pl.gitems.size()));
+        SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+        MTile* d_tiles = (MTile*)sl.mtiles.ptr;
         uint32_t* d_gitems = (uint32_t*)(d_tiles + nt);
         HIP_TRY(hipMemcpyAsync(d_tiles, pl.tiles.data(), sizeof(MTile) * nt,
                                hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size(),
                                hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_query_corr<N>, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
-                           (uint64_t*)ctx->q_corr.ptr);
+                           (uint64_t*)sl.q_corr.ptr);
         if (kernel == KERNEL_MFMA_WAVE) {
             const uint32_t blocks = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>((nt + 3) / 4, (uint64_t)ctx->n_cu * 8));
             hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists,
                                (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
                                (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                               (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+                               (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         } else {
             const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
-            hipLaunchKernelGGL((k_fp_gemm_tiles<N, 0>), dim3(blocks), dim3(256), 0, st, d_lists,
-                               (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
-                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                               (const uint64_t*)ctx->q_corr.ptr, (uint64_t*)ctx->r.ptr);
+                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
+            const TileArgs ta{d_lists, d_tiles, nt, d_gitems, d_items, d_cand_ptr};
+            launch_gemm<N, 0>(variant, st, blocks, ta, (const uint64_t*)d_venc,
+                              (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         }
         ctx->launches += 2;
         HIP_TRY(hipGetLastError());
     }
-    SVS_TRY(prof_mark(ctx, 1, false));
-    ctx->prof.gemv_bytes += pl.distinct_bytes;
-    ctx->prof.gemv_rows += pl.n_cands;
+    SVS_TRY(prof_mark(sl, 1, false));
+    sl.prof.gemv_bytes += pl.distinct_bytes;
+    sl.prof.gemv_rows += pl.n_cands;
     if (ctx->fault_list >= 0 && pl.n_items) {
         hipLaunchKernelGGL(k_inject_fault, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
                            d_items, pl.n_items, (const uint32_t*)d_cand_ptr,
                            (uint32_t)ctx->fault_list, ctx->fault_row, ctx->fault_delta,
-                           (uint64_t*)ctx->r.ptr);
+                           (uint64_t*)sl.r.ptr);
         HIP_TRY(hipGetLastError());
     }
     if (product_only) return SVS_OK;
     // verification
-    SVS_TRY(prof_mark(ctx, 2, true));
+    SVS_TRY(prof_mark(sl, 2, true));
     if (pl.any_verify) {
-        SVS_TRY(ctx->partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
+        SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
         hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS),
                            0, st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
                            (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (const uint64_t*)ctx->r.ptr, (const uint32_t*)d_vseed,
-                           (const uint8_t*)ctx->flags.ptr, (uint64_t*)ctx->partials.ptr,
+                           (const uint64_t*)sl.r.ptr, (const uint32_t*)d_vseed,
+                           (const uint8_t*)sl.flags.ptr, (uint64_t*)sl.partials.ptr,
                            d_rng_flag);
         hipLaunchKernelGGL(k_verify_finalize, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
                            d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
-                           (const uint8_t*)ctx->flags.ptr, (const uint64_t*)ctx->partials.ptr,
+                           (const uint8_t*)sl.flags.ptr, (const uint64_t*)sl.partials.ptr,
                            d_item_verdict);
         ctx->launches += 2;
         HIP_TRY(hipGetLastError());
     }
-    SVS_TRY(prof_mark(ctx, 2, false));
+    SVS_TRY(prof_mark(sl, 2, false));
     hipLaunchKernelGGL(k_query_status, dim3((nq + 255) / 256), dim3(256), 0, st, d_items,
                        (const uint32_t*)d_item_ptr, nq, (const int32_t*)d_item_verdict,
                        d_qstatus, pl.any_verify ? 1 : 0);
     // decode
-    SVS_TRY(ctx->scores.ensure(ctx, sizeof(int64_t) * std::max<uint32_t>(pl.n_cands, 1)));
-    SVS_TRY(prof_mark(ctx, 3, true));
+    SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+    SVS_TRY(prof_mark(sl, 3, true));
     if (pl.n_cands) {
         // The dense decode terms AL G + H L^T v_enc: the same MFMA tiles over
         // the row-major [AL | H] when the batch runs on block tiles, else
@@ -870,106 +961,105 @@ static int run_plan(svs_index* idx, const BatchPlan& pl, uint32_t k, bool produc
         const uint64_t* dense = nullptr;
         const bool tiles = kernel == KERNEL_MFMA && pl.n_chunks;
         const uint32_t nt = (uint32_t)pl.tiles.size();
-        const MTile* d_tiles = (const MTile*)ctx->mtiles.ptr;
+        const MTile* d_tiles = (const MTile*)sl.mtiles.ptr;
         const uint32_t* d_gitems = (const uint32_t*)(d_tiles + nt);
-        const uint32_t tblocks =
-            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
+        const uint32_t tblocks = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
+        const TileArgs ta{d_lists, d_tiles, nt, d_gitems, d_items, d_cand_ptr};
         if (tiles) {
-            SVS_TRY(ctx->dense.ensure(ctx, sizeof(uint64_t) * pl.n_cands));
-            uint64_t* qc_gl = (uint64_t*)ctx->q_corr.ptr + nq;
+            SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * pl.n_cands));
+            uint64_t* qc_gl = (uint64_t*)sl.q_corr.ptr + nq;
             hipLaunchKernelGGL(k_query_corr<2 * N1>, dim3(nq), dim3(WAVE), 0, st,
                                (const uint64_t*)d_gl, qc_gl);
-            hipLaunchKernelGGL((k_fp_gemm_tiles<2 * N1, 1>), dim3(tblocks), dim3(256), 0, st,
-                               d_lists, d_tiles, nt, d_gitems, d_items,
-                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_gl,
-                               (const uint64_t*)qc_gl, (uint64_t*)ctx->dense.ptr);
+            launch_gemm<2 * N1, 1>(variant, st, tblocks, ta, (const uint64_t*)d_gl,
+                                   (const uint64_t*)qc_gl, (uint64_t*)sl.dense.ptr);
             ctx->launches += 2;
-            dense = (const uint64_t*)ctx->dense.ptr;
+            dense = (const uint64_t*)sl.dense.ptr;
         }
         if (tiles && pl.planes_ok) {
             // M T and S v_enc on the same tiles; the decode ends there.
-            SVS_TRY(ctx->q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
-            uint64_t* mt_q = (uint64_t*)ctx->q_corr.ptr + 2 * nq;
+            SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+            uint64_t* mt_q = (uint64_t*)sl.q_corr.ptr + 2 * nq;
             hipLaunchKernelGGL(k_decode_prep, dim3(nq), dim3(256), 0, st, (const uint64_t*)d_venc,
-                               (const int8_t*)d_tdense, (int8_t*)ctx->q_vplanes.ptr, mt_q);
-            hipLaunchKernelGGL(k_decode_tiles, dim3(tblocks), dim3(256), 0, st, d_lists, d_tiles,
-                               nt, d_gitems, d_items, (const uint32_t*)d_cand_ptr,
-                               (const int8_t*)d_tdense, (const int8_t*)ctx->q_vplanes.ptr,
-                               (const uint64_t*)mt_q, (const uint64_t*)ctx->r.ptr, dense,
-                               (int64_t*)ctx->scores.ptr);
+                               (const int8_t*)d_tdense, (int8_t*)sl.q_vplanes.ptr, mt_q);
+            launch_decode_tiles(variant, st, tblocks, ta, (const int8_t*)d_tdense,
+                                (const int8_t*)sl.q_vplanes.ptr, (const uint64_t*)mt_q,
+                                (const uint64_t*)sl.r.ptr, dense, (int64_t*)sl.scores.ptr);
             ctx->launches += 2;
         } else {
             const uint32_t blocks = (uint32_t)std::min<uint64_t>((pl.n_cands + 255) / 256,
                                                                  (uint64_t)ctx->n_cu * 32);
             hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
                                pl.n_items, (const uint32_t*)d_cand_ptr,
-                               (const uint64_t*)ctx->r.ptr, (const uint64_t*)d_gl,
+                               (const uint64_t*)sl.r.ptr, (const uint64_t*)d_gl,
                                (const uint64_t*)d_venc, (const uint16_t*)d_tlist,
-                               (const uint32_t*)d_tcnt, dense, (int64_t*)ctx->scores.ptr,
+                               (const uint32_t*)d_tcnt, dense, (int64_t*)sl.scores.ptr,
                                (uint64_t*)nullptr);
             ctx->launches++;
         }
         HIP_TRY(hipGetLastError());
     }
-    SVS_TRY(prof_mark(ctx, 3, false));
+    SVS_TRY(prof_mark(sl, 3, false));
     // top-k
-    SVS_TRY(prof_mark(ctx, 4, true));
-    SVS_TRY(ctx->out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
+    SVS_TRY(prof_mark(sl, 4, true));
+    SVS_TRY(sl.out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
                                      sizeof(uint32_t) * nq + 64));
-    uint32_t* d_oids = (uint32_t*)ctx->out.ptr;
+    uint32_t* d_oids = (uint32_t*)sl.out.ptr;
     int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
     uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
     if (k <= (uint32_t)TOPK_LANE) {
-        SVS_TRY(ctx->topk_part.ensure(ctx, sizeof(Cand) * (size_t)std::max<uint32_t>(pl.n_tchunks, 1) * k));
+        SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * (size_t)std::max<uint32_t>(pl.n_tchunks, 1) * k));
         if (pl.n_tchunks)
             hipLaunchKernelGGL(k_topk_chunks, dim3((pl.n_tchunks + 3) / 4), dim3(256), 0, st,
                                (const uint32_t*)d_qchunk_ptr, nq, pl.n_tchunks,
                                (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr,
-                               (const int64_t*)ctx->scores.ptr, k, (Cand*)ctx->topk_part.ptr);
+                               (const int64_t*)sl.scores.ptr, k, (Cand*)sl.topk_part.ptr);
         hipLaunchKernelGGL((k_topk_final<256>), dim3(nq), dim3(256), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const uint32_t*)d_qchunk_ptr, (const Cand*)ctx->topk_part.ptr,
+                           (const uint32_t*)d_qchunk_ptr, (const Cand*)sl.topk_part.ptr,
                            (const int32_t*)d_qstatus, k, d_oids, d_oscores, d_ocount);
         ctx->launches++;
     } else if (k <= 128)
         hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
+                           (const int64_t*)sl.scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
                            d_oscores, d_ocount);
     else
         hipLaunchKernelGGL(k_topk_large, dim3(nq), dim3(TOPK_LARGE_THREADS), 0, st, d_items,
                            (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const int64_t*)ctx->scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
+                           (const int64_t*)sl.scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
                            d_oscores, d_ocount);
     ctx->launches += 2;
     HIP_TRY(hipGetLastError());
-    SVS_TRY(prof_mark(ctx, 4, false));
+    SVS_TRY(prof_mark(sl, 4, false));
     return SVS_OK;
 }
 
-static int stage_queries(svs_ctx* ctx, uint32_t nq, const uint64_t* v_q, const uint8_t* seeds) {
-    SVS_TRY(ctx->q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
-    SVS_TRY(ctx->q_seeds.ensure(ctx, (size_t)nq * 32));
-    SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
-    SVS_TRY(ctx->q_gl.ensure(ctx, sizeof(uint64_t) * (size_t)nq * 2 * N1));
-    SVS_TRY(ctx->q_tdense.ensure(ctx, (size_t)nq * N));
-    SVS_TRY(ctx->q_tlist.ensure(ctx, sizeof(uint16_t) * (size_t)nq * N));
-    SVS_TRY(ctx->q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
-    SVS_TRY(ctx->q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
-    HIP_TRY(hipMemcpyAsync(ctx->q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
-                           hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyHostToDevice,
-                           ctx->stream));
+static int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
+                         const uint8_t* seeds) {
+    SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_seeds.ensure(ctx, (size_t)nq * 32));
+    SVS_TRY(sl.q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_gl.ensure(ctx, sizeof(uint64_t) * (size_t)nq * 2 * N1));
+    SVS_TRY(sl.q_tdense.ensure(ctx, (size_t)nq * N));
+    SVS_TRY(sl.q_tlist.ensure(ctx, sizeof(uint16_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
+    SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
+    HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
+                           hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyHostToDevice,
+                           sl.stream));
     return SVS_OK;
 }
 
-static int collect(svs_ctx* ctx, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,
+static int collect(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items,
+                   uint32_t* ids,
                    int64_t* scores, uint32_t* count, int32_t* status) {
-    hipStream_t st = ctx->stream;
-    uint32_t* d_oids = (uint32_t*)ctx->out.ptr;
+    hipStream_t st = sl.stream;
+    uint32_t* d_oids = (uint32_t*)sl.out.ptr;
     int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
     uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
-    int32_t* d_qstatus = (int32_t*)ctx->verdict.ptr + std::max<uint32_t>(n_items, 1);
+    int32_t* d_qstatus = (int32_t*)sl.verdict.ptr + std::max<uint32_t>(n_items, 1);
     uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
     uint32_t rng_flag = 0;
     HIP_TRY(hipMemcpyAsync(ids, d_oids, sizeof(uint32_t) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
@@ -978,7 +1068,7 @@ static int collect(svs_ctx* ctx, uint32_t nq, uint32_t k, uint32_t n_items, uint
     HIP_TRY(hipMemcpyAsync(status, d_qstatus, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(&rng_flag, d_rng_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    SVS_TRY(prof_collect(ctx));
+    SVS_TRY(prof_collect(sl));
     if (rng_flag & 1u)
         return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p (probability 2^-61 per draw)");
     if (rng_flag & 2u)
@@ -1007,21 +1097,81 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
         std::vector<uint32_t> pptr(nq + 1);
         for (uint32_t q = 0; q <= nq; ++q) pptr[q] = b->probe_ptr[q0 + q] - b->probe_ptr[q0];
         const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
+        const int variant = ctx->tile_variant[nq_class(nq)];
         uint32_t trows, tqg;
-        tile_shape(kernel, trows, tqg);
+        tile_shape(kernel, variant, trows, tqg);
         BatchPlan pl;
         SVS_TRY(make_plan(idx, nq, pptr.data(), b->probe_lists + b->probe_ptr[q0],
                           b->verify ? b->verify + q0 : nullptr, pl, trows, tqg));
         for (const Item& it : pl.items)
             if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
                 return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
-        SVS_TRY(stage_queries(ctx, nq, b->v_q + (size_t)q0 * N, b->q_seeds + (size_t)q0 * 32));
-        SVS_TRY(run_plan(idx, pl, b->k, false, kernel, true));
-        SVS_TRY(collect(ctx, nq, b->k, pl.n_items, b->out_ids + (size_t)q0 * b->k,
+        SVS_TRY(stage_queries(ctx, *ctx, nq, b->v_q + (size_t)q0 * N,
+                              b->q_seeds + (size_t)q0 * 32));
+        SVS_TRY(run_plan(idx, *ctx, pl, b->k, false, kernel, true, variant));
+        SVS_TRY(collect(ctx, *ctx, nq, b->k, pl.n_items, b->out_ids + (size_t)q0 * b->k,
                         b->out_scores + (size_t)q0 * b->k, b->out_count + q0,
                         b->out_status + q0));
     }
     return SVS_OK;
+}
+
+// Times every block-tile variant on a real batch (MODE_BATCHED, `reps` timed
+// runs after one warm-up each), checks that all variants return identical
+// results, and keeps the fastest for the batch's size class.
+extern "C" int svs_autotune(svs_index* idx, const svs_query_batch* b, uint32_t reps, int* best,
+                            double* ms_per_variant) {
+    if (!idx || !b || reps == 0) return fail(SVS_ERR_ARG, "null argument or reps == 0");
+    if (b->n_queries == 0) return fail(SVS_ERR_ARG, "empty batch");
+    svs_ctx* ctx = idx->ctx;
+    const size_t nk = (size_t)b->n_queries * b->k;
+    std::vector<uint32_t> ids0(nk), cnt0(b->n_queries);
+    std::vector<int64_t> sc0(nk);
+    std::vector<int32_t> st0(b->n_queries);
+    int prev;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        prev = ctx_variant_for(ctx, b->n_queries);
+    }
+    int win = -1;
+    double win_ms = 0;
+    for (int v = 0; v < SVS_TILE_VARIANTS; ++v) {
+        SVS_TRY(svs_ctx_set_tile_variant(ctx, b->n_queries, v));
+        double tot = 0;
+        for (uint32_t r = 0; r <= reps; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = svs_score_batch(idx, b, SVS_MODE_BATCHED);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (rc != SVS_OK) {
+                svs_ctx_set_tile_variant(ctx, b->n_queries, prev);
+                return rc;
+            }
+            if (r) tot += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+        const double ms = tot / reps;
+        if (ms_per_variant) ms_per_variant[v] = ms;
+        if (v == 0) {
+            std::copy(b->out_ids, b->out_ids + nk, ids0.begin());
+            std::copy(b->out_scores, b->out_scores + nk, sc0.begin());
+            std::copy(b->out_count, b->out_count + b->n_queries, cnt0.begin());
+            std::copy(b->out_status, b->out_status + b->n_queries, st0.begin());
+        } else {
+            for (uint32_t q = 0; q < b->n_queries; ++q) {
+                bool same = b->out_count[q] == cnt0[q] && b->out_status[q] == st0[q];
+                for (uint32_t i = 0; same && i < b->out_count[q]; ++i)
+                    same = b->out_ids[(size_t)q * b->k + i] == ids0[(size_t)q * b->k + i] &&
+                           b->out_scores[(size_t)q * b->k + i] == sc0[(size_t)q * b->k + i];
+                if (!same) {
+                    svs_ctx_set_tile_variant(ctx, b->n_queries, prev);
+                    return fail(SVS_ERR_STATE, "tile variant %d disagrees with variant 0 on query %u",
+                                v, q);
+                }
+            }
+        }
+        if (win < 0 || ms < win_ms) win = v, win_ms = ms;
+    }
+    if (best) *best = win;
+    return svs_ctx_set_tile_variant(ctx, b->n_queries, win);
 }
 
 // ============================================================ per-list drop-ins
@@ -1262,7 +1412,8 @@ extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
     const int kern = kernel == SVS_KERNEL_MFMA ? KERNEL_MFMA
                      : kernel == SVS_KERNEL_MFMA_WAVE ? KERNEL_MFMA_WAVE : KERNEL_VALU;
     uint32_t trows, tqg;
-    tile_shape(kern, trows, tqg);
+    const int variant = ctx_variant_for(idx->ctx, n_queries);
+    tile_shape(kern, variant, trows, tqg);
     if (n_queries == 0) return SVS_OK;
     SVS_TRY(check_canonical(v_enc, (size_t)n_queries * N, "v_enc"));
     svs_ctx* ctx = idx->ctx;
@@ -1273,12 +1424,12 @@ extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
     SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)n_queries * N));
     HIP_TRY(hipMemcpyAsync(ctx->q_venc.ptr, v_enc, sizeof(uint64_t) * (size_t)n_queries * N,
                            hipMemcpyHostToDevice, ctx->stream));
-    SVS_TRY(run_plan(idx, pl, 1, true, kern, false));
+    SVS_TRY(run_plan(idx, *ctx, pl, 1, true, kern, false, variant));
     if (pl.n_cands)
         HIP_TRY(hipMemcpyAsync(r, ctx->r.ptr, sizeof(uint64_t) * pl.n_cands,
                                hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    SVS_TRY(prof_collect(ctx));
+    SVS_TRY(prof_collect(*ctx));
     return SVS_OK;
 }
 

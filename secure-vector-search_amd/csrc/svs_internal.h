@@ -74,19 +74,32 @@ struct svs_prof {
     uint32_t pending = 0;
 };
 
-struct svs_ctx {
+// Per-batch device scratch and the stream a batch runs on. The context is
+// itself the default slot (used by every synchronous entry point); a server
+// owns further slots so that batches overlap on the device.
+struct svs_slot {
+    hipStream_t stream = nullptr;
+    svs::DevBuf items, prefixes, flags;
+    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
+    svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part;
+    svs_prof prof;
+    std::vector<svs::DevBuf*> slot_scratch() {
+        return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
+                &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &r, &scores, &partials, &verdict,
+                &out, &mtiles, &q_corr, &dense, &topk_part};
+    }
+};
+
+struct svs_ctx : svs_slot {
     int device = 0;
     int n_cu = 256;
-    hipStream_t stream = nullptr;
     uint64_t budget_bytes = 0, hbm_total = 0;
     uint64_t resident_bytes = 0, scratch_bytes = 0, peak_bytes = 0;
     uint64_t uploads = 0, launches = 0;
     std::mutex mu;
-    // scratch
+    // scratch of the per-list entry points and index builds
     svs::DevBuf tmp_a, tmp_b, tmp_c;
-    svs::DevBuf items, prefixes, flags, items1, prefixes1;
-    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
-    svs::DevBuf r, scores, partials, verdict, out, io_a, io_b, mtiles, q_corr, dense, topk_part;
+    svs::DevBuf items1, prefixes1, io_a, io_b;
     // cached single-item plan of svs_compute_products_dev
     int64_t cached_plan_list = -1;
     const void* cached_plan_index = nullptr;
@@ -98,12 +111,15 @@ struct svs_ctx {
     bool destroy_pending = false;
     uint32_t fault_row = 0;
     uint64_t fault_delta = 0;
-    svs_prof prof;
+    // chosen block-tile variant of the MFMA path per batch-size class
+    // (n <= 16, <= 64, <= 256, larger)
+    int tile_variant[4] = {0, 0, 0, 0};
     void note_peak();
     std::vector<svs::DevBuf*> all_scratch() {
-        return {&tmp_a, &tmp_b, &tmp_c, &items, &prefixes, &flags, &items1, &prefixes1,
-                &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense, &q_tlist, &q_tcnt, &q_vseed, &q_vplanes,
-                &r, &scores, &partials, &verdict, &out, &io_a, &io_b, &mtiles, &q_corr, &dense, &topk_part};
+        std::vector<svs::DevBuf*> v = slot_scratch();
+        for (svs::DevBuf* b : {&tmp_a, &tmp_b, &tmp_c, &items1, &prefixes1, &io_a, &io_b})
+            v.push_back(b);
+        return v;
     }
 };
 

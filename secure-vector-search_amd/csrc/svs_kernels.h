@@ -1112,30 +1112,37 @@ __device__ __forceinline__ uint32_t xcd_worker() {
 // ---------------------------------------------------------------------------
 // Block-cooperative F_p GEMM tiles on int8 MFMA (same identity (*) as
 // k_gemv_mfma). A work tile is PB_ROWS rows of one list's row-major KE-wide
-// matrix (SRC 0: M_enc, KE = 1024; SRC 1: [AL | H], KE = 256) times up to
-// PB_QG query vectors. The B operand windows of a K'-step (query x output
-// column x element) are built once per block into double-buffered LDS and
-// read by all four waves; each wave owns PB_RT 16-row tiles. The next step's
-// A rows and query words are loaded before the current step's MFMAs, so HBM
-// latency overlaps the matrix-core work.
-template <int KE, int SRC>
-__global__ void __launch_bounds__(256)
+// matrix (SRC 0: M_enc, KE = 1024; SRC 1: [AL | H], KE = 256) times up to QG
+// query vectors. The B operand windows of a K'-step (query x output column x
+// element) are built once per block into double-buffered LDS and read by all
+// four waves; each wave owns PB_RT 16-row tiles. A PF-deep register ring
+// keeps the A rows and query words of the next PF steps in flight while the
+// current step's MFMAs run. OCC is the waves per SIMD the register budget
+// is sized for (1: deep ring, 2: shallow ring with a second wave to switch
+// to); the host picks a (QG, PF, OCC) variant by timing them.
+template <int KE, int SRC, int QG, int PF, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                 uint32_t n_tiles, const uint32_t* __restrict__ gitems,
                 const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
                 const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
                 uint64_t* __restrict__ out) {
     constexpr int STEPS = KE / 8;  // 64 K'-bytes (8 elements) per step
+    static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
+    static_assert(STEPS % PF == 0 && PF % 2 == 0, "ring must tile the steps");
     // [buf][query][column][element], each column row padded from 8 to 10
     // words so that the 16 columns of a ds_read_b128 pass start on distinct
     // bank groups
-    __shared__ uint64_t bw[2][PB_QG][16][10];
+    __shared__ uint64_t bw[2][QG][16][10];
     __shared__ int32_t dsm[PB_WAVES][4][16][17];
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int grp = lane / 16, col = lane % 16;
-    // producer role: query pj, element pe, columns pd0 .. pd0 + 3
-    const int pj = tid >> 5, pe = (tid >> 2) & 7, pd0 = (tid & 3) * 4;
+    // producer role: query pj, element pe, columns pd0 .. pd0 + CPT - 1
+    constexpr int TPQ = 256 / QG;  // producer threads per query
+    constexpr int CPT = QG / 2;    // columns per producer thread
+    const int pj = tid / TPQ, pw = tid % TPQ;
+    const int pe = pw / (16 / CPT), pd0 = (pw % (16 / CPT)) * CPT;
     for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
@@ -1154,7 +1161,7 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
         auto produce = [&](int buf, uint64_t v) {
             const uint64_t x = __builtin_bswap64(v ^ 0x8080808080808080ull);
 #pragma unroll
-            for (int dd = 0; dd < 4; ++dd) {
+            for (int dd = 0; dd < CPT; ++dd) {
                 const int d = pd0 + dd;
                 uint64_t w = 0;
                 if (d <= 7) w = x >> (8 * (7 - d));
@@ -1162,36 +1169,35 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 bw[buf][pj][d][pe] = w;
             }
         };
-        // PB_PF-deep register ring: ar[d] / vr[d] hold step s with s % PB_PF == d.
-        v4i ar[PB_PF][PB_RT];
-        uint64_t vr[PB_PF];
+        // ring: ar[d] / vr[d] hold step s with s % PF == d
+        v4i ar[PF][PB_RT];
+        uint64_t vr[PF];
 #pragma unroll
-        for (int d = 0; d < PB_PF; ++d) {
+        for (int d = 0; d < PF; ++d) {
 #pragma unroll
             for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * d);
             vr[d] = gload_u64(pv + 8 * d);
         }
         if (producer) produce(0, vr[0]);
         __syncthreads();
-        v4i acc[PB_RT][PB_QG];
+        v4i acc[PB_RT][QG];
 #pragma unroll
         for (int rt = 0; rt < PB_RT; ++rt)
 #pragma unroll
-            for (int j = 0; j < PB_QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
-        static_assert(STEPS % PB_PF == 0 && PB_PF % 2 == 0, "ring must tile the steps");
-        for (int s0 = 0; s0 < STEPS; s0 += PB_PF) {
+            for (int j = 0; j < QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
+        for (int s0 = 0; s0 < STEPS; s0 += PF) {
 #pragma unroll
-            for (int d = 0; d < PB_PF; ++d) {
-                // Loads are unconditional (the last PB_PF steps re-load the
+            for (int d = 0; d < PF; ++d) {
+                // Loads are unconditional (the last PF steps re-load the
                 // final step, in bounds) so the wait counts stay exact.
-                const int nxt = min(s0 + d + PB_PF, STEPS - 1);
+                const int nxt = min(s0 + d + PF, STEPS - 1);
                 v4i a[PB_RT];
 #pragma unroll
                 for (int rt = 0; rt < PB_RT; ++rt) a[rt] = ar[d][rt] ^ (int)0x80808080;
 #pragma unroll
                 for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * nxt);
 #pragma unroll
-                for (int j = 0; j < PB_QG; ++j) {
+                for (int j = 0; j < QG; ++j) {
                     if (j < gn) {
                         const v4i b = *reinterpret_cast<const v4i*>(&bw[d & 1][j][col][2 * grp]);
 #pragma unroll
@@ -1202,33 +1208,33 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 }
                 // (after the last step this fills the idle buffer with a
                 // repeat of the final step; nothing reads it)
-                if (producer) produce((d + 1) & 1, vr[(d + 1) % PB_PF]);
+                if (producer) produce((d + 1) & 1, vr[(d + 1) % PF]);
                 vr[d] = gload_u64(pv + 8 * nxt);
                 __syncthreads();
             }
         }
         // Epilogue, four (row tile, query) pairs per round (see k_gemv_mfma).
 #pragma unroll
-        for (int rnd = 0; rnd < PB_RT * PB_QG / 4; ++rnd) {
+        for (int rnd = 0; rnd < PB_RT * QG / 4; ++rnd) {
 #pragma unroll
             for (int pp = 0; pp < 4; ++pp) {
                 const int pair = rnd * 4 + pp;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    dsm[wave][pp][4 * grp + i][col] = acc[pair / PB_QG][pair % PB_QG][i];
+                    dsm[wave][pp][4 * grp + i][col] = acc[pair / QG][pair % QG][i];
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int pp = lane / 16, row = lane % 16;
             const int pair = rnd * 4 + pp;
-            const int rt = pair / PB_QG, j = pair % PB_QG;
+            const int rt = pair / QG, j = pair % QG;
             const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;
             if (j < gn && grow < m) {
                 __int128 sacc = 0;
 #pragma unroll
                 for (int d = 0; d < 15; ++d)
-                    sacc += (__int128)dsm[wave][pp][row][d] << ((8 * d) % 61);
+                    sacc += (__int128)dsm[wave][pp][row][d] * (__int128)(1ull << ((8 * d) % 61));
                 const unsigned __int128 u = (unsigned __int128)(sacc + ((__int128)P << 27));
                 uint64_t v = reduce128((uint64_t)u, (uint64_t)(u >> 64));
                 const uint32_t it = gitems[tile.g0 + j];
@@ -1326,7 +1332,9 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 //   S v_j   = sum_b 256^b (S . v'_{j,b}) + c sum S,
 // with A = the row's three M planes and S, B = T_j (one column per query,
 // columns 8..15 zero) and the eight v_enc byte planes of query j (two
-// queries per 16-column tile). Each wave owns PB_RT 16-row tiles.
+// queries per 16-column tile). Each wave owns PB_RT 16-row tiles; QG is the
+// tile's query group (the product's).
+template <int QG>
 __global__ void __launch_bounds__(256)
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                uint32_t n_tiles, const uint32_t* __restrict__ gitems,
@@ -1335,7 +1343,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                const uint64_t* __restrict__ mt_qterm, const uint64_t* __restrict__ r,
                const uint64_t* __restrict__ dense, int64_t* __restrict__ scores) {
     constexpr int KC = N / 64;  // 16 K-steps of 64 bytes
-    constexpr int NCT = PB_QG / 2;  // S v column tiles
+    constexpr int NCT = QG / 2;  // S v column tiles
+    static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
     __shared__ int32_t dsm[PB_WAVES][3 + NCT][16][17];
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
@@ -1427,7 +1436,7 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < QG / 4; ++h) {
                 const int pr = lane + 64 * h;
                 const int row = pr % 16, j = pr / 16;
                 const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;

@@ -31,7 +31,9 @@ EXPORTS = [
     "svs_emvp_create", "svs_emvp_destroy", "svs_emvp_compute_products", "svs_tiptoe_create",
     "svs_tiptoe_destroy", "svs_tiptoe_stored_bytes", "svs_tiptoe_answer_u32",
     "svs_tiptoe_answer_u32_dev", "svs_tiptoe_answer_u64", "svs_ctx_copy_probe",
+    "svs_ctx_set_tile_variant", "svs_ctx_get_tile_variant", "svs_autotune",
 ]
+TILE_VARIANTS = 5
 
 
 class GpuError(RuntimeError):
@@ -118,6 +120,11 @@ def lib() -> C.CDLL:
             "svs_tiptoe_answer_u32_dev": ([vp, u32, vp, vp, i32], i32),
             "svs_tiptoe_answer_u64": ([vp, u32, vp, vp], i32),
             "svs_ctx_copy_probe": ([vp, u64, C.POINTER(C.c_double)], i32),
+            "svs_ctx_set_tile_variant": ([vp, u32, i32], i32),
+            "svs_ctx_get_tile_variant": ([vp, u32, C.POINTER(i32), C.POINTER(u32),
+                                          C.POINTER(u32), C.POINTER(u32)], i32),
+            "svs_autotune": ([vp, C.POINTER(QueryBatch), u32, C.POINTER(i32),
+                              C.POINTER(C.c_double)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

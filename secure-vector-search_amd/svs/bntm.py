@@ -190,6 +190,18 @@ class Context:
         check(L.lib().svs_ctx_copy_probe(self.h, nbytes, C.byref(ms)))
         return ms.value
 
+    def set_tile_variant(self, variant: int, n_queries: int = 0):
+        """Block-tile variant of the MFMA batch path for batches of
+        n_queries (0: every batch size)."""
+        check(L.lib().svs_ctx_set_tile_variant(self.h, n_queries, variant))
+
+    def tile_variant(self, n_queries: int) -> dict:
+        v, qg, pf, occ = C.c_int(0), C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        check(L.lib().svs_ctx_get_tile_variant(self.h, n_queries, C.byref(v), C.byref(qg),
+                                               C.byref(pf), C.byref(occ)))
+        return dict(variant=v.value, query_group=qg.value, ring_depth=pf.value,
+                    waves_per_simd=occ.value)
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""
@@ -326,8 +338,7 @@ class GpuIndex:
                                       ptr(out)))
         return None if rc == L.SVS_NONE else out[:m]
 
-    def score_batch(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k,
-                    mode=L.MODE_BATCHED):
+    def _batch(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k):
         v_q = u64(v_q).reshape(-1, N)
         nq = v_q.shape[0]
         seeds = np.ascontiguousarray(q_seeds, np.uint8).reshape(nq, 32)
@@ -342,8 +353,23 @@ class GpuIndex:
         status = np.zeros(nq, np.int32)
         b = L.QueryBatch(nq, ptr(v_q), ptr(seeds), ptr(pp), ptr(pl), ptr(ver), k, ptr(ids),
                          ptr(scores), ptr(count), ptr(status))
+        # the arrays stay referenced by the returned tuple while b is used
+        return b, (v_q, seeds, pp, pl, ver), (ids, scores, count, status)
+
+    def score_batch(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k,
+                    mode=L.MODE_BATCHED):
+        b, _keep, out = self._batch(v_q, q_seeds, probe_ptr, probe_lists, verify, k)
         check(L.lib().svs_score_batch(self.h, C.byref(b), mode))
-        return ids, scores, count, status
+        return out
+
+    def autotune(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k, reps: int = 2):
+        """Times each block-tile variant on this batch and keeps the fastest
+        for its size class; returns (best, ms per variant)."""
+        b, _keep, _out = self._batch(v_q, q_seeds, probe_ptr, probe_lists, verify, k)
+        best = C.c_int(0)
+        ms = (C.c_double * L.TILE_VARIANTS)()
+        check(L.lib().svs_autotune(self.h, C.byref(b), reps, C.byref(best), ms))
+        return best.value, list(ms)
 
     def close(self):
         if getattr(self, "h", None):

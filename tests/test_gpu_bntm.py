@@ -487,3 +487,55 @@ def test_topk_ties_across_chunks(ctx):
                                                 False, k)
             assert [h.id for h in got[i]] == ids.tolist(), f"k {k} query {i}"
             assert len(set(scores.tolist())) < k or k == 1
+
+
+@pytest.mark.parametrize("variant", range(5))
+def test_tile_variants_match_oracle(ctx, ivf_setup, l_sub, variant):
+    """Every block-tile variant (query group 4/8/16, ring depth, waves per
+    SIMD) of the MFMA batch path is bit-exact: whole IVF queries and the
+    hot-list product."""
+    import svs
+    s = ivf_setup
+    ctx.set_tile_variant(variant)
+    try:
+        assert ctx.tile_variant(1)["variant"] == variant
+        seeds = np.arange(12 * 32, dtype=np.uint64).astype(np.uint8) ^ np.uint8(0x33)
+        say("tile variant", variant, ctx.tile_variant(12))
+        got = s["scorer"].score_batch(s["handle"], s["qs"], 10, q_seeds=seeds,
+                                      mode=svs._lib.MODE_BATCHED)
+        for i, q in enumerate(s["qs"]):
+            ids, scores = O.ivf_score(s["clusters"], s["lists"], s["centres"], s["l"], q, 4,
+                                      seeds[i * 32:(i + 1) * 32], True, 10)
+            assert [h.id for h in got[i]] == ids.tolist(), f"variant {variant} query {i}"
+        rng = np.random.default_rng(500 + variant)
+        m = 300
+        cl = _random_cluster(rng, m)
+        idx = svs.GpuIndex(ctx, 1, l_sub)
+        idx.upload_list(0, cl)
+        nq = 37
+        v = _field(rng, (nq, 1024))
+        got = idx.compute_products_batch(v, np.arange(nq + 1, dtype=np.uint32),
+                                         np.zeros(nq, np.uint32), svs._lib.KERNEL_MFMA)
+        want = O.matvec_batch(cl["m_enc"], m, 1024, v)
+        for q in range(nq):
+            assert np.array_equal(got[q], want[q]), f"variant {variant} query {q}"
+    finally:
+        ctx.set_tile_variant(0)
+
+
+def test_autotune_picks_a_variant(ctx, ivf_setup):
+    import svs
+    s = ivf_setup
+    h = s["handle"]
+    probes = svs.probe_route(s["centres"], s["qs"], 4)
+    v_q = svs.quantise(s["qs"])
+    seeds = np.zeros((12, 32), np.uint8)
+    pp = np.arange(0, 12 * 4 + 1, 4, dtype=np.uint32)
+    say("autotune")
+    try:
+        best, ms = h.index.autotune(v_q, seeds, pp, probes.reshape(-1), None, 10, reps=1)
+        assert 0 <= best < svs._lib.TILE_VARIANTS and len(ms) == svs._lib.TILE_VARIANTS
+        assert ctx.tile_variant(12)["variant"] == best
+        assert all(x > 0 for x in ms)
+    finally:
+        ctx.set_tile_variant(0)
