@@ -57,6 +57,75 @@ def run_batches(index, v_q, seeds, probes, verify, k, batch, nbatches, mode, off
     return lat, res
 
 
+class DeviceQueries:
+    """The workload's query vectors and seeds resident in HBM, plus device
+    outputs for one batch (svs_dev_alloc buffers)."""
+
+    def __init__(self, ctx, v_q, seeds, batch, k):
+        import svs
+        self.n = v_q.shape[0]
+        self.v = svs.DeviceArray.from_host(ctx, v_q)
+        self.s = svs.DeviceArray.from_host(ctx, seeds)
+        self.ids = svs.DeviceArray(ctx, (batch, k), np.uint32)
+        self.sc = svs.DeviceArray(ctx, (batch, k), np.int64)
+        self.cnt = svs.DeviceArray(ctx, (batch,), np.uint32)
+        self.st = svs.DeviceArray(ctx, (batch,), np.int32)
+
+    def results(self, nq):
+        return (self.ids.to_host()[:nq], self.sc.to_host()[:nq], self.cnt.to_host()[:nq],
+                self.st.to_host()[:nq])
+
+
+def run_batches_dev(index, dq, probes, verify, k, batch, nbatches, offset=0):
+    """Device-resident batches: per-batch wall seconds; results of the last."""
+    nprobe = probes.shape[1]
+    lat = []
+    for b in range(nbatches):
+        q0 = offset + b * batch
+        nq = min(batch, dq.n - q0)
+        pp = np.arange(0, nq * nprobe + 1, nprobe, dtype=np.uint32)
+        t0 = time.perf_counter()
+        index.score_batch_dev(nq, dq.v.ptr(q0 * 1024 * 8), dq.s.ptr(q0 * 32), pp,
+                              probes[q0:q0 + nq].reshape(-1), verify[q0:q0 + nq], k,
+                              dq.ids.ptr(), dq.sc.ptr(), dq.cnt.ptr(), dq.st.ptr())
+        lat.append(time.perf_counter() - t0)
+    return lat
+
+
+def served(index, v_q, probes, verify, k, levels, seconds, max_batch, slots, pool, seed=5):
+    """Open-loop Poisson arrivals through the continuous-batching server at
+    each offered rate; per level: achieved q/s and latency percentiles
+    (scheduled arrival -> result)."""
+    import svs
+    rng = np.random.default_rng(seed)
+    out = []
+    with svs.Server(index, max_batch=max_batch, slots=slots) as srv:
+        # warm the server's slots (scratch sizes) once
+        warm = [dict(v_q=v_q[i], q_seed=rng.integers(0, 256, 32, np.uint8), probes=probes[i],
+                     verify=bool(verify[i]), k=k) for i in range(min(pool, 512))]
+        srv.run_poisson(warm, qps=levels[0], seed=seed)
+        for qps in levels:
+            n = max(64, int(qps * seconds))
+            reqs = []
+            for j in range(n):
+                i = j % pool
+                reqs.append(dict(v_q=v_q[i], q_seed=rng.integers(0, 256, 32, np.uint8),
+                                 probes=probes[i], verify=bool(verify[i]), k=k))
+            s0 = srv.stats()
+            res, wall = srv.run_poisson(reqs, qps=qps, seed=seed + int(qps))
+            s1 = srv.stats()
+            lat = [r.latency_ms for r in res]
+            nb = s1["batches"] - s0["batches"]
+            lvl = dict(offered_qps=qps, achieved_qps=round(n / wall, 1), requests=n,
+                       p50_ms=round(pct(lat, 50), 3), p99_ms=round(pct(lat, 99), 3),
+                       p999_ms=round(pct(lat, 99.9), 3),
+                       mean_batch=round((s1["batched_queries"] - s0["batched_queries"]) /
+                                        max(nb, 1), 1), max_in_flight=s1["max_in_flight"])
+            log(f"served: {lvl}")
+            out.append(lvl)
+    return out
+
+
 def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threads):
     """The reference CPU path (BnTmIvfScorer::score, ivf.rs:746-822) as the C
     oracle, one query per thread, on a bounded sample of the workload's
@@ -134,6 +203,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile-variant", type=int, default=-1,
+                    help="force a block-tile variant (skips the autotuner)")
+    ap.add_argument("--serve-qps", default="10000,20000,30000,40000",
+                    help="offered rates of the served (open-loop Poisson) levels; '' = skip")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-max-batch", type=int, default=512)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--slo-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain-copy calibration dispatch (0 = skip)")
     args = ap.parse_args()
@@ -174,7 +251,9 @@ def main():
 
     # block-tile variant for this batch size: timed on the first warm-up batch
     tune = None
-    if not args.no_autotune:
+    if args.tile_variant >= 0:
+        ctx.set_tile_variant(args.tile_variant)
+    elif not args.no_autotune:
         s0 = slice(0, args.batch)
         pp0 = np.arange(0, v_q[s0].shape[0] * args.nprobe + 1, args.nprobe, dtype=np.uint32)
         best, tms = index.autotune(v_q[s0], seeds[s0], pp0, probes[s0].reshape(-1), verify[s0],
@@ -183,20 +262,28 @@ def main():
         log(f"autotune: {tune}")
     variant = ctx.tile_variant(args.batch)
 
-    # batched (served-batched) path
-    run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.warmup, 1)
+    # batched path, query inputs and outputs resident in HBM
+    dq = DeviceQueries(ctx, v_q, seeds, args.batch, args.k)
+    run_batches_dev(index, dq, probes, verify, args.k, args.batch, args.warmup)
     ctx.profile(True, True)
-    lat, res_b = run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.steps, 1,
-                             offset=args.warmup * args.batch)
+    off = args.warmup * args.batch
+    lat = run_batches_dev(index, dq, probes, verify, args.k, args.batch, args.steps, offset=off)
     prof = ctx.profile(False, False)
     wall = sum(lat)
     qps = args.steps * args.batch / wall
     log(f"batched: {qps:.0f} q/s, batch p50 {pct(lat, 50) * 1e3:.2f} ms; stages {prof}")
+    # batch 0 of the timed region again, for the bit-exactness check below
+    run_batches_dev(index, dq, probes, verify, args.k, args.batch, 1, offset=off)
+    res_b = [dq.results(min(args.batch, v_q.shape[0] - off))]
+    # the same batches through host buffers (PCIe-inclusive rate)
+    lat_h, _ = run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.steps, 1,
+                           offset=off)
+    qps_host = args.steps * args.batch / sum(lat_h)
+    log(f"batched, host buffers: {qps_host:.0f} q/s")
 
     # plain path: one query per launch sequence, same queries as batch 0 of
     # the timed region -> also a bit-exactness check between the two modes
     npl = min(args.plain_queries, args.batch)
-    off = args.warmup * args.batch
     run_batches(index, v_q, seeds, probes, verify, args.k, 1, 2, 0, offset=off)
     lat_p, res_p = run_batches(index, v_q, seeds, probes, verify, args.k, 1, npl, 0, offset=off)
     qps_plain = npl / sum(lat_p)
@@ -204,6 +291,13 @@ def main():
     exact = all(np.array_equal(res_p[i][0][0], ids_b[i]) and np.array_equal(res_p[i][1][0], sc_b[i])
                 for i in range(npl))
     log(f"plain: {qps_plain:.0f} q/s, p50 {pct(lat_p, 50) * 1e3:.2f} ms; batched==plain {exact}")
+
+    levels = [float(x) for x in args.serve_qps.split(",") if x.strip()]
+    serve = None
+    if levels:
+        pool = qs.shape[0]
+        serve = served(index, v_q, probes, verify, args.k, levels, args.serve_seconds,
+                       args.serve_max_batch, args.serve_slots, pool)
 
     gemv_ms, gemv_calls = prof["product"]
     achieved = prof["gemv_bytes"] / (gemv_ms * 1e-3) / 1e9 if gemv_ms else 0.0
@@ -255,6 +349,8 @@ def main():
                   "p50_ms": round(pct(lat_p, 50) * 1e3, 3),
                   "p99_ms": round(pct(lat_p, 99) * 1e3, 3), "queries": npl},
         "speedup_vs_plain": round(qps / qps_plain, 2),
+        "batched_host_buffers_qps": round(qps_host, 2),
+        "served": serve,
         "bit_exact_batched_vs_plain": bool(exact),
         "stages_ms_per_batch": {s: round(prof[s][0] / max(prof[s][1], 1), 4)
                                 for s in ("encode", "product", "verify", "decode", "topk")},

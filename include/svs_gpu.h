@@ -207,6 +207,16 @@ typedef struct {
 } svs_query_batch;
 
 int svs_score_batch(svs_index* idx, const svs_query_batch* batch_host, int mode);
+/* Device buffers for callers that keep query inputs and outputs in HBM (no
+ * reference counterpart; plumbing for svs_score_batch_dev). Counted against
+ * the context budget; svs_memcpy copies in any direction (synchronous). */
+int svs_dev_alloc(svs_ctx* ctx, uint64_t bytes, void** out);
+int svs_dev_free(svs_ctx* ctx, void* p);
+int svs_memcpy(svs_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Same with v_q, q_seeds and the four output arrays in device memory
+ * (probe_ptr, probe_lists and verify stay on the host: they plan the
+ * batch). Non-canonical v_q is reported as SVS_ERR_ARG after the batch. */
+int svs_score_batch_dev(svs_index* idx, const svs_query_batch* batch_dev, int mode);
 
 /* ---- block-tile variants and the autotuner ---------------------------------
  * No reference counterpart (the reference has one CUDA kernel shape): the
@@ -227,6 +237,64 @@ int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* variant,
  * wall time per run; the batch's outputs hold the last variant's results. */
 int svs_autotune(svs_index* idx, const svs_query_batch* batch_host, uint32_t reps, int* best,
                  double* ms_per_variant);
+
+/* ---- serving: continuous batching -----------------------------------------
+ * No single reference counterpart: the reference scores one query per call
+ * (BnTmIvfScorer::score, scorer-bntm/src/ivf.rs:746-822) and batches only a
+ * chunk handed to it (eval.rs:2428-2433). A server keeps one or more
+ * resident indexes (tenants: each its own keys and lists, all on one
+ * context) and a queue of requests. A dispatcher thread packs the pending
+ * requests of one tenant into a batch whenever one of `slots` device
+ * scratch/stream slots is free, admitting requests while the batch's device
+ * scratch estimate stays within slot_bytes (at least one request), and runs
+ * the batch asynchronously; batches on different slots overlap. Results are
+ * those of svs_score_batch for the same request alone (bit-exact). Indexes
+ * must outlive the server and must not be modified while it runs. */
+typedef struct svs_server svs_server;
+typedef struct {
+    uint32_t max_batch;   /* queries per batch, 1..4096 */
+    uint32_t slots;       /* batches in flight, 1..8 */
+    uint64_t slot_bytes;  /* device scratch budget per batch (0 = 1 GiB) */
+    int mode;             /* SVS_MODE_BATCHED or SVS_MODE_BATCHED_VALU */
+} svs_server_config;
+typedef struct {
+    uint32_t tenant;         /* position in the tenants array */
+    const uint64_t* v_q;     /* 1024 canonical u64 (quantised query) */
+    const uint8_t* q_seed;   /* 32 bytes */
+    const uint32_t* probes;  /* probed list ids, in probe order */
+    uint32_t n_probes;
+    uint8_t verify;          /* Freivalds on every probed list */
+    uint32_t k;              /* 1..4096 */
+    double arrival_s;        /* 0 (= now), or a scheduled arrival on the
+                                library's steady clock (load generator) */
+} svs_request;
+typedef struct {
+    uint32_t* ids;        /* caller buffer of k (may be NULL) */
+    int64_t* scores;      /* caller buffer of k (may be NULL) */
+    uint32_t count;
+    int32_t status;       /* as svs_query_batch.out_status */
+    int32_t rc;           /* SVS_OK or the batch's error code */
+    uint32_t batch_size;  /* queries in the batch the request ran in */
+    double latency_ms;    /* submit (or scheduled arrival) to completion */
+} svs_result;
+typedef struct {
+    uint64_t submitted, completed, batches, batched_queries;
+    uint32_t max_batch, max_in_flight, pending;
+} svs_server_counters;
+
+int svs_server_create(svs_index* const* tenants, uint32_t n_tenants, const svs_server_config* cfg,
+                      svs_server** out);
+/* Finishes every pending request, then stops. */
+int svs_server_destroy(svs_server* s);
+int svs_server_submit(svs_server* s, const svs_request* req, uint64_t* ticket);
+/* Blocks until the request is done; each ticket is waited for once. */
+int svs_server_wait(svs_server* s, uint64_t ticket, svs_result* out);
+int svs_server_stats(svs_server* s, svs_server_counters* out);
+/* Open-loop Poisson load generator: request i is submitted at its scheduled
+ * arrival (exponential gaps at `qps`, seeded) regardless of earlier ones;
+ * results[i] as svs_server_wait, latency from the scheduled arrival. */
+int svs_server_run_poisson(svs_server* s, const svs_request* reqs, uint32_t n, double qps,
+                           uint64_t seed, svs_result* results, double* wall_s);
 
 /* ---- EMVP score GEMV (same kernel family) ---------------------------------
  * Resident EMVP cluster M_hat (m x 1292 canonical u64, SEC128 n = 1292,

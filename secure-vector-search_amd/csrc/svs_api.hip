@@ -51,7 +51,8 @@ int DevBuf::ensure(svs_ctx* ctx, size_t bytes) {
         ptr = nullptr;
         cap = 0;
     }
-    size_t want = std::max(bytes, (size_t)4096);
+    // geometric growth: a buffer that keeps growing is reallocated O(log) times
+    size_t want = std::max({bytes, (size_t)4096, cap + cap / 2});
     want = (want + 4095) & ~(size_t)4095;
     HIP_TRY(hipMalloc(&ptr, want));
     cap = want;
@@ -94,6 +95,40 @@ void svs::dev_free(svs_ctx* ctx, void* p, size_t bytes) {
     ctx->resident_bytes -= bytes;
 }
 
+// Caller-held device buffers (query inputs and outputs kept in HBM for
+// svs_score_batch_dev), counted against the context budget.
+extern "C" int svs_dev_alloc(svs_ctx* ctx, uint64_t bytes, void** out) {
+    if (!ctx || !out || bytes == 0) return fail(SVS_ERR_ARG, "null argument or zero size");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(dev_alloc(ctx, out, bytes));
+    ctx->user_allocs[*out] = bytes;
+    return SVS_OK;
+}
+
+extern "C" int svs_dev_free(svs_ctx* ctx, void* p) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    if (!p) return SVS_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->user_allocs.find(p);
+    if (it == ctx->user_allocs.end()) return fail(SVS_ERR_ARG, "pointer not from svs_dev_alloc");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dev_free(ctx, p, it->second);
+    ctx->user_allocs.erase(it);
+    return SVS_OK;
+}
+
+extern "C" int svs_memcpy(svs_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(SVS_ERR_ARG, "null argument");
+    if (bytes == 0) return SVS_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
 extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) {
     if (!out) return fail(SVS_ERR_ARG, "out is null");
     *out = nullptr;
@@ -131,6 +166,8 @@ static void ctx_free(svs_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : ctx->all_scratch()) b->release(ctx);
+    for (auto& kv : ctx->user_allocs) dev_free(ctx, kv.first, kv.second);
+    ctx->user_allocs.clear();
     if (ctx->prof.created)
         for (auto& e : ctx->prof.ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
@@ -564,25 +601,10 @@ extern "C" int svs_index_download_list(svs_index* idx, uint32_t list, uint64_t* 
 
 // ================================================================== pipeline
 // Device-side work description of a batch: items, prefixes, flags.
-struct BatchPlan {
-    uint32_t n_queries = 0, n_items = 0, n_chunks = 0, n_cands = 0;
-    uint64_t distinct_bytes = 0;  // sum of m_c * n * 8 over distinct lists
-    bool any_verify = false;
-    bool planes_ok = true;  // every probed list has the MFMA decode planes
-    std::vector<Item> items;
-    std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
-    std::vector<uint32_t> qchunk_ptr;  // per query: its top-k wave chunks
-    uint32_t n_tchunks = 0;
-    std::vector<uint8_t> item_verify;
-    // MFMA product: items grouped by list (probe order kept within a list)
-    // and work tiles of MT_ROWS rows x up to MT_QG of those items.
-    std::vector<uint32_t> gitems;
-    std::vector<MTile> tiles;
-};
 
-static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
-                     const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
-                     uint32_t tile_rows = PB_ROWS, uint32_t tile_qg = PB_QG) {
+int svs::make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
+              const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
+              uint32_t tile_rows, uint32_t tile_qg) {
     pl = BatchPlan();
     pl.n_queries = nq;
     if (probe_ptr[0] != 0) return fail(SVS_ERR_ARG, "probe_ptr[0] must be 0");
@@ -640,16 +662,24 @@ static int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_pt
     std::stable_sort(pl.gitems.begin(), pl.gitems.end(), [&](uint32_t x, uint32_t y) {
         return pl.items[x].list < pl.items[y].list;
     });
-    const uint32_t ng = (uint32_t)pl.gitems.size();
-    for (uint32_t g = 0; g < ng;) {
-        uint32_t e = g;
-        while (e < ng && pl.items[pl.gitems[e]].list == pl.items[pl.gitems[g]].list) ++e;
-        const uint32_t m = pl.items[pl.gitems[g]].m;
-        for (uint32_t r0 = 0; r0 < m; r0 += tile_rows)
-            for (uint32_t q0 = g; q0 < e; q0 += tile_qg)
-                pl.tiles.push_back(MTile{pl.items[pl.gitems[g]].list, r0, q0,
-                                         std::min<uint32_t>(tile_qg, e - q0)});
-        g = e;
+    auto make_tiles = [&](const std::vector<uint32_t>& gi, std::vector<MTile>& tiles) {
+        const uint32_t ng = (uint32_t)gi.size();
+        for (uint32_t g = 0; g < ng;) {
+            uint32_t e = g;
+            while (e < ng && pl.items[gi[e]].list == pl.items[gi[g]].list) ++e;
+            const uint32_t m = pl.items[gi[g]].m;
+            for (uint32_t r0 = 0; r0 < m; r0 += tile_rows)
+                for (uint32_t q0 = g; q0 < e; q0 += tile_qg)
+                    tiles.push_back(MTile{pl.items[gi[g]].list, r0, q0,
+                                          std::min<uint32_t>(tile_qg, e - q0)});
+            g = e;
+        }
+    };
+    make_tiles(pl.gitems, pl.tiles);
+    if (pl.any_verify) {
+        for (uint32_t i : pl.gitems)
+            if (pl.item_verify[i]) pl.vgitems.push_back(i);
+        make_tiles(pl.vgitems, pl.vtiles);
     }
     return SVS_OK;
 }
@@ -824,7 +854,19 @@ static void tile_shape(int kernel, int variant, uint32_t& rows, uint32_t& qg) {
     else rows = PB_ROWS, qg = kTileVariants[variant].qg;
 }
 
-static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
+// Device tile buffer: [tiles][gitems] then, 16-byte aligned, [vtiles][vgitems].
+static size_t vtiles_offset(const BatchPlan& pl) {
+    return (sizeof(MTile) * pl.tiles.size() + sizeof(uint32_t) * pl.gitems.size() + 15) & ~(size_t)15;
+}
+static size_t mtile_bytes(const BatchPlan& pl) {
+    return vtiles_offset(pl) + sizeof(MTile) * pl.vtiles.size() +
+           sizeof(uint32_t) * pl.vgitems.size();
+}
+static MTile* vtiles_ptr(svs_slot& sl, const BatchPlan& pl) {
+    return (MTile*)((char*)sl.mtiles.ptr + vtiles_offset(pl));
+}
+
+int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
                     bool product_only,
                     int kernel, bool encode, int variant) {
     svs_ctx* ctx = idx->ctx;
@@ -892,7 +934,7 @@ static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t 
         HIP_TRY(hipGetLastError());
     } else if (pl.n_chunks) {
         const uint32_t nt = (uint32_t)pl.tiles.size();
-        SVS_TRY(sl.mtiles.ensure(ctx, sizeof(MTile) * nt + sizeof(uint32_t) * pl.gitems.size()));
+        SVS_TRY(sl.mtiles.ensure(ctx, mtile_bytes(pl)));
         SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
         MTile* d_tiles = (MTile*)sl.mtiles.ptr;
         uint32_t* d_gitems = (uint32_t*)(d_tiles + nt);
@@ -900,6 +942,13 @@ static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t 
                                hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size(),
                                hipMemcpyHostToDevice, st));
+        if (!pl.vtiles.empty()) {
+            HIP_TRY(hipMemcpyAsync(vtiles_ptr(sl, pl), pl.vtiles.data(),
+                                   sizeof(MTile) * pl.vtiles.size(), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(vtiles_ptr(sl, pl) + pl.vtiles.size(), pl.vgitems.data(),
+                                   sizeof(uint32_t) * pl.vgitems.size(), hipMemcpyHostToDevice,
+                                   st));
+        }
         hipLaunchKernelGGL(k_query_corr<N>, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
                            (uint64_t*)sl.q_corr.ptr);
         if (kernel == KERNEL_MFMA_WAVE) {
@@ -934,10 +983,29 @@ static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t 
     SVS_TRY(prof_mark(sl, 2, true));
     if (pl.any_verify) {
         SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
+        // The check's right-hand side sum_i u_t[i] (M_enc[i] . v_enc) needs
+        // the row products again: on the MFMA path they are recomputed by
+        // the product tiles over the verified items only, independently of
+        // the r the server returned; otherwise k_verify_chunks recomputes
+        // them row by row.
+        const uint64_t* r_check = nullptr;
+        if (kernel == KERNEL_MFMA && !pl.vtiles.empty()) {
+            SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+            const uint32_t nvt = (uint32_t)pl.vtiles.size();
+            const MTile* d_vtiles = vtiles_ptr(sl, pl);
+            const TileArgs va{d_lists, d_vtiles, nvt, (const uint32_t*)(d_vtiles + nvt), d_items,
+                              d_cand_ptr};
+            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(nvt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
+            launch_gemm<N, 0>(variant, st, blocks, va, (const uint64_t*)d_venc,
+                              (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.rv.ptr);
+            ctx->launches++;
+            r_check = (const uint64_t*)sl.rv.ptr;
+        }
         hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS),
                            0, st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
                            (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (const uint64_t*)sl.r.ptr, (const uint32_t*)d_vseed,
+                           (const uint64_t*)sl.r.ptr, r_check, (const uint32_t*)d_vseed,
                            (const uint8_t*)sl.flags.ptr, (uint64_t*)sl.partials.ptr,
                            d_rng_flag);
         hipLaunchKernelGGL(k_verify_finalize, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
@@ -1035,8 +1103,8 @@ static int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t 
     return SVS_OK;
 }
 
-static int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
-                         const uint8_t* seeds) {
+int svs::stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
+                  const uint8_t* seeds) {
     SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
     SVS_TRY(sl.q_seeds.ensure(ctx, (size_t)nq * 32));
     SVS_TRY(sl.q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
@@ -1046,29 +1114,33 @@ static int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t
     SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
     SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
     HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
-                           hipMemcpyHostToDevice, sl.stream));
-    HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyHostToDevice,
+                           hipMemcpyDefault, sl.stream));
+    HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyDefault,
                            sl.stream));
     return SVS_OK;
 }
 
-static int collect(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items,
-                   uint32_t* ids,
-                   int64_t* scores, uint32_t* count, int32_t* status) {
+// Enqueues the batch's result copies (ids, scores, counts, per-query status
+// and the RNG flag) on the slot's stream.
+int svs::collect_async(svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,
+                  int64_t* scores, uint32_t* count, int32_t* status, uint32_t* rng_flag) {
     hipStream_t st = sl.stream;
     uint32_t* d_oids = (uint32_t*)sl.out.ptr;
     int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
     uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
     int32_t* d_qstatus = (int32_t*)sl.verdict.ptr + std::max<uint32_t>(n_items, 1);
     uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
-    uint32_t rng_flag = 0;
-    HIP_TRY(hipMemcpyAsync(ids, d_oids, sizeof(uint32_t) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(scores, d_oscores, sizeof(int64_t) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(count, d_ocount, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(status, d_qstatus, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&rng_flag, d_rng_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    SVS_TRY(prof_collect(sl));
+    HIP_TRY(hipMemcpyAsync(ids, d_oids, sizeof(uint32_t) * (size_t)nq * k, hipMemcpyDefault, st));
+    HIP_TRY(hipMemcpyAsync(scores, d_oscores, sizeof(int64_t) * (size_t)nq * k, hipMemcpyDefault, st));
+    HIP_TRY(hipMemcpyAsync(count, d_ocount, sizeof(uint32_t) * nq, hipMemcpyDefault, st));
+    HIP_TRY(hipMemcpyAsync(status, d_qstatus, sizeof(int32_t) * nq, hipMemcpyDefault, st));
+    HIP_TRY(hipMemcpyAsync(rng_flag, d_rng_flag, sizeof(uint32_t), hipMemcpyDefault, st));
+    return SVS_OK;
+}
+
+// The batch's RNG flag: a Freivalds u-draw equal to p, or an encode
+// keystream that ran out (both have probability < 2^-60).
+int svs::check_rng_flag(uint32_t rng_flag) {
     if (rng_flag & 1u)
         return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p (probability 2^-61 per draw)");
     if (rng_flag & 2u)
@@ -1076,7 +1148,34 @@ static int collect(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t k, uint32_t
     return SVS_OK;
 }
 
-extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mode) {
+static int collect(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items,
+                   uint32_t* ids, int64_t* scores, uint32_t* count, int32_t* status) {
+    (void)ctx;
+    uint32_t rng_flag = 0;
+    SVS_TRY(collect_async(sl, nq, k, n_items, ids, scores, count, status, &rng_flag));
+    HIP_TRY(hipStreamSynchronize(sl.stream));
+    SVS_TRY(prof_collect(sl));
+    return check_rng_flag(rng_flag);
+}
+
+// Kernel, tile variant and plan for a batch run in `mode`.
+int svs::plan_for_mode(svs_index* idx, int mode, uint32_t nq, const uint32_t* probe_ptr,
+                  const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
+                  int& kernel, int& variant) {
+    kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
+    variant = ctx_variant_for(idx->ctx, nq);
+    uint32_t trows, tqg;
+    tile_shape(kernel, variant, trows, tqg);
+    SVS_TRY(make_plan(idx, nq, probe_ptr, probe_lists, verify, pl, trows, tqg));
+    for (const Item& it : pl.items)
+        if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
+            return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
+    return SVS_OK;
+}
+
+// dev: v_q, q_seeds and the outputs are device pointers (checked for
+// canonical values on the device); otherwise host pointers.
+static int score_batch_impl(svs_index* idx, const svs_query_batch* b, int mode, bool dev) {
     if (!idx || !b) return fail(SVS_ERR_ARG, "null argument");
     if (b->n_queries == 0) return SVS_OK;
     if (!b->v_q || !b->q_seeds || !b->probe_ptr || !b->probe_lists || !b->out_ids ||
@@ -1089,23 +1188,28 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
     svs_ctx* ctx = idx->ctx;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    SVS_TRY(check_canonical(b->v_q, (size_t)b->n_queries * N, "v_q"));
+    uint32_t* d_badv = nullptr;
+    if (dev) {
+        SVS_TRY(ctx->io_b.ensure(ctx, 64));
+        d_badv = (uint32_t*)ctx->io_b.ptr;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, d_badv, 1u, 0u);
+        const uint64_t n = (uint64_t)b->n_queries * N;
+        hipLaunchKernelGGL(k_check_canonical, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 4096)),
+                           dim3(256), 0, ctx->stream, b->v_q, n, d_badv);
+        HIP_TRY(hipGetLastError());
+    } else {
+        SVS_TRY(check_canonical(b->v_q, (size_t)b->n_queries * N, "v_q"));
+    }
     const uint32_t nq_total = b->n_queries;
     const uint32_t step = mode == SVS_MODE_PLAIN ? 1 : nq_total;
     for (uint32_t q0 = 0; q0 < nq_total; q0 += step) {
         const uint32_t nq = std::min(step, nq_total - q0);
         std::vector<uint32_t> pptr(nq + 1);
         for (uint32_t q = 0; q <= nq; ++q) pptr[q] = b->probe_ptr[q0 + q] - b->probe_ptr[q0];
-        const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
-        const int variant = ctx->tile_variant[nq_class(nq)];
-        uint32_t trows, tqg;
-        tile_shape(kernel, variant, trows, tqg);
         BatchPlan pl;
-        SVS_TRY(make_plan(idx, nq, pptr.data(), b->probe_lists + b->probe_ptr[q0],
-                          b->verify ? b->verify + q0 : nullptr, pl, trows, tqg));
-        for (const Item& it : pl.items)
-            if (it.m && !idx->lists[it.list].d.mp_t32 && !idx->lists[it.list].d.mp_t64)
-                return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", it.list);
+        int kernel, variant;
+        SVS_TRY(plan_for_mode(idx, mode, nq, pptr.data(), b->probe_lists + b->probe_ptr[q0],
+                              b->verify ? b->verify + q0 : nullptr, pl, kernel, variant));
         SVS_TRY(stage_queries(ctx, *ctx, nq, b->v_q + (size_t)q0 * N,
                               b->q_seeds + (size_t)q0 * 32));
         SVS_TRY(run_plan(idx, *ctx, pl, b->k, false, kernel, true, variant));
@@ -1113,7 +1217,20 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
                         b->out_scores + (size_t)q0 * b->k, b->out_count + q0,
                         b->out_status + q0));
     }
+    if (dev) {
+        uint32_t bad = 0;
+        HIP_TRY(hipMemcpy(&bad, d_badv, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (bad) return fail(SVS_ERR_ARG, "v_q holds a non-canonical value (>= p)");
+    }
     return SVS_OK;
+}
+
+extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mode) {
+    return score_batch_impl(idx, b, mode, false);
+}
+
+extern "C" int svs_score_batch_dev(svs_index* idx, const svs_query_batch* b, int mode) {
+    return score_batch_impl(idx, b, mode, true);
 }
 
 // Times every block-tile variant on a real batch (MODE_BATCHED, `reps` timed
@@ -1178,7 +1295,7 @@ extern "C" int svs_autotune(svs_index* idx, const svs_query_batch* b, uint32_t r
 static int one_item_plan(svs_index* idx, uint32_t list, BatchPlan& pl, bool verify) {
     uint32_t pptr[2] = {0, 1};
     uint8_t v = verify ? 1 : 0;
-    return make_plan(idx, 1, pptr, &list, &v, pl);
+    return make_plan(idx, 1, pptr, &list, &v, pl, PB_ROWS, PB_QG);
 }
 
 static int list_ok(svs_index* idx, uint32_t list) {
@@ -1292,8 +1409,9 @@ extern "C" int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r,
     hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
                        ctx->stream, (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr,
                        1u, pre, pre + 2, (const uint64_t*)ctx->q_venc.ptr,
-                       (const uint64_t*)ctx->r.ptr, (const uint32_t*)ctx->q_vseed.ptr,
-                       (const uint8_t*)ctx->flags.ptr, (uint64_t*)ctx->partials.ptr, d_flag);
+                       (const uint64_t*)ctx->r.ptr, (const uint64_t*)nullptr,
+                       (const uint32_t*)ctx->q_vseed.ptr, (const uint8_t*)ctx->flags.ptr,
+                       (uint64_t*)ctx->partials.ptr, d_flag);
     hipLaunchKernelGGL(k_verify_finalize, dim3(1), dim3(64), 0, ctx->stream,
                        (const Item*)ctx->items.ptr, 1u, pre, (const uint8_t*)ctx->flags.ptr,
                        (const uint64_t*)ctx->partials.ptr, d_verdict);

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -81,12 +82,12 @@ struct svs_slot {
     hipStream_t stream = nullptr;
     svs::DevBuf items, prefixes, flags;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
-    svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part;
+    svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part, rv;
     svs_prof prof;
     std::vector<svs::DevBuf*> slot_scratch() {
         return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
                 &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &r, &scores, &partials, &verdict,
-                &out, &mtiles, &q_corr, &dense, &topk_part};
+                &out, &mtiles, &q_corr, &dense, &topk_part, &rv};
     }
 };
 
@@ -114,6 +115,8 @@ struct svs_ctx : svs_slot {
     // chosen block-tile variant of the MFMA path per batch-size class
     // (n <= 16, <= 64, <= 256, larger)
     int tile_variant[4] = {0, 0, 0, 0};
+    // svs_dev_alloc buffers and their sizes
+    std::unordered_map<void*, size_t> user_allocs;
     void note_peak();
     std::vector<svs::DevBuf*> all_scratch() {
         std::vector<svs::DevBuf*> v = slot_scratch();
@@ -130,3 +133,43 @@ struct svs_index {
     svs::ListDev* lists_dev = nullptr;
     std::vector<svs::ListHost> lists;
 };
+
+// ---------------------------------------------------------------- batches
+namespace svs {
+
+struct BatchPlan {
+    uint32_t n_queries = 0, n_items = 0, n_chunks = 0, n_cands = 0;
+    uint64_t distinct_bytes = 0;  // sum of m_c * n * 8 over distinct lists
+    bool any_verify = false;
+    bool planes_ok = true;  // every probed list has the MFMA decode planes
+    std::vector<Item> items;
+    std::vector<uint32_t> item_ptr, chunk_ptr, cand_ptr;
+    std::vector<uint32_t> qchunk_ptr;  // per query: its top-k wave chunks
+    uint32_t n_tchunks = 0;
+    std::vector<uint8_t> item_verify;
+    // MFMA product: items grouped by list (probe order kept within a list)
+    // and work tiles of MT_ROWS rows x up to MT_QG of those items.
+    std::vector<uint32_t> gitems;
+    std::vector<MTile> tiles;
+    // the same over the items to verify (the Freivalds recompute)
+    std::vector<uint32_t> vgitems;
+    std::vector<MTile> vtiles;
+};
+
+int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
+              const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
+              uint32_t tile_rows, uint32_t tile_qg);
+int plan_for_mode(svs_index* idx, int mode, uint32_t nq, const uint32_t* probe_ptr,
+                  const uint32_t* probe_lists, const uint8_t* verify, BatchPlan& pl,
+                  int& kernel, int& variant);
+// Encode + product + verify + decode + top-k of one planned batch whose
+// query inputs are staged on the slot; asynchronous on the slot's stream.
+int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k, bool product_only,
+             int kernel, bool encode, int variant);
+int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
+                  const uint8_t* seeds);
+int collect_async(svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,
+                  int64_t* scores, uint32_t* count, int32_t* status, uint32_t* rng_flag);
+int check_rng_flag(uint32_t rng_flag);
+
+}  // namespace svs

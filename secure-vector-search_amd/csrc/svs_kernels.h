@@ -96,7 +96,8 @@ __global__ void __launch_bounds__(ROW_THREADS)
 k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ items,
                 uint32_t n_items, const uint32_t* __restrict__ chunk_ptr,
                 const uint32_t* __restrict__ cand_ptr, const uint64_t* __restrict__ venc,
-                const uint64_t* __restrict__ r_claim, const uint32_t* __restrict__ vseed_words,
+                const uint64_t* __restrict__ r_claim, const uint64_t* __restrict__ r_check,
+                const uint32_t* __restrict__ vseed_words,
                 const uint8_t* __restrict__ item_verify, uint64_t* __restrict__ partials,
                 uint32_t* __restrict__ rng_flag) {
     const uint32_t lane = threadIdx.x % WAVE;
@@ -133,8 +134,10 @@ k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ item
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(venc + (size_t)item.q * N);
         ulonglong2 v[8];
+        if (!r_check) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = vq[j * WAVE + lane];
+            for (int j = 0; j < 8; ++j) v[j] = vq[j * WAVE + lane];
+        }
         const uint64_t* rc = r_claim + cand_ptr[it] + row0;
         // lane t accumulates trial t's sums (lanes >= LAMBDA idle in this part)
         Acc128 ur, zv;
@@ -142,17 +145,22 @@ k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ item
         zv.zero();
         bool rejected = false;
         for (uint32_t i = 0; i < rows; ++i) {
-            const ulonglong2* row =
-                reinterpret_cast<const ulonglong2*>(L.menc + (size_t)(row0 + i) * N);
-            Acc128 acc;
-            acc.zero();
+            uint64_t d;
+            if (r_check) {
+                d = r_check[cand_ptr[it] + row0 + i];  // recomputed M_enc[i] . v_enc
+            } else {
+                const ulonglong2* row =
+                    reinterpret_cast<const ulonglong2*>(L.menc + (size_t)(row0 + i) * N);
+                Acc128 acc;
+                acc.zero();
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                ulonglong2 a = row[j * WAVE + lane];
-                acc.mac(a.x, v[j].x);
-                acc.mac(a.y, v[j].y);
+                for (int j = 0; j < 8; ++j) {
+                    ulonglong2 a = row[j * WAVE + lane];
+                    acc.mac(a.x, v[j].x);
+                    acc.mac(a.y, v[j].y);
+                }
+                d = wave_sum_mod(acc.reduce());
             }
-            const uint64_t d = wave_sum_mod(acc.reduce());
             if (lane < LAMBDA) {
                 const uint64_t fb = (2ull * ((uint64_t)lane * m + row0)) / 16;
                 const uint64_t wpos = 2ull * ((uint64_t)lane * m + row0 + i) - fb * 16;
@@ -1481,6 +1489,16 @@ k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n1
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
          i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// Sets *bad when any of v[0..n) is not canonical (>= p).
+__global__ void k_check_canonical(const uint64_t* __restrict__ v, uint64_t n,
+                                  uint32_t* __restrict__ bad) {
+    bool any = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        any |= v[i] >= P;
+    if (any) atomicOr(bad, 1u);
 }
 
 // Fill kernel (used instead of hipMemsetAsync so a captured graph stays valid).

@@ -1,0 +1,495 @@
+// svs_server.cpp — the serving loop: continuous batching of BN-IVF queries
+// over resident indexes (one or more tenants sharing a context).
+//
+// The reference serves one query per `score` call (scorer-bntm/src/ivf.rs:
+// 746-822, eval.rs:2339-2344) and batches only when the caller hands it a
+// chunk (eval.rs:2428-2433). Here requests are queued as they arrive; a
+// dispatcher thread packs whatever is pending into a batch whenever one of
+// the server's slots (device scratch + stream) is free, subject to a
+// per-batch memory budget, and launches it asynchronously. Batches on
+// different slots overlap on the device; results are copied back into
+// pinned memory and handed to the waiting callers when the slot's event
+// fires. A batch's k is the largest k of its requests: top-k lists are
+// prefixes of each other under the (score desc, position asc) order, so
+// each request keeps the first k of its own list.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "svs_internal.h"
+
+using namespace svs;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_s() {
+    return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t ticket = 0;
+    uint32_t tenant = 0;
+    std::vector<uint64_t> v_q;
+    uint8_t seed[32];
+    std::vector<uint32_t> probes;
+    uint8_t verify = 0;
+    uint32_t k = 0;
+    double t_start = 0;  // submit time, or the scheduled arrival
+    // results
+    bool done = false;
+    int rc = SVS_OK;
+    std::string err;
+    std::vector<uint32_t> ids;
+    std::vector<int64_t> scores;
+    uint32_t count = 0;
+    int32_t status = 0;
+    uint32_t batch_size = 0;
+    double t_done = 0;
+};
+
+// Pinned host buffer that grows geometrically.
+template <typename T>
+struct Pinned {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n) {
+        if (n <= cap) return SVS_OK;
+        size_t want = std::max(n, cap + cap / 2);
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&p, sizeof(T) * want, hipHostMallocDefault));
+        cap = want;
+        return SVS_OK;
+    }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct Slot {
+    svs_slot sl;
+    hipEvent_t done_ev = nullptr;
+    bool busy = false;
+    uint64_t seq = 0;  // launch order
+    std::vector<Request*> reqs;
+    BatchPlan plan;
+    uint32_t k = 0;
+    Pinned<uint64_t> h_vq;
+    Pinned<uint8_t> h_seeds;
+    Pinned<uint32_t> h_ids, h_count, h_flag;
+    Pinned<int64_t> h_scores;
+    Pinned<int32_t> h_status;
+};
+
+}  // namespace
+
+struct svs_server {
+    svs_ctx* ctx = nullptr;
+    std::vector<svs_index*> tenants;
+    svs_server_config cfg{};
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> by_ticket;
+    uint64_t next_ticket = 1, next_seq = 1;
+    bool stop = false;
+    std::thread loop;
+    // counters (under mu)
+    uint64_t submitted = 0, completed = 0, batches = 0, batched_queries = 0;
+    uint32_t max_batch_seen = 0, max_in_flight = 0;
+
+    void run();
+    void launch(Slot& s, std::vector<Request*>& batch);
+    void retire(Slot& s);
+    uint64_t batch_bytes(const Request& r) const;
+};
+
+// Device scratch a request adds to a batch: per candidate row the product,
+// score and dense-term words (24 B) plus the top-k partials; per query the
+// encode state (v_q, v_enc, [g ; lt], T, byte planes, ...), about 30 KiB.
+uint64_t svs_server::batch_bytes(const Request& r) const {
+    const svs_index* idx = tenants[r.tenant];
+    uint64_t rows = 0;
+    for (uint32_t c : r.probes)
+        if (c < idx->lists.size()) rows += idx->lists[c].m;
+    return rows * 26 + 30 * 1024 + (uint64_t)r.k * 16;
+}
+
+void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
+    const uint32_t nq = (uint32_t)batch.size();
+    svs_index* idx = tenants[batch[0]->tenant];
+    uint32_t k = 0;
+    for (Request* r : batch) k = std::max(k, r->k);
+    int rc = SVS_OK;
+    std::vector<uint32_t> pptr(nq + 1, 0), plist;
+    std::vector<uint8_t> ver(nq);
+    for (uint32_t q = 0; q < nq; ++q) {
+        plist.insert(plist.end(), batch[q]->probes.begin(), batch[q]->probes.end());
+        pptr[q + 1] = (uint32_t)plist.size();
+        ver[q] = batch[q]->verify;
+    }
+    if (plist.empty()) plist.push_back(0);  // (pptr says no probes)
+    int kernel = 0, variant = 0;
+    rc = plan_for_mode(idx, cfg.mode, nq, pptr.data(), plist.data(), ver.data(), s.plan, kernel,
+                       variant);
+    if (rc == SVS_OK) rc = s.h_vq.ensure((size_t)nq * N);
+    if (rc == SVS_OK) rc = s.h_seeds.ensure((size_t)nq * 32);
+    if (rc == SVS_OK) rc = s.h_ids.ensure((size_t)nq * k);
+    if (rc == SVS_OK) rc = s.h_scores.ensure((size_t)nq * k);
+    if (rc == SVS_OK) rc = s.h_count.ensure(nq);
+    if (rc == SVS_OK) rc = s.h_status.ensure(nq);
+    if (rc == SVS_OK) rc = s.h_flag.ensure(1);
+    if (rc == SVS_OK) {
+        for (uint32_t q = 0; q < nq; ++q) {
+            std::memcpy(s.h_vq.p + (size_t)q * N, batch[q]->v_q.data(), sizeof(uint64_t) * N);
+            std::memcpy(s.h_seeds.p + (size_t)q * 32, batch[q]->seed, 32);
+        }
+        std::lock_guard<std::mutex> g(ctx->mu);
+        rc = hipSetDevice(ctx->device) == hipSuccess ? SVS_OK : SVS_ERR_HIP;
+        if (rc == SVS_OK) rc = stage_queries(ctx, s.sl, nq, s.h_vq.p, s.h_seeds.p);
+        if (rc == SVS_OK) rc = run_plan(idx, s.sl, s.plan, k, false, kernel, true, variant);
+        if (rc == SVS_OK)
+            rc = collect_async(s.sl, nq, k, s.plan.n_items, s.h_ids.p, s.h_scores.p,
+                               s.h_count.p, s.h_status.p, s.h_flag.p);
+        if (rc == SVS_OK && hipEventRecord(s.done_ev, s.sl.stream) != hipSuccess)
+            rc = fail(SVS_ERR_HIP, "hipEventRecord failed");
+    }
+    s.reqs = batch;
+    s.k = k;
+    if (rc != SVS_OK) {
+        char buf[512];
+        svs_last_error(buf, sizeof(buf));
+        const double t = now_s();
+        std::lock_guard<std::mutex> g(mu);
+        for (Request* r : batch) {
+            r->rc = rc;
+            r->err = buf;
+            r->batch_size = nq;
+            r->t_done = t;
+            r->done = true;
+            ++completed;
+        }
+        s.reqs.clear();
+        cv_done.notify_all();
+        return;
+    }
+    s.busy = true;
+    s.seq = next_seq++;
+    std::lock_guard<std::mutex> g(mu);
+    ++batches;
+    batched_queries += nq;
+    max_batch_seen = std::max(max_batch_seen, nq);
+    uint32_t in_flight = 0;
+    for (auto& x : slots) in_flight += x->busy ? 1 : 0;
+    max_in_flight = std::max(max_in_flight, in_flight);
+}
+
+void svs_server::retire(Slot& s) {
+    const hipError_t e = hipEventSynchronize(s.done_ev);
+    int rc = SVS_OK;
+    std::string err;
+    if (e != hipSuccess) {
+        rc = fail(SVS_ERR_HIP, "batch failed on the device: %s", hipGetErrorString(e));
+    } else {
+        rc = check_rng_flag(*s.h_flag.p);
+    }
+    if (rc != SVS_OK) {
+        char buf[512];
+        svs_last_error(buf, sizeof(buf));
+        err = buf;
+    }
+    const double t = now_s();
+    const uint32_t nq = (uint32_t)s.reqs.size();
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (uint32_t q = 0; q < nq; ++q) {
+            Request* r = s.reqs[q];
+            r->rc = rc;
+            r->err = err;
+            r->batch_size = nq;
+            if (rc == SVS_OK) {
+                r->count = std::min(s.h_count.p[q], r->k);
+                r->status = s.h_status.p[q];
+                r->ids.assign(s.h_ids.p + (size_t)q * s.k, s.h_ids.p + (size_t)q * s.k + r->count);
+                r->scores.assign(s.h_scores.p + (size_t)q * s.k,
+                                 s.h_scores.p + (size_t)q * s.k + r->count);
+            }
+            r->t_done = t;
+            r->done = true;
+            ++completed;
+        }
+    }
+    s.reqs.clear();
+    s.busy = false;
+    cv_done.notify_all();
+}
+
+void svs_server::run() {
+    hipSetDevice(ctx->device);
+    for (;;) {
+        bool progressed = false;
+        // retire finished batches
+        for (auto& sp : slots)
+            if (sp->busy && hipEventQuery(sp->done_ev) != hipErrorNotReady) {
+                retire(*sp);
+                progressed = true;
+            }
+        Slot* free_slot = nullptr;
+        for (auto& sp : slots)
+            if (!sp->busy) {
+                free_slot = sp.get();
+                break;
+            }
+        std::vector<Request*> batch;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            bool any_busy = false;
+            for (auto& sp : slots) any_busy |= sp->busy;
+            if (stop && pending.empty() && !any_busy) return;
+            if (free_slot && !pending.empty()) {
+                // FIFO over requests; a batch holds one tenant's requests
+                const uint32_t tenant = pending.front()->tenant;
+                uint64_t bytes = 0;
+                for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch;) {
+                    Request* r = *it;
+                    if (r->tenant != tenant) {
+                        ++it;
+                        continue;
+                    }
+                    const uint64_t b = batch_bytes(*r);
+                    if (!batch.empty() && bytes + b > cfg.slot_bytes) break;
+                    bytes += b;
+                    batch.push_back(r);
+                    it = pending.erase(it);
+                }
+            } else if (!progressed) {
+                if (!any_busy) {
+                    cv_submit.wait(lk, [&] { return stop || !pending.empty(); });
+                    continue;
+                }
+                if (!free_slot) {
+                    // every slot busy: block on the oldest batch outside the lock
+                    lk.unlock();
+                    Slot* oldest = nullptr;
+                    for (auto& sp : slots)
+                        if (sp->busy && (!oldest || sp->seq < oldest->seq)) oldest = sp.get();
+                    if (oldest) hipEventSynchronize(oldest->done_ev);
+                    continue;
+                }
+                cv_submit.wait_for(lk, std::chrono::microseconds(50));
+                continue;
+            }
+        }
+        if (!batch.empty()) launch(*free_slot, batch);
+    }
+}
+
+extern "C" int svs_server_create(svs_index* const* tenants, uint32_t n_tenants,
+                                 const svs_server_config* cfg, svs_server** out) {
+    if (!tenants || n_tenants == 0 || !cfg || !out) return fail(SVS_ERR_ARG, "null argument");
+    if (cfg->max_batch == 0 || cfg->max_batch > 4096)
+        return fail(SVS_ERR_ARG, "max_batch = %u must be in 1..4096", cfg->max_batch);
+    if (cfg->slots == 0 || cfg->slots > 8)
+        return fail(SVS_ERR_ARG, "slots = %u must be in 1..8", cfg->slots);
+    if (cfg->mode != SVS_MODE_BATCHED && cfg->mode != SVS_MODE_BATCHED_VALU)
+        return fail(SVS_ERR_ARG, "mode must be SVS_MODE_BATCHED or SVS_MODE_BATCHED_VALU");
+    svs_ctx* ctx = nullptr;
+    for (uint32_t t = 0; t < n_tenants; ++t) {
+        if (!tenants[t]) return fail(SVS_ERR_ARG, "tenant %u is null", t);
+        if (ctx && tenants[t]->ctx != ctx) return fail(SVS_ERR_ARG, "tenants on different contexts");
+        ctx = tenants[t]->ctx;
+    }
+    auto s = std::make_unique<svs_server>();
+    s->ctx = ctx;
+    s->tenants.assign(tenants, tenants + n_tenants);
+    s->cfg = *cfg;
+    if (s->cfg.slot_bytes == 0) s->cfg.slot_bytes = 1ull << 30;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (uint32_t i = 0; i < cfg->slots; ++i) {
+            auto sl = std::make_unique<Slot>();
+            HIP_TRY(hipStreamCreateWithFlags(&sl->sl.stream, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&sl->done_ev, hipEventDisableTiming));
+            s->slots.push_back(std::move(sl));
+        }
+    }
+    ctx_retain(ctx);
+    svs_server* raw = s.release();
+    raw->loop = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return SVS_OK;
+}
+
+extern "C" int svs_server_destroy(svs_server* s) {
+    if (!s) return SVS_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_submit.notify_all();
+    if (s->loop.joinable()) s->loop.join();
+    svs_ctx* ctx = s->ctx;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        for (auto& sp : s->slots) {
+            hipStreamSynchronize(sp->sl.stream);
+            for (DevBuf* b : sp->sl.slot_scratch()) b->release(ctx);
+            if (sp->sl.prof.created)
+                for (auto& e : sp->sl.prof.ev) hipEventDestroy(e);
+            hipEventDestroy(sp->done_ev);
+            hipStreamDestroy(sp->sl.stream);
+            sp->h_vq.release();
+            sp->h_seeds.release();
+            sp->h_ids.release();
+            sp->h_count.release();
+            sp->h_flag.release();
+            sp->h_scores.release();
+            sp->h_status.release();
+        }
+    }
+    for (auto& kv : s->by_ticket) delete kv.second;
+    delete s;
+    ctx_release(ctx);
+    return SVS_OK;
+}
+
+extern "C" int svs_server_submit(svs_server* s, const svs_request* r, uint64_t* ticket) {
+    if (!s || !r || !ticket || !r->v_q || !r->q_seed || (r->n_probes && !r->probes))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (r->tenant >= s->tenants.size()) return fail(SVS_ERR_ARG, "tenant %u out of range", r->tenant);
+    if (r->k == 0 || r->k > 4096) return fail(SVS_ERR_ARG, "k = %u must be in 1..4096", r->k);
+    for (int i = 0; i < N; ++i)
+        if (r->v_q[i] >= P) return fail(SVS_ERR_ARG, "v_q[%d] is not canonical", i);
+    auto q = std::make_unique<Request>();
+    q->tenant = r->tenant;
+    q->v_q.assign(r->v_q, r->v_q + N);
+    std::memcpy(q->seed, r->q_seed, 32);
+    q->probes.assign(r->probes, r->probes + r->n_probes);
+    q->verify = r->verify ? 1 : 0;
+    q->k = r->k;
+    q->t_start = r->arrival_s > 0 ? r->arrival_s : now_s();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(SVS_ERR_STATE, "server is shutting down");
+        q->ticket = s->next_ticket++;
+        *ticket = q->ticket;
+        s->by_ticket[q->ticket] = q.get();
+        s->pending.push_back(q.release());
+        ++s->submitted;
+    }
+    s->cv_submit.notify_one();
+    return SVS_OK;
+}
+
+extern "C" int svs_server_wait(svs_server* s, uint64_t ticket, svs_result* out) {
+    if (!s || !out) return fail(SVS_ERR_ARG, "null argument");
+    Request* r = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->by_ticket.find(ticket);
+        if (it == s->by_ticket.end()) return fail(SVS_ERR_ARG, "unknown ticket %llu",
+                                                  (unsigned long long)ticket);
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->by_ticket.erase(it);
+    }
+    std::unique_ptr<Request> hold(r);
+    out->count = r->count;
+    out->status = r->status;
+    out->rc = r->rc;
+    out->batch_size = r->batch_size;
+    out->latency_ms = (r->t_done - r->t_start) * 1e3;
+    if (out->ids) std::copy(r->ids.begin(), r->ids.end(), out->ids);
+    if (out->scores) std::copy(r->scores.begin(), r->scores.end(), out->scores);
+    if (r->rc != SVS_OK) return fail(r->rc, "%s", r->err.c_str());
+    return SVS_OK;
+}
+
+extern "C" int svs_server_stats(svs_server* s, svs_server_counters* out) {
+    if (!s || !out) return fail(SVS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->batches = s->batches;
+    out->batched_queries = s->batched_queries;
+    out->max_batch = s->max_batch_seen;
+    out->max_in_flight = s->max_in_flight;
+    out->pending = (uint32_t)s->pending.size();
+    return SVS_OK;
+}
+
+// Open-loop Poisson load: request i is submitted at its scheduled arrival
+// t0 + sum of exponential gaps (rate qps, seeded), whatever the state of
+// earlier requests; latency runs from the scheduled arrival to completion.
+extern "C" int svs_server_run_poisson(svs_server* s, const svs_request* reqs, uint32_t n,
+                                      double qps, uint64_t seed, svs_result* results,
+                                      double* wall_s) {
+    if (!s || (n && (!reqs || !results)) || !(qps > 0)) return fail(SVS_ERR_ARG, "bad argument");
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(qps);
+    std::vector<double> at(n);
+    const double t0 = now_s() + 1e-3;
+    double t = t0;
+    for (uint32_t i = 0; i < n; ++i) {
+        t += gap(rng);
+        at[i] = t;
+    }
+    std::vector<uint64_t> tickets(n, 0);
+    std::vector<int> src(n, SVS_OK);
+    std::thread submitter([&] {
+        for (uint32_t i = 0; i < n; ++i) {
+            const double wait = at[i] - now_s();
+            if (wait > 2e-4)
+                std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+            while (now_s() < at[i]) {
+            }
+            svs_request r = reqs[i];
+            r.arrival_s = at[i];
+            uint64_t tk = 0;
+            src[i] = svs_server_submit(s, &r, &tk);
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                tickets[i] = tk;
+            }
+            s->cv_done.notify_all();
+        }
+    });
+    int rc = SVS_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t tk = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_done.wait(lk, [&] { return tickets[i] != 0 || src[i] != SVS_OK; });
+            tk = tickets[i];
+        }
+        if (src[i] != SVS_OK) {
+            results[i].rc = src[i];
+            if (rc == SVS_OK) rc = src[i];
+            continue;
+        }
+        const int w = svs_server_wait(s, tk, &results[i]);
+        if (w != SVS_OK && rc == SVS_OK) rc = w;
+    }
+    submitter.join();
+    if (wall_s) *wall_s = now_s() - t0;
+    return rc;
+}

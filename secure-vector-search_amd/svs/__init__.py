@@ -5,8 +5,9 @@ SimplePIR answer). Native code: libsvs_gpu.so (include/svs_gpu.h)."""
 from ._lib import GpuError, BudgetTooSmall, lib, LIB_PATH, EXPORTS  # noqa: F401
 from .bntm import (  # noqa: F401
     BnTmConfig, BnTmError, BnTmHandle, BnTmIvfConfig, BnTmIvfHandle, BnTmIvfScorer, BnTmScorer,
-    BuildOutcome, CommunicationCost, Context, DimensionMismatch, DimensionTooLarge, EmptyCorpus,
+    BuildOutcome, CommunicationCost, Context, DeviceArray, DimensionMismatch, DimensionTooLarge, EmptyCorpus,
     GpuIndex, Hit, InvalidNprobe, IvfIndex, TooManyCentroids, VerificationFailed, default_context,
     derive_h_seed, domain_separate, fp_to_signed, generate_subspace, probe_route, quantise,
     regenerate_h_s,
 )
+from .serve import Server, ServedResult  # noqa: F401,E402

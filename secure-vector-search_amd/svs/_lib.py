@@ -32,6 +32,9 @@ EXPORTS = [
     "svs_tiptoe_destroy", "svs_tiptoe_stored_bytes", "svs_tiptoe_answer_u32",
     "svs_tiptoe_answer_u32_dev", "svs_tiptoe_answer_u64", "svs_ctx_copy_probe",
     "svs_ctx_set_tile_variant", "svs_ctx_get_tile_variant", "svs_autotune",
+    "svs_server_create", "svs_server_destroy", "svs_server_submit", "svs_server_wait",
+    "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
+    "svs_dev_free", "svs_memcpy",
 ]
 TILE_VARIANTS = 5
 
@@ -73,6 +76,29 @@ class QueryBatch(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_uint32), ("slots", C.c_uint32), ("slot_bytes", C.c_uint64),
+                ("mode", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("tenant", C.c_uint32), ("v_q", C.c_void_p), ("q_seed", C.c_void_p),
+                ("probes", C.c_void_p), ("n_probes", C.c_uint32), ("verify", C.c_uint8),
+                ("k", C.c_uint32), ("arrival_s", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("ids", C.c_void_p), ("scores", C.c_void_p), ("count", C.c_uint32),
+                ("status", C.c_int32), ("rc", C.c_int32), ("batch_size", C.c_uint32),
+                ("latency_ms", C.c_double)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("batched_queries", C.c_uint64), ("max_batch", C.c_uint32),
+                ("max_in_flight", C.c_uint32), ("pending", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -101,6 +127,10 @@ def lib() -> C.CDLL:
             "svs_verify": ([vp, u32, vp, vp, vp, u32, C.POINTER(C.c_int32)], i32),
             "svs_decode": ([vp, u32, vp, vp, vp, vp, vp, u32, vp], i32),
             "svs_score_batch": ([vp, C.POINTER(QueryBatch), i32], i32),
+            "svs_score_batch_dev": ([vp, C.POINTER(QueryBatch), i32], i32),
+            "svs_dev_alloc": ([vp, u64, C.POINTER(vp)], i32),
+            "svs_dev_free": ([vp, vp], i32),
+            "svs_memcpy": ([vp, vp, vp, u64], i32),
             "svs_keygen_subspace": ([vp, vp], i32),
             "svs_keygen_derive_h_seed": ([u64, u64, vp], i32),
             "svs_keygen_domain_separate": ([vp, vp, sz, vp], i32),
@@ -125,6 +155,13 @@ def lib() -> C.CDLL:
                                           C.POINTER(u32), C.POINTER(u32)], i32),
             "svs_autotune": ([vp, C.POINTER(QueryBatch), u32, C.POINTER(i32),
                               C.POINTER(C.c_double)], i32),
+            "svs_server_create": ([vp, u32, C.POINTER(ServerConfig), C.POINTER(vp)], i32),
+            "svs_server_destroy": ([vp], i32),
+            "svs_server_submit": ([vp, C.POINTER(Request), C.POINTER(u64)], i32),
+            "svs_server_wait": ([vp, u64, C.POINTER(Result)], i32),
+            "svs_server_stats": ([vp, C.POINTER(ServerCounters)], i32),
+            "svs_server_run_poisson": ([vp, vp, u32, C.c_double, u64, vp,
+                                        C.POINTER(C.c_double)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -221,12 +221,15 @@ class Context:
 
 _default_ctx: Context | None = None
 _live_indexes: "weakref.WeakSet[GpuIndex]" = weakref.WeakSet()
+_live_servers: weakref.WeakSet = weakref.WeakSet()
 
 
 @atexit.register
 def _close_all():
-    """Release indexes before contexts at interpreter exit (finalisers run in
-    no particular order then)."""
+    """Release servers, then indexes, then contexts at interpreter exit
+    (finalisers run in no particular order then)."""
+    for srv in list(_live_servers):
+        srv.close()
     for idx in list(_live_indexes):
         idx.close()
     if _default_ctx is not None:
@@ -361,6 +364,20 @@ class GpuIndex:
         b, _keep, out = self._batch(v_q, q_seeds, probe_ptr, probe_lists, verify, k)
         check(L.lib().svs_score_batch(self.h, C.byref(b), mode))
         return out
+
+    def score_batch_dev(self, n_queries, v_q_dev, q_seeds_dev, probe_ptr, probe_lists, verify,
+                        k, ids_dev, scores_dev, count_dev, status_dev, mode=L.MODE_BATCHED):
+        """svs_score_batch_dev: query inputs and outputs already in device
+        memory (device addresses as ints, e.g. torch's data_ptr()); the probe
+        plan and verification flags stay on the host."""
+        pp = np.ascontiguousarray(probe_ptr, np.uint32)
+        pl = np.ascontiguousarray(probe_lists, np.uint32)
+        if pl.size == 0:
+            pl = np.zeros(1, np.uint32)
+        ver = None if verify is None else np.ascontiguousarray(verify, np.uint8)
+        b = L.QueryBatch(n_queries, v_q_dev, q_seeds_dev, ptr(pp), ptr(pl), ptr(ver), k,
+                         ids_dev, scores_dev, count_dev, status_dev)
+        check(L.lib().svs_score_batch_dev(self.h, C.byref(b), mode))
 
     def autotune(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k, reps: int = 2):
         """Times each block-tile variant on this batch and keeps the fastest
@@ -598,3 +615,49 @@ class BnTmIvfScorer:
         cost.cluster_response_bytes = sum(per) // len(per) if per else 0
         cost.effective_bytes_per_query = int(sizes.sum()) * N * FIELD_BYTES
         return _hits(ids[0], scores[0], min(int(count[0]), k)), cost
+
+
+class DeviceArray:
+    """A numpy-shaped buffer in HBM (svs_dev_alloc), for the device-resident
+    batch entry point; `ptr(offset)` is a device address."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = C.c_void_p()
+        check(L.lib().svs_dev_alloc(ctx.h, max(self.nbytes, 1), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_host(cls, ctx: Context, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        check(L.lib().svs_memcpy(ctx.h, d.h, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return d
+
+    def ptr(self, offset_bytes: int = 0) -> int:
+        return self.h.value + offset_bytes
+
+    def copy_from(self, a, offset_bytes: int = 0):
+        a = np.ascontiguousarray(a)
+        check(L.lib().svs_memcpy(self.ctx.h, C.c_void_p(self.ptr(offset_bytes)),
+                                 a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(L.lib().svs_memcpy(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.h,
+                                 self.nbytes))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            L.lib().svs_dev_free(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -539,3 +539,38 @@ def test_autotune_picks_a_variant(ctx, ivf_setup):
         assert all(x > 0 for x in ms)
     finally:
         ctx.set_tile_variant(0)
+
+
+def test_score_batch_dev_matches_host(ctx, ivf_setup):
+    """svs_score_batch_dev (device-resident queries and outputs) returns the
+    host entry point's results; a non-canonical v_q is rejected."""
+    import svs
+    s = ivf_setup
+    h = s["handle"]
+    probes = svs.probe_route(s["centres"], s["qs"], 4)
+    v_q = svs.quantise(s["qs"])
+    seeds = np.arange(12 * 32, dtype=np.uint64).astype(np.uint8).reshape(12, 32)
+    pp = np.arange(0, 12 * 4 + 1, 4, dtype=np.uint32)
+    ver = (np.arange(12) % 2).astype(np.uint8)
+    want = h.index.score_batch(v_q, seeds, pp, probes.reshape(-1), ver, 10)
+    dv = svs.DeviceArray.from_host(ctx, v_q)
+    ds = svs.DeviceArray.from_host(ctx, seeds)
+    ids = svs.DeviceArray(ctx, (12, 10), np.uint32)
+    sc = svs.DeviceArray(ctx, (12, 10), np.int64)
+    cnt = svs.DeviceArray(ctx, (12,), np.uint32)
+    st = svs.DeviceArray(ctx, (12,), np.int32)
+    say("score_batch_dev")
+    h.index.score_batch_dev(12, dv.ptr(), ds.ptr(), pp, probes.reshape(-1), ver, 10,
+                            ids.ptr(), sc.ptr(), cnt.ptr(), st.ptr())
+    assert np.array_equal(ids.to_host(), want[0])
+    assert np.array_equal(sc.to_host(), want[1])
+    assert np.array_equal(cnt.to_host(), want[2])
+    assert np.array_equal(st.to_host(), want[3])
+    bad = v_q.copy()
+    bad[3, 7] = np.uint64(O.P)
+    dv.copy_from(bad)
+    with pytest.raises(svs.GpuError):
+        h.index.score_batch_dev(12, dv.ptr(), ds.ptr(), pp, probes.reshape(-1), ver, 10,
+                                ids.ptr(), sc.ptr(), cnt.ptr(), st.ptr())
+    for d in (dv, ds, ids, sc, cnt, st):
+        d.close()
