@@ -254,11 +254,14 @@ def main():
     if args.tile_variant >= 0:
         ctx.set_tile_variant(args.tile_variant)
     elif not args.no_autotune:
-        s0 = slice(0, args.batch)
-        pp0 = np.arange(0, v_q[s0].shape[0] * args.nprobe + 1, args.nprobe, dtype=np.uint32)
-        best, tms = index.autotune(v_q[s0], seeds[s0], pp0, probes[s0].reshape(-1), verify[s0],
-                                   args.k, reps=2)
-        tune = dict(best=best, ms_per_variant=[round(x, 3) for x in tms])
+        # the timed batch size, plus the small-batch classes the server uses
+        tune = {}
+        for nb in sorted({args.batch, 16, 64}):
+            s0 = slice(0, nb)
+            pp0 = np.arange(0, v_q[s0].shape[0] * args.nprobe + 1, args.nprobe, dtype=np.uint32)
+            best, tms = index.autotune(v_q[s0], seeds[s0], pp0, probes[s0].reshape(-1),
+                                       verify[s0], args.k, reps=2)
+            tune[str(nb)] = dict(best=best, ms_per_variant=[round(x, 3) for x in tms])
         log(f"autotune: {tune}")
     variant = ctx.tile_variant(args.batch)
 
@@ -318,6 +321,11 @@ def main():
         ms = min(ctx.copy_probe(nb) for _ in range(3))
         copy = dict(bytes=nb, ms=round(ms, 4), gbs=round(2 * nb / (ms * 1e-3) / 1e9, 1),
                     note="k_copy_probe reads and writes `bytes`; gbs counts both")
+        ctx.read_probe(nb)
+        rms = min(ctx.read_probe(nb) for _ in range(3))
+        copy["read_probe"] = dict(bytes=nb, ms=round(rms, 4),
+                                  gbs=round(nb / (rms * 1e-3) / 1e9, 1),
+                                  note="k_read_probe: reads only, in the product's A pattern")
         log(f"copy probe: {copy}")
 
     cpu = None

@@ -88,6 +88,10 @@ int svs_ctx_profile(svs_ctx* ctx, int enable, int reset, svs_profile* out);
  * reference dispatch for calibrating HBM byte counters (FETCH_SIZE /
  * WRITE_SIZE) and the measured copy bandwidth. */
 int svs_ctx_copy_probe(svs_ctx* ctx, uint64_t bytes, double* ms);
+/* Read-only probe of `bytes` (a multiple of 8192) in the batched product's
+ * A-operand access pattern (64-byte segments of 16 rows of 8 KiB per load
+ * instruction): the HBM-counter calibration for that kernel. */
+int svs_ctx_read_probe(svs_ctx* ctx, uint64_t bytes, double* ms);
 
 /* ---- resident index -----------------------------------------------------
  * An index holds n_lists encrypted lists (a flat scorer is one list) and the

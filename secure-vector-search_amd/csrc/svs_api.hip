@@ -1551,6 +1551,31 @@ extern "C" int svs_compute_products_batch(svs_index* idx, uint32_t n_queries,
     return SVS_OK;
 }
 
+extern "C" int svs_ctx_read_probe(svs_ctx* ctx, uint64_t bytes, double* ms) {
+    if (!ctx || bytes == 0 || bytes % 8192)
+        return fail(SVS_ERR_ARG, "bytes must be a multiple of 8192");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(ctx->tmp_a.ensure(ctx, bytes));
+    const uint32_t blocks = ctx->n_cu * 8;
+    SVS_TRY(ctx->tmp_b.ensure(ctx, sizeof(uint32_t) * blocks * 256));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    hipLaunchKernelGGL(k_read_probe, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)ctx->tmp_a.ptr, bytes / 8192, (uint32_t*)ctx->tmp_b.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return SVS_OK;
+}
+
 extern "C" int svs_ctx_copy_probe(svs_ctx* ctx, uint64_t bytes, double* ms) {
     if (!ctx || bytes == 0 || bytes % 16) return fail(SVS_ERR_ARG, "bytes must be a multiple of 16");
     std::lock_guard<std::mutex> g(ctx->mu);

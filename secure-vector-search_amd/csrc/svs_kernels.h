@@ -1204,13 +1204,18 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 for (int rt = 0; rt < PB_RT; ++rt) a[rt] = ar[d][rt] ^ (int)0x80808080;
 #pragma unroll
                 for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * nxt);
+                // All B windows are read first (one LDS wait per step; the
+                // rows of absent queries hold stale data and are not used).
+                v4i b[QG];
+#pragma unroll
+                for (int j = 0; j < QG; ++j)
+                    b[j] = *reinterpret_cast<const v4i*>(&bw[d & 1][j][col][2 * grp]);
 #pragma unroll
                 for (int j = 0; j < QG; ++j) {
                     if (j < gn) {
-                        const v4i b = *reinterpret_cast<const v4i*>(&bw[d & 1][j][col][2 * grp]);
 #pragma unroll
                         for (int rt = 0; rt < PB_RT; ++rt)
-                            acc[rt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt], b,
+                            acc[rt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt], b[j],
                                                                                  acc[rt][j], 0, 0, 0);
                     }
                 }
@@ -1395,7 +1400,11 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
         }
-        // two-deep register ring over the K-steps
+        // Two-slot register ring over the K-steps: slot d's MFMAs run, then
+        // slot d is reloaded for the step two ahead (the tail re-loads the
+        // last step, in bounds), so one step's loads are always in flight
+        // behind the current step's MFMAs. The outer loop stays rolled so
+        // the compiler keeps that order.
         v4i ra[2][PB_RT][4], rb[2][1 + NCT];
         auto issue = [&](int slot, int kc) {
 #pragma unroll
@@ -1409,24 +1418,27 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             for (int ct = 0; ct < NCT; ++ct) rb[slot][1 + ct] = load16b(bv[ct] + 64 * kc);
         };
         issue(0, 0);
+        issue(1, 1);
+#pragma unroll 1
+        for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-            const int slot = kc & 1;
-            if (kc + 1 < KC) issue(slot ^ 1, kc + 1);
-            const v4i b_t = rb[slot][0] & tmask;
-#pragma unroll
-            for (int rt = 0; rt < PB_RT; ++rt)
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][a], b_t,
-                                                                        amt[rt][a], 0, 0, 0);
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const v4i b_v = rb[slot][1 + ct] & vmask[ct];
+            for (int slot = 0; slot < 2; ++slot) {
+                const v4i b_t = rb[slot][0] & tmask;
 #pragma unroll
                 for (int rt = 0; rt < PB_RT; ++rt)
-                    asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][3], b_v,
-                                                                         asv[rt][ct], 0, 0, 0);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][a], b_t,
+                                                                            amt[rt][a], 0, 0, 0);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) {
+                    const v4i b_v = rb[slot][1 + ct] & vmask[ct];
+#pragma unroll
+                    for (int rt = 0; rt < PB_RT; ++rt)
+                        asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][3], b_v,
+                                                                             asv[rt][ct], 0, 0, 0);
+                }
+                issue(slot, min(kc0 + slot + 2, KC - 1));
             }
         }
         // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i, column
@@ -1489,6 +1501,29 @@ k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n1
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
          i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// Read-only calibration probe in the product's A-operand pattern: a wave
+// owns 16 rows of 8 KiB; lane (row = lane % 16, grp = lane / 16) reads 16
+// bytes at row * 8192 + 64 s + 16 grp for s = 0..127, i.e. 64-byte segments
+// of 16 rows per instruction. XOR-folded so the loads are kept; one word per
+// thread is written.
+__global__ void __launch_bounds__(256)
+k_read_probe(const uint64_t* __restrict__ src, uint64_t rows, uint32_t* __restrict__ sink) {
+    const uint32_t lane = threadIdx.x % WAVE, grp = lane / 16, r16 = lane % 16;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+    uint32_t x = 0;
+    for (uint64_t w = wave; w * 16 < rows; w += nwaves) {
+        const uint64_t row = min(w * 16 + r16, rows - 1);
+        const uint64_t* base = src + row * (N) + 2 * grp;
+#pragma unroll 8
+        for (int st = 0; st < N / 8; ++st) {
+            const v4i v = load16(base + 8 * st);
+            x ^= (uint32_t)(v[0] ^ v[1] ^ v[2] ^ v[3]);
+        }
+    }
+    sink[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
 // Sets *bad when any of v[0..n) is not canonical (>= p).

@@ -34,7 +34,7 @@ EXPORTS = [
     "svs_ctx_set_tile_variant", "svs_ctx_get_tile_variant", "svs_autotune",
     "svs_server_create", "svs_server_destroy", "svs_server_submit", "svs_server_wait",
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
-    "svs_dev_free", "svs_memcpy",
+    "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe",
 ]
 TILE_VARIANTS = 5
 
@@ -150,6 +150,7 @@ def lib() -> C.CDLL:
             "svs_tiptoe_answer_u32_dev": ([vp, u32, vp, vp, i32], i32),
             "svs_tiptoe_answer_u64": ([vp, u32, vp, vp], i32),
             "svs_ctx_copy_probe": ([vp, u64, C.POINTER(C.c_double)], i32),
+            "svs_ctx_read_probe": ([vp, u64, C.POINTER(C.c_double)], i32),
             "svs_ctx_set_tile_variant": ([vp, u32, i32], i32),
             "svs_ctx_get_tile_variant": ([vp, u32, C.POINTER(i32), C.POINTER(u32),
                                           C.POINTER(u32), C.POINTER(u32)], i32),

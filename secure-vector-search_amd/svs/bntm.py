@@ -202,6 +202,12 @@ class Context:
         return dict(variant=v.value, query_group=qg.value, ring_depth=pf.value,
                     waves_per_simd=occ.value)
 
+    def read_probe(self, nbytes: int) -> float:
+        """ms for a read of nbytes in the batched product's access pattern."""
+        ms = C.c_double(0)
+        check(L.lib().svs_ctx_read_probe(self.h, nbytes, C.byref(ms)))
+        return ms.value
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""
