@@ -92,6 +92,21 @@ def run_batches_dev(index, dq, probes, verify, k, batch, nbatches, offset=0):
     return lat
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-probe factors), when
+    it was taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_bn_ivf_1M_nprobe32_b256.json")
+    if not (args.n == 1_000_000 and args.nprobe == 32 and args.batch == 256 and
+            args.nlist == 1024 and os.path.exists(path)):
+        return None, None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    if not k:
+        return None, None
+    return k["traffic_bytes"], "profiles/hbm_traffic_bn_ivf_1M_nprobe32_b256.json"
+
+
 def served(index, v_q, probes, verify, k, levels, seconds, max_batch, slots, pool, seed=5):
     """Open-loop Poisson arrivals through the continuous-batching server at
     each offered rate; per level: achieved q/s and latency percentiles
@@ -306,10 +321,13 @@ def main():
     achieved = prof["gemv_bytes"] / (gemv_ms * 1e-3) / 1e9 if gemv_ms else 0.0
     rows = prof["gemv_rows"]
     mulmods = rows * 1024
+    kname = (f"k_fp_gemm_tiles<1024,0,{variant['query_group']},{variant['ring_depth']},"
+             f"{variant['waves_per_simd']}>")
+    traffic, traffic_src = measured_traffic(kname, args)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=f"k_fp_gemm_tiles<1024,0,{variant['query_group']},"
-                           f"{variant['ring_depth']},{variant['waves_per_simd']}>",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=traffic_src,
+                    kernel=kname,
                     algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
                     avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
                     mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0)
