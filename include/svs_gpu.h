@@ -228,7 +228,7 @@ int svs_score_batch_dev(svs_index* idx, const svs_query_batch* batch_dev, int mo
  * tiles whose query group, prefetch depth and waves per SIMD come from a
  * small table of variants, chosen per batch-size class (n_queries <= 16,
  * <= 64, <= 256, larger). Every variant returns identical results. */
-#define SVS_TILE_VARIANTS 5
+#define SVS_TILE_VARIANTS 8
 /* n_queries == 0 sets every class. */
 int svs_ctx_set_tile_variant(svs_ctx* ctx, uint32_t n_queries, int variant);
 int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* variant,

@@ -763,9 +763,11 @@ enum { KERNEL_VALU = 0, KERNEL_MFMA = 1, KERNEL_MFMA_WAVE = 2 };
 // SIMD. All give identical results; svs_autotune times them.
 struct TileVariant {
     uint32_t qg, pf, occ;
+    uint32_t regs;  // 1: B windows built in registers (k_fp_gemm_vb), 0: in LDS
 };
 static constexpr TileVariant kTileVariants[SVS_TILE_VARIANTS] = {
-    {8, 8, 1}, {8, 2, 2}, {16, 4, 1}, {4, 4, 2}, {16, 2, 1}};
+    {8, 8, 1, 0}, {8, 2, 2, 0}, {16, 4, 1, 0}, {4, 4, 2, 0}, {16, 2, 1, 0},
+    {8, 4, 1, 1}, {8, 2, 1, 1}, {16, 2, 1, 1}};
 
 // batch-size class of a tile-variant choice
 static int nq_class(uint32_t nq) { return nq <= 16 ? 0 : nq <= 64 ? 1 : nq <= 256 ? 2 : 3; }
@@ -812,6 +814,13 @@ static void launch_gemm_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
                        a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
 }
 
+template <int KE, int SRC, int QG, int PF>
+static void launch_gemm_vb(hipStream_t st, uint32_t blocks, const TileArgs& a,
+                           const uint64_t* vecs, const uint64_t* qc, uint64_t* out) {
+    hipLaunchKernelGGL((k_fp_gemm_vb<KE, SRC, QG, PF>), dim3(blocks), dim3(256), 0, st, a.lists,
+                       a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
+}
+
 // The product (SRC 0, KE = n) or dense decode (SRC 1, KE = 2 n1) tiles.
 template <int KE, int SRC>
 static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& a,
@@ -820,6 +829,9 @@ static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& 
         case 1: launch_gemm_v<KE, SRC, 8, 2, 2>(st, blocks, a, vecs, qc, out); break;
         case 2: launch_gemm_v<KE, SRC, 16, 4, 1>(st, blocks, a, vecs, qc, out); break;
         case 4: launch_gemm_v<KE, SRC, 16, 2, 1>(st, blocks, a, vecs, qc, out); break;
+        case 5: launch_gemm_vb<KE, SRC, 8, 4>(st, blocks, a, vecs, qc, out); break;
+        case 6: launch_gemm_vb<KE, SRC, 8, 2>(st, blocks, a, vecs, qc, out); break;
+        case 7: launch_gemm_vb<KE, SRC, 16, 2>(st, blocks, a, vecs, qc, out); break;
         case 3: launch_gemm_v<KE, SRC, 4, 4, 2>(st, blocks, a, vecs, qc, out); break;
         default: launch_gemm_v<KE, SRC, 8, 8, 1>(st, blocks, a, vecs, qc, out); break;
     }

@@ -1263,6 +1263,133 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same F_p GEMM tiles with the B windows built in registers: no LDS, no
+// block barrier. Lane (column d, group grp) needs, per query and K'-step,
+// the windows of two elements: output column d at K' byte (element e, limb
+// a) holds t_{d-a}(e) (0 outside 0..7). Each window half is one v_perm_b32
+// of the element's two (XOR 0x80) dwords with a lane-constant selector, so
+// a query costs one 16-byte load (shared by the 16 lanes of a group) and
+// eight VALU ops per step. Waves of a block run independently on their 64
+// rows; A rows and query words ride a PF-deep register ring.
+template <int KE, int SRC, int QG, int PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_fp_gemm_vb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+             uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+             const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+             const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
+             uint64_t* __restrict__ out) {
+    constexpr int STEPS = KE / 8;
+    static_assert(STEPS % PF == 0, "ring must tile the steps");
+    __shared__ int32_t dsm[PB_WAVES][4][16][17];
+    const int tid = threadIdx.x;
+    const int lane = tid % WAVE, wave = tid / WAVE;
+    const int grp = lane / 16, col = lane % 16;
+    // v_perm_b32 selectors (source bytes 0..7 = the element's low then high
+    // dword; 0x0C yields a zero byte): window byte a takes element byte
+    // col - a when that lies in 0..7.
+    uint32_t sel0 = 0, sel1 = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int b0 = col - a, b1 = col - 4 - a;
+        sel0 |= (uint32_t)(b0 >= 0 && b0 <= 7 ? b0 : 0x0C) << (8 * a);
+        sel1 |= (uint32_t)(b1 >= 0 && b1 <= 7 ? b1 : 0x0C) << (8 * a);
+    }
+    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+        const MTile tile = tiles[t];
+        const ListDev L = lists[tile.list];
+        const uint32_t m = L.m;
+        const int gn = (int)tile.gn;
+        const uint64_t* base = SRC == 0 ? L.menc : L.alh;
+        const uint64_t* arow[PB_RT];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt) {
+            const uint32_t row = min(tile.row0 + 64 * wave + 16 * rt + col, m - 1);
+            arow[rt] = base + (size_t)row * KE + 2 * grp;
+        }
+        const uint64_t* qv[QG];
+#pragma unroll
+        for (int j = 0; j < QG; ++j)
+            qv[j] = vecs + (size_t)items[gitems[tile.g0 + min(j, gn - 1)]].q * KE + 2 * grp;
+        v4i ar[PF][PB_RT], vr[PF][QG];
+#pragma unroll
+        for (int d = 0; d < PF; ++d) {
+#pragma unroll
+            for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * d);
+#pragma unroll
+            for (int j = 0; j < QG; ++j) vr[d][j] = load16(qv[j] + 8 * d);
+        }
+        v4i acc[PB_RT][QG];
+#pragma unroll
+        for (int rt = 0; rt < PB_RT; ++rt)
+#pragma unroll
+            for (int j = 0; j < QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
+        for (int s0 = 0; s0 < STEPS; s0 += PF) {
+#pragma unroll
+            for (int d = 0; d < PF; ++d) {
+                const int nxt = min(s0 + d + PF, STEPS - 1);
+                v4i a[PB_RT];
+#pragma unroll
+                for (int rt = 0; rt < PB_RT; ++rt) a[rt] = ar[d][rt] ^ (int)0x80808080;
+#pragma unroll
+                for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * nxt);
+#pragma unroll
+                for (int j = 0; j < QG; ++j) {
+                    if (j < gn) {
+                        const v4i x = vr[d][j] ^ (int)0x80808080;
+                        v4i b;
+                        b[0] = (int)__builtin_amdgcn_perm((uint32_t)x[1], (uint32_t)x[0], sel0);
+                        b[1] = (int)__builtin_amdgcn_perm((uint32_t)x[1], (uint32_t)x[0], sel1);
+                        b[2] = (int)__builtin_amdgcn_perm((uint32_t)x[3], (uint32_t)x[2], sel0);
+                        b[3] = (int)__builtin_amdgcn_perm((uint32_t)x[3], (uint32_t)x[2], sel1);
+#pragma unroll
+                        for (int rt = 0; rt < PB_RT; ++rt)
+                            acc[rt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt], b, acc[rt][j],
+                                                                                 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < QG; ++j) vr[d][j] = load16(qv[j] + 8 * nxt);
+            }
+        }
+        // Epilogue, four (row tile, query) pairs per round; rounds whose
+        // queries are all absent are skipped (uniform per tile).
+#pragma unroll
+        for (int rnd = 0; rnd < PB_RT * QG / 4; ++rnd) {
+            if ((rnd * 4) % QG >= gn) continue;
+#pragma unroll
+            for (int pp = 0; pp < 4; ++pp) {
+                const int pair = rnd * 4 + pp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    dsm[wave][pp][4 * grp + i][col] = acc[pair / QG][pair % QG][i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int pp = lane / 16, row = lane % 16;
+            const int pair = rnd * 4 + pp;
+            const int rt = pair / QG, j = pair % QG;
+            const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;
+            if (j < gn && grow < m) {
+                __int128 sacc = 0;
+#pragma unroll
+                for (int d = 0; d < 15; ++d)
+                    sacc += (__int128)dsm[wave][pp][row][d] * (__int128)(1ull << ((8 * d) % 61));
+                const unsigned __int128 u = (unsigned __int128)(sacc + ((__int128)P << 27));
+                uint64_t v = reduce128((uint64_t)u, (uint64_t)(u >> 64));
+                const uint32_t it = gitems[tile.g0 + j];
+                v = add_mod(v, (SRC == 0 ? L.rowcorr : L.alh_rowcorr)[grow]);
+                v = add_mod(v, qcorr[items[it].q]);
+                out[cand_ptr[it] + grow] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
 // ------------------------------------------------------------ MFMA decode
 // Signed integers to F_p.
 __device__ __forceinline__ uint64_t i64_to_fp(int64_t x) {

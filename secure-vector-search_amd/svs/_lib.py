@@ -36,7 +36,7 @@ EXPORTS = [
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe",
 ]
-TILE_VARIANTS = 5
+TILE_VARIANTS = 8
 
 
 class GpuError(RuntimeError):
