@@ -114,8 +114,11 @@ def served(index, v_q, probes, verify, k, levels, seconds, max_batch, slots, poo
     import svs
     rng = np.random.default_rng(seed)
     out = []
+    sizes = np.array([index.rows(c) for c in range(int(probes.max()) + 1)], np.int64)
+    rows = np.sort(sizes[probes[:pool]].sum(axis=1))[::-1]
     with svs.Server(index, max_batch=max_batch, slots=slots) as srv:
-        # warm the server's slots (scratch sizes) once
+        # size the slots for the largest possible batch of this pool, then warm up
+        srv.reserve(max_batch, max_batch * probes.shape[1], int(rows[:max_batch].sum()), k)
         warm = [dict(v_q=v_q[i], q_seed=rng.integers(0, 256, 32, np.uint8), probes=probes[i],
                      verify=bool(verify[i]), k=k) for i in range(min(pool, 512))]
         srv.run_poisson(warm, qps=levels[0], seed=seed)

@@ -228,7 +228,7 @@ int svs_score_batch_dev(svs_index* idx, const svs_query_batch* batch_dev, int mo
  * tiles whose query group, prefetch depth and waves per SIMD come from a
  * small table of variants, chosen per batch-size class (n_queries <= 16,
  * <= 64, <= 256, larger). Every variant returns identical results. */
-#define SVS_TILE_VARIANTS 8
+#define SVS_TILE_VARIANTS 10
 /* n_queries == 0 sets every class. */
 int svs_ctx_set_tile_variant(svs_ctx* ctx, uint32_t n_queries, int variant);
 int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* variant,
@@ -288,6 +288,13 @@ typedef struct {
 
 int svs_server_create(svs_index* const* tenants, uint32_t n_tenants, const svs_server_config* cfg,
                       svs_server** out);
+/* Sizes every slot's device scratch and pinned staging for batches of up to
+ * n_queries queries, n_items (query, list) items, n_candidates candidate
+ * rows and top-k of k, so that serving never reallocates (a device free
+ * synchronises all streams). Optional, and only before the first request;
+ * buffers otherwise grow on demand. */
+int svs_server_reserve(svs_server* s, uint32_t n_queries, uint32_t n_items,
+                       uint64_t n_candidates, uint32_t k);
 /* Finishes every pending request, then stops. */
 int svs_server_destroy(svs_server* s);
 int svs_server_submit(svs_server* s, const svs_request* req, uint64_t* ticket);

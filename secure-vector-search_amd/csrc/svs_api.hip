@@ -764,10 +764,11 @@ enum { KERNEL_VALU = 0, KERNEL_MFMA = 1, KERNEL_MFMA_WAVE = 2 };
 struct TileVariant {
     uint32_t qg, pf, occ;
     uint32_t regs;  // 1: B windows built in registers (k_fp_gemm_vb), 0: in LDS
+    uint32_t vpf;   // query-word ring depth (LDS kernel)
 };
 static constexpr TileVariant kTileVariants[SVS_TILE_VARIANTS] = {
-    {8, 8, 1, 0}, {8, 2, 2, 0}, {16, 4, 1, 0}, {4, 4, 2, 0}, {16, 2, 1, 0},
-    {8, 4, 1, 1}, {8, 2, 1, 1}, {16, 2, 1, 1}};
+    {8, 8, 1, 0, 8}, {8, 2, 2, 0, 2}, {16, 4, 1, 0, 4}, {4, 4, 2, 0, 4}, {16, 2, 1, 0, 2},
+    {8, 4, 1, 1, 4}, {8, 2, 1, 1, 2}, {16, 2, 1, 1, 2}, {8, 4, 1, 0, 8}, {16, 4, 1, 0, 8}};
 
 // batch-size class of a tile-variant choice
 static int nq_class(uint32_t nq) { return nq <= 16 ? 0 : nq <= 64 ? 1 : nq <= 256 ? 2 : 3; }
@@ -807,10 +808,10 @@ struct TileArgs {
     const uint32_t* cand_ptr;
 };
 
-template <int KE, int SRC, int QG, int PF, int OCC>
+template <int KE, int SRC, int QG, int PF, int OCC, int VPF = PF>
 static void launch_gemm_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
                           const uint64_t* vecs, const uint64_t* qc, uint64_t* out) {
-    hipLaunchKernelGGL((k_fp_gemm_tiles<KE, SRC, QG, PF, OCC>), dim3(blocks), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_fp_gemm_tiles<KE, SRC, QG, PF, OCC, VPF>), dim3(blocks), dim3(256), 0, st,
                        a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
 }
 
@@ -832,31 +833,32 @@ static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& 
         case 5: launch_gemm_vb<KE, SRC, 8, 4>(st, blocks, a, vecs, qc, out); break;
         case 6: launch_gemm_vb<KE, SRC, 8, 2>(st, blocks, a, vecs, qc, out); break;
         case 7: launch_gemm_vb<KE, SRC, 16, 2>(st, blocks, a, vecs, qc, out); break;
+        case 8: launch_gemm_v<KE, SRC, 8, 4, 1, 8>(st, blocks, a, vecs, qc, out); break;
+        case 9: launch_gemm_v<KE, SRC, 16, 4, 1, 8>(st, blocks, a, vecs, qc, out); break;
         case 3: launch_gemm_v<KE, SRC, 4, 4, 2>(st, blocks, a, vecs, qc, out); break;
         default: launch_gemm_v<KE, SRC, 8, 8, 1>(st, blocks, a, vecs, qc, out); break;
     }
 }
 
+template <int QG, int RTW>
+static void launch_decode_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
+                            const int8_t* t_dense, const int8_t* vplanes, const uint64_t* mt_q,
+                            const uint64_t* r, const uint64_t* dense, int64_t* scores) {
+    hipLaunchKernelGGL((k_decode_tiles<QG, RTW>), dim3(blocks), dim3(256), 0, st, a.lists,
+                       a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes, mt_q, r,
+                       dense, scores);
+}
+
+// Decode tiles for the variant's query group: all four row tiles of a wave
+// in one pass (one wave per SIMD measured faster than two passes of two).
 static void launch_decode_tiles(int v, hipStream_t st, uint32_t blocks, const TileArgs& a,
                                 const int8_t* t_dense, const int8_t* vplanes,
                                 const uint64_t* mt_q, const uint64_t* r, const uint64_t* dense,
                                 int64_t* scores) {
     switch (kTileVariants[v].qg) {
-        case 16:
-            hipLaunchKernelGGL(k_decode_tiles<16>, dim3(blocks), dim3(256), 0, st, a.lists,
-                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
-                               mt_q, r, dense, scores);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_decode_tiles<4>, dim3(blocks), dim3(256), 0, st, a.lists,
-                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
-                               mt_q, r, dense, scores);
-            break;
-        default:
-            hipLaunchKernelGGL(k_decode_tiles<8>, dim3(blocks), dim3(256), 0, st, a.lists,
-                               a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes,
-                               mt_q, r, dense, scores);
-            break;
+        case 16: launch_decode_v<16, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        case 4: launch_decode_v<4, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        default: launch_decode_v<8, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
     }
 }
 
@@ -1062,7 +1064,9 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
             uint64_t* mt_q = (uint64_t*)sl.q_corr.ptr + 2 * nq;
             hipLaunchKernelGGL(k_decode_prep, dim3(nq), dim3(256), 0, st, (const uint64_t*)d_venc,
                                (const int8_t*)d_tdense, (int8_t*)sl.q_vplanes.ptr, mt_q);
-            launch_decode_tiles(variant, st, tblocks, ta, (const int8_t*)d_tdense,
+            const uint32_t dblocks = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
+            launch_decode_tiles(variant, st, dblocks, ta, (const int8_t*)d_tdense,
                                 (const int8_t*)sl.q_vplanes.ptr, (const uint64_t*)mt_q,
                                 (const uint64_t*)sl.r.ptr, dense, (int64_t*)sl.scores.ptr);
             ctx->launches += 2;
@@ -1129,6 +1133,40 @@ int svs::stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* 
                            hipMemcpyDefault, sl.stream));
     HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyDefault,
                            sl.stream));
+    return SVS_OK;
+}
+
+// Grows a slot's batch scratch to hold a batch of nq queries, n_items
+// (query, list) items and n_cands candidate rows with top-k of k, so that
+// serving does not reallocate (a device free synchronises every stream).
+int svs::reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
+                      uint64_t n_cands, uint32_t k) {
+    const uint64_t ni = std::max<uint32_t>(n_items, 1), nc = std::max<uint64_t>(n_cands, 1);
+    const uint64_t chunks = nc / CHUNK_ROWS + ni;
+    const uint64_t tiles = nc / 64 + ni;  // generous for any tile shape
+    SVS_TRY(sl.items.ensure(ctx, sizeof(Item) * ni));
+    SVS_TRY(sl.prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (ni + 1) + 2 * (nq + 1))));
+    SVS_TRY(sl.flags.ensure(ctx, ni + 64));
+    SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_seeds.ensure(ctx, (size_t)nq * 32));
+    SVS_TRY(sl.q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_gl.ensure(ctx, sizeof(uint64_t) * (size_t)nq * 2 * N1));
+    SVS_TRY(sl.q_tdense.ensure(ctx, (size_t)nq * N));
+    SVS_TRY(sl.q_tlist.ensure(ctx, sizeof(uint16_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
+    SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
+    SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+    SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc));
+    SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc));
+    SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc));
+    SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc));
+    SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * chunks));
+    SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (ni + nq) + 64));
+    SVS_TRY(sl.out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
+                                   sizeof(uint32_t) * nq + 64));
+    SVS_TRY(sl.mtiles.ensure(ctx, 2 * (sizeof(MTile) * tiles + sizeof(uint32_t) * ni) + 16));
+    SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * (nc / TOPK_WCHUNK + nq) * k));
     return SVS_OK;
 }
 

@@ -171,5 +171,7 @@ int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
 int collect_async(svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,
                   int64_t* scores, uint32_t* count, int32_t* status, uint32_t* rng_flag);
 int check_rng_flag(uint32_t rng_flag);
+int reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items, uint64_t n_cands,
+                 uint32_t k);
 
 }  // namespace svs

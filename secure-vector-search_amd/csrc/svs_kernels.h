@@ -1128,7 +1128,7 @@ __device__ __forceinline__ uint32_t xcd_worker() {
 // current step's MFMAs run. OCC is the waves per SIMD the register budget
 // is sized for (1: deep ring, 2: shallow ring with a second wave to switch
 // to); the host picks a (QG, PF, OCC) variant by timing them.
-template <int KE, int SRC, int QG, int PF, int OCC>
+template <int KE, int SRC, int QG, int PF, int OCC, int VPF = 8>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                 uint32_t n_tiles, const uint32_t* __restrict__ gitems,
@@ -1137,7 +1137,9 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 uint64_t* __restrict__ out) {
     constexpr int STEPS = KE / 8;  // 64 K'-bytes (8 elements) per step
     static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
-    static_assert(STEPS % PF == 0 && PF % 2 == 0, "ring must tile the steps");
+    static_assert(STEPS % PF == 0 && PF % 2 == 0 && STEPS % VPF == 0 && VPF % PF == 0 &&
+                      VPF % 2 == 0,
+                  "rings must tile the steps");
     // [buf][query][column][element], each column row padded from 8 to 10
     // words so that the 16 columns of a ds_read_b128 pass start on distinct
     // bank groups
@@ -1177,15 +1179,18 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 bw[buf][pj][d][pe] = w;
             }
         };
-        // ring: ar[d] / vr[d] hold step s with s % PF == d
+        // rings: ar[d] holds the A rows of step s with s % PF == d; vr[e]
+        // the query word of step s with s % VPF == e (a deeper ring: the
+        // producers need step s + 1's word one step ahead of the MFMAs)
         v4i ar[PF][PB_RT];
-        uint64_t vr[PF];
+        uint64_t vr[VPF];
 #pragma unroll
         for (int d = 0; d < PF; ++d) {
 #pragma unroll
             for (int rt = 0; rt < PB_RT; ++rt) ar[d][rt] = load16(arow[rt] + 8 * d);
-            vr[d] = gload_u64(pv + 8 * d);
         }
+#pragma unroll
+        for (int e = 0; e < VPF; ++e) vr[e] = gload_u64(pv + 8 * e);
         if (producer) produce(0, vr[0]);
         __syncthreads();
         v4i acc[PB_RT][QG];
@@ -1193,12 +1198,15 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
         for (int rt = 0; rt < PB_RT; ++rt)
 #pragma unroll
             for (int j = 0; j < QG; ++j) acc[rt][j] = v4i{0, 0, 0, 0};
-        for (int s0 = 0; s0 < STEPS; s0 += PF) {
+        constexpr int U = PF > VPF ? PF : VPF;  // steps per unrolled group
+        for (int s0 = 0; s0 < STEPS; s0 += U) {
 #pragma unroll
-            for (int d = 0; d < PF; ++d) {
-                // Loads are unconditional (the last PF steps re-load the
-                // final step, in bounds) so the wait counts stay exact.
-                const int nxt = min(s0 + d + PF, STEPS - 1);
+            for (int u = 0; u < U; ++u) {
+                const int d = u % PF;
+                // Loads are unconditional (the last steps re-load the final
+                // step, in bounds) so the wait counts stay exact.
+                const int nxt = min(s0 + u + PF, STEPS - 1);
+                const int vnxt = min(s0 + u + VPF, STEPS - 1);
                 v4i a[PB_RT];
 #pragma unroll
                 for (int rt = 0; rt < PB_RT; ++rt) a[rt] = ar[d][rt] ^ (int)0x80808080;
@@ -1209,7 +1217,7 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 v4i b[QG];
 #pragma unroll
                 for (int j = 0; j < QG; ++j)
-                    b[j] = *reinterpret_cast<const v4i*>(&bw[d & 1][j][col][2 * grp]);
+                    b[j] = *reinterpret_cast<const v4i*>(&bw[u & 1][j][col][2 * grp]);
 #pragma unroll
                 for (int j = 0; j < QG; ++j) {
                     if (j < gn) {
@@ -1221,8 +1229,8 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 }
                 // (after the last step this fills the idle buffer with a
                 // repeat of the final step; nothing reads it)
-                if (producer) produce((d + 1) & 1, vr[(d + 1) % PF]);
-                vr[d] = gload_u64(pv + 8 * nxt);
+                if (producer) produce((u + 1) & 1, vr[(u + 1) % VPF]);
+                vr[u % VPF] = gload_u64(pv + 8 * vnxt);
                 __syncthreads();
             }
         }
@@ -1474,8 +1482,8 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 // columns 8..15 zero) and the eight v_enc byte planes of query j (two
 // queries per 16-column tile). Each wave owns PB_RT 16-row tiles; QG is the
 // tile's query group (the product's).
-template <int QG>
-__global__ void __launch_bounds__(256)
+template <int QG, int RTW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW == PB_RT ? 1 : 2, RTW == PB_RT ? 1 : 2)))
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                uint32_t n_tiles, const uint32_t* __restrict__ gitems,
                const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
@@ -1485,6 +1493,7 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
     constexpr int KC = N / 64;  // 16 K-steps of 64 bytes
     constexpr int NCT = QG / 2;  // S v column tiles
     static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
+    static_assert(PB_RT % RTW == 0, "passes must tile the wave's rows");
     __shared__ int32_t dsm[PB_WAVES][3 + NCT][16][17];
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
@@ -1494,14 +1503,6 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
         const ListDev L = lists[tile.list];
         const uint32_t m = L.m;
         const int gn = (int)tile.gn;
-        const int8_t* amp[PB_RT];
-        const int8_t* as8[PB_RT];
-#pragma unroll
-        for (int rt = 0; rt < PB_RT; ++rt) {
-            const uint32_t row = min(tile.row0 + 64 * wave + 16 * rt + col, m - 1);
-            amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
-            as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
-        }
         // B sources; a column without a query reads query 0's data and is
         // masked to zero (loads stay unconditional).
         const uint32_t q0 = items[gitems[tile.g0]].q;
@@ -1519,94 +1520,111 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             bv[ct] = vplanes + ((size_t)qj * 8 + col % 8) * N + 16 * grp;
             vmask[ct] = ok ? -1 : 0;
         }
-        v4i amt[PB_RT][3], asv[PB_RT][NCT];
-#pragma unroll
-        for (int rt = 0; rt < PB_RT; ++rt) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) amt[rt][a] = v4i{0, 0, 0, 0};
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
-        }
-        // Two-slot register ring over the K-steps: slot d's MFMAs run, then
-        // slot d is reloaded for the step two ahead (the tail re-loads the
-        // last step, in bounds), so one step's loads are always in flight
-        // behind the current step's MFMAs. The outer loop stays rolled so
-        // the compiler keeps that order.
-        v4i ra[2][PB_RT][4], rb[2][1 + NCT];
-        auto issue = [&](int slot, int kc) {
-#pragma unroll
-            for (int rt = 0; rt < PB_RT; ++rt) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kc);
-                ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
-            }
-            rb[slot][0] = load16b(bt + 64 * kc);
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) rb[slot][1 + ct] = load16b(bv[ct] + 64 * kc);
-        };
-        issue(0, 0);
-        issue(1, 1);
+        // The wave's 64 rows in passes of RTW 16-row tiles (fewer registers,
+        // so two waves per SIMD when RTW < PB_RT).
 #pragma unroll 1
-        for (int kc0 = 0; kc0 < KC; kc0 += 2) {
+        for (int pass = 0; pass < PB_RT / RTW; ++pass) {
+            const uint32_t prow0 = tile.row0 + 64 * wave + 16 * RTW * pass;
+            const int8_t* amp[RTW];
+            const int8_t* as8[RTW];
 #pragma unroll
-            for (int slot = 0; slot < 2; ++slot) {
-                const v4i b_t = rb[slot][0] & tmask;
+            for (int rt = 0; rt < RTW; ++rt) {
+                const uint32_t row = min(prow0 + 16 * rt + col, m - 1);
+                amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
+                as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
+            }
+            v4i amt[RTW][3], asv[RTW][NCT];
 #pragma unroll
-                for (int rt = 0; rt < PB_RT; ++rt)
+            for (int rt = 0; rt < RTW; ++rt) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) amt[rt][a] = v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
+            }
+            // Two-slot register ring over the K-steps: slot d's MFMAs run,
+            // then slot d is reloaded for the step two ahead (the tail
+            // re-loads the last step, in bounds), so one step's loads are
+            // always in flight behind the current step's MFMAs. The outer
+            // loop stays rolled so the compiler keeps that order.
+            v4i ra[2][RTW][4], rb[2][1 + NCT];
+            auto issue = [&](int slot, int kc) {
+#pragma unroll
+                for (int rt = 0; rt < RTW; ++rt) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a)
-                        amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][a], b_t,
-                                                                            amt[rt][a], 0, 0, 0);
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) {
-                    const v4i b_v = rb[slot][1 + ct] & vmask[ct];
-#pragma unroll
-                    for (int rt = 0; rt < PB_RT; ++rt)
-                        asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][3], b_v,
-                                                                             asv[rt][ct], 0, 0, 0);
+                        ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kc);
+                    ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
                 }
-                issue(slot, min(kc0 + slot + 2, KC - 1));
-            }
-        }
-        // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i, column
-        // col), then each lane finishes (row, query) pairs lane and lane + 64.
+                rb[slot][0] = load16b(bt + 64 * kc);
 #pragma unroll
-        for (int rt = 0; rt < PB_RT; ++rt) {
+                for (int ct = 0; ct < NCT; ++ct) rb[slot][1 + ct] = load16b(bv[ct] + 64 * kc);
+            };
+            issue(0, 0);
+            issue(1, 1);
+#pragma unroll 1
+            for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+                for (int slot = 0; slot < 2; ++slot) {
+                    const v4i b_t = rb[slot][0] & tmask;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) dsm[wave][a][4 * grp + i][col] = amt[rt][a][i];
+                    for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) dsm[wave][3 + ct][4 * grp + i][col] = asv[rt][ct][i];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        for (int a = 0; a < 3; ++a)
+                            amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                ra[slot][rt][a], b_t, amt[rt][a], 0, 0, 0);
 #pragma unroll
-            for (int h = 0; h < QG / 4; ++h) {
-                const int pr = lane + 64 * h;
-                const int row = pr % 16, j = pr / 16;
-                const uint32_t grow = tile.row0 + 64 * wave + 16 * rt + row;
-                if (j < gn && grow < m) {
-                    const uint32_t it = gitems[tile.g0 + j];
-                    const int64_t mt = (int64_t)dsm[wave][0][row][j] +
-                                       (int64_t)dsm[wave][1][row][j] * 256 +
-                                       (int64_t)dsm[wave][2][row][j] * 65536;
-                    __int128 sv = 0;
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        const v4i b_v = rb[slot][1 + ct] & vmask[ct];
 #pragma unroll
-                    for (int b = 0; b < 8; ++b)
-                        sv += (__int128)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
-                              (__int128)(1ull << (8 * b));
-                    const uint64_t g = cand_ptr[it] + grow;
-                    uint64_t corr = add_mod(i64_to_fp(mt), mt_qterm[items[it].q]);
-                    corr = add_mod(corr, add_mod(i128_to_fp(sv), L.s_rowterm[grow]));
-                    corr = add_mod(corr, dense[g]);
-                    scores[g] = fp_to_signed(sub_mod(r[g], corr));
+                        for (int rt = 0; rt < RTW; ++rt)
+                            asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                ra[slot][rt][3], b_v, asv[rt][ct], 0, 0, 0);
+                    }
+                    issue(slot, min(kc0 + slot + 2, KC - 1));
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i,
+            // column col), then each lane finishes (row, query) pairs lane,
+            // lane + 64, ...
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) dsm[wave][a][4 * grp + i][col] = amt[rt][a][i];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct)
+                        dsm[wave][3 + ct][4 * grp + i][col] = asv[rt][ct][i];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int h = 0; h < QG / 4; ++h) {
+                    const int pr = lane + 64 * h;
+                    const int row = pr % 16, j = pr / 16;
+                    const uint32_t grow = prow0 + 16 * rt + row;
+                    if (j < gn && grow < m) {
+                        const uint32_t it = gitems[tile.g0 + j];
+                        const int64_t mt = (int64_t)dsm[wave][0][row][j] +
+                                           (int64_t)dsm[wave][1][row][j] * 256 +
+                                           (int64_t)dsm[wave][2][row][j] * 65536;
+                        __int128 sv = 0;
+#pragma unroll
+                        for (int b = 0; b < 8; ++b)
+                            sv += (__int128)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
+                                  (__int128)(1ull << (8 * b));
+                        const uint64_t g = cand_ptr[it] + grow;
+                        uint64_t corr = add_mod(i64_to_fp(mt), mt_qterm[items[it].q]);
+                        corr = add_mod(corr, add_mod(i128_to_fp(sv), L.s_rowterm[grow]));
+                        corr = add_mod(corr, dense[g]);
+                        scores[g] = fp_to_signed(sub_mod(r[g], corr));
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
     }
 }

@@ -372,6 +372,29 @@ extern "C" int svs_server_destroy(svs_server* s) {
     return SVS_OK;
 }
 
+extern "C" int svs_server_reserve(svs_server* s, uint32_t n_queries, uint32_t n_items,
+                                  uint64_t n_candidates, uint32_t k) {
+    if (!s || n_queries == 0 || k == 0 || k > 4096) return fail(SVS_ERR_ARG, "bad argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->submitted)
+            return fail(SVS_ERR_STATE, "svs_server_reserve must come before the first request");
+    }
+    std::lock_guard<std::mutex> g(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    for (auto& sp : s->slots) {
+        SVS_TRY(reserve_slot(s->ctx, sp->sl, n_queries, n_items, n_candidates, k));
+        SVS_TRY(sp->h_vq.ensure((size_t)n_queries * N));
+        SVS_TRY(sp->h_seeds.ensure((size_t)n_queries * 32));
+        SVS_TRY(sp->h_ids.ensure((size_t)n_queries * k));
+        SVS_TRY(sp->h_scores.ensure((size_t)n_queries * k));
+        SVS_TRY(sp->h_count.ensure(n_queries));
+        SVS_TRY(sp->h_status.ensure(n_queries));
+        SVS_TRY(sp->h_flag.ensure(1));
+    }
+    return SVS_OK;
+}
+
 extern "C" int svs_server_submit(svs_server* s, const svs_request* r, uint64_t* ticket) {
     if (!s || !r || !ticket || !r->v_q || !r->q_seed || (r->n_probes && !r->probes))
         return fail(SVS_ERR_ARG, "null argument");

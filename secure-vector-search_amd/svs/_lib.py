@@ -34,9 +34,9 @@ EXPORTS = [
     "svs_ctx_set_tile_variant", "svs_ctx_get_tile_variant", "svs_autotune",
     "svs_server_create", "svs_server_destroy", "svs_server_submit", "svs_server_wait",
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
-    "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe",
+    "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
 ]
-TILE_VARIANTS = 8
+TILE_VARIANTS = 10
 
 
 class GpuError(RuntimeError):
@@ -158,6 +158,7 @@ def lib() -> C.CDLL:
                               C.POINTER(C.c_double)], i32),
             "svs_server_create": ([vp, u32, C.POINTER(ServerConfig), C.POINTER(vp)], i32),
             "svs_server_destroy": ([vp], i32),
+            "svs_server_reserve": ([vp, u32, u32, u64, u32], i32),
             "svs_server_submit": ([vp, C.POINTER(Request), C.POINTER(u64)], i32),
             "svs_server_wait": ([vp, u64, C.POINTER(Result)], i32),
             "svs_server_stats": ([vp, C.POINTER(ServerCounters)], i32),

@@ -56,6 +56,11 @@ class Server:
         self._keep = {}
         _bntm._live_servers.add(self)
 
+    def reserve(self, n_queries: int, n_items: int, n_candidates: int, k: int = 10):
+        """Pre-size every slot for batches up to these sizes (no reallocation
+        while serving)."""
+        check(L.lib().svs_server_reserve(self.h, n_queries, n_items, n_candidates, k))
+
     def submit(self, v_q, q_seed, probes, verify: bool = False, k: int = 10,
                tenant: int = 0) -> int:
         keep = []

@@ -489,7 +489,7 @@ def test_topk_ties_across_chunks(ctx):
             assert len(set(scores.tolist())) < k or k == 1
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(10))
 def test_tile_variants_match_oracle(ctx, ivf_setup, l_sub, variant):
     """Every block-tile variant (query group 4/8/16, ring depth, waves per
     SIMD) of the MFMA batch path is bit-exact: whole IVF queries and the

@@ -181,3 +181,20 @@ def test_served_verification_failure(ctx, tenant_a):
             _check(res, tenant_a, rq, f"request {i}")
     finally:
         ctx.inject_fault(-1)
+
+
+def test_served_after_reserve(ctx, tenant_a):
+    """Pre-sized slots serve the same results; reserving after the first
+    request is refused."""
+    import svs
+    rng = np.random.default_rng(6)
+    reqs = _requests(tenant_a, rng, 20)
+    with svs.Server(tenant_a["handle"].index, max_batch=16, slots=2) as srv:
+        srv.reserve(16, 16 * 8, 16 * 2500, 40)
+        tickets = [srv.submit(r["v_q"], r["q_seed"], r["probes"], r["verify"], r["k"])
+                   for r in reqs]
+        results = [srv.wait(tk) for tk in tickets]
+        with pytest.raises(svs.GpuError):
+            srv.reserve(16, 128, 1000, 10)
+    for i, (res, rq) in enumerate(zip(results, reqs)):
+        _check(res, tenant_a, rq, f"request {i}")
