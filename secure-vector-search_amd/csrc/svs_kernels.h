@@ -1483,7 +1483,7 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 // queries per 16-column tile). Each wave owns PB_RT 16-row tiles; QG is the
 // tile's query group (the product's).
 template <int QG, int RTW>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW == PB_RT ? 1 : 2, RTW == PB_RT ? 1 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                uint32_t n_tiles, const uint32_t* __restrict__ gitems,
                const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
@@ -1520,8 +1520,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             bv[ct] = vplanes + ((size_t)qj * 8 + col % 8) * N + 16 * grp;
             vmask[ct] = ok ? -1 : 0;
         }
-        // The wave's 64 rows in passes of RTW 16-row tiles (fewer registers,
-        // so two waves per SIMD when RTW < PB_RT).
+        // The wave's 64 rows in passes of RTW 16-row tiles (fewer live
+        // registers per pass for the wide query group).
 #pragma unroll 1
         for (int pass = 0; pass < PB_RT / RTW; ++pass) {
             const uint32_t prow0 = tile.row0 + 64 * wave + 16 * RTW * pass;
@@ -1541,6 +1541,28 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
             }
+            // The epilogue's inputs for this lane's (row, query) pairs are
+            // loaded now, so their latency hides behind the K loop: the
+            // candidate index, r, the dense term, the row and query terms.
+            constexpr int NH = QG / 4;
+            uint32_t eg[RTW][NH];
+            uint64_t er[RTW][NH], ed[RTW][NH], es[RTW][NH], eq[RTW][NH];
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    const int pr = lane + 64 * h;
+                    const int row = pr % 16, j = pr / 16;
+                    const uint32_t grow = prow0 + 16 * rt + row;
+                    const bool ok = j < gn && grow < m;
+                    const uint32_t it = gitems[tile.g0 + (ok ? j : 0)];
+                    const uint32_t g = cand_ptr[it] + (ok ? grow : 0);
+                    eg[rt][h] = ok ? g : 0xFFFFFFFFu;
+                    er[rt][h] = r[g];
+                    ed[rt][h] = dense[g];
+                    es[rt][h] = L.s_rowterm[ok ? grow : 0];
+                    eq[rt][h] = mt_qterm[items[it].q];
+                }
             // Two-slot register ring over the K-steps: slot d's MFMAs run,
             // then slot d is reloaded for the step two ahead (the tail
             // re-loads the last step, in bounds), so one step's loads are
@@ -1600,12 +1622,10 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-                for (int h = 0; h < QG / 4; ++h) {
+                for (int h = 0; h < NH; ++h) {
                     const int pr = lane + 64 * h;
                     const int row = pr % 16, j = pr / 16;
-                    const uint32_t grow = prow0 + 16 * rt + row;
-                    if (j < gn && grow < m) {
-                        const uint32_t it = gitems[tile.g0 + j];
+                    if (eg[rt][h] != 0xFFFFFFFFu) {
                         const int64_t mt = (int64_t)dsm[wave][0][row][j] +
                                            (int64_t)dsm[wave][1][row][j] * 256 +
                                            (int64_t)dsm[wave][2][row][j] * 65536;
@@ -1614,11 +1634,10 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                         for (int b = 0; b < 8; ++b)
                             sv += (__int128)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
                                   (__int128)(1ull << (8 * b));
-                        const uint64_t g = cand_ptr[it] + grow;
-                        uint64_t corr = add_mod(i64_to_fp(mt), mt_qterm[items[it].q]);
-                        corr = add_mod(corr, add_mod(i128_to_fp(sv), L.s_rowterm[grow]));
-                        corr = add_mod(corr, dense[g]);
-                        scores[g] = fp_to_signed(sub_mod(r[g], corr));
+                        uint64_t corr = add_mod(i64_to_fp(mt), eq[rt][h]);
+                        corr = add_mod(corr, add_mod(i128_to_fp(sv), es[rt][h]));
+                        corr = add_mod(corr, ed[rt][h]);
+                        scores[eg[rt][h]] = fp_to_signed(sub_mod(er[rt][h], corr));
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
