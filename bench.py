@@ -6,9 +6,11 @@ seeded 1024-component Gaussian mixture with Zipf(1.1) component weights;
 nlist = 1024 IVF lists (the mixture components); nprobe = 32; k = 10;
 Sec128 BN encryption (n = 1024, n1 = 128, p = 2^61 - 1). A step is one batch
 of queries through the whole device path: encode -> server product ->
-[Freivalds] -> decode -> top-k. Routing and quantisation (client-side CPU
-work in the reference) happen before the timed region; the batch's query
-vectors cross PCIe inside it.
+[Freivalds] -> decode -> top-k, with the quantised queries, their probe
+lists and the outputs resident in HBM (`value`). Routing and quantisation
+(client-side CPU work in the reference's score) run on the device too in the
+`raw_queries` figures and in the served levels, whose requests carry the f32
+query (host buffers, PCIe inside the timing).
 
 Prints one JSON line (the contract) with the batched throughput as `value`,
 the plain one-query-per-launch throughput beside it, p50/p99 latency, the
@@ -57,6 +59,19 @@ def run_batches(index, v_q, seeds, probes, verify, k, batch, nbatches, mode, off
     return lat, res
 
 
+def run_raw(index, qs, seeds, nprobe, verify, k, batch, nbatches, mode, offset=0):
+    """Raw f32 queries (host buffers): quantise + route on the device, then
+    the pipeline; per-batch wall seconds and results."""
+    lat, res = [], []
+    for b in range(nbatches):
+        s = slice(offset + b * batch, offset + (b + 1) * batch)
+        t0 = time.perf_counter()
+        out = index.score_raw_batch(qs[s], seeds[s], nprobe, verify[s], k, mode=mode)
+        lat.append(time.perf_counter() - t0)
+        res.append(out)
+    return lat, res
+
+
 class DeviceQueries:
     """The workload's query vectors and seeds resident in HBM, plus device
     outputs for one batch (svs_dev_alloc buffers)."""
@@ -92,6 +107,72 @@ def run_batches_dev(index, dq, probes, verify, k, batch, nbatches, offset=0):
     return lat
 
 
+def schemes(ctx, seconds=1.0):
+    """The other schemes on the kernel family, device-resident inputs and
+    outputs: Tiptoe's SimplePIR answer (32768 x 8192 u32 database, entries
+    in [0, 32), mod 2^32) at batch 1 and 64, and EMVP's partials GEMV
+    (20,000 rows x 1292, batch 1 and 16, host buffers)."""
+    import ctypes as C
+    import svs
+    from svs import _lib as L
+    from svs import schemes as S
+    rng = np.random.default_rng(11)
+    out = {}
+    rows, cols = 32768, 8192
+    db = rng.integers(0, 32, size=(rows, cols), dtype=np.uint32)
+    tt = S.TiptoeDatabase(db, ctx)
+    stored, elem = tt.stored()
+    del db
+    res = {}
+    for b in (1, 64):
+        ct = svs.DeviceArray.from_host(ctx, rng.integers(0, 2**32, size=(b, cols), dtype=np.uint32))
+        ans = svs.DeviceArray(ctx, (b, rows), np.uint32)
+        one = np.zeros(1, np.uint32)
+
+        def run(n):
+            for _ in range(n):
+                L.check(L.lib().svs_tiptoe_answer_u32_dev(tt.h, b, C.c_void_p(ct.ptr()),
+                                                          C.c_void_p(ans.ptr()), -1))
+            L.check(L.lib().svs_memcpy(ctx.h, one.ctypes.data_as(C.c_void_p),
+                                       C.c_void_p(ans.ptr()), 4))
+        run(3)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < seconds / 2:
+            run(10)
+            n += 10
+        dt = (time.perf_counter() - t0) / n
+        res[str(b)] = dict(ms_per_batch=round(dt * 1e3, 4), queries_per_s=round(b / dt, 1),
+                           u32_layout_gbs=round(rows * cols * 4 / dt / 1e9, 1),
+                           stored_gbs=round(stored / dt / 1e9, 1))
+        ct.close()
+        ans.close()
+    tt.close()
+    out["tiptoe_simplepir_32768x8192_mod2^32"] = dict(
+        stored_bytes=int(stored), stored_entry_bytes=int(elem), batches=res,
+        note="algorithmic bytes per batch 1.07 GB in the u32 layout; the database is held "
+             "one byte per entry (all entries < 128)")
+    m = 20000
+    em = S.EmvpCluster(rng.integers(0, (1 << 61) - 1, size=(m, 1292), dtype=np.uint64),
+                                 ctx)
+    eres = {}
+    for b in (1, 16):
+        q = rng.integers(0, (1 << 61) - 1, size=(b, 1292), dtype=np.uint64)
+        em.compute_products_batch(q)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < seconds / 2:
+            em.compute_products_batch(q)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+        eres[str(b)] = dict(ms_per_batch=round(dt * 1e3, 3), queries_per_s=round(b / dt, 1),
+                            gbs=round(m * 1292 * 8 / dt / 1e9, 1))
+    em.close()
+    out["emvp_partials_20000x1292"] = dict(batches=eres, note="host buffers: each query returns "
+                                           "76 x m partials (12 MB) over PCIe")
+    return out
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-probe factors), when
@@ -107,28 +188,38 @@ def measured_traffic(kernel, args):
     return k["traffic_bytes"], "profiles/hbm_traffic_bn_ivf_1M_nprobe32_b256.json"
 
 
-def served(index, v_q, probes, verify, k, levels, seconds, max_batch, slots, pool, seed=5):
+def served(index, qs, v_q, probes, verify, k, levels, seconds, max_batch, slots, pool, raw,
+           seed=5):
     """Open-loop Poisson arrivals through the continuous-batching server at
     each offered rate; per level: achieved q/s and latency percentiles
-    (scheduled arrival -> result)."""
+    (scheduled arrival -> result). raw: requests carry the f32 query and
+    nprobe (quantised and routed on the device inside each batch); else the
+    quantised query and its probe list."""
     import svs
     rng = np.random.default_rng(seed)
     out = []
-    sizes = np.array([index.rows(c) for c in range(int(probes.max()) + 1)], np.int64)
+    nprobe = probes.shape[1]
+    sizes = np.array([index.rows(c) for c in range(index.n_lists)], np.int64)
+    # raw batches are sized for their bound (the nprobe largest lists)
+    per_q = np.sort(sizes)[::-1][:nprobe].sum() if raw else None
     rows = np.sort(sizes[probes[:pool]].sum(axis=1))[::-1]
+    cand = int(per_q) * max_batch if raw else int(rows[:max_batch].sum())
+
+    def req(i):
+        base = dict(q_seed=rng.integers(0, 256, 32, np.uint8), verify=bool(verify[i]), k=k)
+        if raw:
+            base.update(query=qs[i], nprobe=nprobe)
+        else:
+            base.update(v_q=v_q[i], probes=probes[i])
+        return base
     with svs.Server(index, max_batch=max_batch, slots=slots) as srv:
         # size the slots for the largest possible batch of this pool, then warm up
-        srv.reserve(max_batch, max_batch * probes.shape[1], int(rows[:max_batch].sum()), k)
-        warm = [dict(v_q=v_q[i], q_seed=rng.integers(0, 256, 32, np.uint8), probes=probes[i],
-                     verify=bool(verify[i]), k=k) for i in range(min(pool, 512))]
+        srv.reserve(max_batch, max_batch * nprobe, cand, k)
+        warm = [req(i) for i in range(min(pool, 512))]
         srv.run_poisson(warm, qps=levels[0], seed=seed)
         for qps in levels:
             n = max(64, int(qps * seconds))
-            reqs = []
-            for j in range(n):
-                i = j % pool
-                reqs.append(dict(v_q=v_q[i], q_seed=rng.integers(0, 256, 32, np.uint8),
-                                 probes=probes[i], verify=bool(verify[i]), k=k))
+            reqs = [req(j % pool) for j in range(n)]
             s0 = srv.stats()
             res, wall = srv.run_poisson(reqs, qps=qps, seed=seed + int(qps))
             s1 = srv.stats()
@@ -228,7 +319,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-routed", action="store_true",
+                    help="served requests carry quantised queries and probe lists "
+                         "(default: raw f32 queries routed on the device)")
     ap.add_argument("--slo-ms", type=float, default=10.0)
+    ap.add_argument("--no-schemes", action="store_true",
+                    help="skip the Tiptoe / EMVP measurements")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain-copy calibration dispatch (0 = skip)")
     args = ap.parse_args()
@@ -313,12 +409,27 @@ def main():
                 for i in range(npl))
     log(f"plain: {qps_plain:.0f} q/s, p50 {pct(lat_p, 50) * 1e3:.2f} ms; batched==plain {exact}")
 
+    # raw f32 queries: quantisation and routing on the device as well (the
+    # reference's score does both on the client CPU), host buffers
+    lat_r, res_r = run_raw(index, qs, seeds, args.nprobe, verify, args.k, args.batch,
+                           args.steps, 1, offset=off)
+    qps_raw = args.steps * args.batch / sum(lat_r)
+    run_raw(index, qs, seeds, args.nprobe, verify, args.k, 1, 2, 0, offset=off)
+    lat_rp, res_rp = run_raw(index, qs, seeds, args.nprobe, verify, args.k, 1, npl, 0,
+                             offset=off)
+    qps_raw_plain = npl / sum(lat_rp)
+    exact_raw = np.array_equal(res_r[0][0][:npl], ids_b) and \
+        np.array_equal(res_r[0][1][:npl], sc_b) and \
+        all(np.array_equal(res_rp[i][0][0], ids_b[i]) for i in range(npl))
+    log(f"raw batched: {qps_raw:.0f} q/s; raw plain: {qps_raw_plain:.0f} q/s; "
+        f"raw == routed {exact_raw}")
+
     levels = [float(x) for x in args.serve_qps.split(",") if x.strip()]
     serve = None
     if levels:
         pool = qs.shape[0]
-        serve = served(index, v_q, probes, verify, args.k, levels, args.serve_seconds,
-                       args.serve_max_batch, args.serve_slots, pool)
+        serve = served(index, qs, v_q, probes, verify, args.k, levels, args.serve_seconds,
+                       args.serve_max_batch, args.serve_slots, pool, not args.serve_routed)
 
     gemv_ms, gemv_calls = prof["product"]
     achieved = prof["gemv_bytes"] / (gemv_ms * 1e-3) / 1e9 if gemv_ms else 0.0
@@ -348,6 +459,10 @@ def main():
                                   gbs=round(nb / (rms * 1e-3) / 1e9, 1),
                                   note="k_read_probe: reads only, in the product's A pattern")
         log(f"copy probe: {copy}")
+
+    other = None if args.no_schemes else schemes(ctx)
+    if other:
+        log(f"schemes: {other}")
 
     cpu = None
     if not args.no_cpu:
@@ -379,13 +494,22 @@ def main():
                   "p99_ms": round(pct(lat_p, 99) * 1e3, 3), "queries": npl},
         "speedup_vs_plain": round(qps / qps_plain, 2),
         "batched_host_buffers_qps": round(qps_host, 2),
+        "raw_queries": {"batched_host_buffers_qps": round(qps_raw, 2),
+                        "plain_qps": round(qps_raw_plain, 2),
+                        "plain_p50_ms": round(pct(lat_rp, 50) * 1e3, 3),
+                        "bit_exact_vs_routed": bool(exact_raw),
+                        "note": "f32 queries in host buffers; quantise_vector and probe_route "
+                                "run on the device inside the batch"},
         "served": serve,
+        "served_requests": "routed (v_q + probes)" if args.serve_routed else
+                           "raw f32 queries, quantised and routed on the device",
         "bit_exact_batched_vs_plain": bool(exact),
         "stages_ms_per_batch": {s: round(prof[s][0] / max(prof[s][1], 1), 4)
                                 for s in ("encode", "product", "verify", "decode", "topk")},
         "tile_variant": variant,
         "autotune": tune,
         "roofline": roofline,
+        "schemes": other,
         "copy_probe": copy,
         "cpu_baseline": cpu,
         "wall_s": round(time.perf_counter() - t_all, 1),

@@ -222,6 +222,55 @@ int svs_memcpy(svs_ctx* ctx, void* dst, const void* src, uint64_t bytes);
  * batch). Non-canonical v_q is reported as SVS_ERR_ARG after the batch. */
 int svs_score_batch_dev(svs_index* idx, const svs_query_batch* batch_dev, int mode);
 
+/* Where batched runs are planned (items, prefix arrays, MFMA tiles): 1 = on
+ * the device by one planning kernel (default), 0 = on the host and uploaded.
+ * Both give the same plan; no reference counterpart (the reference plans
+ * per list on the host, ivf.rs:1289-1296). */
+int svs_ctx_set_planner(svs_ctx* ctx, int on_device);
+
+/* ---- client routing and quantisation on the device -------------------------
+ * The reference client routes and quantises each query on the host before
+ * encryption (BnTmIvfScorer::score steps 1-2, scorer-bntm/src/ivf.rs:
+ * 762-766): probe_route (ivf-index/src/ivf.rs:150-164) picks the nprobe
+ * centroids nearest by l2_f32_scalar (ivf-index/src/distance.rs:30-37), and
+ * quantise_vector (crypto.rs:105-122) scales by Q. Here both run on the
+ * device with the same f32 / f64 results: each distance is a left-to-right
+ * f32 sum of rounded squared differences and a correctly rounded sqrt;
+ * probes ascend by f32::total_cmp with ties in centroid order (the
+ * reference's unstable sort leaves ties unspecified). */
+/* The IVF centroids (IvfIndex::centroids): n_lists rows of d <= 1024 f32,
+ * row-major. Replaces any earlier set. Device routing takes up to 8192. */
+int svs_index_set_centroids(svs_index* idx, const float* centroids_host, uint32_t d);
+/* probe_route for n_queries queries (n_queries x d f32, host): out
+ * (n_queries x nprobe, host) receives each query's probe list. nprobe is
+ * clamped to n_lists as in the reference. */
+int svs_route_batch(svs_index* idx, const float* queries_host, uint32_t n_queries,
+                    uint32_t nprobe, uint32_t* out_host);
+
+/* A batch of raw queries: the whole BnTmIvfScorer::score (scorer-bntm/src/
+ * ivf.rs:746-822) per query on the device: quantise, route, encode, product,
+ * verify, decode, top-k. SVS_MODE_BATCHED runs the batch as one stream of
+ * kernels with no host round trip (routing feeds the device planner);
+ * SVS_MODE_PLAIN routes and runs one query at a time. Outputs as
+ * svs_query_batch; out_probes (optional) receives the routed lists, query
+ * after query (sum of the clamped nprobe words). */
+typedef struct {
+    uint32_t n_queries;
+    const float* queries;      /* n_queries x d f32, d = the centroids' d */
+    const uint32_t* nprobe;    /* per query (NULL: nprobe_all for every query) */
+    uint32_t nprobe_all;
+    uint64_t quantisation_q;   /* Q of quantise_vector (params.rs:19: 2^20) */
+    const uint8_t* q_seeds;    /* n_queries x 32 */
+    const uint8_t* verify;     /* n_queries flags (NULL = none) */
+    uint32_t k;                /* 1..4096 */
+    uint32_t* out_ids;         /* n_queries x k */
+    int64_t* out_scores;       /* n_queries x k */
+    uint32_t* out_count;       /* n_queries */
+    int32_t* out_status;       /* n_queries */
+    uint32_t* out_probes;      /* optional */
+} svs_raw_batch;
+int svs_score_raw_batch(svs_index* idx, const svs_raw_batch* batch_host, int mode);
+
 /* ---- block-tile variants and the autotuner ---------------------------------
  * No reference counterpart (the reference has one CUDA kernel shape): the
  * MFMA batch path (SVS_MODE_BATCHED, SVS_KERNEL_MFMA) runs on 256-row block
@@ -258,7 +307,9 @@ typedef struct svs_server svs_server;
 typedef struct {
     uint32_t max_batch;   /* queries per batch, 1..4096 */
     uint32_t slots;       /* batches in flight, 1..8 */
-    uint64_t slot_bytes;  /* device scratch budget per batch (0 = 1 GiB) */
+    uint64_t slot_bytes;  /* device scratch budget per batch (0 = 1/16 of the
+                             context budget, at least 1 GiB); a request routed
+                             on the device counts its nprobe largest lists */
     int mode;             /* SVS_MODE_BATCHED or SVS_MODE_BATCHED_VALU */
 } svs_server_config;
 typedef struct {
@@ -271,6 +322,12 @@ typedef struct {
     uint32_t k;              /* 1..4096 */
     double arrival_s;        /* 0 (= now), or a scheduled arrival on the
                                 library's steady clock (load generator) */
+    /* A raw query instead (as svs_score_raw_batch): when `query` is set,
+     * v_q and probes are ignored and the batch quantises with
+     * quantisation_q and routes to `nprobe` lists on the device. */
+    const float* query;      /* d f32, d = the tenant's centroids' d */
+    uint32_t nprobe;
+    uint64_t quantisation_q;
 } svs_request;
 typedef struct {
     uint32_t* ids;        /* caller buffer of k (may be NULL) */

@@ -155,11 +155,16 @@ void or_quantise_vector(const float* v, size_t d, size_t n, uint64_t q, uint64_t
     for (size_t i = 0; i < n; ++i) out[i] = 0;
     for (size_t i = 0; i < d; ++i) {
         double scaled = (double)v[i] * qs;
-        int64_t rounded = (int64_t)round(scaled); /* f64::round: half away from 0 */
+        double rd = round(scaled); /* f64::round: half away from 0 */
+        /* `as i64` saturates and maps NaN to 0; the negation below wraps */
+        int64_t rounded = rd != rd ? 0
+                          : rd >= 9223372036854775808.0 ? INT64_MAX
+                          : rd <= -9223372036854775808.0 ? INT64_MIN
+                          : (int64_t)rd;
         if (rounded >= 0) {
             out[i] = (uint64_t)rounded % OR_P;
         } else {
-            uint64_t k = (uint64_t)(-rounded);
+            uint64_t k = 0ull - (uint64_t)rounded;
             uint64_t rr = k % OR_P;
             out[i] = rr == 0 ? 0 : OR_P - rr;
         }
@@ -433,8 +438,9 @@ size_t or_topk(const uint32_t* ids, const int64_t* scores, size_t count, size_t 
  * IVF routing: l2_f32_scalar (ivf-index/src/distance.rs:30-37) = sqrt of a
  * left-to-right f32 sum of squared differences (no FMA contraction: this
  * file is built with -ffp-contract=off); probe_route (ivf-index/src/ivf.rs:
- * 150-164) sorts ascending by distance and keeps the first nprobe. Ties in
- * distance keep centroid order.
+ * 150-164) sorts ascending by distance under f32::total_cmp and keeps the
+ * first nprobe. The reference's sort is unstable, so ties in distance have
+ * no fixed order there; here they keep centroid order.
  * ---------------------------------------------------------------------- */
 float or_l2_f32(const float* a, const float* b, size_t d) {
     float acc = -0.0f;
@@ -445,13 +451,24 @@ float or_l2_f32(const float* a, const float* b, size_t d) {
     return sqrtf(acc);
 }
 
+/* f32::total_cmp order as an unsigned key (-inf < ... < -0 < +0 < ... <
+ * +inf). Every NaN is taken as the largest value: the sign of a NaN
+ * distance follows the platform's NaN propagation (x86 keeps an operand's
+ * NaN, and its default NaN is negative), which the reference does not fix. */
+static uint32_t total_key(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    if (f != f) return 0xFFFFFFFFu;
+    return (b & 0x80000000u) ? ~b : b | 0x80000000u;
+}
+
 void or_probe_route(const float* centroids, size_t n_centroids, size_t d, const float* query,
                     size_t nprobe, uint32_t* out) {
     if (nprobe > n_centroids) nprobe = n_centroids;
-    float* best_d = (float*)malloc((nprobe ? nprobe : 1) * sizeof(float));
+    uint32_t* best_d = (uint32_t*)malloc((nprobe ? nprobe : 1) * sizeof(uint32_t));
     size_t filled = 0;
     for (size_t c = 0; c < n_centroids; ++c) {
-        float dist = or_l2_f32(query, centroids + c * d, d);
+        uint32_t dist = total_key(or_l2_f32(query, centroids + c * d, d));
         if (filled == nprobe && (nprobe == 0 || dist >= best_d[nprobe - 1])) continue;
         size_t pos = filled < nprobe ? filled : nprobe - 1;
         while (pos > 0 && best_d[pos - 1] > dist) {

@@ -17,6 +17,8 @@
 #include "../../include/svs_gpu.h"
 #include "svs_internal.h"
 #include "svs_kernels.h"
+#include "svs_plan.h"
+#include "svs_route.h"
 
 using namespace svs;
 
@@ -268,6 +270,8 @@ extern "C" int svs_index_destroy(svs_index* idx) {
         dev_free(ctx, idx->l_dev, sizeof(uint64_t) * N * N1);
         dev_free(ctx, idx->lt_dev, sizeof(uint64_t) * N * N1);
         dev_free(ctx, idx->lists_dev, sizeof(ListDev) * idx->lists.size());
+        if (idx->cent_dev)
+            dev_free(ctx, idx->cent_dev, sizeof(float) * idx->lists.size() * idx->cent_d);
         if (ctx->cached_plan_index == idx) ctx->cached_plan_index = nullptr;
         delete idx;
     }
@@ -413,6 +417,7 @@ static int publish_list(svs_index* idx, uint32_t list) {
     HIP_TRY(hipMemcpy(idx->lists_dev + list, &d, sizeof(ListDev), hipMemcpyHostToDevice));
     idx->ctx->uploads++;
     idx->ctx->cached_plan_list = -1;
+    idx->top_rows_dirty = true;
     return SVS_OK;
 }
 
@@ -681,6 +686,9 @@ int svs::make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
             if (pl.item_verify[i]) pl.vgitems.push_back(i);
         make_tiles(pl.vgitems, pl.vtiles);
     }
+    pl.hdr[PH_NT] = (uint32_t)pl.tiles.size();
+    pl.hdr[PH_NVT] = (uint32_t)pl.vtiles.size();
+    pl.hdr[PH_NCANDS] = pl.n_cands;
     return SVS_OK;
 }
 
@@ -802,7 +810,7 @@ extern "C" int svs_ctx_get_tile_variant(svs_ctx* ctx, uint32_t n_queries, int* v
 struct TileArgs {
     const ListDev* lists;
     const MTile* tiles;
-    uint32_t nt;
+    const uint32_t* nt;  // tile count, in device memory
     const uint32_t* gitems;
     const Item* items;
     const uint32_t* cand_ptr;
@@ -868,46 +876,204 @@ static void tile_shape(int kernel, int variant, uint32_t& rows, uint32_t& qg) {
     else rows = PB_ROWS, qg = kTileVariants[variant].qg;
 }
 
-// Device tile buffer: [tiles][gitems] then, 16-byte aligned, [vtiles][vgitems].
-static size_t vtiles_offset(const BatchPlan& pl) {
-    return (sizeof(MTile) * pl.tiles.size() + sizeof(uint32_t) * pl.gitems.size() + 15) & ~(size_t)15;
+// Plan layout in a slot. prefixes: chunk_ptr [n_items + 1], cand_ptr
+// [n_items + 1], item_ptr [nq + 1], qchunk_ptr [nq + 1], hdr [PH_WORDS],
+// planner scratch [2 (n_items + 1)]. mtiles: tiles [cap_tiles], gitems
+// [n_items], then 16-byte aligned vtiles [cap_tiles], vgitems [n_items].
+int svs::plan_layout(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
+                     uint64_t cap_tiles, PlanView& v) {
+    const uint64_t ni = n_items;
+    SVS_TRY(sl.items.ensure(ctx, sizeof(Item) * std::max<uint64_t>(ni, 1)));
+    SVS_TRY(sl.prefixes.ensure(ctx, sizeof(uint32_t) * (4 * (ni + 1) + 2 * (nq + 1) + PH_WORDS)));
+    SVS_TRY(sl.flags.ensure(ctx, ni + 64));
+    const size_t half = (sizeof(MTile) * cap_tiles + sizeof(uint32_t) * ni + 15) & ~(size_t)15;
+    SVS_TRY(sl.mtiles.ensure(ctx, 2 * half + 16));
+    uint32_t* pre = (uint32_t*)sl.prefixes.ptr;
+    v.nq = nq;
+    v.n_items = n_items;
+    v.items = (const Item*)sl.items.ptr;
+    v.chunk_ptr = pre;
+    v.cand_ptr = pre + ni + 1;
+    v.item_ptr = pre + 2 * (ni + 1);
+    v.qchunk_ptr = v.item_ptr + nq + 1;
+    v.hdr = v.qchunk_ptr + nq + 1;
+    v.flags = (const uint8_t*)sl.flags.ptr;
+    v.tiles = (const MTile*)sl.mtiles.ptr;
+    v.gitems = (const uint32_t*)(v.tiles + cap_tiles);
+    v.vtiles = (const MTile*)((char*)sl.mtiles.ptr + half);
+    v.vgitems = (const uint32_t*)(v.vtiles + cap_tiles);
+    v.cap_tiles = v.cap_vtiles = cap_tiles;
+    return SVS_OK;
 }
-static size_t mtile_bytes(const BatchPlan& pl) {
-    return vtiles_offset(pl) + sizeof(MTile) * pl.vtiles.size() +
-           sizeof(uint32_t) * pl.vgitems.size();
+
+// Uploads a host plan into the slot's plan layout.
+static int upload_plan(svs_ctx* ctx, svs_slot& sl, const BatchPlan& pl, PlanView& v) {
+    hipStream_t st = sl.stream;
+    const uint64_t cap_tiles = std::max<size_t>(pl.tiles.size(), pl.vtiles.size());
+    SVS_TRY(plan_layout(ctx, sl, pl.n_queries, pl.n_items, cap_tiles, v));
+    v.cap_cands = pl.n_cands;
+    v.cap_chunks = pl.n_chunks;
+    v.cap_tiles = pl.tiles.size();
+    v.cap_vtiles = pl.vtiles.size();
+    v.cap_tchunks = pl.n_tchunks;
+    v.any_verify = pl.any_verify;
+    v.planes_ok = pl.planes_ok;
+    v.distinct_bytes = pl.distinct_bytes;
+    v.n_cands = pl.n_cands;
+    const uint32_t nq = pl.n_queries, ni = pl.n_items;
+    auto up = [&](const void* dst, const void* src, size_t bytes) -> int {
+        if (bytes)
+            HIP_TRY(hipMemcpyAsync(const_cast<void*>(dst), src, bytes, hipMemcpyHostToDevice, st));
+        return SVS_OK;
+    };
+    SVS_TRY(up(v.items, pl.items.data(), sizeof(Item) * ni));
+    SVS_TRY(up(v.chunk_ptr, pl.chunk_ptr.data(), sizeof(uint32_t) * (ni + 1)));
+    SVS_TRY(up(v.cand_ptr, pl.cand_ptr.data(), sizeof(uint32_t) * (ni + 1)));
+    SVS_TRY(up(v.item_ptr, pl.item_ptr.data(), sizeof(uint32_t) * (nq + 1)));
+    SVS_TRY(up(v.qchunk_ptr, pl.qchunk_ptr.data(), sizeof(uint32_t) * (nq + 1)));
+    SVS_TRY(up(v.hdr, pl.hdr, sizeof(pl.hdr)));
+    SVS_TRY(up(v.flags, pl.item_verify.data(), ni));
+    SVS_TRY(up(v.tiles, pl.tiles.data(), sizeof(MTile) * pl.tiles.size()));
+    SVS_TRY(up(v.gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size()));
+    SVS_TRY(up(v.vtiles, pl.vtiles.data(), sizeof(MTile) * pl.vtiles.size()));
+    SVS_TRY(up(v.vgitems, pl.vgitems.data(), sizeof(uint32_t) * pl.vgitems.size()));
+    return SVS_OK;
 }
-static MTile* vtiles_ptr(svs_slot& sl, const BatchPlan& pl) {
-    return (MTile*)((char*)sl.mtiles.ptr + vtiles_offset(pl));
+
+bool svs::device_plan_ok(const svs_index* idx, uint32_t n_items) {
+    return n_items <= PLAN_MAX_ITEMS && idx->lists.size() < PLAN_MAX_LISTS;
+}
+
+uint64_t svs::top_rows(svs_index* idx, uint32_t n) {
+    if (idx->top_rows_dirty) {
+        std::vector<uint32_t> m(idx->lists.size());
+        for (size_t c = 0; c < m.size(); ++c) m[c] = idx->lists[c].m;
+        std::sort(m.begin(), m.end(), std::greater<uint32_t>());
+        idx->top_rows_prefix.assign(m.size() + 1, 0);
+        for (size_t c = 0; c < m.size(); ++c)
+            idx->top_rows_prefix[c + 1] = idx->top_rows_prefix[c] + m[c];
+        idx->top_rows_dirty = false;
+    }
+    return idx->top_rows_prefix[std::min<size_t>(n, idx->lists.size())];
+}
+
+int svs::plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t* probe_ptr,
+                      const uint32_t* probe_lists, const uint8_t* verify, uint64_t cands_bound,
+                      int variant, PlanView& v) {
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t n_items = probe_ptr[nq];
+    if (!device_plan_ok(idx, n_items))
+        return fail(SVS_ERR_ARG, "device planning takes up to %u items and %u lists",
+                    PLAN_MAX_ITEMS, PLAN_MAX_LISTS - 1);
+    const uint64_t cap_cands = std::min<uint64_t>(cands_bound, 0xFFFFFFFEull);
+    uint32_t trows, tqg;
+    tile_shape(KERNEL_MFMA, variant, trows, tqg);
+    // tiles = sum over list groups of ceil(m / trows) ceil(n_g / qg)
+    //      <= sum of n_g (m / trows + 1) = rows / trows + items
+    const uint64_t cap_tiles = cap_cands / trows + n_items;
+    SVS_TRY(plan_layout(ctx, sl, nq, n_items, cap_tiles, v));
+    v.cap_cands = cap_cands;
+    v.cap_chunks = cap_cands / CHUNK_ROWS + n_items;
+    v.cap_tchunks = cap_cands / TOPK_WCHUNK + nq;
+    v.any_verify = false;
+    if (verify)
+        for (uint32_t q = 0; q < nq; ++q) v.any_verify |= verify[q] != 0;
+    v.planes_ok = true;
+    for (const ListHost& L : idx->lists)
+        if (L.m && !L.d.mp8) v.planes_ok = false;
+    SVS_TRY(sl.q_probes.ensure(ctx, sizeof(uint32_t) * ((size_t)n_items + nq + 1) + nq + 64));
+    uint32_t* d_probes = (uint32_t*)sl.q_probes.ptr;
+    uint32_t* d_pptr = d_probes + n_items;
+    uint8_t* d_ver = (uint8_t*)(d_pptr + nq + 1);
+    hipStream_t st = sl.stream;
+    if (probe_lists && n_items)
+        HIP_TRY(hipMemcpyAsync(d_probes, probe_lists, sizeof(uint32_t) * n_items,
+                               hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_pptr, probe_ptr, sizeof(uint32_t) * (nq + 1), hipMemcpyHostToDevice,
+                           st));
+    if (verify)
+        HIP_TRY(hipMemcpyAsync(d_ver, verify, nq, hipMemcpyHostToDevice, st));
+    else
+        hipLaunchKernelGGL(k_fill_u32, dim3(((nq + 3) / 4 + 255) / 256), dim3(256), 0, st,
+                           (uint32_t*)d_ver, (nq + 3) / 4, 0u);
+    v.probes = d_probes;
+    return SVS_OK;
+}
+
+int svs::plan_launch(svs_index* idx, svs_slot& sl, const PlanView& v, int variant) {
+    svs_ctx* ctx = idx->ctx;
+    uint32_t trows, tqg;
+    tile_shape(KERNEL_MFMA, variant, trows, tqg);
+    const uint32_t n_items = v.n_items, nq = v.nq;
+    const uint32_t* d_probes = (const uint32_t*)sl.q_probes.ptr;
+    const uint32_t* d_pptr = d_probes + n_items;
+    const uint8_t* d_ver = (const uint8_t*)(d_pptr + nq + 1);
+    uint32_t* tmp = const_cast<uint32_t*>(v.hdr) + PH_WORDS;
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_THREADS), 0, sl.stream, idx->lists_dev,
+                       (uint32_t)idx->lists.size(), d_probes, d_pptr, d_ver, nq, n_items, trows,
+                       tqg, (uint32_t)v.cap_cands, (uint32_t)v.cap_tchunks,
+                       (uint32_t)v.cap_tiles, const_cast<Item*>(v.items),
+                       const_cast<uint8_t*>(v.flags), const_cast<uint32_t*>(v.chunk_ptr),
+                       const_cast<uint32_t*>(v.cand_ptr), const_cast<uint32_t*>(v.item_ptr),
+                       const_cast<uint32_t*>(v.qchunk_ptr), tmp, const_cast<uint32_t*>(v.gitems),
+                       const_cast<MTile*>(v.tiles), const_cast<uint32_t*>(v.vgitems),
+                       const_cast<MTile*>(v.vtiles), const_cast<uint32_t*>(v.hdr));
+    ctx->launches++;
+    HIP_TRY(hipGetLastError());
+    return SVS_OK;
+}
+
+int svs::route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items,
+                      double q_scale) {
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t nl = (uint32_t)idx->lists.size(), d = idx->cent_d;
+    hipStream_t st = sl.stream;
+    SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.route_dist.ensure(ctx, sizeof(float) * (size_t)nq * nl));
+    const float* x = (const float*)sl.q_raw.ptr;
+    hipLaunchKernelGGL(k_quantise, dim3((uint32_t)std::min<uint64_t>(((uint64_t)nq * N + 255) / 256,
+                                                                      (uint64_t)ctx->n_cu * 8)),
+                       dim3(256), 0, st, x, nq, d, q_scale, (uint64_t*)sl.q_vq.ptr);
+    hipLaunchKernelGGL(k_route_dist, dim3((nq + RT_Q - 1) / RT_Q, (nl + RT_C - 1) / RT_C),
+                       dim3(256), 0, st, x, nq, (const float*)idx->cent_dev, nl, d,
+                       (float*)sl.route_dist.ptr);
+    uint32_t p2 = 1;
+    while (p2 < nl) p2 <<= 1;
+    uint32_t* d_probes = (uint32_t*)sl.q_probes.ptr;
+    hipLaunchKernelGGL(k_route_select, dim3(nq), dim3(1024), sizeof(uint64_t) * p2, st,
+                       (const float*)sl.route_dist.ptr, nl, (const uint32_t*)(d_probes + n_items),
+                       d_probes);
+    ctx->launches += 3;
+    HIP_TRY(hipGetLastError());
+    return SVS_OK;
+}
+
+// blocks of a tile kernel: one per tile up to the resident limit
+static uint32_t tile_blocks(svs_ctx* ctx, uint64_t cap, uint32_t per_cu) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, (uint64_t)ctx->n_cu * per_cu));
 }
 
 int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
-                    bool product_only,
-                    int kernel, bool encode, int variant) {
+                  bool product_only, int kernel, bool encode, int variant) {
+    PlanView v;
+    SVS_TRY(upload_plan(idx->ctx, sl, pl, v));
+    return run_view(idx, sl, v, k, product_only, kernel, encode, variant);
+}
+
+int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
+                  bool product_only, int kernel, bool encode, int variant) {
     svs_ctx* ctx = idx->ctx;
     hipStream_t st = sl.stream;
-    const uint32_t nq = pl.n_queries;
-    // plan arrays
-    SVS_TRY(sl.items.ensure(ctx, sizeof(Item) * std::max<uint32_t>(pl.n_items, 1)));
-    SVS_TRY(sl.prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (pl.n_items + 1) + 2 * (nq + 1))));
-    SVS_TRY(sl.flags.ensure(ctx, std::max<uint32_t>(pl.n_items, 1) + 64));
-    uint32_t* d_chunk_ptr = (uint32_t*)sl.prefixes.ptr;
-    uint32_t* d_cand_ptr = d_chunk_ptr + pl.n_items + 1;
-    uint32_t* d_item_ptr = d_cand_ptr + pl.n_items + 1;
-    uint32_t* d_qchunk_ptr = d_item_ptr + nq + 1;
-    HIP_TRY(hipMemcpyAsync(sl.items.ptr, pl.items.data(), sizeof(Item) * pl.n_items,
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_chunk_ptr, pl.chunk_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_cand_ptr, pl.cand_ptr.data(), sizeof(uint32_t) * (pl.n_items + 1),
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_item_ptr, pl.item_ptr.data(), sizeof(uint32_t) * (nq + 1),
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_qchunk_ptr, pl.qchunk_ptr.data(), sizeof(uint32_t) * (nq + 1),
-                           hipMemcpyHostToDevice, st));
-    if (pl.n_items)
-        HIP_TRY(hipMemcpyAsync(sl.flags.ptr, pl.item_verify.data(), pl.n_items,
-                               hipMemcpyHostToDevice, st));
-    const Item* d_items = (const Item*)sl.items.ptr;
+    const uint32_t nq = v.nq;
+    const uint32_t ni1 = std::max<uint32_t>(v.n_items, 1);
+    const uint64_t nc1 = std::max<uint64_t>(v.cap_cands, 1);
+    const Item* d_items = v.items;
+    const uint32_t* d_chunk_ptr = v.chunk_ptr;
+    const uint32_t* d_cand_ptr = v.cand_ptr;
+    const uint32_t* d_item_ptr = v.item_ptr;
+    const uint32_t* d_qchunk_ptr = v.qchunk_ptr;
+    const uint32_t* d_nt = v.hdr + PH_NT;
+    const uint32_t* d_nvt = v.hdr + PH_NVT;
     const ListDev* d_lists = idx->lists_dev;
     // per-query encode
     uint64_t* d_venc = (uint64_t*)sl.q_venc.ptr;
@@ -917,9 +1083,9 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
     uint32_t* d_tcnt = (uint32_t*)sl.q_tcnt.ptr;
     uint32_t* d_vseed = (uint32_t*)sl.q_vseed.ptr;
     // verdicts, per-query status and the RNG flag (set by encode or verify)
-    SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (std::max<uint32_t>(pl.n_items, 1) + nq) + 64));
+    SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (ni1 + nq) + 64));
     int32_t* d_item_verdict = (int32_t*)sl.verdict.ptr;
-    int32_t* d_qstatus = d_item_verdict + std::max<uint32_t>(pl.n_items, 1);
+    int32_t* d_qstatus = d_item_verdict + ni1;
     uint32_t* d_rng_flag = (uint32_t*)(d_qstatus + nq);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_rng_flag, 1u, 0u);
     SVS_TRY(prof_mark(sl, 0, true));
@@ -938,44 +1104,25 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
     SVS_TRY(prof_mark(sl, 0, false));
     // server product
     SVS_TRY(prof_mark(sl, 1, true));
-    SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
-    if (pl.n_chunks && kernel == KERNEL_VALU) {
-        hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS), 0,
-                           st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
-                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (uint64_t*)sl.r.ptr);
+    SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc1));
+    if (v.cap_chunks && kernel == KERNEL_VALU) {
+        hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, v.cap_chunks)), dim3(ROW_THREADS), 0,
+                           st, d_lists, d_items, v.n_items, d_chunk_ptr, d_cand_ptr,
+                           (const uint64_t*)d_venc, (uint64_t*)sl.r.ptr);
         ctx->launches++;
         HIP_TRY(hipGetLastError());
-    } else if (pl.n_chunks) {
-        const uint32_t nt = (uint32_t)pl.tiles.size();
-        SVS_TRY(sl.mtiles.ensure(ctx, mtile_bytes(pl)));
+    } else if (v.cap_chunks) {
         SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
-        MTile* d_tiles = (MTile*)sl.mtiles.ptr;
-        uint32_t* d_gitems = (uint32_t*)(d_tiles + nt);
-        HIP_TRY(hipMemcpyAsync(d_tiles, pl.tiles.data(), sizeof(MTile) * nt,
-                               hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d_gitems, pl.gitems.data(), sizeof(uint32_t) * pl.gitems.size(),
-                               hipMemcpyHostToDevice, st));
-        if (!pl.vtiles.empty()) {
-            HIP_TRY(hipMemcpyAsync(vtiles_ptr(sl, pl), pl.vtiles.data(),
-                                   sizeof(MTile) * pl.vtiles.size(), hipMemcpyHostToDevice, st));
-            HIP_TRY(hipMemcpyAsync(vtiles_ptr(sl, pl) + pl.vtiles.size(), pl.vgitems.data(),
-                                   sizeof(uint32_t) * pl.vgitems.size(), hipMemcpyHostToDevice,
-                                   st));
-        }
         hipLaunchKernelGGL(k_query_corr<N>, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
                            (uint64_t*)sl.q_corr.ptr);
         if (kernel == KERNEL_MFMA_WAVE) {
-            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>((nt + 3) / 4, (uint64_t)ctx->n_cu * 8));
-            hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists,
-                               (const MTile*)d_tiles, nt, (const uint32_t*)d_gitems, d_items,
-                               (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
+            const uint32_t blocks = tile_blocks(ctx, (v.cap_tiles + 3) / 4, 8);
+            hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, st, d_lists, v.tiles, d_nt,
+                               v.gitems, d_items, d_cand_ptr, (const uint64_t*)d_venc,
                                (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         } else {
-            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
-            const TileArgs ta{d_lists, d_tiles, nt, d_gitems, d_items, d_cand_ptr};
+            const uint32_t blocks = tile_blocks(ctx, v.cap_tiles, 4 * kTileVariants[variant].occ);
+            const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
             launch_gemm<N, 0>(variant, st, blocks, ta, (const uint64_t*)d_venc,
                               (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         }
@@ -983,73 +1130,63 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
         HIP_TRY(hipGetLastError());
     }
     SVS_TRY(prof_mark(sl, 1, false));
-    sl.prof.gemv_bytes += pl.distinct_bytes;
-    sl.prof.gemv_rows += pl.n_cands;
-    if (ctx->fault_list >= 0 && pl.n_items) {
-        hipLaunchKernelGGL(k_inject_fault, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
-                           d_items, pl.n_items, (const uint32_t*)d_cand_ptr,
-                           (uint32_t)ctx->fault_list, ctx->fault_row, ctx->fault_delta,
-                           (uint64_t*)sl.r.ptr);
+    sl.prof.gemv_bytes += v.distinct_bytes;
+    sl.prof.gemv_rows += v.n_cands;
+    if (ctx->fault_list >= 0 && v.n_items) {
+        hipLaunchKernelGGL(k_inject_fault, dim3((v.n_items + 255) / 256), dim3(256), 0, st,
+                           d_items, v.n_items, d_cand_ptr, (uint32_t)ctx->fault_list,
+                           ctx->fault_row, ctx->fault_delta, (uint64_t*)sl.r.ptr);
         HIP_TRY(hipGetLastError());
     }
     if (product_only) return SVS_OK;
     // verification
     SVS_TRY(prof_mark(sl, 2, true));
-    if (pl.any_verify) {
-        SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA * std::max<uint32_t>(pl.n_chunks, 1)));
+    if (v.any_verify) {
+        SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA *
+                                            std::max<uint64_t>(v.cap_chunks, 1)));
         // The check's right-hand side sum_i u_t[i] (M_enc[i] . v_enc) needs
         // the row products again: on the MFMA path they are recomputed by
         // the product tiles over the verified items only, independently of
         // the r the server returned; otherwise k_verify_chunks recomputes
         // them row by row.
         const uint64_t* r_check = nullptr;
-        if (kernel == KERNEL_MFMA && !pl.vtiles.empty()) {
-            SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * std::max<uint32_t>(pl.n_cands, 1)));
-            const uint32_t nvt = (uint32_t)pl.vtiles.size();
-            const MTile* d_vtiles = vtiles_ptr(sl, pl);
-            const TileArgs va{d_lists, d_vtiles, nvt, (const uint32_t*)(d_vtiles + nvt), d_items,
-                              d_cand_ptr};
-            const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(nvt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
+        if (kernel == KERNEL_MFMA && v.cap_vtiles) {
+            SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc1));
+            const TileArgs va{d_lists, v.vtiles, d_nvt, v.vgitems, d_items, d_cand_ptr};
+            const uint32_t blocks = tile_blocks(ctx, v.cap_vtiles, 4 * kTileVariants[variant].occ);
             launch_gemm<N, 0>(variant, st, blocks, va, (const uint64_t*)d_venc,
                               (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.rv.ptr);
             ctx->launches++;
             r_check = (const uint64_t*)sl.rv.ptr;
         }
-        hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, pl.n_chunks)), dim3(ROW_THREADS),
-                           0, st, d_lists, d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
-                           (const uint32_t*)d_cand_ptr, (const uint64_t*)d_venc,
-                           (const uint64_t*)sl.r.ptr, r_check, (const uint32_t*)d_vseed,
-                           (const uint8_t*)sl.flags.ptr, (uint64_t*)sl.partials.ptr,
+        hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, std::max<uint64_t>(v.cap_chunks, 1))),
+                           dim3(ROW_THREADS), 0, st, d_lists, d_items, v.n_items, d_chunk_ptr,
+                           d_cand_ptr, (const uint64_t*)d_venc, (const uint64_t*)sl.r.ptr, r_check,
+                           (const uint32_t*)d_vseed, v.flags, (uint64_t*)sl.partials.ptr,
                            d_rng_flag);
-        hipLaunchKernelGGL(k_verify_finalize, dim3((pl.n_items + 255) / 256), dim3(256), 0, st,
-                           d_items, pl.n_items, (const uint32_t*)d_chunk_ptr,
-                           (const uint8_t*)sl.flags.ptr, (const uint64_t*)sl.partials.ptr,
+        hipLaunchKernelGGL(k_verify_finalize, dim3((ni1 + 255) / 256), dim3(256), 0, st, d_items,
+                           v.n_items, d_chunk_ptr, v.flags, (const uint64_t*)sl.partials.ptr,
                            d_item_verdict);
         ctx->launches += 2;
         HIP_TRY(hipGetLastError());
     }
     SVS_TRY(prof_mark(sl, 2, false));
     hipLaunchKernelGGL(k_query_status, dim3((nq + 255) / 256), dim3(256), 0, st, d_items,
-                       (const uint32_t*)d_item_ptr, nq, (const int32_t*)d_item_verdict,
-                       d_qstatus, pl.any_verify ? 1 : 0);
+                       d_item_ptr, nq, (const int32_t*)d_item_verdict, d_qstatus,
+                       v.any_verify ? 1 : 0);
     // decode
-    SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * std::max<uint32_t>(pl.n_cands, 1)));
+    SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc1));
     SVS_TRY(prof_mark(sl, 3, true));
-    if (pl.n_cands) {
+    if (v.cap_cands) {
         // The dense decode terms AL G + H L^T v_enc: the same MFMA tiles over
         // the row-major [AL | H] when the batch runs on block tiles, else
         // inside the per-candidate decode kernel.
         const uint64_t* dense = nullptr;
-        const bool tiles = kernel == KERNEL_MFMA && pl.n_chunks;
-        const uint32_t nt = (uint32_t)pl.tiles.size();
-        const MTile* d_tiles = (const MTile*)sl.mtiles.ptr;
-        const uint32_t* d_gitems = (const uint32_t*)(d_tiles + nt);
-        const uint32_t tblocks = (uint32_t)std::max<uint64_t>(
-            1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4 * kTileVariants[variant].occ));
-        const TileArgs ta{d_lists, d_tiles, nt, d_gitems, d_items, d_cand_ptr};
+        const bool tiles = kernel == KERNEL_MFMA && v.cap_chunks;
+        const uint32_t tblocks = tile_blocks(ctx, v.cap_tiles, 4 * kTileVariants[variant].occ);
+        const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
         if (tiles) {
-            SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * pl.n_cands));
+            SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
             uint64_t* qc_gl = (uint64_t*)sl.q_corr.ptr + nq;
             hipLaunchKernelGGL(k_query_corr<2 * N1>, dim3(nq), dim3(WAVE), 0, st,
                                (const uint64_t*)d_gl, qc_gl);
@@ -1058,27 +1195,25 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
             ctx->launches += 2;
             dense = (const uint64_t*)sl.dense.ptr;
         }
-        if (tiles && pl.planes_ok) {
+        if (tiles && v.planes_ok) {
             // M T and S v_enc on the same tiles; the decode ends there.
             SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
             uint64_t* mt_q = (uint64_t*)sl.q_corr.ptr + 2 * nq;
             hipLaunchKernelGGL(k_decode_prep, dim3(nq), dim3(256), 0, st, (const uint64_t*)d_venc,
                                (const int8_t*)d_tdense, (int8_t*)sl.q_vplanes.ptr, mt_q);
-            const uint32_t dblocks = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(nt, (uint64_t)ctx->n_cu * 4));
-            launch_decode_tiles(variant, st, dblocks, ta, (const int8_t*)d_tdense,
-                                (const int8_t*)sl.q_vplanes.ptr, (const uint64_t*)mt_q,
-                                (const uint64_t*)sl.r.ptr, dense, (int64_t*)sl.scores.ptr);
+            launch_decode_tiles(variant, st, tile_blocks(ctx, v.cap_tiles, 4), ta,
+                                (const int8_t*)d_tdense, (const int8_t*)sl.q_vplanes.ptr,
+                                (const uint64_t*)mt_q, (const uint64_t*)sl.r.ptr, dense,
+                                (int64_t*)sl.scores.ptr);
             ctx->launches += 2;
         } else {
-            const uint32_t blocks = (uint32_t)std::min<uint64_t>((pl.n_cands + 255) / 256,
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((v.cap_cands + 255) / 256,
                                                                  (uint64_t)ctx->n_cu * 32);
             hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
-                               pl.n_items, (const uint32_t*)d_cand_ptr,
-                               (const uint64_t*)sl.r.ptr, (const uint64_t*)d_gl,
-                               (const uint64_t*)d_venc, (const uint16_t*)d_tlist,
-                               (const uint32_t*)d_tcnt, dense, (int64_t*)sl.scores.ptr,
-                               (uint64_t*)nullptr);
+                               v.n_items, d_cand_ptr, (const uint64_t*)sl.r.ptr,
+                               (const uint64_t*)d_gl, (const uint64_t*)d_venc,
+                               (const uint16_t*)d_tlist, (const uint32_t*)d_tcnt, dense,
+                               (int64_t*)sl.scores.ptr, (uint64_t*)nullptr);
             ctx->launches++;
         }
         HIP_TRY(hipGetLastError());
@@ -1092,27 +1227,25 @@ int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
     int64_t* d_oscores = (int64_t*)(d_oids + (size_t)nq * k + ((nq * k) & 1));
     uint32_t* d_ocount = (uint32_t*)(d_oscores + (size_t)nq * k);
     if (k <= (uint32_t)TOPK_LANE) {
-        SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * (size_t)std::max<uint32_t>(pl.n_tchunks, 1) * k));
-        if (pl.n_tchunks)
-            hipLaunchKernelGGL(k_topk_chunks, dim3((pl.n_tchunks + 3) / 4), dim3(256), 0, st,
-                               (const uint32_t*)d_qchunk_ptr, nq, pl.n_tchunks,
-                               (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr,
+        SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * std::max<uint64_t>(v.cap_tchunks, 1) * k));
+        if (v.cap_tchunks)
+            hipLaunchKernelGGL(k_topk_chunks,
+                               dim3((uint32_t)std::min<uint64_t>((v.cap_tchunks + 3) / 4,
+                                                                 (uint64_t)ctx->n_cu * 16)),
+                               dim3(256), 0, st, d_qchunk_ptr, nq, d_item_ptr, d_cand_ptr,
                                (const int64_t*)sl.scores.ptr, k, (Cand*)sl.topk_part.ptr);
-        hipLaunchKernelGGL((k_topk_final<256>), dim3(nq), dim3(256), 0, st, d_items,
-                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const uint32_t*)d_qchunk_ptr, (const Cand*)sl.topk_part.ptr,
+        hipLaunchKernelGGL((k_topk_final<256>), dim3(nq), dim3(256), 0, st, d_items, d_item_ptr,
+                           d_cand_ptr, d_lists, d_qchunk_ptr, (const Cand*)sl.topk_part.ptr,
                            (const int32_t*)d_qstatus, k, d_oids, d_oscores, d_ocount);
         ctx->launches++;
     } else if (k <= 128)
-        hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items,
-                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const int64_t*)sl.scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
-                           d_oscores, d_ocount);
+        hipLaunchKernelGGL((k_topk<128, 32>), dim3(nq), dim3(32), 0, st, d_items, d_item_ptr,
+                           d_cand_ptr, d_lists, (const int64_t*)sl.scores.ptr,
+                           (const int32_t*)d_qstatus, k, d_oids, d_oscores, d_ocount);
     else
         hipLaunchKernelGGL(k_topk_large, dim3(nq), dim3(TOPK_LARGE_THREADS), 0, st, d_items,
-                           (const uint32_t*)d_item_ptr, (const uint32_t*)d_cand_ptr, d_lists,
-                           (const int64_t*)sl.scores.ptr, (const int32_t*)d_qstatus, k, d_oids,
-                           d_oscores, d_ocount);
+                           d_item_ptr, d_cand_ptr, d_lists, (const int64_t*)sl.scores.ptr,
+                           (const int32_t*)d_qstatus, k, d_oids, d_oscores, d_ocount);
     ctx->launches += 2;
     HIP_TRY(hipGetLastError());
     SVS_TRY(prof_mark(sl, 4, false));
@@ -1129,10 +1262,12 @@ int svs::stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* 
     SVS_TRY(sl.q_tlist.ensure(ctx, sizeof(uint16_t) * (size_t)nq * N));
     SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
     SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
-    HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
-                           hipMemcpyDefault, sl.stream));
-    HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyDefault,
-                           sl.stream));
+    if (v_q)
+        HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, v_q, sizeof(uint64_t) * (size_t)nq * N,
+                               hipMemcpyDefault, sl.stream));
+    if (seeds)
+        HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyDefault,
+                               sl.stream));
     return SVS_OK;
 }
 
@@ -1144,9 +1279,9 @@ int svs::reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
     const uint64_t ni = std::max<uint32_t>(n_items, 1), nc = std::max<uint64_t>(n_cands, 1);
     const uint64_t chunks = nc / CHUNK_ROWS + ni;
     const uint64_t tiles = nc / 64 + ni;  // generous for any tile shape
-    SVS_TRY(sl.items.ensure(ctx, sizeof(Item) * ni));
-    SVS_TRY(sl.prefixes.ensure(ctx, sizeof(uint32_t) * (3 * (ni + 1) + 2 * (nq + 1))));
-    SVS_TRY(sl.flags.ensure(ctx, ni + 64));
+    PlanView v;
+    SVS_TRY(plan_layout(ctx, sl, nq, (uint32_t)ni, tiles, v));
+    SVS_TRY(sl.q_probes.ensure(ctx, sizeof(uint32_t) * ni + nq + 64));
     SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
     SVS_TRY(sl.q_seeds.ensure(ctx, (size_t)nq * 32));
     SVS_TRY(sl.q_venc.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
@@ -1165,7 +1300,6 @@ int svs::reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
     SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (ni + nq) + 64));
     SVS_TRY(sl.out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
                                    sizeof(uint32_t) * nq + 64));
-    SVS_TRY(sl.mtiles.ensure(ctx, 2 * (sizeof(MTile) * tiles + sizeof(uint32_t) * ni) + 16));
     SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * (nc / TOPK_WCHUNK + nq) * k));
     return SVS_OK;
 }
@@ -1223,6 +1357,57 @@ int svs::plan_for_mode(svs_index* idx, int mode, uint32_t nq, const uint32_t* pr
     return SVS_OK;
 }
 
+// Rows and validity of a batch's probe lists (host inputs): the exact
+// candidate-row count, and the checks make_plan / plan_for_mode apply.
+static int probe_rows(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
+                      const uint32_t* probe_lists, uint64_t& rows) {
+    if (probe_ptr[0] != 0) return fail(SVS_ERR_ARG, "probe_ptr[0] must be 0");
+    rows = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (probe_ptr[q + 1] < probe_ptr[q]) return fail(SVS_ERR_ARG, "probe_ptr not monotone");
+        for (uint32_t p = probe_ptr[q]; p < probe_ptr[q + 1]; ++p) {
+            const uint32_t c = probe_lists[p];
+            if (c >= idx->lists.size() || !idx->lists[c].m) continue;
+            const ListHost& L = idx->lists[c];
+            if (!L.d.mp_t32 && !L.d.mp_t64)
+                return fail(SVS_ERR_STATE, "list %u was uploaded without M_plain", c);
+            rows += L.m;
+        }
+    }
+    if (rows >= 0xFFFFFFFFull) return fail(SVS_ERR_ARG, "batch has too many candidate rows");
+    return SVS_OK;
+}
+
+// Reads back a device plan's overflow word (bounds are exact or upper
+// bounds, so a set flag is an internal error).
+static int check_plan_overflow(svs_slot& sl, const PlanView& v) {
+    uint32_t over = 0;
+    HIP_TRY(hipMemcpyAsync(&over, v.hdr + PH_OVERFLOW, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           sl.stream));
+    HIP_TRY(hipStreamSynchronize(sl.stream));
+    if (over) return fail(SVS_ERR_STATE, "device plan exceeded its bounds");
+    return SVS_OK;
+}
+
+// One batched run (mode BATCHED or BATCHED_VALU) planned on the device: the
+// host only validates the probe lists and counts their rows.
+static int batch_device_planned(svs_index* idx, const svs_query_batch* b, int mode) {
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t nq = b->n_queries;
+    uint64_t rows = 0;
+    SVS_TRY(probe_rows(idx, nq, b->probe_ptr, b->probe_lists, rows));
+    const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
+    const int variant = ctx_variant_for(ctx, nq);
+    PlanView v;
+    SVS_TRY(plan_prepare(idx, *ctx, nq, b->probe_ptr, b->probe_lists, b->verify, rows, variant, v));
+    SVS_TRY(stage_queries(ctx, *ctx, nq, b->v_q, b->q_seeds));
+    SVS_TRY(plan_launch(idx, *ctx, v, variant));
+    SVS_TRY(run_view(idx, *ctx, v, b->k, false, kernel, true, variant));
+    SVS_TRY(collect(ctx, *ctx, nq, b->k, v.n_items, b->out_ids, b->out_scores, b->out_count,
+                    b->out_status));
+    return check_plan_overflow(*ctx, v);
+}
+
 // dev: v_q, q_seeds and the outputs are device pointers (checked for
 // canonical values on the device); otherwise host pointers.
 static int score_batch_impl(svs_index* idx, const svs_query_batch* b, int mode, bool dev) {
@@ -1251,6 +1436,16 @@ static int score_batch_impl(svs_index* idx, const svs_query_batch* b, int mode, 
         SVS_TRY(check_canonical(b->v_q, (size_t)b->n_queries * N, "v_q"));
     }
     const uint32_t nq_total = b->n_queries;
+    if (mode != SVS_MODE_PLAIN && ctx->planner &&
+        device_plan_ok(idx, b->probe_ptr[nq_total] - b->probe_ptr[0])) {
+        SVS_TRY(batch_device_planned(idx, b, mode));
+        if (dev) {
+            uint32_t bad = 0;
+            HIP_TRY(hipMemcpy(&bad, d_badv, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (bad) return fail(SVS_ERR_ARG, "v_q holds a non-canonical value (>= p)");
+        }
+        return SVS_OK;
+    }
     const uint32_t step = mode == SVS_MODE_PLAIN ? 1 : nq_total;
     for (uint32_t q0 = 0; q0 < nq_total; q0 += step) {
         const uint32_t nq = std::min(step, nq_total - q0);
@@ -1281,6 +1476,158 @@ extern "C" int svs_score_batch(svs_index* idx, const svs_query_batch* b, int mod
 
 extern "C" int svs_score_batch_dev(svs_index* idx, const svs_query_batch* b, int mode) {
     return score_batch_impl(idx, b, mode, true);
+}
+
+extern "C" int svs_ctx_set_planner(svs_ctx* ctx, int on_device) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->planner = on_device ? 1 : 0;
+    return SVS_OK;
+}
+
+// ======================================================= routing on device
+extern "C" int svs_index_set_centroids(svs_index* idx, const float* centroids, uint32_t d) {
+    if (!idx || !centroids) return fail(SVS_ERR_ARG, "null argument");
+    if (d == 0 || d > (uint32_t)N) return fail(SVS_ERR_ARG, "d = %u must be in 1..%d", d, N);
+    svs_ctx* ctx = idx->ctx;
+    const size_t nl = idx->lists.size();
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (idx->cent_dev) dev_free(ctx, idx->cent_dev, sizeof(float) * nl * idx->cent_d);
+    idx->cent_dev = nullptr;
+    idx->cent_d = 0;
+    float* p = nullptr;
+    SVS_TRY(dev_alloc(ctx, (void**)&p, sizeof(float) * nl * d));
+    HIP_TRY(hipMemcpy(p, centroids, sizeof(float) * nl * d, hipMemcpyHostToDevice));
+    idx->cent_dev = p;
+    idx->cent_d = d;
+    return SVS_OK;
+}
+
+static int route_ok(const svs_index* idx) {
+    if (!idx->cent_dev) return fail(SVS_ERR_STATE, "the index has no centroids (svs_index_set_centroids)");
+    if (idx->lists.size() > (size_t)RT_SELECT_MAX)
+        return fail(SVS_ERR_ARG, "device routing takes up to %d lists", RT_SELECT_MAX);
+    return SVS_OK;
+}
+
+// Stages nq raw queries and a route-only probe layout in the slot:
+// q_raw <- queries, q_probes = [probes][probe_ptr][verify], q_seeds.
+static int stage_raw(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t d, const float* x,
+                     const uint32_t* probe_ptr, const uint8_t* seeds) {
+    const uint32_t n_items = probe_ptr[nq];
+    hipStream_t st = sl.stream;
+    SVS_TRY(sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * d));
+    HIP_TRY(hipMemcpyAsync(sl.q_raw.ptr, x, sizeof(float) * (size_t)nq * d, hipMemcpyHostToDevice,
+                           st));
+    SVS_TRY(sl.q_probes.ensure(ctx, sizeof(uint32_t) * ((size_t)n_items + nq + 1) + nq + 64));
+    HIP_TRY(hipMemcpyAsync((uint32_t*)sl.q_probes.ptr + n_items, probe_ptr,
+                           sizeof(uint32_t) * (nq + 1), hipMemcpyHostToDevice, st));
+    if (seeds) {
+        SVS_TRY(sl.q_seeds.ensure(ctx, (size_t)nq * 32));
+        HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, seeds, (size_t)nq * 32, hipMemcpyHostToDevice, st));
+    }
+    return SVS_OK;
+}
+
+extern "C" int svs_route_batch(svs_index* idx, const float* queries, uint32_t n_queries,
+                               uint32_t nprobe, uint32_t* out) {
+    if (!idx || (n_queries && (!queries || !out))) return fail(SVS_ERR_ARG, "null argument");
+    SVS_TRY(route_ok(idx));
+    if (n_queries == 0) return SVS_OK;
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t np = std::min<uint32_t>(nprobe, (uint32_t)idx->lists.size());
+    std::vector<uint32_t> pptr(n_queries + 1);
+    for (uint32_t q = 0; q <= n_queries; ++q) pptr[q] = q * np;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(stage_raw(ctx, *ctx, n_queries, idx->cent_d, queries, pptr.data(), nullptr));
+    SVS_TRY(route_launch(idx, *ctx, n_queries, pptr[n_queries], (double)(1u << 20)));
+    HIP_TRY(hipMemcpyAsync(out, ctx->q_probes.ptr, sizeof(uint32_t) * pptr[n_queries],
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+extern "C" int svs_score_raw_batch(svs_index* idx, const svs_raw_batch* b, int mode) {
+    if (!idx || !b) return fail(SVS_ERR_ARG, "null argument");
+    if (b->n_queries == 0) return SVS_OK;
+    if (!b->queries || !b->q_seeds || !b->out_ids || !b->out_scores || !b->out_count ||
+        !b->out_status)
+        return fail(SVS_ERR_ARG, "missing batch array");
+    if (b->k == 0 || b->k > (uint32_t)TOPK_LARGE_MAX)
+        return fail(SVS_ERR_ARG, "k = %u must be in 1..%d", b->k, TOPK_LARGE_MAX);
+    if (mode != SVS_MODE_PLAIN && mode != SVS_MODE_BATCHED && mode != SVS_MODE_BATCHED_VALU)
+        return fail(SVS_ERR_ARG, "unknown mode %d", mode);
+    SVS_TRY(route_ok(idx));
+    svs_ctx* ctx = idx->ctx;
+    const uint32_t nq_total = b->n_queries, nl = (uint32_t)idx->lists.size(), d = idx->cent_d;
+    for (const ListHost& L : idx->lists)
+        if (L.m && !L.d.mp_t32 && !L.d.mp_t64)
+            return fail(SVS_ERR_STATE, "a list was uploaded without M_plain");
+    std::vector<uint32_t> np(nq_total);
+    for (uint32_t q = 0; q < nq_total; ++q)
+        np[q] = std::min<uint32_t>(b->nprobe ? b->nprobe[q] : b->nprobe_all, nl);
+    const double q_scale = (double)b->quantisation_q;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t step = mode == SVS_MODE_PLAIN ? 1 : nq_total;
+    uint32_t probe_base = 0;
+    for (uint32_t q0 = 0; q0 < nq_total; q0 += step) {
+        const uint32_t nq = std::min(step, nq_total - q0);
+        std::vector<uint32_t> pptr(nq + 1, 0);
+        uint64_t bound = 0;
+        for (uint32_t q = 0; q < nq; ++q) {
+            pptr[q + 1] = pptr[q] + np[q0 + q];
+            bound += top_rows(idx, np[q0 + q]);
+        }
+        const uint32_t n_items = pptr[nq];
+        const uint8_t* ver = b->verify ? b->verify + q0 : nullptr;
+        const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
+        const int variant = ctx_variant_for(ctx, nq);
+        const float* x = b->queries + (size_t)q0 * d;
+        const uint8_t* seeds = b->q_seeds + (size_t)q0 * 32;
+        uint32_t* o_ids = b->out_ids + (size_t)q0 * b->k;
+        int64_t* o_scores = b->out_scores + (size_t)q0 * b->k;
+        if (mode != SVS_MODE_PLAIN && ctx->planner && device_plan_ok(idx, n_items) &&
+            bound < 0xFFFFFFFFull) {
+            // route -> plan -> pipeline, one stream, no host round trip
+            PlanView v;
+            SVS_TRY(plan_prepare(idx, *ctx, nq, pptr.data(), nullptr, ver, bound, variant, v));
+            SVS_TRY(stage_raw(ctx, *ctx, nq, d, x, pptr.data(), seeds));
+            SVS_TRY(stage_queries(ctx, *ctx, nq, nullptr, nullptr));
+            SVS_TRY(route_launch(idx, *ctx, nq, n_items, q_scale));
+            SVS_TRY(plan_launch(idx, *ctx, v, variant));
+            SVS_TRY(run_view(idx, *ctx, v, b->k, false, kernel, true, variant));
+            if (b->out_probes && n_items)
+                HIP_TRY(hipMemcpyAsync(b->out_probes + probe_base, v.probes,
+                                       sizeof(uint32_t) * n_items, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            SVS_TRY(collect(ctx, *ctx, nq, b->k, n_items, o_ids, o_scores, b->out_count + q0,
+                            b->out_status + q0));
+            SVS_TRY(check_plan_overflow(*ctx, v));
+        } else {
+            // route on the device, then plan on the host
+            SVS_TRY(stage_raw(ctx, *ctx, nq, d, x, pptr.data(), seeds));
+            SVS_TRY(stage_queries(ctx, *ctx, nq, nullptr, nullptr));
+            SVS_TRY(route_launch(idx, *ctx, nq, n_items, q_scale));
+            std::vector<uint32_t> probes(std::max<uint32_t>(n_items, 1));
+            HIP_TRY(hipMemcpyAsync(probes.data(), ctx->q_probes.ptr, sizeof(uint32_t) * n_items,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (b->out_probes)
+                std::copy(probes.begin(), probes.begin() + n_items, b->out_probes + probe_base);
+            BatchPlan pl;
+            int kern, var;
+            SVS_TRY(plan_for_mode(idx, mode, nq, pptr.data(), probes.data(), ver, pl, kern, var));
+            SVS_TRY(run_plan(idx, *ctx, pl, b->k, false, kern, true, var));
+            SVS_TRY(collect(ctx, *ctx, nq, b->k, pl.n_items, o_ids, o_scores, b->out_count + q0,
+                            b->out_status + q0));
+        }
+        probe_base += n_items;
+    }
+    return SVS_OK;
 }
 
 // Times every block-tile variant on a real batch (MODE_BATCHED, `reps` timed

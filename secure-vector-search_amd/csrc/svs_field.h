@@ -65,6 +65,22 @@ SVS_HD uint64_t mul_pow2(uint64_t x, int s) {
     return r == P ? 0 : r;
 }
 
+// quantise_vector's element map (crypto.rs:105-122): round(x * q) with f64
+// round-half-away-from-zero, then into [0, p) with negatives as p - (-k mod p).
+// The f64 -> i64 conversion follows Rust's `as` (NaN -> 0, out of range
+// saturates) and the negation wraps, as in a release build.
+SVS_HD uint64_t quantise_scalar(float x, double q_scale) {
+    const double r = round((double)x * q_scale);
+    int64_t k;
+    if (r != r) k = 0;
+    else if (r >= 9223372036854775808.0) k = INT64_MAX;
+    else if (r <= -9223372036854775808.0) k = INT64_MIN;
+    else k = (int64_t)r;
+    if (k >= 0) return (uint64_t)k % P;
+    const uint64_t rr = (0ull - (uint64_t)k) % P;
+    return rr == 0 ? 0 : P - rr;
+}
+
 // ------------------------------------------------------------------ ChaCha20
 SVS_HD uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
 

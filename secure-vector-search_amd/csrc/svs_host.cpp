@@ -174,15 +174,7 @@ extern "C" int svs_quantise(const float* v, uint32_t rows, uint32_t d, uint64_t 
         for (uint32_t r = r0; r < r1; ++r) {
             uint64_t* o = out + (size_t)r * N;
             for (uint32_t j = 0; j < (uint32_t)N; ++j) o[j] = 0;
-            for (uint32_t j = 0; j < d; ++j) {
-                const int64_t k = (int64_t)std::round((double)v[(size_t)r * d + j] * qs);
-                if (k >= 0) {
-                    o[j] = (uint64_t)k % P;
-                } else {
-                    const uint64_t rr = (uint64_t)(-k) % P;
-                    o[j] = rr == 0 ? 0 : P - rr;
-                }
-            }
+            for (uint32_t j = 0; j < d; ++j) o[j] = quantise_scalar(v[(size_t)r * d + j], qs);
         }
     };
     const unsigned nt = rows >= 4096 ? worker_count() : 1u;
@@ -197,14 +189,24 @@ extern "C" int svs_quantise(const float* v, uint32_t rows, uint32_t d, uint64_t 
 // probe_route (ivf-index/src/ivf.rs:150-164) with l2_f32_scalar
 // (distance.rs:30-37): f32 squared differences summed left to right (this
 // file is built with -ffp-contract=off), correctly rounded sqrt, ascending
-// sort, first nprobe. Equal distances keep centroid order.
+// sort under f32::total_cmp, first nprobe. Equal distances keep centroid
+// order.
+// (every NaN is taken as the largest value: a NaN distance's sign depends
+// on the platform's NaN propagation)
+static uint32_t total_key(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if (f != f) return 0xFFFFFFFFu;
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 extern "C" int svs_probe_route(const float* centroids, uint32_t n_centroids, uint32_t d,
                                const float* queries, uint32_t n_queries, uint32_t nprobe,
                                uint32_t* out) {
     if (!centroids || !queries || !out || nprobe == 0 || nprobe > n_centroids)
         return fail(SVS_ERR_ARG, "bad argument");
     auto work = [&](uint32_t q0, uint32_t q1) {
-        std::vector<std::pair<float, uint32_t>> dist(n_centroids);
+        std::vector<std::pair<uint32_t, uint32_t>> dist(n_centroids);
         for (uint32_t q = q0; q < q1; ++q) {
             const float* x = queries + (size_t)q * d;
             for (uint32_t c = 0; c < n_centroids; ++c) {
@@ -214,14 +216,9 @@ extern "C" int svs_probe_route(const float* centroids, uint32_t n_centroids, uin
                     const float diff = x[j] - y[j];
                     acc = acc + diff * diff;
                 }
-                dist[c] = {std::sqrt(acc), c};
+                dist[c] = {total_key(std::sqrt(acc)), c};
             }
-            std::partial_sort(dist.begin(), dist.begin() + nprobe, dist.end(),
-                              [](const std::pair<float, uint32_t>& a,
-                                 const std::pair<float, uint32_t>& b) {
-                                  return a.first < b.first ||
-                                         (a.first == b.first && a.second < b.second);
-                              });
+            std::partial_sort(dist.begin(), dist.begin() + nprobe, dist.end());
             for (uint32_t p = 0; p < nprobe; ++p) out[(size_t)q * nprobe + p] = dist[p].second;
         }
     };

@@ -83,11 +83,15 @@ struct svs_slot {
     svs::DevBuf items, prefixes, flags;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
     svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part, rv;
+    // inputs of device-planned batches: probes + per-query flags, raw f32
+    // queries, routing distances
+    svs::DevBuf q_probes, q_raw, route_dist;
     svs_prof prof;
     std::vector<svs::DevBuf*> slot_scratch() {
         return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
                 &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &r, &scores, &partials, &verdict,
-                &out, &mtiles, &q_corr, &dense, &topk_part, &rv};
+                &out, &mtiles, &q_corr, &dense, &topk_part, &rv, &q_probes, &q_raw,
+                &route_dist};
     }
 };
 
@@ -115,6 +119,8 @@ struct svs_ctx : svs_slot {
     // chosen block-tile variant of the MFMA path per batch-size class
     // (n <= 16, <= 64, <= 256, larger)
     int tile_variant[4] = {0, 0, 0, 0};
+    // batched runs planned on the device (1) or on the host (0)
+    int planner = 1;
     // svs_dev_alloc buffers and their sizes
     std::unordered_map<void*, size_t> user_allocs;
     void note_peak();
@@ -132,6 +138,13 @@ struct svs_index {
     uint64_t* lt_dev = nullptr;  // L^T, n1 x n
     svs::ListDev* lists_dev = nullptr;
     std::vector<svs::ListHost> lists;
+    // IVF centroids for device routing (n_lists x cent_d f32), or null
+    float* cent_dev = nullptr;
+    uint32_t cent_d = 0;
+    // prefix sums of the list sizes sorted descending (rebuilt when a list
+    // changes; top_rows)
+    std::vector<uint64_t> top_rows_prefix;
+    bool top_rows_dirty = true;
 };
 
 // ---------------------------------------------------------------- batches
@@ -154,6 +167,28 @@ struct BatchPlan {
     // the same over the items to verify (the Freivalds recompute)
     std::vector<uint32_t> vgitems;
     std::vector<MTile> vtiles;
+    // device header words: tile and verify-tile counts, overflow, rows
+    uint32_t hdr[PH_WORDS] = {};
+};
+
+// What the pipeline kernels read about a batch: device arrays in the slot's
+// buffers, plus host-side bounds that size grids and scratch. A plan built
+// on the device knows its row and tile counts only there (hdr words and the
+// prefix arrays' last entries); the caps bound them. For an uploaded host
+// plan the caps are the exact counts.
+struct PlanView {
+    uint32_t nq = 0, n_items = 0;
+    uint64_t cap_cands = 0, cap_chunks = 0, cap_tiles = 0, cap_vtiles = 0, cap_tchunks = 0;
+    bool any_verify = false;  // some item may need verification
+    const uint32_t* probes = nullptr;  // device-planned: the probe lists
+    bool planes_ok = true;    // every list the batch can touch has the decode planes
+    uint64_t distinct_bytes = 0, n_cands = 0;  // profile counters (host plans only)
+    const Item* items = nullptr;
+    const uint32_t *chunk_ptr = nullptr, *cand_ptr = nullptr, *item_ptr = nullptr,
+                   *qchunk_ptr = nullptr, *hdr = nullptr;
+    const uint8_t* flags = nullptr;
+    const MTile *tiles = nullptr, *vtiles = nullptr;
+    const uint32_t *gitems = nullptr, *vgitems = nullptr;
 };
 
 int make_plan(const svs_index* idx, uint32_t nq, const uint32_t* probe_ptr,
@@ -166,6 +201,32 @@ int plan_for_mode(svs_index* idx, int mode, uint32_t nq, const uint32_t* probe_p
 // query inputs are staged on the slot; asynchronous on the slot's stream.
 int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k, bool product_only,
              int kernel, bool encode, int variant);
+// The same from a plan view (uploaded or built on the device).
+int run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k, bool product_only,
+             int kernel, bool encode, int variant);
+// Device planning of a batched run. plan_prepare sizes the slot for nq
+// queries whose items are given by probe_ptr (host, nq + 1), and stages the
+// planner's inputs in sl.q_probes = [probes: n_items u32][probe_ptr: nq + 1
+// u32][verify: nq bytes]: probe_ptr and verify from the host, the probe lists
+// from the host too unless probe_lists is null (then a routing kernel writes
+// them). cands_bound bounds the batch's candidate rows. plan_launch enqueues
+// the planner; a plan beyond the bounds comes back flagged in
+// hdr[PH_OVERFLOW] and empty.
+int plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t* probe_ptr,
+                 const uint32_t* probe_lists, const uint8_t* verify, uint64_t cands_bound,
+                 int variant, PlanView& v);
+int plan_launch(svs_index* idx, svs_slot& sl, const PlanView& v, int variant);
+// Quantises and routes nq raw queries already in sl.q_raw (f32, d each)
+// into sl.q_vq and the probe lists of sl.q_probes (layout above).
+int route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items, double q_scale);
+// Plan layout in the slot for nq queries, n_items items and cap_tiles tiles:
+// ensures the buffers and sets v's array pointers.
+int plan_layout(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items, uint64_t cap_tiles,
+                PlanView& v);
+// Whether device planning can take a batch of this index and size.
+bool device_plan_ok(const svs_index* idx, uint32_t n_items);
+// Sum of the n largest list sizes of the index, n = 0 .. n_lists.
+uint64_t top_rows(svs_index* idx, uint32_t n);
 int stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
                   const uint8_t* seeds);
 int collect_async(svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items, uint32_t* ids,

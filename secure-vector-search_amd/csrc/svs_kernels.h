@@ -378,36 +378,38 @@ __device__ __forceinline__ void wave_best(int64_t& score, uint32_t& pos) {
 }
 
 __global__ void __launch_bounds__(256)
-k_topk_chunks(const uint32_t* __restrict__ qchunk_ptr, uint32_t nq, uint32_t n_chunks,
+k_topk_chunks(const uint32_t* __restrict__ qchunk_ptr, uint32_t nq,
               const uint32_t* __restrict__ item_ptr, const uint32_t* __restrict__ cand_ptr,
               const int64_t* __restrict__ scores, uint32_t k, Cand* __restrict__ part) {
-    const uint32_t c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (c >= n_chunks) return;  // uniform per wave
+    const uint32_t n_chunks = qchunk_ptr[nq];
     const uint32_t lane = threadIdx.x % WAVE;
-    const uint32_t q = find_seg(qchunk_ptr, nq, c);
-    const uint32_t c0 = cand_ptr[item_ptr[q]], c1 = cand_ptr[item_ptr[q + 1]];
-    const uint32_t lo = c0 + (c - qchunk_ptr[q]) * TOPK_WCHUNK;
-    int64_t v[TOPK_LANE];
+    for (uint32_t c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < n_chunks;
+         c += gridDim.x * (blockDim.x / WAVE)) {  // uniform per wave
+        const uint32_t q = find_seg(qchunk_ptr, nq, c);
+        const uint32_t c0 = cand_ptr[item_ptr[q]], c1 = cand_ptr[item_ptr[q + 1]];
+        const uint32_t lo = c0 + (c - qchunk_ptr[q]) * TOPK_WCHUNK;
+        int64_t v[TOPK_LANE];
 #pragma unroll
-    for (int i = 0; i < TOPK_LANE; ++i) {
-        const uint32_t g = lo + i * WAVE + lane;
-        v[i] = g < c1 ? scores[g] : INT64_MIN;  // scores are in (-p/2, p/2)
-    }
-    uint32_t used = 0;
-    for (uint32_t r = 0; r < k; ++r) {
-        int64_t best = INT64_MIN;
-        int bi = -1;
+        for (int i = 0; i < TOPK_LANE; ++i) {
+            const uint32_t g = lo + i * WAVE + lane;
+            v[i] = g < c1 ? scores[g] : INT64_MIN;  // scores are in (-p/2, p/2)
+        }
+        uint32_t used = 0;
+        for (uint32_t r = 0; r < k; ++r) {
+            int64_t best = INT64_MIN;
+            int bi = -1;
 #pragma unroll
-        for (int i = 0; i < TOPK_LANE; ++i)
-            if (!((used >> i) & 1) && v[i] > best) {
-                best = v[i];
-                bi = i;
-            }
-        const uint32_t mine = bi >= 0 ? lo - c0 + bi * WAVE + lane : 0xFFFFFFFFu;
-        uint32_t pos = mine;
-        wave_best(best, pos);
-        if (lane == 0) part[(size_t)c * k + r] = Cand{best, pos, 0};
-        if (bi >= 0 && pos == mine) used |= 1u << bi;
+            for (int i = 0; i < TOPK_LANE; ++i)
+                if (!((used >> i) & 1) && v[i] > best) {
+                    best = v[i];
+                    bi = i;
+                }
+            const uint32_t mine = bi >= 0 ? lo - c0 + bi * WAVE + lane : 0xFFFFFFFFu;
+            uint32_t pos = mine;
+            wave_best(best, pos);
+            if (lane == 0) part[(size_t)c * k + r] = Cand{best, pos, 0};
+            if (bi >= 0 && pos == mine) used |= 1u << bi;
+        }
     }
 }
 
@@ -975,8 +977,9 @@ __device__ __forceinline__ uint64_t gload_u64(const uint64_t* p) {
 }
 
 __global__ void __launch_bounds__(256)
-k_gemv_mfma(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles, uint32_t n_tiles,
-            const uint32_t* __restrict__ gitems, const Item* __restrict__ items,
+k_gemv_mfma(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+            const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
+            const Item* __restrict__ items,
             const uint32_t* __restrict__ cand_ptr, const uint64_t* __restrict__ venc,
             const uint64_t* __restrict__ qcorr, uint64_t* __restrict__ r_out) {
     __shared__ int32_t dsm[ROW_THREADS / WAVE][4][16][17];
@@ -988,6 +991,7 @@ k_gemv_mfma(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles, 
     const int shr = col <= 7 ? 8 * (7 - col) : 0;
     const int shl = (col >= 8 && col <= 14) ? 8 * (col - 7) : 0;
     const uint32_t waves_per_block = blockDim.x / WAVE;
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t t = blockIdx.x * waves_per_block + wave; t < n_tiles;
          t += gridDim.x * waves_per_block) {
         const MTile tile = tiles[t];
@@ -1131,7 +1135,7 @@ __device__ __forceinline__ uint32_t xcd_worker() {
 template <int KE, int SRC, int QG, int PF, int OCC, int VPF = 8>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
-                uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+                const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
                 const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
                 const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
                 uint64_t* __restrict__ out) {
@@ -1153,6 +1157,7 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
     constexpr int CPT = QG / 2;    // columns per producer thread
     const int pj = tid / TPQ, pw = tid % TPQ;
     const int pe = pw / (16 / CPT), pd0 = (pw % (16 / CPT)) * CPT;
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
@@ -1283,7 +1288,7 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
 template <int KE, int SRC, int QG, int PF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fp_gemm_vb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
-             uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+             const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
              const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
              const uint64_t* __restrict__ vecs, const uint64_t* __restrict__ qcorr,
              uint64_t* __restrict__ out) {
@@ -1303,6 +1308,7 @@ k_fp_gemm_vb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
         sel0 |= (uint32_t)(b0 >= 0 && b0 <= 7 ? b0 : 0x0C) << (8 * a);
         sel1 |= (uint32_t)(b1 >= 0 && b1 <= 7 ? b1 : 0x0C) << (8 * a);
     }
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
@@ -1485,7 +1491,7 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 template <int QG, int RTW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
-               uint32_t n_tiles, const uint32_t* __restrict__ gitems,
+               const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
                const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
                const int8_t* __restrict__ t_dense, const int8_t* __restrict__ vplanes,
                const uint64_t* __restrict__ mt_qterm, const uint64_t* __restrict__ r,
@@ -1498,6 +1504,7 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int grp = lane / 16, col = lane % 16;
+    const uint32_t n_tiles = *n_tiles_p;
     for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];

@@ -47,6 +47,11 @@ struct Request {
     std::vector<uint32_t> probes;
     uint8_t verify = 0;
     uint32_t k = 0;
+    // raw query (routed and quantised on the device) instead of v_q + probes
+    bool raw = false;
+    std::vector<float> x;
+    uint32_t nprobe = 0;
+    uint64_t qq = 0;
     double t_start = 0;  // submit time, or the scheduled arrival
     // results
     bool done = false;
@@ -89,9 +94,14 @@ struct Slot {
     uint64_t seq = 0;  // launch order
     std::vector<Request*> reqs;
     BatchPlan plan;
+    PlanView view;
     uint32_t k = 0;
     Pinned<uint64_t> h_vq;
     Pinned<uint8_t> h_seeds;
+    // device-planned batches: probe_ptr, probe lists, verify flags, raw queries
+    Pinned<uint32_t> h_pptr, h_plist;
+    Pinned<uint8_t> h_ver;
+    Pinned<float> h_x;
     Pinned<uint32_t> h_ids, h_count, h_flag;
     Pinned<int64_t> h_scores;
     Pinned<int32_t> h_status;
@@ -102,6 +112,9 @@ struct Slot {
 struct svs_server {
     svs_ctx* ctx = nullptr;
     std::vector<svs_index*> tenants;
+    // per tenant: sum of its n largest list sizes (the row bound of a
+    // routed request with nprobe n); indexes do not change while serving
+    std::vector<std::vector<uint64_t>> top_rows;
     svs_server_config cfg{};
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
@@ -127,48 +140,106 @@ struct svs_server {
 uint64_t svs_server::batch_bytes(const Request& r) const {
     const svs_index* idx = tenants[r.tenant];
     uint64_t rows = 0;
-    for (uint32_t c : r.probes)
-        if (c < idx->lists.size()) rows += idx->lists[c].m;
+    if (r.raw) {
+        rows = top_rows[r.tenant][r.nprobe];
+    } else {
+        for (uint32_t c : r.probes)
+            if (c < idx->lists.size()) rows += idx->lists[c].m;
+    }
     return rows * 26 + 30 * 1024 + (uint64_t)r.k * 16;
 }
 
 void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
     const uint32_t nq = (uint32_t)batch.size();
     svs_index* idx = tenants[batch[0]->tenant];
+    const bool raw = batch[0]->raw;  // batches are of one kind
     uint32_t k = 0;
     for (Request* r : batch) k = std::max(k, r->k);
     int rc = SVS_OK;
-    std::vector<uint32_t> pptr(nq + 1, 0), plist;
-    std::vector<uint8_t> ver(nq);
-    for (uint32_t q = 0; q < nq; ++q) {
-        plist.insert(plist.end(), batch[q]->probes.begin(), batch[q]->probes.end());
-        pptr[q + 1] = (uint32_t)plist.size();
-        ver[q] = batch[q]->verify;
+    uint32_t n_items = 0;
+    uint64_t rows = 0;
+    for (Request* r : batch) {
+        n_items += raw ? r->nprobe : (uint32_t)r->probes.size();
+        if (raw) {
+            rows += top_rows[r->tenant][r->nprobe];
+        } else {
+            for (uint32_t c : r->probes)
+                if (c < idx->lists.size()) rows += idx->lists[c].m;
+        }
     }
-    if (plist.empty()) plist.push_back(0);  // (pptr says no probes)
-    int kernel = 0, variant = 0;
-    rc = plan_for_mode(idx, cfg.mode, nq, pptr.data(), plist.data(), ver.data(), s.plan, kernel,
-                       variant);
-    if (rc == SVS_OK) rc = s.h_vq.ensure((size_t)nq * N);
+    const bool dplan = cfg.mode != SVS_MODE_PLAIN && ctx->planner &&
+                       device_plan_ok(idx, n_items) && rows < 0xFFFFFFFFull;
+    const int kernel = cfg.mode == SVS_MODE_BATCHED ? 1 : 0;  // MFMA tiles or VALU
+    int variant = 0, hkernel = 0;
+    const uint32_t d = idx->cent_d;
+    if (raw && !dplan)
+        rc = fail(SVS_ERR_ARG, "a raw batch needs device planning (%u items)", n_items);
+    if (rc == SVS_OK) rc = s.h_pptr.ensure(nq + 1);
+    if (rc == SVS_OK) rc = s.h_plist.ensure(std::max<uint32_t>(n_items, 1));
+    if (rc == SVS_OK) rc = s.h_ver.ensure(nq);
+    if (rc == SVS_OK && raw) rc = s.h_x.ensure((size_t)nq * d);
+    if (rc == SVS_OK && !raw) rc = s.h_vq.ensure((size_t)nq * N);
     if (rc == SVS_OK) rc = s.h_seeds.ensure((size_t)nq * 32);
     if (rc == SVS_OK) rc = s.h_ids.ensure((size_t)nq * k);
     if (rc == SVS_OK) rc = s.h_scores.ensure((size_t)nq * k);
     if (rc == SVS_OK) rc = s.h_count.ensure(nq);
     if (rc == SVS_OK) rc = s.h_status.ensure(nq);
-    if (rc == SVS_OK) rc = s.h_flag.ensure(1);
+    if (rc == SVS_OK) rc = s.h_flag.ensure(2);
     if (rc == SVS_OK) {
+        s.h_pptr.p[0] = 0;
         for (uint32_t q = 0; q < nq; ++q) {
-            std::memcpy(s.h_vq.p + (size_t)q * N, batch[q]->v_q.data(), sizeof(uint64_t) * N);
-            std::memcpy(s.h_seeds.p + (size_t)q * 32, batch[q]->seed, 32);
+            const Request* r = batch[q];
+            const uint32_t np = raw ? r->nprobe : (uint32_t)r->probes.size();
+            if (!raw) std::copy(r->probes.begin(), r->probes.end(), s.h_plist.p + s.h_pptr.p[q]);
+            s.h_pptr.p[q + 1] = s.h_pptr.p[q] + np;
+            s.h_ver.p[q] = r->verify;
+            std::memcpy(s.h_seeds.p + (size_t)q * 32, r->seed, 32);
+            if (raw)
+                std::memcpy(s.h_x.p + (size_t)q * d, r->x.data(), sizeof(float) * d);
+            else
+                std::memcpy(s.h_vq.p + (size_t)q * N, r->v_q.data(), sizeof(uint64_t) * N);
         }
+        s.h_flag.p[1] = 0;
+        if (!dplan)
+            rc = plan_for_mode(idx, cfg.mode, nq, s.h_pptr.p, s.h_plist.p, s.h_ver.p, s.plan,
+                               hkernel, variant);
+    }
+    if (rc == SVS_OK) {
         std::lock_guard<std::mutex> g(ctx->mu);
         rc = hipSetDevice(ctx->device) == hipSuccess ? SVS_OK : SVS_ERR_HIP;
-        if (rc == SVS_OK) rc = stage_queries(ctx, s.sl, nq, s.h_vq.p, s.h_seeds.p);
-        if (rc == SVS_OK) rc = run_plan(idx, s.sl, s.plan, k, false, kernel, true, variant);
+        hipStream_t st = s.sl.stream;
+        if (dplan) {
+            variant = ctx->tile_variant[nq <= 16 ? 0 : nq <= 64 ? 1 : nq <= 256 ? 2 : 3];
+            PlanView& v = s.view;
+            if (rc == SVS_OK)
+                rc = plan_prepare(idx, s.sl, nq, s.h_pptr.p, raw ? nullptr : s.h_plist.p,
+                                  s.h_ver.p, rows, variant, v);
+            if (rc == SVS_OK && raw) {
+                rc = s.sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * d);
+                if (rc == SVS_OK &&
+                    hipMemcpyAsync(s.sl.q_raw.ptr, s.h_x.p, sizeof(float) * (size_t)nq * d,
+                                   hipMemcpyHostToDevice, st) != hipSuccess)
+                    rc = fail(SVS_ERR_HIP, "query upload failed");
+            }
+            if (rc == SVS_OK)
+                rc = stage_queries(ctx, s.sl, nq, raw ? nullptr : s.h_vq.p, s.h_seeds.p);
+            if (rc == SVS_OK && raw)
+                rc = route_launch(idx, s.sl, nq, n_items, (double)batch[0]->qq);
+            if (rc == SVS_OK) rc = plan_launch(idx, s.sl, v, variant);
+            if (rc == SVS_OK) rc = run_view(idx, s.sl, v, k, false, kernel, true, variant);
+            if (rc == SVS_OK &&
+                hipMemcpyAsync(s.h_flag.p + 1, v.hdr + PH_OVERFLOW, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st) != hipSuccess)
+                rc = fail(SVS_ERR_HIP, "plan flag copy failed");
+            s.plan.n_items = n_items;
+        } else {
+            if (rc == SVS_OK) rc = stage_queries(ctx, s.sl, nq, s.h_vq.p, s.h_seeds.p);
+            if (rc == SVS_OK) rc = run_plan(idx, s.sl, s.plan, k, false, hkernel, true, variant);
+        }
         if (rc == SVS_OK)
             rc = collect_async(s.sl, nq, k, s.plan.n_items, s.h_ids.p, s.h_scores.p,
                                s.h_count.p, s.h_status.p, s.h_flag.p);
-        if (rc == SVS_OK && hipEventRecord(s.done_ev, s.sl.stream) != hipSuccess)
+        if (rc == SVS_OK && hipEventRecord(s.done_ev, st) != hipSuccess)
             rc = fail(SVS_ERR_HIP, "hipEventRecord failed");
     }
     s.reqs = batch;
@@ -208,7 +279,9 @@ void svs_server::retire(Slot& s) {
     if (e != hipSuccess) {
         rc = fail(SVS_ERR_HIP, "batch failed on the device: %s", hipGetErrorString(e));
     } else {
-        rc = check_rng_flag(*s.h_flag.p);
+        rc = check_rng_flag(s.h_flag.p[0]);
+        if (rc == SVS_OK && s.h_flag.p[1])
+            rc = fail(SVS_ERR_STATE, "device plan exceeded its bounds");
     }
     if (rc != SVS_OK) {
         char buf[512];
@@ -266,16 +339,21 @@ void svs_server::run() {
             if (free_slot && !pending.empty()) {
                 // FIFO over requests; a batch holds one tenant's requests
                 const uint32_t tenant = pending.front()->tenant;
+                const bool raw = pending.front()->raw;
                 uint64_t bytes = 0;
+                uint32_t items = 0;
                 for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch;) {
                     Request* r = *it;
-                    if (r->tenant != tenant) {
+                    if (r->tenant != tenant || r->raw != raw) {
                         ++it;
                         continue;
                     }
                     const uint64_t b = batch_bytes(*r);
                     if (!batch.empty() && bytes + b > cfg.slot_bytes) break;
+                    // a raw batch stays within what the device planner takes
+                    if (raw && !batch.empty() && items + r->nprobe > PLAN_MAX_ITEMS) break;
                     bytes += b;
+                    items += raw ? r->nprobe : 0;
                     batch.push_back(r);
                     it = pending.erase(it);
                 }
@@ -319,8 +397,17 @@ extern "C" int svs_server_create(svs_index* const* tenants, uint32_t n_tenants,
     auto s = std::make_unique<svs_server>();
     s->ctx = ctx;
     s->tenants.assign(tenants, tenants + n_tenants);
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        for (svs_index* t : s->tenants) {
+            std::vector<uint64_t> tr(t->lists.size() + 1);
+            for (size_t n = 0; n < tr.size(); ++n) tr[n] = top_rows(t, (uint32_t)n);
+            s->top_rows.push_back(std::move(tr));
+        }
+    }
     s->cfg = *cfg;
-    if (s->cfg.slot_bytes == 0) s->cfg.slot_bytes = 1ull << 30;
+    if (s->cfg.slot_bytes == 0)
+        s->cfg.slot_bytes = std::max<uint64_t>(1ull << 30, ctx->budget_bytes / 16);
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         HIP_TRY(hipSetDevice(ctx->device));
@@ -359,6 +446,10 @@ extern "C" int svs_server_destroy(svs_server* s) {
             hipStreamDestroy(sp->sl.stream);
             sp->h_vq.release();
             sp->h_seeds.release();
+            sp->h_pptr.release();
+            sp->h_plist.release();
+            sp->h_ver.release();
+            sp->h_x.release();
             sp->h_ids.release();
             sp->h_count.release();
             sp->h_flag.release();
@@ -396,17 +487,32 @@ extern "C" int svs_server_reserve(svs_server* s, uint32_t n_queries, uint32_t n_
 }
 
 extern "C" int svs_server_submit(svs_server* s, const svs_request* r, uint64_t* ticket) {
-    if (!s || !r || !ticket || !r->v_q || !r->q_seed || (r->n_probes && !r->probes))
+    if (!s || !r || !ticket || !r->q_seed || (!r->query && !r->v_q) ||
+        (!r->query && r->n_probes && !r->probes))
         return fail(SVS_ERR_ARG, "null argument");
     if (r->tenant >= s->tenants.size()) return fail(SVS_ERR_ARG, "tenant %u out of range", r->tenant);
     if (r->k == 0 || r->k > 4096) return fail(SVS_ERR_ARG, "k = %u must be in 1..4096", r->k);
-    for (int i = 0; i < N; ++i)
-        if (r->v_q[i] >= P) return fail(SVS_ERR_ARG, "v_q[%d] is not canonical", i);
     auto q = std::make_unique<Request>();
     q->tenant = r->tenant;
-    q->v_q.assign(r->v_q, r->v_q + N);
+    if (r->query) {
+        const svs_index* idx = s->tenants[r->tenant];
+        if (!idx->cent_dev)
+            return fail(SVS_ERR_STATE, "tenant %u has no centroids for routing", r->tenant);
+        if (idx->lists.size() > (size_t)RT_SELECT_MAX)
+            return fail(SVS_ERR_ARG, "device routing takes up to %d lists", RT_SELECT_MAX);
+        if (s->cfg.mode == SVS_MODE_PLAIN)
+            return fail(SVS_ERR_ARG, "raw requests need a batched server mode");
+        q->raw = true;
+        q->x.assign(r->query, r->query + idx->cent_d);
+        q->nprobe = std::min<uint32_t>(r->nprobe, (uint32_t)idx->lists.size());
+        q->qq = r->quantisation_q;
+    } else {
+        for (int i = 0; i < N; ++i)
+            if (r->v_q[i] >= P) return fail(SVS_ERR_ARG, "v_q[%d] is not canonical", i);
+        q->v_q.assign(r->v_q, r->v_q + N);
+        q->probes.assign(r->probes, r->probes + r->n_probes);
+    }
     std::memcpy(q->seed, r->q_seed, 32);
-    q->probes.assign(r->probes, r->probes + r->n_probes);
     q->verify = r->verify ? 1 : 0;
     q->k = r->k;
     q->t_start = r->arrival_s > 0 ? r->arrival_s : now_s();

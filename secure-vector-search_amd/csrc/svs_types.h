@@ -60,6 +60,17 @@ constexpr int PB_PF = 8;   // K'-steps of A rows and query words loaded ahead
 // signed-byte-limb value (each byte b read as b - 128).
 constexpr uint64_t LIMB_OFFSET_C = 0x8080808080808080ull % 0x1FFFFFFFFFFFFFFFull;
 
+// Device batch planning (k_plan): a sort key packs (list, item) into 32
+// bits, so a batch has at most PLAN_MAX_ITEMS items and an index fewer than
+// PLAN_MAX_LISTS lists.
+constexpr uint32_t PLAN_ITEM_BITS = 14;
+constexpr uint32_t PLAN_MAX_ITEMS = 1u << PLAN_ITEM_BITS;
+constexpr uint32_t PLAN_MAX_LISTS = 1u << (32 - PLAN_ITEM_BITS);
+// Header words of a plan in device memory.
+enum : uint32_t { PH_NT = 0, PH_NVT = 1, PH_OVERFLOW = 2, PH_NCANDS = 3, PH_WORDS = 8 };
+// Device routing sorts a query's centroid distances in LDS (64 KiB).
+constexpr int RT_SELECT_MAX = 8192;
+
 struct Item {
     uint32_t q;          // query slot in the batch
     uint32_t list;       // list id

@@ -35,8 +35,11 @@ EXPORTS = [
     "svs_server_create", "svs_server_destroy", "svs_server_submit", "svs_server_wait",
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
+    "svs_ctx_set_planner", "svs_index_set_centroids", "svs_route_batch", "svs_score_raw_batch",
 ]
 TILE_VARIANTS = 10
+ROUTE_MAX_LISTS = 8192  # device routing sorts a query's distances in LDS
+PLAN_MAX_ITEMS = 16384  # items of a batch planned on the device
 
 
 class GpuError(RuntimeError):
@@ -76,6 +79,16 @@ class QueryBatch(C.Structure):
     ]
 
 
+class RawBatch(C.Structure):
+    _fields_ = [
+        ("n_queries", C.c_uint32), ("queries", C.c_void_p), ("nprobe", C.c_void_p),
+        ("nprobe_all", C.c_uint32), ("quantisation_q", C.c_uint64), ("q_seeds", C.c_void_p),
+        ("verify", C.c_void_p), ("k", C.c_uint32), ("out_ids", C.c_void_p),
+        ("out_scores", C.c_void_p), ("out_count", C.c_void_p), ("out_status", C.c_void_p),
+        ("out_probes", C.c_void_p),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("slots", C.c_uint32), ("slot_bytes", C.c_uint64),
                 ("mode", C.c_int)]
@@ -84,7 +97,8 @@ class ServerConfig(C.Structure):
 class Request(C.Structure):
     _fields_ = [("tenant", C.c_uint32), ("v_q", C.c_void_p), ("q_seed", C.c_void_p),
                 ("probes", C.c_void_p), ("n_probes", C.c_uint32), ("verify", C.c_uint8),
-                ("k", C.c_uint32), ("arrival_s", C.c_double)]
+                ("k", C.c_uint32), ("arrival_s", C.c_double), ("query", C.c_void_p),
+                ("nprobe", C.c_uint32), ("quantisation_q", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -164,6 +178,10 @@ def lib() -> C.CDLL:
             "svs_server_stats": ([vp, C.POINTER(ServerCounters)], i32),
             "svs_server_run_poisson": ([vp, vp, u32, C.c_double, u64, vp,
                                         C.POINTER(C.c_double)], i32),
+            "svs_ctx_set_planner": ([vp, i32], i32),
+            "svs_index_set_centroids": ([vp, vp, u32], i32),
+            "svs_route_batch": ([vp, vp, u32, u32, vp], i32),
+            "svs_score_raw_batch": ([vp, C.POINTER(RawBatch), i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

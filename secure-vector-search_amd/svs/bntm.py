@@ -208,6 +208,10 @@ class Context:
         check(L.lib().svs_ctx_read_probe(self.h, nbytes, C.byref(ms)))
         return ms.value
 
+    def set_planner(self, on_device: bool = True):
+        """Plan batched runs on the device (default) or on the host."""
+        check(L.lib().svs_ctx_set_planner(self.h, 1 if on_device else 0))
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""
@@ -259,7 +263,50 @@ class GpuIndex:
         check(L.lib().svs_index_create(ctx.h, n_lists, ptr(self.l), C.byref(h)))
         self.h = h
         self.n_lists = n_lists
+        self.cent_d = 0
         _live_indexes.add(self)
+
+    def set_centroids(self, centroids):
+        """The IVF centroids (n_lists x d f32) for routing on the device."""
+        c = np.ascontiguousarray(centroids, np.float32)
+        if c.ndim != 2 or c.shape[0] != self.n_lists:
+            raise ValueError(f"centroids must be {self.n_lists} x d")
+        check(L.lib().svs_index_set_centroids(self.h, ptr(c), c.shape[1]))
+        self.cent_d = c.shape[1]
+
+    def route(self, queries, nprobe: int) -> np.ndarray:
+        """probe_route on the device: (n_queries, min(nprobe, n_lists))."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, max(self.cent_d, 1))
+        np_ = min(int(nprobe), self.n_lists)
+        out = np.zeros((q.shape[0], max(np_, 1)), np.uint32)
+        check(L.lib().svs_route_batch(self.h, ptr(q), q.shape[0], np_, ptr(out)))
+        return out[:, :np_]
+
+    def score_raw_batch(self, queries, q_seeds, nprobe, verify, k, quantisation_q=Q_DEFAULT,
+                        mode=L.MODE_BATCHED):
+        """svs_score_raw_batch: f32 queries quantised and routed on the
+        device, then scored. nprobe: one value or one per query. Returns
+        (ids, scores, count, status, probes) with probes concatenated."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, max(self.cent_d, 1))
+        nq = q.shape[0]
+        seeds = np.ascontiguousarray(q_seeds, np.uint8).reshape(nq, 32)
+        per = None
+        if np.ndim(nprobe):
+            per = np.minimum(np.ascontiguousarray(nprobe, np.uint32), self.n_lists)
+            total = int(per.sum())
+        else:
+            total = nq * min(int(nprobe), self.n_lists)
+        ver = None if verify is None else np.ascontiguousarray(verify, np.uint8)
+        ids = np.zeros((nq, k), np.uint32)
+        scores = np.zeros((nq, k), np.int64)
+        count = np.zeros(nq, np.uint32)
+        status = np.zeros(nq, np.int32)
+        probes = np.zeros(max(total, 1), np.uint32)
+        b = L.RawBatch(nq, ptr(q), ptr(per), 0 if per is not None else int(nprobe),
+                       int(quantisation_q), ptr(seeds), ptr(ver), k, ptr(ids), ptr(scores),
+                       ptr(count), ptr(status), ptr(probes))
+        check(L.lib().svs_score_raw_batch(self.h, C.byref(b), mode))
+        return ids, scores, count, status, probes[:total]
 
     def upload_list(self, list_id, cl: dict, ids=None):
         m = int(cl["m"])
@@ -561,6 +608,9 @@ class BnTmIvfScorer:
             mq = quantise(v[members], config.quantisation_q) if members.size else \
                 np.zeros((0, N), np.uint64)
             idx.build_list(ci, mq, derive_h_seed(config.upload_seed, ci), members)
+        cent = np.asarray(config.index.centroids, np.float32)
+        if nc <= L.ROUTE_MAX_LISTS and cent.ndim == 2 and cent.shape[1] == dim:
+            idx.set_centroids(cent)
         h = BnTmIvfHandle(idx, config.index, l, sizes, config.nprobe, dim,
                           config.verification_enabled, config.quantisation_q)
         return h, BuildOutcome(False, time.perf_counter() - t0)
@@ -572,11 +622,16 @@ class BnTmIvfScorer:
         if qs.shape[1] != handle.dim:
             raise DimensionMismatch(qs.shape[1], handle.dim)
         nq = qs.shape[0]
+        seeds = _seeds(nq, q_seeds)
+        ver = np.full(nq, 1 if handle.verification_enabled else 0, np.uint8)
+        if handle.index.cent_d:
+            # steps 1-2 of score (routing, quantisation) on the device too
+            *res, probes = handle.index.score_raw_batch(qs, seeds, handle.nprobe, ver, max(k, 1),
+                                                        handle.quantisation_q, mode)
+            return probes.reshape(nq, -1), tuple(res)
         probes = probe_route(handle.ivf.centroids, qs, handle.nprobe)
         v_q = quantise(qs, handle.quantisation_q)
-        seeds = _seeds(nq, q_seeds)
         pp = np.arange(0, nq * handle.nprobe + 1, handle.nprobe, dtype=np.uint32)
-        ver = np.full(nq, 1 if handle.verification_enabled else 0, np.uint8)
         res = handle.index.score_batch(v_q, seeds, pp, probes.reshape(-1), ver, max(k, 1), mode)
         return probes, res
 

@@ -40,7 +40,24 @@ def _request(tenant, v_q, q_seed, probes, verify, k, keep):
     p = np.ascontiguousarray(probes, np.uint32).reshape(-1)
     keep.extend((v, s, p))
     return L.Request(tenant, ptr(v), ptr(s), ptr(p) if p.size else None, p.size,
-                     1 if verify else 0, k, 0.0)
+                     1 if verify else 0, k, 0.0, None, 0, 0)
+
+
+def _raw_request(tenant, query, q_seed, nprobe, verify, k, quantisation_q, keep):
+    x = np.ascontiguousarray(query, np.float32).reshape(-1)
+    s = seed_array(q_seed)
+    keep.extend((x, s))
+    return L.Request(tenant, None, ptr(s), None, 0, 1 if verify else 0, k, 0.0, ptr(x),
+                     int(nprobe), int(quantisation_q))
+
+
+def _any_request(rq, keep):
+    if rq.get("query") is not None:
+        return _raw_request(rq.get("tenant", 0), rq["query"], rq["q_seed"], rq["nprobe"],
+                            rq.get("verify", False), rq.get("k", 10),
+                            rq.get("quantisation_q", _bntm.Q_DEFAULT), keep)
+    return _request(rq.get("tenant", 0), rq["v_q"], rq["q_seed"], rq["probes"],
+                    rq.get("verify", False), rq.get("k", 10), keep)
 
 
 class Server:
@@ -70,6 +87,17 @@ class Server:
         self._keep[t.value] = k
         return t.value
 
+    def submit_raw(self, query, q_seed, nprobe: int, verify: bool = False, k: int = 10,
+                   tenant: int = 0, quantisation_q: int = _bntm.Q_DEFAULT) -> int:
+        """A raw f32 query: quantised and routed to `nprobe` lists on the
+        device (the tenant's index needs its centroids)."""
+        keep = []
+        r = _raw_request(tenant, query, q_seed, nprobe, verify, k, quantisation_q, keep)
+        t = C.c_uint64(0)
+        check(L.lib().svs_server_submit(self.h, C.byref(r), C.byref(t)))
+        self._keep[t.value] = k
+        return t.value
+
     def wait(self, ticket: int) -> ServedResult:
         k = self._keep.pop(ticket)
         ids = np.zeros(k, np.uint32)
@@ -81,15 +109,15 @@ class Server:
 
     def run_poisson(self, requests, qps: float, seed: int = 1):
         """Open-loop Poisson arrivals at `qps` over `requests` (dicts with
-        v_q, q_seed, probes, verify, k, tenant); returns (results, wall_s)."""
+        v_q, q_seed, probes, verify, k, tenant, or a raw `query` with
+        `nprobe` instead of v_q and probes); returns (results, wall_s)."""
         keep = []
         n = len(requests)
         reqs = (L.Request * n)()
         outs = []
         res = (L.Result * n)()
         for i, rq in enumerate(requests):
-            reqs[i] = _request(rq.get("tenant", 0), rq["v_q"], rq["q_seed"], rq["probes"],
-                               rq.get("verify", False), rq.get("k", 10), keep)
+            reqs[i] = _any_request(rq, keep)
             ids = np.zeros(rq.get("k", 10), np.uint32)
             sc = np.zeros(rq.get("k", 10), np.int64)
             outs.append((ids, sc))

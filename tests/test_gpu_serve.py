@@ -198,3 +198,45 @@ def test_served_after_reserve(ctx, tenant_a):
             srv.reserve(16, 128, 1000, 10)
     for i, (res, rq) in enumerate(zip(results, reqs)):
         _check(res, tenant_a, rq, f"request {i}")
+
+
+def _alone_raw(t, rq):
+    ids, scores, count, status, _ = t["handle"].index.score_raw_batch(
+        rq["query"][None, :], rq["q_seed"][None, :], rq["nprobe"],
+        np.array([rq["verify"]], np.uint8), rq["k"])
+    return ids[0, :count[0]], scores[0, :count[0]], int(status[0])
+
+
+def test_served_raw_queries_equal_alone(ctx, tenant_a, tenant_b):
+    """Raw f32 requests (routed and quantised on the device inside the
+    batch) mixed with pre-routed ones and across tenants: each result equals
+    the same request alone."""
+    import svs
+    rng = np.random.default_rng(8)
+    reqs = []
+    for i in range(60):
+        t = tenant_a if i % 3 else tenant_b
+        qi = int(rng.integers(0, t["qs"].shape[0]))
+        rq = dict(tenant=0 if t is tenant_a else 1, q_seed=rng.integers(0, 256, 32, np.uint8),
+                  verify=bool(rng.integers(0, 2)), k=int(rng.choice([1, 10, 16, 33])))
+        if i % 4:
+            rq.update(query=np.asarray(t["qs"][qi], np.float32),
+                      nprobe=int(rng.choice([1, 3, 8, 40])))
+        else:
+            rq.update(_requests(t, rng, 1, tenant=rq["tenant"])[0])
+        reqs.append(rq)
+    with svs.Server([tenant_a["handle"].index, tenant_b["handle"].index], max_batch=16,
+                    slots=2) as srv:
+        results, _ = srv.run_poisson(reqs, qps=5000.0, seed=4)
+        st = srv.stats()
+    say("raw served stats", st)
+    assert st["completed"] == len(reqs)
+    for i, (res, rq) in enumerate(zip(results, reqs)):
+        t = tenant_a if rq["tenant"] == 0 else tenant_b
+        if rq.get("query") is not None:
+            ids, scores, status = _alone_raw(t, rq)
+            assert res.status == status, f"request {i}"
+            assert res.ids.tolist() == ids.tolist(), f"request {i}"
+            assert res.scores.tolist() == scores.tolist(), f"request {i}"
+        else:
+            _check(res, t, rq, f"request {i}")
