@@ -53,7 +53,9 @@ typedef struct {
     uint64_t hbm_total_bytes;     /* hipMemGetInfo total */
     uint64_t hbm_free_bytes;      /* hipMemGetInfo free, sampled at this call */
     uint64_t uploads;             /* lists uploaded */
-    uint64_t launches;            /* kernels launched */
+    uint64_t launches;            /* kernels launched (a graph's once, at capture) */
+    uint64_t graph_launches;      /* batches replayed from captured HIP graphs */
+    uint64_t graph_captures;      /* HIP graphs captured */
 } svs_stats;
 
 /* Thread-local message of the last failing call; returns its length. */
@@ -228,6 +230,14 @@ int svs_score_batch_dev(svs_index* idx, const svs_query_batch* batch_dev, int mo
  * per list on the host, ivf.rs:1289-1296). */
 int svs_ctx_set_planner(svs_ctx* ctx, int on_device);
 
+/* Device-planned batches (svs_score_batch[_dev] and svs_score_raw_batch in
+ * the batched modes, and every server batch) run as one HIP graph per batch
+ * shape: the shape's second batch is captured (staging copies from pinned
+ * memory, routing, planning, the whole pipeline) and later batches of that
+ * shape replay it; 0 launches every kernel instead (default 1). Results are
+ * identical either way. No reference counterpart. */
+int svs_ctx_set_graphs(svs_ctx* ctx, int on);
+
 /* ---- client routing and quantisation on the device -------------------------
  * The reference client routes and quantises each query on the host before
  * encryption (BnTmIvfScorer::score steps 1-2, scorer-bntm/src/ivf.rs:
@@ -341,6 +351,7 @@ typedef struct {
 typedef struct {
     uint64_t submitted, completed, batches, batched_queries;
     uint32_t max_batch, max_in_flight, pending;
+    uint64_t graph_launches, graph_captures;  /* over the server's slots */
 } svs_server_counters;
 
 int svs_server_create(svs_index* const* tenants, uint32_t n_tenants, const svs_server_config* cfg,

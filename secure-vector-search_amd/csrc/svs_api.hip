@@ -72,6 +72,23 @@ void DevBuf::release(svs_ctx* ctx) {
     cap = 0;
 }
 
+int HostBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return SVS_OK;
+    const size_t want = std::max(bytes, cap + cap / 2);
+    if (ptr) hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    HIP_TRY(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+    cap = want;
+    return SVS_OK;
+}
+
+void HostBuf::release() {
+    if (ptr) hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
 void svs_ctx::note_peak() {
     uint64_t now = resident_bytes + scratch_bytes;
     if (now > peak_bytes) peak_bytes = now;
@@ -167,6 +184,8 @@ static std::mutex g_ctx_refs;
 static void ctx_free(svs_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    release_graphs(*ctx);
+    ctx->h_stage.release();
     for (DevBuf* b : ctx->all_scratch()) b->release(ctx);
     for (auto& kv : ctx->user_allocs) dev_free(ctx, kv.first, kv.second);
     ctx->user_allocs.clear();
@@ -216,6 +235,15 @@ extern "C" int svs_ctx_stats(svs_ctx* ctx, svs_stats* out) {
     out->hbm_free_bytes = free_b;
     out->uploads = ctx->uploads;
     out->launches = ctx->launches;
+    out->graph_launches = ctx->graph_launches;
+    out->graph_captures = ctx->graph_captures;
+    return SVS_OK;
+}
+
+extern "C" int svs_ctx_set_graphs(svs_ctx* ctx, int on) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->graphs = on ? 1 : 0;
     return SVS_OK;
 }
 
@@ -957,11 +985,9 @@ uint64_t svs::top_rows(svs_index* idx, uint32_t n) {
     return idx->top_rows_prefix[std::min<size_t>(n, idx->lists.size())];
 }
 
-int svs::plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t* probe_ptr,
-                      const uint32_t* probe_lists, const uint8_t* verify, uint64_t cands_bound,
-                      int variant, PlanView& v) {
+int svs::plan_setup(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items,
+                    const uint8_t* verify, uint64_t cands_bound, int variant, PlanView& v) {
     svs_ctx* ctx = idx->ctx;
-    const uint32_t n_items = probe_ptr[nq];
     if (!device_plan_ok(idx, n_items))
         return fail(SVS_ERR_ARG, "device planning takes up to %u items and %u lists",
                     PLAN_MAX_ITEMS, PLAN_MAX_LISTS - 1);
@@ -982,6 +1008,13 @@ int svs::plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t*
     for (const ListHost& L : idx->lists)
         if (L.m && !L.d.mp8) v.planes_ok = false;
     SVS_TRY(sl.q_probes.ensure(ctx, sizeof(uint32_t) * ((size_t)n_items + nq + 1) + nq + 64));
+    v.probes = (const uint32_t*)sl.q_probes.ptr;
+    return SVS_OK;
+}
+
+int svs::plan_stage(svs_slot& sl, const PlanView& v, const uint32_t* probe_ptr,
+                    const uint32_t* probe_lists, const uint8_t* verify) {
+    const uint32_t n_items = v.n_items, nq = v.nq;
     uint32_t* d_probes = (uint32_t*)sl.q_probes.ptr;
     uint32_t* d_pptr = d_probes + n_items;
     uint8_t* d_ver = (uint8_t*)(d_pptr + nq + 1);
@@ -996,7 +1029,6 @@ int svs::plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t*
     else
         hipLaunchKernelGGL(k_fill_u32, dim3(((nq + 3) / 4 + 255) / 256), dim3(256), 0, st,
                            (uint32_t*)d_ver, (nq + 3) / 4, 0u);
-    v.probes = d_probes;
     return SVS_OK;
 }
 
@@ -1023,13 +1055,19 @@ int svs::plan_launch(svs_index* idx, svs_slot& sl, const PlanView& v, int varian
     return SVS_OK;
 }
 
+int svs::route_setup(svs_index* idx, svs_slot& sl, uint32_t nq) {
+    svs_ctx* ctx = idx->ctx;
+    SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
+    SVS_TRY(sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * idx->cent_d));
+    SVS_TRY(sl.route_dist.ensure(ctx, sizeof(float) * (size_t)nq * idx->lists.size()));
+    return SVS_OK;
+}
+
 int svs::route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items,
                       double q_scale) {
     svs_ctx* ctx = idx->ctx;
     const uint32_t nl = (uint32_t)idx->lists.size(), d = idx->cent_d;
     hipStream_t st = sl.stream;
-    SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
-    SVS_TRY(sl.route_dist.ensure(ctx, sizeof(float) * (size_t)nq * nl));
     const float* x = (const float*)sl.q_raw.ptr;
     hipLaunchKernelGGL(k_quantise, dim3((uint32_t)std::min<uint64_t>(((uint64_t)nq * N + 255) / 256,
                                                                       (uint64_t)ctx->n_cu * 8)),
@@ -1252,6 +1290,91 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
     return SVS_OK;
 }
 
+int svs::view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k) {
+    const uint32_t nq = v.nq;
+    const uint64_t ni1 = std::max<uint32_t>(v.n_items, 1), nc1 = std::max<uint64_t>(v.cap_cands, 1);
+    SVS_TRY(sl.verdict.ensure(ctx, sizeof(int32_t) * (ni1 + nq) + 64));
+    SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc1));
+    SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc1));
+    SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc1));
+    SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
+    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+    SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA *
+                                        std::max<uint64_t>(v.cap_chunks, 1)));
+    SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+    SVS_TRY(sl.out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
+                                   sizeof(uint32_t) * nq + 64));
+    if (k <= (uint32_t)TOPK_LANE)
+        SVS_TRY(sl.topk_part.ensure(ctx, sizeof(Cand) * std::max<uint64_t>(v.cap_tchunks, 1) * k));
+    return SVS_OK;
+}
+
+uint64_t svs::cap_bucket(uint64_t rows) {
+    if (rows <= (1ull << 16)) return 1ull << 16;
+    int e = 63 - __builtin_clzll(rows) - 3;
+    const uint64_t unit = 1ull << e;
+    return (rows + unit - 1) / unit * unit;
+}
+
+void svs::release_graphs(svs_slot& sl) {
+    for (auto& g : sl.graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
+    sl.graphs.clear();
+    sl.graph_seen.clear();
+}
+
+int svs::run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
+                     const std::function<int()>& enqueue) {
+    if (!ctx->graphs || sl.prof.on || ctx->fault_list >= 0) return enqueue();
+    for (DevBuf* b : sl.slot_scratch()) {
+        key.push_back((uint64_t)(uintptr_t)b->ptr);
+        key.push_back(b->cap);
+    }
+    const uint64_t now = ++sl.graph_clock;
+    for (auto& g : sl.graphs)
+        if (g.key == key) {
+            g.last_use = now;
+            HIP_TRY(hipGraphLaunch(g.exec, sl.stream));
+            ++sl.graph_launches;
+            return SVS_OK;
+        }
+    auto it = std::find(sl.graph_seen.begin(), sl.graph_seen.end(), key);
+    if (it == sl.graph_seen.end()) {
+        if (sl.graph_seen.size() >= 64) sl.graph_seen.erase(sl.graph_seen.begin());
+        sl.graph_seen.push_back(std::move(key));
+        return enqueue();
+    }
+    sl.graph_seen.erase(it);
+    HIP_TRY(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(sl.stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    hipError_t ei = hipErrorUnknown;
+    if (rc == SVS_OK && e == hipSuccess && graph)
+        ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (rc != SVS_OK || e != hipSuccess || ei != hipSuccess) {
+        // not capturable after all: run it as launches
+        (void)hipGetLastError();
+        if (exec) hipGraphExecDestroy(exec);
+        return enqueue();
+    }
+    if (sl.graphs.size() >= 32) {
+        size_t lru = 0;
+        for (size_t i = 1; i < sl.graphs.size(); ++i)
+            if (sl.graphs[i].last_use < sl.graphs[lru].last_use) lru = i;
+        HIP_TRY(hipStreamSynchronize(sl.stream));
+        hipGraphExecDestroy(sl.graphs[lru].exec);
+        sl.graphs.erase(sl.graphs.begin() + lru);
+    }
+    sl.graphs.push_back(svs_slot::Graph{std::move(key), exec, now});
+    ++sl.graph_captures;
+    HIP_TRY(hipGraphLaunch(exec, sl.stream));
+    ++sl.graph_launches;
+    return SVS_OK;
+}
+
 int svs::stage_queries(svs_ctx* ctx, svs_slot& sl, uint32_t nq, const uint64_t* v_q,
                   const uint8_t* seeds) {
     SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
@@ -1391,21 +1514,97 @@ static int check_plan_overflow(svs_slot& sl, const PlanView& v) {
 
 // One batched run (mode BATCHED or BATCHED_VALU) planned on the device: the
 // host only validates the probe lists and counts their rows.
-static int batch_device_planned(svs_index* idx, const svs_query_batch* b, int mode) {
+// Pinned staging of a batch's host inputs in ctx->h_stage: probe_ptr,
+// probe lists, verify flags, then (optionally) raw f32 queries, v_q and seeds.
+struct Staged {
+    uint32_t* pptr = nullptr;
+    uint32_t* plist = nullptr;
+    uint8_t* ver = nullptr;
+    float* x = nullptr;
+    uint64_t* vq = nullptr;
+    uint8_t* seeds = nullptr;
+};
+static int stage_host(svs_ctx* ctx, uint32_t nq, const uint32_t* probe_ptr,
+                      const uint32_t* probe_lists, const uint8_t* verify, uint32_t d,
+                      const float* x, const uint64_t* vq, const uint8_t* seeds, Staged& o) {
+    const uint32_t ni = probe_ptr[nq];
+    auto al = [](size_t b) { return (b + 63) & ~(size_t)63; };
+    const size_t o_plist = al(sizeof(uint32_t) * (nq + 1));
+    const size_t o_ver = o_plist + al(sizeof(uint32_t) * ni);
+    const size_t o_x = o_ver + al(nq);
+    const size_t o_vq = o_x + al(x ? sizeof(float) * (size_t)nq * d : 0);
+    const size_t o_seeds = o_vq + al(vq ? sizeof(uint64_t) * (size_t)nq * N : 0);
+    const size_t total = o_seeds + al(seeds ? (size_t)nq * 32 : 0);
+    SVS_TRY(ctx->h_stage.ensure(total));
+    char* base = (char*)ctx->h_stage.ptr;
+    o.pptr = (uint32_t*)base;
+    for (uint32_t q = 0; q <= nq; ++q) o.pptr[q] = probe_ptr[q] - probe_ptr[0];
+    o.plist = (uint32_t*)(base + o_plist);
+    if (probe_lists) std::memcpy(o.plist, probe_lists + probe_ptr[0], sizeof(uint32_t) * ni);
+    o.ver = (uint8_t*)(base + o_ver);
+    if (verify) std::memcpy(o.ver, verify, nq);
+    else std::memset(o.ver, 0, nq);
+    if (x) std::memcpy(o.x = (float*)(base + o_x), x, sizeof(float) * (size_t)nq * d);
+    if (vq) std::memcpy(o.vq = (uint64_t*)(base + o_vq), vq, sizeof(uint64_t) * (size_t)nq * N);
+    if (seeds) std::memcpy(o.seeds = (uint8_t*)(base + o_seeds), seeds, (size_t)nq * 32);
+    return SVS_OK;
+}
+
+// One batched run (mode BATCHED or BATCHED_VALU) planned on the device: the
+// host only validates the probe lists and counts their rows. dev: v_q,
+// seeds and outputs are device pointers.
+static int batch_device_planned(svs_index* idx, const svs_query_batch* b, int mode, bool dev) {
     svs_ctx* ctx = idx->ctx;
+    svs_slot& sl = *ctx;
     const uint32_t nq = b->n_queries;
     uint64_t rows = 0;
     SVS_TRY(probe_rows(idx, nq, b->probe_ptr, b->probe_lists, rows));
     const int kernel = mode == SVS_MODE_BATCHED ? KERNEL_MFMA : KERNEL_VALU;
     const int variant = ctx_variant_for(ctx, nq);
+    const uint32_t n_items = b->probe_ptr[nq];
+    const uint64_t cap = ctx->graphs ? cap_bucket(rows) : rows;
+    Staged h;
+    SVS_TRY(stage_host(ctx, nq, b->probe_ptr, b->probe_lists, b->verify, 0, nullptr,
+                       dev ? nullptr : b->v_q, dev ? nullptr : b->q_seeds, h));
     PlanView v;
-    SVS_TRY(plan_prepare(idx, *ctx, nq, b->probe_ptr, b->probe_lists, b->verify, rows, variant, v));
-    SVS_TRY(stage_queries(ctx, *ctx, nq, b->v_q, b->q_seeds));
-    SVS_TRY(plan_launch(idx, *ctx, v, variant));
-    SVS_TRY(run_view(idx, *ctx, v, b->k, false, kernel, true, variant));
-    SVS_TRY(collect(ctx, *ctx, nq, b->k, v.n_items, b->out_ids, b->out_scores, b->out_count,
+    SVS_TRY(plan_setup(idx, sl, nq, n_items, h.ver, cap, variant, v));
+    SVS_TRY(stage_queries(ctx, sl, nq, nullptr, nullptr));
+    SVS_TRY(view_reserve(ctx, sl, v, b->k));
+    if (sl.prof.on) {
+        // profile counters: the product's algorithmic bytes (distinct lists)
+        std::vector<uint8_t> seen(idx->lists.size(), 0);
+        for (uint32_t i = 0; i < n_items; ++i) {
+            const uint32_t c = h.plist[i];
+            if (c < seen.size() && !seen[c]) {
+                seen[c] = 1;
+                v.distinct_bytes += (uint64_t)idx->lists[c].m * N * 8;
+            }
+        }
+        v.n_cands = rows;
+    }
+    if (dev) {
+        HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, b->v_q, sizeof(uint64_t) * (size_t)nq * N,
+                               hipMemcpyDeviceToDevice, sl.stream));
+        HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, b->q_seeds, (size_t)nq * 32,
+                               hipMemcpyDeviceToDevice, sl.stream));
+    }
+    const std::vector<uint64_t> key{1, (uint64_t)(uintptr_t)idx, nq, n_items, b->k,
+                                    (uint64_t)kernel, (uint64_t)variant, v.any_verify,
+                                    v.planes_ok, cap, dev, (uint64_t)(uintptr_t)h.pptr};
+    SVS_TRY(run_graphed(ctx, sl, key, [&]() -> int {
+        SVS_TRY(plan_stage(sl, v, h.pptr, h.plist, h.ver));
+        if (!dev) {
+            HIP_TRY(hipMemcpyAsync(sl.q_vq.ptr, h.vq, sizeof(uint64_t) * (size_t)nq * N,
+                                   hipMemcpyHostToDevice, sl.stream));
+            HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, h.seeds, (size_t)nq * 32,
+                                   hipMemcpyHostToDevice, sl.stream));
+        }
+        SVS_TRY(plan_launch(idx, sl, v, variant));
+        return run_view(idx, sl, v, b->k, false, kernel, true, variant);
+    }));
+    SVS_TRY(collect(ctx, sl, nq, b->k, v.n_items, b->out_ids, b->out_scores, b->out_count,
                     b->out_status));
-    return check_plan_overflow(*ctx, v);
+    return check_plan_overflow(sl, v);
 }
 
 // dev: v_q, q_seeds and the outputs are device pointers (checked for
@@ -1438,7 +1637,7 @@ static int score_batch_impl(svs_index* idx, const svs_query_batch* b, int mode, 
     const uint32_t nq_total = b->n_queries;
     if (mode != SVS_MODE_PLAIN && ctx->planner &&
         device_plan_ok(idx, b->probe_ptr[nq_total] - b->probe_ptr[0])) {
-        SVS_TRY(batch_device_planned(idx, b, mode));
+        SVS_TRY(batch_device_planned(idx, b, mode, dev));
         if (dev) {
             uint32_t bad = 0;
             HIP_TRY(hipMemcpy(&bad, d_badv, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1514,11 +1713,12 @@ static int route_ok(const svs_index* idx) {
 
 // Stages nq raw queries and a route-only probe layout in the slot:
 // q_raw <- queries, q_probes = [probes][probe_ptr][verify], q_seeds.
-static int stage_raw(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t d, const float* x,
+static int stage_raw(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t d, const float* x,
                      const uint32_t* probe_ptr, const uint8_t* seeds) {
+    svs_ctx* ctx = idx->ctx;
     const uint32_t n_items = probe_ptr[nq];
     hipStream_t st = sl.stream;
-    SVS_TRY(sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * d));
+    SVS_TRY(route_setup(idx, sl, nq));
     HIP_TRY(hipMemcpyAsync(sl.q_raw.ptr, x, sizeof(float) * (size_t)nq * d, hipMemcpyHostToDevice,
                            st));
     SVS_TRY(sl.q_probes.ensure(ctx, sizeof(uint32_t) * ((size_t)n_items + nq + 1) + nq + 64));
@@ -1542,7 +1742,7 @@ extern "C" int svs_route_batch(svs_index* idx, const float* queries, uint32_t n_
     for (uint32_t q = 0; q <= n_queries; ++q) pptr[q] = q * np;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    SVS_TRY(stage_raw(ctx, *ctx, n_queries, idx->cent_d, queries, pptr.data(), nullptr));
+    SVS_TRY(stage_raw(idx, *ctx, n_queries, idx->cent_d, queries, pptr.data(), nullptr));
     SVS_TRY(route_launch(idx, *ctx, n_queries, pptr[n_queries], (double)(1u << 20)));
     HIP_TRY(hipMemcpyAsync(out, ctx->q_probes.ptr, sizeof(uint32_t) * pptr[n_queries],
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -1593,23 +1793,40 @@ extern "C" int svs_score_raw_batch(svs_index* idx, const svs_raw_batch* b, int m
         if (mode != SVS_MODE_PLAIN && ctx->planner && device_plan_ok(idx, n_items) &&
             bound < 0xFFFFFFFFull) {
             // route -> plan -> pipeline, one stream, no host round trip
+            svs_slot& sl = *ctx;
+            Staged h;
+            SVS_TRY(stage_host(ctx, nq, pptr.data(), nullptr, ver, d, x, nullptr, seeds, h));
             PlanView v;
-            SVS_TRY(plan_prepare(idx, *ctx, nq, pptr.data(), nullptr, ver, bound, variant, v));
-            SVS_TRY(stage_raw(ctx, *ctx, nq, d, x, pptr.data(), seeds));
-            SVS_TRY(stage_queries(ctx, *ctx, nq, nullptr, nullptr));
-            SVS_TRY(route_launch(idx, *ctx, nq, n_items, q_scale));
-            SVS_TRY(plan_launch(idx, *ctx, v, variant));
-            SVS_TRY(run_view(idx, *ctx, v, b->k, false, kernel, true, variant));
+            SVS_TRY(plan_setup(idx, sl, nq, n_items, h.ver, bound, variant, v));
+            SVS_TRY(route_setup(idx, sl, nq));
+            SVS_TRY(stage_queries(ctx, sl, nq, nullptr, nullptr));
+            SVS_TRY(view_reserve(ctx, sl, v, b->k));
+            uint64_t qbits;
+            std::memcpy(&qbits, &q_scale, 8);
+            const std::vector<uint64_t> key{2, (uint64_t)(uintptr_t)idx, nq, n_items, b->k,
+                                            (uint64_t)kernel, (uint64_t)variant, v.any_verify,
+                                            v.planes_ok, bound, d, qbits,
+                                            (uint64_t)(uintptr_t)h.pptr};
+            SVS_TRY(run_graphed(ctx, sl, key, [&]() -> int {
+                SVS_TRY(plan_stage(sl, v, h.pptr, nullptr, h.ver));
+                HIP_TRY(hipMemcpyAsync(sl.q_raw.ptr, h.x, sizeof(float) * (size_t)nq * d,
+                                       hipMemcpyHostToDevice, sl.stream));
+                HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, h.seeds, (size_t)nq * 32,
+                                       hipMemcpyHostToDevice, sl.stream));
+                SVS_TRY(route_launch(idx, sl, nq, n_items, q_scale));
+                SVS_TRY(plan_launch(idx, sl, v, variant));
+                return run_view(idx, sl, v, b->k, false, kernel, true, variant);
+            }));
             if (b->out_probes && n_items)
                 HIP_TRY(hipMemcpyAsync(b->out_probes + probe_base, v.probes,
                                        sizeof(uint32_t) * n_items, hipMemcpyDeviceToHost,
-                                       ctx->stream));
-            SVS_TRY(collect(ctx, *ctx, nq, b->k, n_items, o_ids, o_scores, b->out_count + q0,
+                                       sl.stream));
+            SVS_TRY(collect(ctx, sl, nq, b->k, n_items, o_ids, o_scores, b->out_count + q0,
                             b->out_status + q0));
-            SVS_TRY(check_plan_overflow(*ctx, v));
+            SVS_TRY(check_plan_overflow(sl, v));
         } else {
             // route on the device, then plan on the host
-            SVS_TRY(stage_raw(ctx, *ctx, nq, d, x, pptr.data(), seeds));
+            SVS_TRY(stage_raw(idx, *ctx, nq, d, x, pptr.data(), seeds));
             SVS_TRY(stage_queries(ctx, *ctx, nq, nullptr, nullptr));
             SVS_TRY(route_launch(idx, *ctx, nq, n_items, q_scale));
             std::vector<uint32_t> probes(std::max<uint32_t>(n_items, 1));

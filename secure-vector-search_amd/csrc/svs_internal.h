@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <unordered_map>
 #include <utility>
@@ -38,6 +39,15 @@ struct DevBuf {
     size_t cap = 0;
     int ensure(svs_ctx* ctx, size_t bytes);
     void release(svs_ctx* ctx);
+};
+
+// Pinned host staging that grows (hipHostMalloc): the host side of copies
+// that a captured graph replays.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
 };
 
 // Objects that live on a context (indexes, EMVP clusters, Tiptoe databases)
@@ -87,6 +97,16 @@ struct svs_slot {
     // queries, routing distances
     svs::DevBuf q_probes, q_raw, route_dist;
     svs_prof prof;
+    // captured batch graphs (LRU) keyed by the batch's shape and the slot's
+    // buffers, and the keys seen once (a key is captured on its second use)
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t last_use = 0;
+    };
+    std::vector<Graph> graphs;
+    std::vector<std::vector<uint64_t>> graph_seen;
+    uint64_t graph_clock = 0, graph_launches = 0, graph_captures = 0;
     std::vector<svs::DevBuf*> slot_scratch() {
         return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
                 &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &r, &scores, &partials, &verdict,
@@ -121,6 +141,10 @@ struct svs_ctx : svs_slot {
     int tile_variant[4] = {0, 0, 0, 0};
     // batched runs planned on the device (1) or on the host (0)
     int planner = 1;
+    // device-planned batches replay captured HIP graphs (1) or launch (0)
+    int graphs = 1;
+    // pinned staging of caller inputs for graphed batches
+    svs::HostBuf h_stage;
     // svs_dev_alloc buffers and their sizes
     std::unordered_map<void*, size_t> user_allocs;
     void note_peak();
@@ -204,21 +228,36 @@ int run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k, bool
 // The same from a plan view (uploaded or built on the device).
 int run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k, bool product_only,
              int kernel, bool encode, int variant);
-// Device planning of a batched run. plan_prepare sizes the slot for nq
-// queries whose items are given by probe_ptr (host, nq + 1), and stages the
-// planner's inputs in sl.q_probes = [probes: n_items u32][probe_ptr: nq + 1
-// u32][verify: nq bytes]: probe_ptr and verify from the host, the probe lists
-// from the host too unless probe_lists is null (then a routing kernel writes
-// them). cands_bound bounds the batch's candidate rows. plan_launch enqueues
-// the planner; a plan beyond the bounds comes back flagged in
-// hdr[PH_OVERFLOW] and empty.
-int plan_prepare(svs_index* idx, svs_slot& sl, uint32_t nq, const uint32_t* probe_ptr,
-                 const uint32_t* probe_lists, const uint8_t* verify, uint64_t cands_bound,
-                 int variant, PlanView& v);
+// Device planning of a batched run. plan_setup sizes the slot for nq queries
+// and n_items items (host work and allocation only): the plan layout, the
+// caps (cands_bound bounds the candidate rows) and sl.q_probes = [probes:
+// n_items u32][probe_ptr: nq + 1 u32][verify: nq bytes]. plan_stage copies
+// probe_ptr, verify and (unless null: a routing kernel writes them) the
+// probe lists from the host. plan_launch enqueues the planner; a plan beyond
+// the bounds comes back flagged in hdr[PH_OVERFLOW] and empty.
+int plan_setup(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items, const uint8_t* verify,
+               uint64_t cands_bound, int variant, PlanView& v);
+int plan_stage(svs_slot& sl, const PlanView& v, const uint32_t* probe_ptr,
+               const uint32_t* probe_lists, const uint8_t* verify);
 int plan_launch(svs_index* idx, svs_slot& sl, const PlanView& v, int variant);
 // Quantises and routes nq raw queries already in sl.q_raw (f32, d each)
-// into sl.q_vq and the probe lists of sl.q_probes (layout above).
+// into sl.q_vq and the probe lists of sl.q_probes (layout above);
+// route_setup sizes the buffers.
+int route_setup(svs_index* idx, svs_slot& sl, uint32_t nq);
 int route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items, double q_scale);
+// Sizes every buffer run_view uses for this view and k.
+int view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k);
+// Enqueues a batch on the slot's stream by replaying the HIP graph captured
+// for `key` (plus the slot's buffer addresses); a key is captured on its
+// second use, and runs eagerly otherwise or when graphs are off, profiling
+// is on or a fault is injected. enqueue() must only enqueue device work and
+// copies from pinned host memory, and must not allocate.
+int run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
+                const std::function<int()>& enqueue);
+void release_graphs(svs_slot& sl);
+// Candidate-row capacity bucket: rows rounded up to at most 1/8 above, so
+// batches of similar size share a capture.
+uint64_t cap_bucket(uint64_t rows);
 // Plan layout in the slot for nq queries, n_items items and cap_tiles tiles:
 // ensures the buffers and sets v's array pointers.
 int plan_layout(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items, uint64_t cap_tiles,

@@ -211,34 +211,57 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
         if (dplan) {
             variant = ctx->tile_variant[nq <= 16 ? 0 : nq <= 64 ? 1 : nq <= 256 ? 2 : 3];
             PlanView& v = s.view;
+            const double qscale = raw ? (double)batch[0]->qq : 0.0;
+            // sizing first (host work, allocations), then the batch as one
+            // enqueue of copies from pinned memory and kernels, replayed as a
+            // captured graph once its shape has been seen
+            // (candidate capacity in buckets, so that batches of similar
+            // size share a capture)
+            const uint64_t cap = ctx->graphs ? cap_bucket(rows) : rows;
+            if (rc == SVS_OK) rc = plan_setup(idx, s.sl, nq, n_items, s.h_ver.p, cap, variant, v);
+            if (rc == SVS_OK && raw) rc = route_setup(idx, s.sl, nq);
+            if (rc == SVS_OK) rc = stage_queries(ctx, s.sl, nq, nullptr, nullptr);
+            if (rc == SVS_OK) rc = view_reserve(ctx, s.sl, v, k);
+            uint64_t qbits;
+            std::memcpy(&qbits, &qscale, 8);
+            const std::vector<uint64_t> key{
+                3, (uint64_t)(uintptr_t)idx, raw, nq, n_items, k, (uint64_t)kernel,
+                (uint64_t)variant, v.any_verify, v.planes_ok, cap, qbits,
+                (uint64_t)(uintptr_t)s.h_pptr.p, (uint64_t)(uintptr_t)s.h_plist.p,
+                (uint64_t)(uintptr_t)s.h_ver.p, (uint64_t)(uintptr_t)s.h_x.p,
+                (uint64_t)(uintptr_t)s.h_vq.p, (uint64_t)(uintptr_t)s.h_seeds.p,
+                (uint64_t)(uintptr_t)s.h_ids.p, (uint64_t)(uintptr_t)s.h_scores.p,
+                (uint64_t)(uintptr_t)s.h_count.p, (uint64_t)(uintptr_t)s.h_status.p,
+                (uint64_t)(uintptr_t)s.h_flag.p};
             if (rc == SVS_OK)
-                rc = plan_prepare(idx, s.sl, nq, s.h_pptr.p, raw ? nullptr : s.h_plist.p,
-                                  s.h_ver.p, rows, variant, v);
-            if (rc == SVS_OK && raw) {
-                rc = s.sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * d);
-                if (rc == SVS_OK &&
-                    hipMemcpyAsync(s.sl.q_raw.ptr, s.h_x.p, sizeof(float) * (size_t)nq * d,
-                                   hipMemcpyHostToDevice, st) != hipSuccess)
-                    rc = fail(SVS_ERR_HIP, "query upload failed");
-            }
-            if (rc == SVS_OK)
-                rc = stage_queries(ctx, s.sl, nq, raw ? nullptr : s.h_vq.p, s.h_seeds.p);
-            if (rc == SVS_OK && raw)
-                rc = route_launch(idx, s.sl, nq, n_items, (double)batch[0]->qq);
-            if (rc == SVS_OK) rc = plan_launch(idx, s.sl, v, variant);
-            if (rc == SVS_OK) rc = run_view(idx, s.sl, v, k, false, kernel, true, variant);
-            if (rc == SVS_OK &&
-                hipMemcpyAsync(s.h_flag.p + 1, v.hdr + PH_OVERFLOW, sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, st) != hipSuccess)
-                rc = fail(SVS_ERR_HIP, "plan flag copy failed");
-            s.plan.n_items = n_items;
+                rc = run_graphed(ctx, s.sl, key, [&]() -> int {
+                    SVS_TRY(plan_stage(s.sl, v, s.h_pptr.p, raw ? nullptr : s.h_plist.p,
+                                       s.h_ver.p));
+                    if (raw)
+                        HIP_TRY(hipMemcpyAsync(s.sl.q_raw.ptr, s.h_x.p,
+                                               sizeof(float) * (size_t)nq * d,
+                                               hipMemcpyHostToDevice, st));
+                    else
+                        HIP_TRY(hipMemcpyAsync(s.sl.q_vq.ptr, s.h_vq.p,
+                                               sizeof(uint64_t) * (size_t)nq * N,
+                                               hipMemcpyHostToDevice, st));
+                    HIP_TRY(hipMemcpyAsync(s.sl.q_seeds.ptr, s.h_seeds.p, (size_t)nq * 32,
+                                           hipMemcpyHostToDevice, st));
+                    if (raw) SVS_TRY(route_launch(idx, s.sl, nq, n_items, qscale));
+                    SVS_TRY(plan_launch(idx, s.sl, v, variant));
+                    SVS_TRY(run_view(idx, s.sl, v, k, false, kernel, true, variant));
+                    HIP_TRY(hipMemcpyAsync(s.h_flag.p + 1, v.hdr + PH_OVERFLOW, sizeof(uint32_t),
+                                           hipMemcpyDeviceToHost, st));
+                    return collect_async(s.sl, nq, k, n_items, s.h_ids.p, s.h_scores.p,
+                                         s.h_count.p, s.h_status.p, s.h_flag.p);
+                });
         } else {
             if (rc == SVS_OK) rc = stage_queries(ctx, s.sl, nq, s.h_vq.p, s.h_seeds.p);
             if (rc == SVS_OK) rc = run_plan(idx, s.sl, s.plan, k, false, hkernel, true, variant);
+            if (rc == SVS_OK)
+                rc = collect_async(s.sl, nq, k, s.plan.n_items, s.h_ids.p, s.h_scores.p,
+                                   s.h_count.p, s.h_status.p, s.h_flag.p);
         }
-        if (rc == SVS_OK)
-            rc = collect_async(s.sl, nq, k, s.plan.n_items, s.h_ids.p, s.h_scores.p,
-                               s.h_count.p, s.h_status.p, s.h_flag.p);
         if (rc == SVS_OK && hipEventRecord(s.done_ev, st) != hipSuccess)
             rc = fail(SVS_ERR_HIP, "hipEventRecord failed");
     }
@@ -439,6 +462,7 @@ extern "C" int svs_server_destroy(svs_server* s) {
         hipSetDevice(ctx->device);
         for (auto& sp : s->slots) {
             hipStreamSynchronize(sp->sl.stream);
+            release_graphs(sp->sl);
             for (DevBuf* b : sp->sl.slot_scratch()) b->release(ctx);
             if (sp->sl.prof.created)
                 for (auto& e : sp->sl.prof.ev) hipEventDestroy(e);
@@ -563,6 +587,11 @@ extern "C" int svs_server_stats(svs_server* s, svs_server_counters* out) {
     out->max_batch = s->max_batch_seen;
     out->max_in_flight = s->max_in_flight;
     out->pending = (uint32_t)s->pending.size();
+    out->graph_launches = out->graph_captures = 0;
+    for (auto& sp : s->slots) {
+        out->graph_launches += sp->sl.graph_launches;
+        out->graph_captures += sp->sl.graph_captures;
+    }
     return SVS_OK;
 }
 

@@ -36,6 +36,7 @@ EXPORTS = [
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
     "svs_ctx_set_planner", "svs_index_set_centroids", "svs_route_batch", "svs_score_raw_batch",
+    "svs_ctx_set_graphs",
 ]
 TILE_VARIANTS = 10
 ROUTE_MAX_LISTS = 8192  # device routing sorts a query's distances in LDS
@@ -58,7 +59,7 @@ class BudgetTooSmall(GpuError):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("budget_bytes", "resident_bytes", "peak_resident_bytes",
                                           "hbm_total_bytes", "hbm_free_bytes", "uploads",
-                                          "launches")]
+                                          "launches", "graph_launches", "graph_captures")]
 
 
 class Profile(C.Structure):
@@ -110,7 +111,8 @@ class Result(C.Structure):
 class ServerCounters(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("batched_queries", C.c_uint64), ("max_batch", C.c_uint32),
-                ("max_in_flight", C.c_uint32), ("pending", C.c_uint32)]
+                ("max_in_flight", C.c_uint32), ("pending", C.c_uint32),
+                ("graph_launches", C.c_uint64), ("graph_captures", C.c_uint64)]
 
 
 _lib = None
@@ -179,6 +181,7 @@ def lib() -> C.CDLL:
             "svs_server_run_poisson": ([vp, vp, u32, C.c_double, u64, vp,
                                         C.POINTER(C.c_double)], i32),
             "svs_ctx_set_planner": ([vp, i32], i32),
+            "svs_ctx_set_graphs": ([vp, i32], i32),
             "svs_index_set_centroids": ([vp, vp, u32], i32),
             "svs_route_batch": ([vp, vp, u32, u32, vp], i32),
             "svs_score_raw_batch": ([vp, C.POINTER(RawBatch), i32], i32),

@@ -212,6 +212,11 @@ class Context:
         """Plan batched runs on the device (default) or on the host."""
         check(L.lib().svs_ctx_set_planner(self.h, 1 if on_device else 0))
 
+    def set_graphs(self, on: bool = True):
+        """Replay device-planned batches from captured HIP graphs (default)
+        or launch every kernel."""
+        check(L.lib().svs_ctx_set_graphs(self.h, 1 if on else 0))
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""
