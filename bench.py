@@ -458,6 +458,12 @@ def main():
                     algorithmic_bytes_per_launch=int(prof["gemv_bytes"] / max(gemv_calls, 1)),
                     avg_launch_ms=round(gemv_ms / max(gemv_calls, 1), 4),
                     mulmods_per_s=round(mulmods / (gemv_ms * 1e-3), 1) if gemv_ms else 0,
+                    # the other bound: 8 v_mfma_i32_16x16x64_i8 (32768 int8 ops each) per
+                    # (row, query) pair, against 5.0 POP/s dense int8
+                    mfma_tops=round(rows * 8 * 32768 / (gemv_ms * 1e-3) / 1e12, 1)
+                    if gemv_ms else 0,
+                    mfma_frac=round(rows * 8 * 32768 / (gemv_ms * 1e-3) / 5.0e15, 4)
+                    if gemv_ms else 0,
                     measured="HIP events on the library's stream around the product stage "
                              "(k_query_corr + the product tiles), over a second pass of the "
                              "timed batches")

@@ -1455,13 +1455,20 @@ int svs::check_rng_flag(uint32_t rng_flag) {
     return SVS_OK;
 }
 
+// plan_hdr: a device plan's header, whose overflow word is checked too
+// (its bounds are exact or upper bounds, so a set flag is an internal error).
 static int collect(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t k, uint32_t n_items,
-                   uint32_t* ids, int64_t* scores, uint32_t* count, int32_t* status) {
+                   uint32_t* ids, int64_t* scores, uint32_t* count, int32_t* status,
+                   const uint32_t* plan_hdr = nullptr) {
     (void)ctx;
-    uint32_t rng_flag = 0;
+    uint32_t rng_flag = 0, over = 0;
     SVS_TRY(collect_async(sl, nq, k, n_items, ids, scores, count, status, &rng_flag));
+    if (plan_hdr)
+        HIP_TRY(hipMemcpyAsync(&over, plan_hdr + PH_OVERFLOW, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, sl.stream));
     HIP_TRY(hipStreamSynchronize(sl.stream));
     SVS_TRY(prof_collect(sl));
+    if (over) return fail(SVS_ERR_STATE, "device plan exceeded its bounds");
     return check_rng_flag(rng_flag);
 }
 
@@ -1501,16 +1508,6 @@ static int probe_rows(const svs_index* idx, uint32_t nq, const uint32_t* probe_p
     return SVS_OK;
 }
 
-// Reads back a device plan's overflow word (bounds are exact or upper
-// bounds, so a set flag is an internal error).
-static int check_plan_overflow(svs_slot& sl, const PlanView& v) {
-    uint32_t over = 0;
-    HIP_TRY(hipMemcpyAsync(&over, v.hdr + PH_OVERFLOW, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           sl.stream));
-    HIP_TRY(hipStreamSynchronize(sl.stream));
-    if (over) return fail(SVS_ERR_STATE, "device plan exceeded its bounds");
-    return SVS_OK;
-}
 
 // One batched run (mode BATCHED or BATCHED_VALU) planned on the device: the
 // host only validates the probe lists and counts their rows.
@@ -1602,9 +1599,8 @@ static int batch_device_planned(svs_index* idx, const svs_query_batch* b, int mo
         SVS_TRY(plan_launch(idx, sl, v, variant));
         return run_view(idx, sl, v, b->k, false, kernel, true, variant);
     }));
-    SVS_TRY(collect(ctx, sl, nq, b->k, v.n_items, b->out_ids, b->out_scores, b->out_count,
-                    b->out_status));
-    return check_plan_overflow(sl, v);
+    return collect(ctx, sl, nq, b->k, v.n_items, b->out_ids, b->out_scores, b->out_count,
+                   b->out_status, v.hdr);
 }
 
 // dev: v_q, q_seeds and the outputs are device pointers (checked for
@@ -1822,8 +1818,7 @@ extern "C" int svs_score_raw_batch(svs_index* idx, const svs_raw_batch* b, int m
                                        sizeof(uint32_t) * n_items, hipMemcpyDeviceToHost,
                                        sl.stream));
             SVS_TRY(collect(ctx, sl, nq, b->k, n_items, o_ids, o_scores, b->out_count + q0,
-                            b->out_status + q0));
-            SVS_TRY(check_plan_overflow(sl, v));
+                            b->out_status + q0, v.hdr));
         } else {
             // route on the device, then plan on the host
             SVS_TRY(stage_raw(idx, *ctx, nq, d, x, pptr.data(), seeds));

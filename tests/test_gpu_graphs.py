@@ -159,3 +159,24 @@ def test_served_fixed_shape_replays(ctx, setup):
         assert res.ids.tolist() == ids[0, :count[0]].tolist(), f"request {i}"
         assert res.scores.tolist() == scores[0, :count[0]].tolist(), f"request {i}"
     assert st["graph_launches"] > 0
+
+
+def test_graph_cache_eviction(ctx, setup):
+    """More shapes than a slot keeps graphs for: least recently used graphs
+    are evicted and recaptured; every result still equals the launched one."""
+    idx = setup["idx"]
+    rng = np.random.default_rng(6)
+    batches = [_routed_batch(setup, rng, nq, 3) for nq in range(1, 41)]
+    ctx.set_graphs(False)
+    want = [_run(idx, x, k=5) for x in batches]
+    ctx.set_graphs(True)
+    s0 = ctx.stats()
+    for rep in range(4):
+        for i, x in enumerate(batches):
+            got = _run(idx, x, k=5)
+            for g, w in zip(got, want[i]):
+                assert np.array_equal(g, w), f"shape {i} pass {rep}"
+    s1 = ctx.stats()
+    say("eviction counters", s1["graph_captures"] - s0["graph_captures"],
+        s1["graph_launches"] - s0["graph_launches"])
+    assert s1["graph_captures"] - s0["graph_captures"] > 32
