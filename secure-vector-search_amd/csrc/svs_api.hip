@@ -1310,10 +1310,9 @@ int svs::view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k)
 }
 
 uint64_t svs::cap_bucket(uint64_t rows) {
-    if (rows <= (1ull << 16)) return 1ull << 16;
-    int e = 63 - __builtin_clzll(rows) - 3;
-    const uint64_t unit = 1ull << e;
-    return (rows + unit - 1) / unit * unit;
+    uint64_t c = 1ull << 16;
+    while (c < rows) c <<= 1;
+    return c;
 }
 
 void svs::release_graphs(svs_slot& sl) {

@@ -255,8 +255,8 @@ int view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k);
 int run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
                 const std::function<int()>& enqueue);
 void release_graphs(svs_slot& sl);
-// Candidate-row capacity bucket: rows rounded up to at most 1/8 above, so
-// batches of similar size share a capture.
+// Candidate-row capacity bucket: rows rounded up to a power of two (at
+// least 2^16), so batches of similar size share a capture.
 uint64_t cap_bucket(uint64_t rows);
 // Plan layout in the slot for nq queries, n_items items and cap_tiles tiles:
 // ensures the buffers and sets v's array pointers.

@@ -21,23 +21,36 @@ constexpr int PLAN_THREADS = 1024;
 constexpr uint32_t PLAN_ITEM_MASK = PLAN_MAX_ITEMS - 1;
 
 // Exclusive scan of a[0..n) in place (u32 values, u64 running sums) by the
-// whole block; a[n] = total. Returns the total to every thread.
+// whole block; a[n] = total. Returns the total to every thread. Each thread
+// sums a contiguous segment; wave prefix sums by shuffles, then one wave
+// scans the wave totals (red: at least 2 x the block's wave count words).
 __device__ uint64_t plan_scan(uint32_t* a, uint32_t n, uint64_t* red) {
     const uint32_t t = threadIdx.x, T = blockDim.x;
+    const uint32_t lane = t % WAVE, w = t / WAVE, nw = T / WAVE;
     const uint32_t seg = (n + T - 1) / T;
     const uint32_t b = min(n, t * seg), e = min(n, b + seg);
     uint64_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += a[i];
-    red[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < T; off <<= 1) {
-        const uint64_t v = t >= off ? red[t - off] : 0;
-        __syncthreads();
-        red[t] += v;
-        __syncthreads();
+    uint64_t x = s;  // inclusive prefix within the wave
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off, WAVE);
+        if ((int)lane >= off) x += y;
     }
-    uint64_t run = red[t] - s;
-    const uint64_t total = red[T - 1];
+    if (lane == WAVE - 1) red[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        uint64_t v = lane < nw ? red[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint64_t y = __shfl_up(v, off, WAVE);
+            if ((int)lane >= off) v += y;
+        }
+        if (lane < nw) red[nw + lane] = v;  // inclusive over the waves
+    }
+    __syncthreads();
+    const uint64_t total = red[2 * nw - 1];
+    uint64_t run = (w ? red[nw + w - 1] : 0) + x - s;
     for (uint32_t i = b; i < e; ++i) {
         const uint32_t v = a[i];
         a[i] = (uint32_t)run;
@@ -48,7 +61,12 @@ __device__ uint64_t plan_scan(uint32_t* a, uint32_t n, uint64_t* red) {
     return total;
 }
 
-// Ascending bitonic sort of keys[0..p2) in LDS, p2 a power of two.
+// Ascending bitonic sort of keys[0..p2) in LDS, p2 a power of two, by a
+// block whose thread t handles positions t, t + blockDim, ... . A stage
+// whose partner distance j is below WAVE pairs positions of one aligned
+// 64-chunk, and every chunk belongs to one wave, so a stage that follows
+// such a stage needs only the wave's own LDS ordering (a wave-scope fence);
+// the block barrier is kept before and after stages that pair across waves.
 __device__ void plan_sort(uint32_t* keys, uint32_t p2) {
     for (uint32_t k = 2; k <= p2; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -62,8 +80,16 @@ __device__ void plan_sort(uint32_t* keys, uint32_t p2) {
                     }
                 }
             }
-            __syncthreads();
+            // the next stage: j / 2, or (k * 2) / 2 = k when j == 1
+            const uint32_t next_j = j > 1 ? j >> 1 : k;
+            if (j < WAVE && next_j < WAVE) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            } else {
+                __syncthreads();
+            }
         }
+    __syncthreads();  // every wave reads the sorted keys next
 }
 
 // End of the list group that starts at g: the first position after g whose
@@ -81,8 +107,9 @@ __device__ __forceinline__ uint32_t plan_group_end(const uint32_t* keys, uint32_
 
 // Tiles of tile_rows rows x up to tile_qg items over the sorted keys
 // keys[0..ng) (list << PLAN_ITEM_BITS | item): per list group, row blocks
-// outer, query groups inner. gitems[g] = item of keys[g]. tmp holds ng + 1
-// words. Returns the tile count; writes no tile past cap.
+// outer, query groups inner. gitems[g] = item of keys[g]. tmp (LDS, ng + 1
+// words) takes the tile offsets, so that each tile's binary search for its
+// group stays in LDS. Returns the tile count; writes no tile past cap.
 __device__ uint64_t plan_tiles(const uint32_t* keys, uint32_t ng, const Item* items,
                                uint32_t tile_rows, uint32_t tile_qg, uint32_t* tmp,
                                uint32_t* gitems, MTile* tiles, uint32_t cap, uint64_t* red) {
@@ -123,10 +150,10 @@ __device__ uint64_t plan_tiles(const uint32_t* keys, uint32_t ng, const Item* it
 // probe order), item_ptr[nq + 1] (each query's items), qverify[nq].
 // Outputs: items, item_verify, chunk_ptr / cand_ptr [n_items + 1], a copy
 // of item_ptr, qchunk_ptr [nq + 1], gitems + tiles, vgitems + vtiles (items
-// to verify),
-// and hdr (tile counts, candidate count, overflow). tmp: 2 (n_items + 1)
-// words. A plan over the capacities (candidate rows, top-k chunks, tiles)
-// is emptied (no rows, no tiles) and flagged in hdr[PH_OVERFLOW].
+// to verify), and hdr (tile counts, candidate count, overflow). tmp:
+// n_items words of global scratch. A plan over the capacities (candidate
+// rows, top-k chunks, tiles) is emptied (no rows, no tiles) and flagged in
+// hdr[PH_OVERFLOW].
 __global__ void __launch_bounds__(PLAN_THREADS)
 k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __restrict__ probes,
        const uint32_t* __restrict__ item_ptr, const uint8_t* __restrict__ qverify, uint32_t nq,
@@ -134,21 +161,21 @@ k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __re
        uint32_t cap_tchunks, uint32_t cap_tiles, Item* __restrict__ items,
        uint8_t* __restrict__ item_verify, uint32_t* __restrict__ chunk_ptr,
        uint32_t* __restrict__ cand_ptr, uint32_t* __restrict__ item_ptr_out,
-       uint32_t* __restrict__ qchunk_ptr, uint32_t* __restrict__ tmp, uint32_t* __restrict__ gitems, MTile* __restrict__ tiles,
-       uint32_t* __restrict__ vgitems, MTile* __restrict__ vtiles, uint32_t* __restrict__ hdr) {
+       uint32_t* __restrict__ qchunk_ptr, uint32_t* __restrict__ tmp,
+       uint32_t* __restrict__ gitems, MTile* __restrict__ tiles, uint32_t* __restrict__ vgitems,
+       MTile* __restrict__ vtiles, uint32_t* __restrict__ hdr) {
     __shared__ uint32_t keys[PLAN_MAX_ITEMS];
+    // per item its query, then the tile offsets and verify flags
+    __shared__ uint32_t scratch[PLAN_MAX_ITEMS + 1];
     __shared__ uint64_t red[PLAN_THREADS];
     const uint32_t T = blockDim.x;
     for (uint32_t q = threadIdx.x; q <= nq; q += T) item_ptr_out[q] = item_ptr[q];
+    for (uint32_t q = threadIdx.x; q < nq; q += T)
+        for (uint32_t i = item_ptr[q]; i < item_ptr[q + 1]; ++i) scratch[i] = q;
+    __syncthreads();
     // items in probe order; rows and chunks per item
     for (uint32_t i = threadIdx.x; i < n_items; i += T) {
-        uint32_t lo = 0, hi = nq;  // item_ptr[lo] <= i < item_ptr[lo + 1]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (item_ptr[mid] <= i) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t q = lo, c = probes[i];
+        const uint32_t q = scratch[i], c = probes[i];
         const bool ok = c < n_lists;
         const uint32_t m = ok ? lists[c].m : 0;
         items[i] = Item{q, ok ? c : 0, i - item_ptr[q], m};
@@ -184,23 +211,23 @@ k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __re
             else hi = mid;
         }
         const uint32_t ng = lo;
-        nt = plan_tiles(keys, ng, items, tile_rows, tile_qg, tmp, gitems, tiles, cap_tiles, red);
+        nt = plan_tiles(keys, ng, items, tile_rows, tile_qg, scratch, gitems, tiles, cap_tiles,
+                        red);
         over = nt > cap_tiles;
         if (!over) {
             // the verified items' keys, in order (a stable compaction)
-            uint32_t* vflag = tmp;
-            uint32_t* vkeys = tmp + n_items + 1;
+            uint32_t* vflag = scratch;
             for (uint32_t g = threadIdx.x; g < ng; g += T)
                 vflag[g] = item_verify[keys[g] & PLAN_ITEM_MASK] ? 1u : 0u;
             __syncthreads();
             const uint32_t nvg = (uint32_t)plan_scan(vflag, ng, red);
             for (uint32_t g = threadIdx.x; g < ng; g += T)
-                if (vflag[g + 1] != vflag[g]) vkeys[vflag[g]] = keys[g];
+                if (vflag[g + 1] != vflag[g]) tmp[vflag[g]] = keys[g];
             __syncthreads();
-            for (uint32_t g = threadIdx.x; g < nvg; g += T) keys[g] = vkeys[g];
+            for (uint32_t g = threadIdx.x; g < nvg; g += T) keys[g] = tmp[g];
             __syncthreads();
-            nvt = plan_tiles(keys, nvg, items, tile_rows, tile_qg, tmp, vgitems, vtiles, cap_tiles,
-                             red);
+            nvt = plan_tiles(keys, nvg, items, tile_rows, tile_qg, scratch, vgitems, vtiles,
+                             cap_tiles, red);
             over = nvt > cap_tiles;
         }
     }

@@ -37,25 +37,40 @@ k_quantise(const float* __restrict__ x, uint32_t nq, uint32_t d, double q_scale,
 // dist[q][c] = l2_f32_scalar(query q, centroid c). Block tile: 32 queries x
 // 32 centroids; thread t owns query t % 32 and centroids 4 (t / 32) + 0..3.
 // Dimensions beyond d are staged as zeros: (0 - 0)^2 = +0 leaves a partial
-// sum unchanged (every partial sum is >= +0 or NaN).
+// sum unchanged (every partial sum is >= +0 or NaN). The next round's 32
+// dimensions are loaded into registers (clamped addresses, unconditional
+// loads) while the current round is summed.
 __global__ void __launch_bounds__(256)
 k_route_dist(const float* __restrict__ x, uint32_t nq, const float* __restrict__ cent,
              uint32_t nc, uint32_t d, float* __restrict__ dist) {
     __shared__ float xs[RT_J][RT_Q + 1];
     __shared__ __align__(16) float cs[RT_J][RT_C + 4];
+    constexpr int PER = RT_Q * RT_J / 256;  // staged elements per thread and array
     const uint32_t q0 = blockIdx.x * RT_Q, c0 = blockIdx.y * RT_C;
     const int t = threadIdx.x;
     const int tq = t % RT_Q, tc = 4 * (t / RT_Q);
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+    float xn[PER], cn[PER];
+    auto fetch = [&](uint32_t j0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = t + 256 * k, r = e / RT_J, jj = e % RT_J;
+            const uint32_t j = min(j0 + jj, d - 1);
+            xn[k] = x[(size_t)min(q0 + r, nq - 1) * d + j];
+            cn[k] = cent[(size_t)min(c0 + r, nc - 1) * d + j];
+        }
+    };
+    fetch(0);
     for (uint32_t j0 = 0; j0 < d; j0 += RT_J) {
-        for (int e = t; e < RT_Q * RT_J; e += 256) {
-            const int r = e / RT_J, jj = e % RT_J;
-            const uint32_t j = j0 + jj;
-            const uint32_t q = q0 + r, c = c0 + r;
-            xs[jj][r] = (q < nq && j < d) ? x[(size_t)q * d + j] : 0.f;
-            cs[jj][r] = (c < nc && j < d) ? cent[(size_t)c * d + j] : 0.f;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = t + 256 * k, r = e / RT_J, jj = e % RT_J;
+            const bool jok = j0 + jj < d;
+            xs[jj][r] = (q0 + r < nq && jok) ? xn[k] : 0.f;
+            cs[jj][r] = (c0 + r < nc && jok) ? cn[k] : 0.f;
         }
         __syncthreads();
+        if (j0 + RT_J < d) fetch(j0 + RT_J);
 #pragma unroll 8
         for (int jj = 0; jj < RT_J; ++jj) {
             const float xv = xs[jj][tq];

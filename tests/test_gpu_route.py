@@ -236,3 +236,29 @@ def test_scorer_routes_on_device(ctx, setup):
     one = [s["scorer"].score(s["handle"], s["qs"][i], 10, q_seed=seeds[i * 32:(i + 1) * 32])
            for i in range(8)]
     assert got == one
+
+
+def test_device_plan_near_item_limit(ctx, setup):
+    """16000 items (the planner's sort at its largest power of two): the
+    device plan equals the host plan."""
+    import svs
+    idx = setup["idx"]
+    nl = len(setup["lists"])
+    rng = np.random.default_rng(8)
+    per = nl + 1
+    nq = 16000 // per
+    v_q = svs.quantise(setup["qs"][rng.integers(0, setup["qs"].shape[0], nq)])
+    seeds = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    plist = np.concatenate([rng.permutation(np.append(np.arange(nl), nl + 3)) for _ in range(nq)])
+    plist = plist.astype(np.uint32)
+    pp = np.arange(0, nq * per + 1, per, dtype=np.uint32)
+    verify = rng.integers(0, 2, nq).astype(np.uint8)
+    try:
+        ctx.set_planner(False)
+        want = idx.score_batch(v_q, seeds, pp, plist, verify, 7, 1)
+        ctx.set_planner(True)
+        got = idx.score_batch(v_q, seeds, pp, plist, verify, 7, 1)
+    finally:
+        ctx.set_planner(True)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
