@@ -165,6 +165,7 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     auto* ctx = new svs_ctx();
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
+    if (const char* g = getenv("SVS_TILE_GRID")) ctx->tile_grid = std::max(1, std::min(8, atoi(g)));
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1086,9 +1087,13 @@ int svs::route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_item
     return SVS_OK;
 }
 
-// blocks of a tile kernel: one per tile up to the resident limit
-static uint32_t tile_blocks(svs_ctx* ctx, uint64_t cap, uint32_t per_cu) {
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, (uint64_t)ctx->n_cu * per_cu));
+// Blocks of a persistent tile kernel: `resident` blocks fit on a CU at
+// once; the grid is that many per CU times ctx->tile_grid (1: every block
+// resident from the start, so the tiles t = w, w + G, ... that the XCDs take
+// in step are processed together), capped at one block per tile.
+static uint32_t tile_blocks(svs_ctx* ctx, uint64_t cap, uint32_t resident) {
+    return (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(cap, (uint64_t)ctx->n_cu * resident * ctx->tile_grid));
 }
 
 int svs::run_plan(svs_index* idx, svs_slot& sl, const BatchPlan& pl, uint32_t k,
@@ -1159,7 +1164,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
                                v.gitems, d_items, d_cand_ptr, (const uint64_t*)d_venc,
                                (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         } else {
-            const uint32_t blocks = tile_blocks(ctx, v.cap_tiles, 4 * kTileVariants[variant].occ);
+            const uint32_t blocks = tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ);
             const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
             launch_gemm<N, 0>(variant, st, blocks, ta, (const uint64_t*)d_venc,
                               (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
@@ -1191,7 +1196,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         if (kernel == KERNEL_MFMA && v.cap_vtiles) {
             SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc1));
             const TileArgs va{d_lists, v.vtiles, d_nvt, v.vgitems, d_items, d_cand_ptr};
-            const uint32_t blocks = tile_blocks(ctx, v.cap_vtiles, 4 * kTileVariants[variant].occ);
+            const uint32_t blocks = tile_blocks(ctx, v.cap_vtiles, kTileVariants[variant].occ);
             launch_gemm<N, 0>(variant, st, blocks, va, (const uint64_t*)d_venc,
                               (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.rv.ptr);
             ctx->launches++;
@@ -1221,7 +1226,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         // inside the per-candidate decode kernel.
         const uint64_t* dense = nullptr;
         const bool tiles = kernel == KERNEL_MFMA && v.cap_chunks;
-        const uint32_t tblocks = tile_blocks(ctx, v.cap_tiles, 4 * kTileVariants[variant].occ);
+        const uint32_t tblocks = tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ);
         const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
         if (tiles) {
             SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
@@ -1239,7 +1244,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             uint64_t* mt_q = (uint64_t*)sl.q_corr.ptr + 2 * nq;
             hipLaunchKernelGGL(k_decode_prep, dim3(nq), dim3(256), 0, st, (const uint64_t*)d_venc,
                                (const int8_t*)d_tdense, (int8_t*)sl.q_vplanes.ptr, mt_q);
-            launch_decode_tiles(variant, st, tile_blocks(ctx, v.cap_tiles, 4), ta,
+            launch_decode_tiles(variant, st, tile_blocks(ctx, v.cap_tiles, 1), ta,
                                 (const int8_t*)d_tdense, (const int8_t*)sl.q_vplanes.ptr,
                                 (const uint64_t*)mt_q, (const uint64_t*)sl.r.ptr, dense,
                                 (int64_t*)sl.scores.ptr);

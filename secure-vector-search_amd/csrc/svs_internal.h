@@ -141,6 +141,8 @@ struct svs_ctx : svs_slot {
     int tile_variant[4] = {0, 0, 0, 0};
     // batched runs planned on the device (1) or on the host (0)
     int planner = 1;
+    // persistent tile grids: resident blocks per CU times this
+    int tile_grid = 1;
     // device-planned batches replay captured HIP graphs (1) or launch (0)
     int graphs = 1;
     // pinned staging of caller inputs for graphed batches
