@@ -180,6 +180,17 @@ int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint
                const uint64_t* g_host, const uint32_t* t_idx_host, const int8_t* t_sign_host,
                uint32_t t_nnz, uint64_t* mv_host);
 
+/* Top-k of decoded scores (fp_to_signed, crypto.rs:128-135, then the
+ * sort by Reverse(score) and truncate of lib.rs:452-469): query q's
+ * candidates are mv[seg_ptr[q] .. seg_ptr[q+1]) (canonical F_p, e.g. from
+ * svs_decode) with their global ids, concatenated in probe order as the IVF
+ * scorer does (ivf.rs:806-820). Outputs up to k (id, signed score) per query
+ * (n_queries x k), descending, ties by position; out_count[q] = min(k,
+ * candidates). k in 1..128. */
+int svs_topk(svs_ctx* ctx, uint32_t n_queries, const uint32_t* seg_ptr_host,
+             const uint64_t* mv_host, const uint32_t* ids_host, uint32_t k,
+             uint32_t* out_ids_host, int64_t* out_scores_host, uint32_t* out_count_host);
+
 /* ---- whole-query pipeline (encode -> product -> verify -> decode -> top-k)
  * For each query q: v_q (quantised, n u64), q_seed (32 B), its probe lists
  * (probe_ptr[q]..probe_ptr[q+1] into probe_lists, in probe order), a verify

@@ -2123,6 +2123,57 @@ extern "C" int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint6
     return SVS_OK;
 }
 
+// Standalone top-k (fp_to_signed + sort by Reverse(score) + truncate,
+// lib.rs:452-469; IVF concatenates the lists' candidates in probe order,
+// ivf.rs:806-820) over caller segments, host buffers.
+extern "C" int svs_topk(svs_ctx* ctx, uint32_t n_queries, const uint32_t* seg_ptr,
+                        const uint64_t* mv, const uint32_t* ids, uint32_t k, uint32_t* out_ids,
+                        int64_t* out_scores, uint32_t* out_count) {
+    if (!ctx || (n_queries && (!seg_ptr || !out_ids || !out_scores || !out_count)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (k == 0 || k > 128) return fail(SVS_ERR_ARG, "k = %u must be in 1..128", k);
+    if (n_queries == 0) return SVS_OK;
+    if (seg_ptr[0] != 0) return fail(SVS_ERR_ARG, "seg_ptr[0] must be 0");
+    for (uint32_t q = 0; q < n_queries; ++q)
+        if (seg_ptr[q + 1] < seg_ptr[q]) return fail(SVS_ERR_ARG, "seg_ptr not monotone");
+    const uint64_t nc = seg_ptr[n_queries];
+    if (nc && (!mv || !ids)) return fail(SVS_ERR_ARG, "null argument");
+    SVS_TRY(check_canonical(mv, nc, "mv"));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t out_b = (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)n_queries * k +
+                         sizeof(uint32_t) * n_queries + 64;
+    SVS_TRY(ctx->tmp_a.ensure(ctx, sizeof(uint64_t) * std::max<uint64_t>(nc, 1)));
+    SVS_TRY(ctx->tmp_b.ensure(ctx, sizeof(uint32_t) * (std::max<uint64_t>(nc, 1) + n_queries + 1)));
+    SVS_TRY(ctx->tmp_c.ensure(ctx, out_b));
+    hipStream_t st = ctx->stream;
+    uint64_t* d_mv = (uint64_t*)ctx->tmp_a.ptr;
+    uint32_t* d_ids = (uint32_t*)ctx->tmp_b.ptr;
+    uint32_t* d_seg = d_ids + std::max<uint64_t>(nc, 1);
+    uint32_t* d_oids = (uint32_t*)ctx->tmp_c.ptr;
+    int64_t* d_osc = (int64_t*)(d_oids + (size_t)n_queries * k + ((n_queries * k) & 1));
+    uint32_t* d_ocnt = (uint32_t*)(d_osc + (size_t)n_queries * k);
+    if (nc) {
+        HIP_TRY(hipMemcpyAsync(d_mv, mv, sizeof(uint64_t) * nc, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_ids, ids, sizeof(uint32_t) * nc, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(d_seg, seg_ptr, sizeof(uint32_t) * (n_queries + 1),
+                           hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_topk_seg<128, 32>), dim3(n_queries), dim3(32), 0, st,
+                       (const uint32_t*)d_seg, (const uint64_t*)d_mv, (const uint32_t*)d_ids, k,
+                       d_oids, d_osc, d_ocnt);
+    ctx->launches++;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out_ids, d_oids, sizeof(uint32_t) * (size_t)n_queries * k,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out_scores, d_osc, sizeof(int64_t) * (size_t)n_queries * k,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out_count, d_ocnt, sizeof(uint32_t) * n_queries,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SVS_OK;
+}
+
 extern "C" int svs_debug_inject_fault(svs_ctx* ctx, int32_t list, uint32_t row, uint64_t delta) {
     if (!ctx) return fail(SVS_ERR_ARG, "null context");
     if (delta >= P) return fail(SVS_ERR_ARG, "delta must be canonical");

@@ -356,6 +356,61 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
     if (tid == 0) out_count[q] = kk;
 }
 
+// Top-k over caller segments (the standalone svs_topk): query q's
+// candidates are mv[seg_ptr[q] .. seg_ptr[q + 1]) with ids alongside, in
+// concatenation order; scores are fp_to_signed(mv), ranked by (score desc,
+// position asc) as k_topk.
+template <int KT, int THREADS>
+__global__ void __launch_bounds__(THREADS)
+k_topk_seg(const uint32_t* __restrict__ seg_ptr, const uint64_t* __restrict__ mv,
+           const uint32_t* __restrict__ ids, uint32_t k, uint32_t* __restrict__ out_ids,
+           int64_t* __restrict__ out_scores, uint32_t* __restrict__ out_count) {
+    __shared__ Cand lst[THREADS][KT];
+    __shared__ int cnt[THREADS];
+    __shared__ int head[THREADS];
+    __shared__ int red_idx[THREADS];
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t c0 = seg_ptr[q], c1 = seg_ptr[q + 1];
+    int n = 0;
+    for (uint32_t g = c0 + tid; g < c1; g += THREADS) {
+        Cand c{fp_to_signed(mv[g]), g - c0, 0};
+        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
+        int pos = n < (int)k ? n : (int)k - 1;
+        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
+            lst[tid][pos] = lst[tid][pos - 1];
+            --pos;
+        }
+        lst[tid][pos] = c;
+        if (n < (int)k) ++n;
+    }
+    cnt[tid] = n;
+    head[tid] = 0;
+    __syncthreads();
+    const uint32_t kk = min(k, c1 - c0);
+    for (uint32_t outi = 0; outi < kk; ++outi) {
+        red_idx[tid] = head[tid] < cnt[tid] ? tid : -1;
+        __syncthreads();
+        for (int s = THREADS / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                int a = red_idx[tid], b = red_idx[tid + s];
+                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                    red_idx[tid] = b;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int w = red_idx[0];
+            const Cand c = lst[w][head[w]];
+            head[w]++;
+            out_ids[(size_t)q * k + outi] = ids[c0 + c.pos];
+            out_scores[(size_t)q * k + outi] = c.score;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out_count[q] = kk;
+}
+
 // Two-phase top-k for k <= TOPK_LANE (the served k = 10 case).
 // Phase 1: every 1024-candidate chunk of a query (one wave, 16 candidates
 // per lane held in registers) yields its k best by k rounds of a wave

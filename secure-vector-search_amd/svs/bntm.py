@@ -217,6 +217,20 @@ class Context:
         or launch every kernel."""
         check(L.lib().svs_ctx_set_graphs(self.h, 1 if on else 0))
 
+    def topk(self, seg_ptr, mv, ids, k: int):
+        """svs_topk: per segment the k best fp_to_signed(mv) with their ids
+        (score desc, position asc); returns (ids, scores, count)."""
+        sp = np.ascontiguousarray(seg_ptr, np.uint32)
+        nq = sp.size - 1
+        m = u64(mv).reshape(-1)
+        i = np.ascontiguousarray(ids, np.uint32).reshape(-1)
+        oi = np.zeros((nq, k), np.uint32)
+        os_ = np.zeros((nq, k), np.int64)
+        oc = np.zeros(nq, np.uint32)
+        check(L.lib().svs_topk(self.h, nq, ptr(sp), ptr(m) if m.size else None,
+                               ptr(i) if i.size else None, k, ptr(oi), ptr(os_), ptr(oc)))
+        return oi, os_, oc
+
     def inject_fault(self, list_id: int, row: int = 0, delta: int = 1):
         """Test hook: a server that adds `delta` to r[row] of `list_id`
         (list_id < 0 disables)."""

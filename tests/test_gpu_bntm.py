@@ -574,3 +574,29 @@ def test_score_batch_dev_matches_host(ctx, ivf_setup):
                                 ids.ptr(), sc.ptr(), cnt.ptr(), st.ptr())
     for d in (dv, ds, ids, sc, cnt, st):
         d.close()
+
+
+def test_topk_standalone_matches_oracle(ctx):
+    """svs_topk (fp_to_signed + Reverse(score) sort + truncate, lib.rs:452-469)
+    against the oracle: empty and short segments, ties (many equal scores),
+    the signed boundary at p/2, k up to 128."""
+    rng = np.random.default_rng(12)
+    sizes = [0, 1, 5, 17, 128, 129, 1000, 3000, 0, 64]
+    seg = np.zeros(len(sizes) + 1, np.uint32)
+    seg[1:] = np.cumsum(sizes)
+    n = int(seg[-1])
+    mv = rng.integers(0, P, n, dtype=np.uint64)
+    mv[::7] = np.uint64(12345)                      # ties
+    mv[1::11] = np.uint64(P // 2)                   # the most negative score
+    mv[2::13] = np.uint64(P // 2 - 1)               # the largest score
+    mv[3::17] = np.uint64(0)
+    ids = rng.permutation(n).astype(np.uint32)
+    for k in (1, 10, 128):
+        say("standalone top-k k =", k)
+        oi, os_, oc = ctx.topk(seg, mv, ids, k)
+        for q in range(len(sizes)):
+            a, b = int(seg[q]), int(seg[q + 1])
+            wi, ws = O.topk(ids[a:b], O.fp_to_signed_vec(mv[a:b]), k)
+            assert oc[q] == wi.size, f"segment {q}"
+            assert oi[q, :oc[q]].tolist() == wi.tolist(), f"segment {q}"
+            assert os_[q, :oc[q]].tolist() == ws.tolist(), f"segment {q}"
