@@ -235,6 +235,36 @@ def served(index, qs, v_q, probes, verify, k, levels, seconds, max_batch, slots,
     return out
 
 
+def served_mix(index, qs, k, qps, seconds, max_batch, slots, seed=9):
+    """One open-loop level with the serving request mix of the reference's
+    served configuration: raw queries, nprobe uniform in {8, 16, 32, 64,
+    128}, half of them verified."""
+    import svs
+    rng = np.random.default_rng(seed)
+    n = max(64, int(qps * seconds))
+    choices = np.array([8, 16, 32, 64, 128])
+
+    def req(i):
+        return dict(query=qs[i % qs.shape[0]], nprobe=int(rng.choice(choices)),
+                    q_seed=rng.integers(0, 256, 32, np.uint8), verify=bool(rng.integers(0, 2)),
+                    k=k)
+    with svs.Server(index, max_batch=max_batch, slots=slots) as srv:
+        srv.run_poisson([req(i) for i in range(512)], qps=qps, seed=seed)  # sizes the slot
+        reqs = [req(i) for i in range(n)]
+        s0 = srv.stats()
+        res, wall = srv.run_poisson(reqs, qps=qps, seed=seed + 1)
+        s1 = srv.stats()
+    lat = [r.latency_ms for r in res]
+    nb = s1["batches"] - s0["batches"]
+    out = dict(offered_qps=qps, achieved_qps=round(n / wall, 1), requests=n,
+               p50_ms=round(pct(lat, 50), 3), p99_ms=round(pct(lat, 99), 3),
+               mean_batch=round((s1["batched_queries"] - s0["batched_queries"]) / max(nb, 1), 1),
+               failed=int(sum(1 for r in res if r.status != 0)),
+               mix="raw queries, nprobe uniform in {8,16,32,64,128}, 50% verified")
+    log(f"served mix: {out}")
+    return out
+
+
 def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threads):
     """The reference CPU path (BnTmIvfScorer::score, ivf.rs:746-822) as the C
     oracle, one query per thread, on a bounded sample of the workload's
@@ -321,6 +351,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1,
                     help="batches in flight (1 measured best: a second slot only splits "
                          "the same HBM bandwidth between two smaller batches)")
+    ap.add_argument("--serve-mix-qps", type=float, default=4000.0,
+                    help="offered rate of the mixed-nprobe, half-verified served level (0 = skip)")
     ap.add_argument("--serve-routed", action="store_true",
                     help="served requests carry quantised queries and probe lists "
                          "(default: raw f32 queries routed on the device)")
@@ -444,6 +476,11 @@ def main():
         serve = served(index, qs, v_q, probes, verify, args.k, levels, args.serve_seconds,
                        args.serve_max_batch, args.serve_slots, pool, not args.serve_routed)
 
+    mix = None
+    if levels and args.serve_mix_qps > 0:
+        mix = served_mix(index, qs, args.k, args.serve_mix_qps, args.serve_seconds,
+                         args.serve_max_batch, args.serve_slots)
+
     gemv_ms, gemv_calls = prof["product"]
     achieved = prof["gemv_bytes"] / (gemv_ms * 1e-3) / 1e9 if gemv_ms else 0.0
     rows = prof["gemv_rows"]
@@ -525,6 +562,7 @@ def main():
                         "note": "f32 queries in host buffers; quantise_vector and probe_route "
                                 "run on the device inside the batch"},
         "served": serve,
+        "served_mix": mix,
         "served_requests": "routed (v_q + probes)" if args.serve_routed else
                            "raw f32 queries, quantised and routed on the device",
         "bit_exact_batched_vs_plain": bool(exact),

@@ -129,6 +129,27 @@ int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m, const uint64
 int svs_index_download_list(svs_index* idx, uint32_t list, uint64_t* m_enc_host,
                             uint64_t* a_l_host, uint64_t* h_host);
 
+/* Load a BN-IVF index from the reference's cache file
+ * (.bntm-ivf-cache-<fingerprint>.bin: layout scorer-bntm/src/ivf.rs:1683-1902,
+ * header scorer-core/src/cache.rs:19-73, IVF payload ivf-index/src/ivf.rs:
+ * 68-131) straight into HBM: the file is memory-mapped and each list's
+ * M_enc, AL, H and M_plain go from the mapping to the device; S is
+ * regenerated from the list's h_seed and the regenerated H must equal the
+ * stored one; the centroids are kept for device routing. A header that does
+ * not match (magic, version, or the fingerprint when check_fingerprint) is
+ * the reference's Ok(None): SVS_NONE and *out = NULL. Replaces
+ * load_bntm_ivf_cache + IvfGpuState::build. */
+int svs_index_load_ivf_cache(svs_ctx* ctx, const char* path, uint64_t fingerprint,
+                             int check_fingerprint, svs_index** out, uint32_t* n_lists,
+                             uint32_t* dim);
+/* fingerprint64 (cache.rs:22-37): FNV-1a 64 over the length-prefixed parts. */
+uint64_t svs_cache_fingerprint64(const uint8_t* const* parts, const uint64_t* lens,
+                                 uint32_t n_parts);
+/* Copies of an index's routing centroids (n_lists x d f32) and subspace L
+ * (n x n1 u64) back to the host. */
+int svs_index_get_centroids(svs_index* idx, float* out_host);
+int svs_index_get_subspace(svs_index* idx, uint64_t* l_out_host);
+
 /* Rows of a list (0 for empty/out-of-range). */
 uint32_t svs_index_list_rows(const svs_index* idx, uint32_t list);
 

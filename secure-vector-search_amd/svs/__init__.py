@@ -8,6 +8,6 @@ from .bntm import (  # noqa: F401
     BuildOutcome, CommunicationCost, Context, DeviceArray, DimensionMismatch, DimensionTooLarge, EmptyCorpus,
     GpuIndex, Hit, InvalidNprobe, IvfIndex, TooManyCentroids, VerificationFailed, default_context,
     derive_h_seed, domain_separate, fp_to_signed, generate_subspace, probe_route, quantise,
-    regenerate_h_s,
+    regenerate_h_s, cache_fingerprint64, load_bntm_ivf_cache,
 )
 from .serve import Server, ServedResult  # noqa: F401,E402

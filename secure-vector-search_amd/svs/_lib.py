@@ -36,7 +36,8 @@ EXPORTS = [
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
     "svs_ctx_set_planner", "svs_index_set_centroids", "svs_route_batch", "svs_score_raw_batch",
-    "svs_ctx_set_graphs", "svs_topk",
+    "svs_ctx_set_graphs", "svs_topk", "svs_index_load_ivf_cache", "svs_cache_fingerprint64",
+    "svs_index_get_centroids", "svs_index_get_subspace",
 ]
 TILE_VARIANTS = 10
 ROUTE_MAX_LISTS = 8192  # device routing sorts a query's distances in LDS
@@ -183,6 +184,11 @@ def lib() -> C.CDLL:
             "svs_ctx_set_planner": ([vp, i32], i32),
             "svs_ctx_set_graphs": ([vp, i32], i32),
             "svs_topk": ([vp, u32, vp, vp, vp, u32, vp, vp, vp], i32),
+            "svs_index_load_ivf_cache": ([vp, C.c_char_p, u64, i32, C.POINTER(vp),
+                                          C.POINTER(u32), C.POINTER(u32)], i32),
+            "svs_cache_fingerprint64": ([C.POINTER(C.c_char_p), C.POINTER(u64), u32], u64),
+            "svs_index_get_centroids": ([vp, vp], i32),
+            "svs_index_get_subspace": ([vp, vp], i32),
             "svs_index_set_centroids": ([vp, vp, u32], i32),
             "svs_route_batch": ([vp, vp, u32, u32, vp], i32),
             "svs_score_raw_batch": ([vp, C.POINTER(RawBatch), i32], i32),

@@ -598,6 +598,43 @@ class BnTmIvfHandle:
     quantisation_q: int
 
 
+def cache_fingerprint64(parts) -> int:
+    """fingerprint64 of the cache key parts (scorer-core/src/cache.rs:22-37)."""
+    bs = [bytes(p) for p in parts]
+    arr = (C.c_char_p * max(len(bs), 1))(*bs)
+    lens = (C.c_uint64 * max(len(bs), 1))(*[len(b) for b in bs])
+    return int(L.lib().svs_cache_fingerprint64(arr, lens, len(bs)))
+
+
+def load_bntm_ivf_cache(path, ctx: "Context | None" = None, fingerprint: int | None = None,
+                        nprobe: int = 1, verification_enabled: bool = False,
+                        quantisation_q: int = Q_DEFAULT):
+    """The reference's BN-IVF cache file loaded straight into HBM
+    (load_bntm_ivf_cache, scorer-bntm/src/ivf.rs:1761-1902): a handle for
+    BnTmIvfScorer, or None when the header does not match (magic, version,
+    and the fingerprint when given)."""
+    ctx = ctx or default_context()
+    h = C.c_void_p()
+    nl, d = C.c_uint32(0), C.c_uint32(0)
+    rc = check(L.lib().svs_index_load_ivf_cache(ctx.h, os.fsencode(path), fingerprint or 0,
+                                                1 if fingerprint is not None else 0, C.byref(h),
+                                                C.byref(nl), C.byref(d)))
+    if rc == L.SVS_NONE:
+        return None
+    idx = GpuIndex.__new__(GpuIndex)
+    idx.ctx, idx.h, idx.n_lists, idx.cent_d = ctx, h, nl.value, d.value
+    idx.l = np.zeros(N * N1, np.uint64)
+    check(L.lib().svs_index_get_subspace(h, ptr(idx.l)))
+    _live_indexes.add(idx)
+    cent = np.zeros((nl.value, d.value), np.float32)
+    check(L.lib().svs_index_get_centroids(h, ptr(cent)))
+    sizes = np.array([idx.rows(c) for c in range(nl.value)], np.int64)
+    if nprobe == 0 or nprobe > nl.value:
+        raise InvalidNprobe(nprobe, nl.value)
+    return BnTmIvfHandle(idx, IvfIndex(cent, []), idx.l, sizes, nprobe, d.value,
+                         verification_enabled, quantisation_q)
+
+
 class BnTmIvfScorer:
     """BnTmIvfScorer (ivf.rs:209) on the gfx950 backend."""
 

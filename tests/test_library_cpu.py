@@ -109,3 +109,23 @@ def test_synthetic_lists_partition_the_corpus():
     for ci, members in enumerate(lists):
         assert np.all(np.diff(members.astype(np.int64)) > 0)
         assert np.all(c.component[members] == ci)
+
+
+def test_cache_fingerprint_is_fnv1a_over_length_prefixed_parts():
+    """fingerprint64 (scorer-core/src/cache.rs:22-37) restated from the
+    FNV-1a definition: offset basis 0xcbf29ce484222325, prime 0x100000001b3,
+    each part preceded by its u64 little-endian length."""
+    import struct
+    import svs
+
+    def fnv(parts):
+        h = 0xcbf29ce484222325
+        for p in parts:
+            for b in struct.pack("<Q", len(p)) + p:
+                h = ((h ^ b) * 0x100000001b3) & ((1 << 64) - 1)
+        return h
+    assert svs.cache_fingerprint64([]) == 0xcbf29ce484222325
+    for parts in ([b""], [b"a"], [b"bntm", b"\x00\x01\x02", b"x" * 300]):
+        assert svs.cache_fingerprint64(parts) == fnv(parts)
+    # the length prefix separates ("ab", "") from ("a", "b")
+    assert svs.cache_fingerprint64([b"ab", b""]) != svs.cache_fingerprint64([b"a", b"b"])
