@@ -131,13 +131,13 @@ struct svs_server {
     void run();
     void launch(Slot& s, std::vector<Request*>& batch);
     void retire(Slot& s);
-    uint64_t batch_bytes(const Request& r) const;
+    uint64_t batch_rows(const Request& r) const;
+    uint64_t batch_bytes(uint64_t rows, uint32_t nq, uint32_t k) const;
 };
 
-// Device scratch a request adds to a batch: per candidate row the product,
-// score and dense-term words (24 B) plus the top-k partials; per query the
-// encode state (v_q, v_enc, [g ; lt], T, byte planes, ...), about 30 KiB.
-uint64_t svs_server::batch_bytes(const Request& r) const {
+// Candidate rows a request adds to a batch (for a routed request, the
+// bound its device plan is sized for: its nprobe largest lists).
+uint64_t svs_server::batch_rows(const Request& r) const {
     const svs_index* idx = tenants[r.tenant];
     uint64_t rows = 0;
     if (r.raw) {
@@ -146,7 +146,16 @@ uint64_t svs_server::batch_bytes(const Request& r) const {
         for (uint32_t c : r.probes)
             if (c < idx->lists.size()) rows += idx->lists[c].m;
     }
-    return rows * 26 + 30 * 1024 + (uint64_t)r.k * 16;
+    return rows;
+}
+
+// Device scratch of a batch: per candidate row of its (bucketed) capacity
+// the product, its verification recompute, the score and the dense term
+// (4 x 8 B) plus verification partials; per query the encode state (v_q,
+// v_enc, [g ; lt], T, byte planes, ...) of about 30 KiB and its top-k.
+uint64_t svs_server::batch_bytes(uint64_t rows, uint32_t nq, uint32_t k) const {
+    const uint64_t cap = ctx->graphs ? cap_bucket(rows) : rows;
+    return cap * 36 + (uint64_t)nq * (30 * 1024 + (uint64_t)k * 16);
 }
 
 void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
@@ -363,19 +372,23 @@ void svs_server::run() {
                 // FIFO over requests; a batch holds one tenant's requests
                 const uint32_t tenant = pending.front()->tenant;
                 const bool raw = pending.front()->raw;
-                uint64_t bytes = 0;
-                uint32_t items = 0;
+                uint64_t rows = 0;
+                uint32_t items = 0, kmax = 0;
                 for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch;) {
                     Request* r = *it;
                     if (r->tenant != tenant || r->raw != raw) {
                         ++it;
                         continue;
                     }
-                    const uint64_t b = batch_bytes(*r);
-                    if (!batch.empty() && bytes + b > cfg.slot_bytes) break;
+                    const uint64_t rr = batch_rows(*r);
+                    const uint32_t kk = std::max(kmax, r->k);
+                    if (!batch.empty() &&
+                        batch_bytes(rows + rr, (uint32_t)batch.size() + 1, kk) > cfg.slot_bytes)
+                        break;
                     // a raw batch stays within what the device planner takes
                     if (raw && !batch.empty() && items + r->nprobe > PLAN_MAX_ITEMS) break;
-                    bytes += b;
+                    rows += rr;
+                    kmax = kk;
                     items += raw ? r->nprobe : 0;
                     batch.push_back(r);
                     it = pending.erase(it);
@@ -498,7 +511,8 @@ extern "C" int svs_server_reserve(svs_server* s, uint32_t n_queries, uint32_t n_
     std::lock_guard<std::mutex> g(s->ctx->mu);
     HIP_TRY(hipSetDevice(s->ctx->device));
     for (auto& sp : s->slots) {
-        SVS_TRY(reserve_slot(s->ctx, sp->sl, n_queries, n_items, n_candidates, k));
+        SVS_TRY(reserve_slot(s->ctx, sp->sl, n_queries, n_items,
+                             s->ctx->graphs ? cap_bucket(n_candidates) : n_candidates, k));
         SVS_TRY(sp->h_vq.ensure((size_t)n_queries * N));
         SVS_TRY(sp->h_seeds.ensure((size_t)n_queries * 32));
         SVS_TRY(sp->h_ids.ensure((size_t)n_queries * k));

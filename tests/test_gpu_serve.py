@@ -155,7 +155,9 @@ def test_served_admission_by_memory(ctx, tenant_a):
         results = [srv.wait(tk) for tk in tickets]
         st = srv.stats()
     say("admission stats", st)
-    assert st["max_batch"] <= 3  # ~30 KiB per query plus its candidate rows
+    # a batch needs >= 2^16 candidate rows of capacity (36 B each) plus ~30 KiB
+    # per query: beyond 100 KiB, so every batch holds just its first request
+    assert st["max_batch"] == 1
     for i, (res, rq) in enumerate(zip(results, reqs)):
         _check(res, tenant_a, rq, f"request {i}")
 
