@@ -422,6 +422,11 @@ int svs_emvp_destroy(svs_emvp* e);
  * an empty cluster. */
 int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const uint64_t* q_hat_host,
                               uint64_t* partials_host);
+/* Same with device pointers (q_hat_dev canonical, n_queries x 1292;
+ * partials_dev n_queries x 76 x m), asynchronous on the context stream:
+ * the resident-input path, without the partials' trip over PCIe. */
+int svs_emvp_compute_products_dev(svs_emvp* e, uint32_t n_queries, const uint64_t* q_hat_dev,
+                                  uint64_t* partials_dev);
 
 /* ---- Tiptoe SimplePIR answer (same kernel family) --------------------------
  * Resident database M_db (rows x cols, row-major) of u32 (elem_bytes 4) or

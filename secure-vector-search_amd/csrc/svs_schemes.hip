@@ -23,20 +23,48 @@ constexpr int TWAVE = 64;
 
 // ===================================================================== EMVP
 // partials[q][si * m + i] = sum_{l < 17} m_hat[i][17 si + l] * q_hat[q][17 si + l]
-// (mod p), one thread per (row, block). The 17 row values stay in registers
-// across the batch's queries, so a batch reads m_hat once. 17 products of
-// canonical values sum below 2^127: one reduction per partial.
+// (mod p). A work unit is 64 rows x EM_G column blocks (EM_G * 17 u64 =
+// 544 contiguous bytes of each row): the block stages it in LDS with 16-byte
+// loads that run along the rows, then wave w takes block si = EM_G g + w with
+// one lane per row, holds its 17 values in registers across the batch's
+// queries (a batch reads m_hat once) and writes 64 consecutive partials per
+// query. 17 products of canonical values sum below 2^127: one reduction per
+// partial. Units are ordered row tile major, column group minor, and the
+// XCD remap gives each XCD a contiguous range of them, so the cache lines
+// two neighbouring column groups share stay in one L2.
+constexpr int EM_ROWS = 64, EM_G = 4, EM_W = EM_G * EMVP_B, EM_GROUPS = EMVP_S / EM_G;
+static_assert(EMVP_S % EM_G == 0 && EM_W % 2 == 0, "EMVP tile shape");
+
+__device__ __forceinline__ uint32_t em_xcd_unit() {
+    const uint32_t b = blockIdx.x, g = gridDim.x;
+    return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
+}
+
 __global__ void __launch_bounds__(256)
 k_emvp(const uint64_t* __restrict__ m_hat, uint32_t m, const uint64_t* __restrict__ q_hat,
        uint32_t nq, uint64_t* __restrict__ partials) {
-    const uint64_t total = (uint64_t)m * EMVP_S;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t i = (uint32_t)(t / EMVP_S), si = (uint32_t)(t % EMVP_S);
-        const uint64_t* row = m_hat + (size_t)i * EMVP_N + (size_t)si * EMVP_B;
+    __shared__ uint64_t tile[EM_ROWS][EM_W + 1];  // odd row stride: conflict-light reads
+    const uint32_t lane = threadIdx.x % TWAVE;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TWAVE);
+    const uint32_t n_units = (m + EM_ROWS - 1) / EM_ROWS * EM_GROUPS;
+    for (uint32_t u = em_xcd_unit(); u < n_units; u += gridDim.x) {
+        const uint32_t i0 = u / EM_GROUPS * EM_ROWS, g = u % EM_GROUPS;
+        __syncthreads();  // the previous unit's LDS reads are done
+        for (uint32_t c = threadIdx.x; c < EM_ROWS * (EM_W / 2); c += blockDim.x) {
+            const uint32_t r = c / (EM_W / 2), k = c % (EM_W / 2);
+            if (i0 + r < m) {
+                const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(
+                    m_hat + (size_t)(i0 + r) * EMVP_N + (size_t)g * EM_W + 2 * k);
+                tile[r][2 * k] = v.x;
+                tile[r][2 * k + 1] = v.y;
+            }
+        }
+        __syncthreads();
+        const uint32_t i = i0 + lane, si = g * EM_G + wave;
+        if (i >= m) continue;
         uint64_t a[EMVP_B];
 #pragma unroll
-        for (int l = 0; l < EMVP_B; ++l) a[l] = row[l];
+        for (int l = 0; l < EMVP_B; ++l) a[l] = tile[lane][wave * EMVP_B + l];
         for (uint32_t q = 0; q < nq; ++q) {
             const uint64_t* qb = q_hat + (size_t)q * EMVP_N + (size_t)si * EMVP_B;
             Acc128 acc;
@@ -263,6 +291,27 @@ extern "C" int svs_emvp_destroy(svs_emvp* e) {
     return SVS_OK;
 }
 
+static int emvp_launch(svs_emvp* e, uint32_t n_queries, const uint64_t* q_dev, uint64_t* out_dev) {
+    svs_ctx* ctx = e->ctx;
+    const uint64_t units = (uint64_t)(e->m + EM_ROWS - 1) / EM_ROWS * EM_GROUPS;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(units, (uint64_t)ctx->n_cu * 4);
+    hipLaunchKernelGGL(k_emvp, dim3(blocks), dim3(256), 0, ctx->stream, (const uint64_t*)e->m_hat,
+                       e->m, q_dev, n_queries, out_dev);
+    HIP_TRY(hipGetLastError());
+    ctx->launches++;
+    return SVS_OK;
+}
+
+extern "C" int svs_emvp_compute_products_dev(svs_emvp* e, uint32_t n_queries,
+                                             const uint64_t* q_hat_dev, uint64_t* partials_dev) {
+    if (!e || (n_queries && (!q_hat_dev || !partials_dev))) return fail(SVS_ERR_ARG, "null argument");
+    if (e->m == 0 || n_queries == 0) return e->m == 0 ? SVS_NONE : SVS_OK;
+    svs_ctx* ctx = e->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return emvp_launch(e, n_queries, q_hat_dev, partials_dev);
+}
+
 extern "C" int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const uint64_t* q_hat,
                                          uint64_t* partials) {
     if (!e || (n_queries && (!q_hat || !partials))) return fail(SVS_ERR_ARG, "null argument");
@@ -277,11 +326,7 @@ extern "C" int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const 
     SVS_TRY(e->out.ensure(ctx, sizeof(uint64_t) * out_n));
     HIP_TRY(hipMemcpyAsync(e->q.ptr, q_hat, sizeof(uint64_t) * (size_t)n_queries * EMVP_N,
                            hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_emvp, dim3(grid_for(ctx, (uint64_t)e->m * EMVP_S, 256)), dim3(256), 0,
-                       ctx->stream, (const uint64_t*)e->m_hat, e->m, (const uint64_t*)e->q.ptr,
-                       n_queries, (uint64_t*)e->out.ptr);
-    HIP_TRY(hipGetLastError());
-    ctx->launches++;
+    SVS_TRY(emvp_launch(e, n_queries, (const uint64_t*)e->q.ptr, (uint64_t*)e->out.ptr));
     HIP_TRY(hipMemcpyAsync(partials, e->out.ptr, sizeof(uint64_t) * out_n, hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -41,6 +41,14 @@ class EmvpCluster:
         rc = check(L.lib().svs_emvp_compute_products(self.h, q.shape[0], ptr(q), ptr(out)))
         return None if rc == L.SVS_NONE else out[:, : EMVP_S * self.m]
 
+    def compute_products_dev(self, n_queries: int, q_hat_ptr: int, partials_ptr: int):
+        """Device pointers (canonical q_hat, n_queries x 1292 u64; partials
+        n_queries x 76 x m u64), asynchronous on the context stream. False
+        for an empty cluster."""
+        rc = check(L.lib().svs_emvp_compute_products_dev(self.h, n_queries, C.c_void_p(q_hat_ptr),
+                                                         C.c_void_p(partials_ptr)))
+        return rc != L.SVS_NONE
+
     def close(self):
         if getattr(self, "h", None):
             L.lib().svs_emvp_destroy(self.h)

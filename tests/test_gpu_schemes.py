@@ -32,7 +32,7 @@ def test_emvp_partials_match_oracle(ctx, golden):
     say("emvp golden")
     cl = EmvpCluster(m_hat, ctx)
     assert [str(int(x)) for x in cl.compute_products(q_hat)] == e["out"]
-    for m in (1, 7, 64, 257):
+    for m in (1, 7, 63, 64, 65, 257):
         say("emvp m =", m)
         mh = rng.integers(0, P, size=(m, 1292), dtype=np.uint64)
         mh[0, :] = P - 1
@@ -43,6 +43,31 @@ def test_emvp_partials_match_oracle(ctx, golden):
         want = O.emvp_compute_products_batch(mh, m, qs)
         assert np.array_equal(got, want)
     assert EmvpCluster(np.zeros((0, 1292), np.uint64), ctx).compute_products(q_hat) is None
+
+
+def test_emvp_dev_grid_stride_matches_oracle(ctx):
+    """Device-pointer entry at a size whose work units outnumber the grid
+    (every block takes several row tiles x column groups), a ragged last
+    row tile, and the host-buffer entry on the same inputs."""
+    import svs
+    from svs.schemes import EmvpCluster
+    rng = np.random.default_rng(12)
+    m, nq = 5001, 16
+    mh = rng.integers(0, P, size=(m, 1292), dtype=np.uint64)
+    mh[m - 1, :] = P - 1
+    qs = rng.integers(0, P, size=(nq, 1292), dtype=np.uint64)
+    qs[nq - 1] = P - 1
+    cl = EmvpCluster(mh, ctx)
+    want = O.emvp_compute_products_batch(mh, m, qs)
+    say("emvp dev m =", m, "nq =", nq)
+    qd = svs.DeviceArray.from_host(ctx, qs)
+    out = svs.DeviceArray(ctx, (nq, 76 * m), np.uint64)
+    assert cl.compute_products_dev(nq, qd.ptr(), out.ptr())
+    assert np.array_equal(out.to_host(), want)
+    assert np.array_equal(cl.compute_products_batch(qs[:3]), want[:3])
+    qd.close()
+    out.close()
+    cl.close()
 
 
 def test_tiptoe_kat(ctx, golden):
