@@ -286,6 +286,201 @@ def served_mix(index, qs, k, qps, seconds, max_batch, slots, seed=9):
     return out
 
 
+def served_tenants(args):
+    """The served configuration at full scale: `tenants` BN-IVF sub-indices
+    of `tenant_rows` rows each (d = 768, `tenant_nlist` Zipf(1.1)-weighted
+    mixture lists, own key seed per tenant) resident in HBM together, raw
+    f32 queries arriving open-loop (Poisson) from uniformly drawn tenants,
+    nprobe uniform in {8, 16, 32, 64, 128}, half of them verified. Prints
+    one JSON line: per offered rate the achieved q/s, p50/p99/p99.9 latency
+    (scheduled arrival -> result) and mean batch, beside the plain
+    one-query-per-launch rate of the same request mix."""
+    import svs
+    from svs import data
+    t_all = time.perf_counter()
+    ctx = svs.Context(0)
+    scorer = svs.BnTmIvfScorer(ctx)
+    handles, pools = [], []
+    for t in range(args.served_tenants):
+        corpus = data.make_corpus(args.tenant_rows, 768, components=args.tenant_nlist,
+                                  zipf_s=1.1, seed=1000 + t)
+        lists = data.ivf_lists(corpus)
+        cfg = svs.BnTmIvfConfig(index=svs.IvfIndex(corpus.centres, lists), nprobe=8,
+                                key_seed=bytes([t + 1] * 32), upload_seed=100 + t)
+        h, _ = scorer.upload_cluster(cfg, corpus.vectors)
+        handles.append(h)
+        pools.append(data.make_queries(corpus, 1024, seed=2000 + t))
+        del corpus
+        log(f"tenant {t}: {args.tenant_rows} rows in {time.perf_counter() - t_all:.0f} s; "
+            f"resident {ctx.stats()['resident_bytes'] / 1e9:.1f} GB")
+    st = ctx.stats()
+    rng = np.random.default_rng(17)
+    choices = np.array([8, 16, 32, 64, 128])
+
+    def req(i):
+        t = int(rng.integers(0, len(handles)))
+        return dict(tenant=t, query=pools[t][int(rng.integers(0, 1024))],
+                    nprobe=int(rng.choice(choices)), q_seed=rng.integers(0, 256, 32, np.uint8),
+                    verify=bool(rng.integers(0, 2)), k=args.k)
+    # plain: the same kind of requests, one query per launch sequence
+    plain = [req(i) for i in range(args.tenant_plain_queries)]
+    for r in plain[:4]:
+        handles[r["tenant"]].index.score_raw_batch(r["query"][None, :], r["q_seed"][None, :],
+                                                   r["nprobe"], np.array([r["verify"]], np.uint8),
+                                                   args.k, mode=0)
+    t0 = time.perf_counter()
+    for r in plain:
+        handles[r["tenant"]].index.score_raw_batch(r["query"][None, :], r["q_seed"][None, :],
+                                                   r["nprobe"], np.array([r["verify"]], np.uint8),
+                                                   args.k, mode=0)
+    plain_qps = len(plain) / (time.perf_counter() - t0)
+    log(f"plain (one query per launch sequence): {plain_qps:.0f} q/s")
+    # one query alone, per nprobe: the plain sequence and a batch of one
+    alone = {}
+    for npb in choices.tolist():
+        r0 = [req(i) for i in range(24)]
+        row = {}
+        for mode, name in ((0, "plain_ms"), (1, "batch_of_one_ms")):
+            ts = []
+            for r in r0:
+                t1 = time.perf_counter()
+                handles[r["tenant"]].index.score_raw_batch(
+                    r["query"][None, :], r["q_seed"][None, :], npb,
+                    np.array([r["verify"]], np.uint8), args.k, mode=mode)
+                ts.append(time.perf_counter() - t1)
+            row[name] = round(float(np.median(ts[4:])) * 1e3, 3)
+        alone[str(npb)] = row
+    log(f"one query alone by nprobe: {alone}")
+    levels = []
+    for slots in [int(x) for x in args.tenant_slots.split(",") if x]:
+        with svs.Server([h.index for h in handles], max_batch=args.serve_max_batch,
+                        slots=slots) as srv:
+            srv.run_poisson([req(i) for i in range(2048)], qps=2000, seed=3)  # sizes the slots
+            for qps in [float(x) for x in args.tenant_qps.split(",") if x]:
+                n = max(64, int(qps * args.serve_seconds))
+                reqs = [req(i) for i in range(n)]
+                s0 = srv.stats()
+                res, wall = srv.run_poisson(reqs, qps=qps, seed=int(qps))
+                s1 = srv.stats()
+                lat = [r.latency_ms for r in res]
+                nb = s1["batches"] - s0["batches"]
+                lvl = dict(slots=slots, offered_qps=qps, achieved_qps=round(n / wall, 1),
+                           requests=n, p50_ms=round(pct(lat, 50), 3),
+                           p99_ms=round(pct(lat, 99), 3), p999_ms=round(pct(lat, 99.9), 3),
+                           mean_batch=round((s1["batched_queries"] - s0["batched_queries"]) /
+                                            max(nb, 1), 1),
+                           failed=int(sum(1 for r in res if r.status != 0)))
+                log(f"served tenants: {lvl}")
+                levels.append(lvl)
+    rows = args.served_tenants * args.tenant_rows
+    line = dict(metric="BN-IVF served queries/s + p50/p99 ms, tenant sub-indices resident in HBM",
+                unit="queries/s", n_gpus=1,
+                value=max((x["achieved_qps"] for x in levels
+                           if x["p99_ms"] < args.slo_ms and x["failed"] == 0), default=None),
+                higher_is_better=True, dtype="u64 (F_p, p = 2^61-1)",
+                data="synthetic: per tenant a seeded Zipf(1.1) Gaussian mixture, d=768, unit-norm",
+                config=dict(workload=f"served bn-ivf {args.served_tenants} tenants x "
+                                     f"{args.tenant_rows} rows (N={rows}) d=768 nlist="
+                                     f"{args.tenant_nlist} per tenant, raw queries, nprobe "
+                                     "uniform in {8,16,32,64,128}, 50% verified, k="
+                                     f"{args.k}", max_batch=args.serve_max_batch),
+                resident_bytes=st["resident_bytes"], plain_qps=round(plain_qps, 1),
+                alone_by_nprobe=alone,
+                levels=levels, slo_ms=args.slo_ms,
+                setup_s=round(time.perf_counter() - t_all, 1))
+    print(json.dumps(line), flush=True)
+
+
+def flat_plain(args):
+    """The plain GPU drop-in at full size: BN flat over N = `flat_n` rows
+    (d = 768, a seeded 1024-component mixture with equal weights), one query
+    per launch sequence, verification off and on. Reports the server
+    product alone (compute_products: M_enc . v_enc for one v_enc, device
+    pointers, the 8.19 GB M_enc read once per call) against HBM, the whole
+    query (encode, product, [verify], decode, top-k) as q/s and p50/p99, the
+    stage split, and the zero-noise check of the returned scores."""
+    import ctypes as C
+    import svs
+    from svs import _lib as L
+    from svs import data
+    t_all = time.perf_counter()
+    corpus = data.make_corpus(args.flat_n, 768, components=1024, zipf_s=None, seed=42)
+    ctx = svs.Context(0)
+    handle, _ = svs.BnTmScorer(ctx).upload_cluster(
+        svs.BnTmConfig(key_seed=bytes([42] * 32)), corpus.vectors, h_seed=bytes([7] * 32))
+    idx = handle.index
+    m = idx.rows(0)
+    log(f"flat index of {m} rows built in {time.perf_counter() - t_all:.1f} s; "
+        f"resident {ctx.stats()['resident_bytes'] / 1e9:.1f} GB")
+    rng = np.random.default_rng(21)
+    v = svs.DeviceArray.from_host(ctx, rng.integers(0, (1 << 61) - 1, 1024, dtype=np.uint64))
+    r = svs.DeviceArray(ctx, (m,), np.uint64)
+    one = np.zeros(1, np.uint64)
+
+    def products(n):
+        for _ in range(n):
+            L.check(L.lib().svs_compute_products_dev(idx.h, 0, C.c_void_p(v.ptr()),
+                                                     C.c_void_p(r.ptr())))
+        L.check(L.lib().svs_memcpy(ctx.h, one.ctypes.data_as(C.c_void_p), C.c_void_p(r.ptr()), 8))
+    products(3)
+    t0 = time.perf_counter()
+    products(args.flat_product_calls)
+    prod_ms = (time.perf_counter() - t0) / args.flat_product_calls * 1e3
+    prod_bytes = m * 1024 * 8
+    log(f"product alone: {prod_ms:.3f} ms per call, {prod_bytes / prod_ms / 1e6:.0f} GB/s")
+    v.close()
+    r.close()
+    qs = data.make_queries(corpus, args.flat_queries, seed=4242)
+    v_q = svs.quantise(qs)
+    seeds = rng.integers(0, 256, (qs.shape[0], 32), dtype=np.uint8)
+    pp = np.array([0, 1], np.uint32)
+    pl = np.zeros(1, np.uint32)
+    legs = {}
+    for ver in (0, 1):
+        vv = np.array([ver], np.uint8)
+        for q in range(3):
+            idx.score_batch(v_q[q:q + 1], seeds[q:q + 1], pp, pl, vv, args.k, 0)
+        ctx.profile(True, True)
+        lat, res = [], []
+        for q in range(qs.shape[0]):
+            t0 = time.perf_counter()
+            out = idx.score_batch(v_q[q:q + 1], seeds[q:q + 1], pp, pl, vv, args.k, 0)
+            lat.append(time.perf_counter() - t0)
+            res.append(out)
+        prof = ctx.profile(False, False)
+        failed = sum(int(o[3][0] != 0) for o in res)
+        stages = {k: round(prof[k][0] / max(prof[k][1], 1), 4) for k in L.STAGES}
+        legs["verify_on" if ver else "verify_off"] = dict(
+            queries=qs.shape[0], queries_per_s=round(len(lat) / sum(lat), 1),
+            p50_ms=round(pct(lat, 50) * 1e3, 3), p99_ms=round(pct(lat, 99) * 1e3, 3),
+            stage_ms=stages, failed=failed)
+        log(f"flat plain verify={ver}: {legs['verify_on' if ver else 'verify_off']}")
+    # zero noise: every returned score equals fp_to_signed(M_q . v_q)
+    p = (1 << 61) - 1
+    checked = 0
+    for q in range(min(4, qs.shape[0])):
+        ids, sc = res[q][0][0], res[q][1][0]
+        for j in range(int(res[q][2][0])):
+            mq = svs.quantise(corpus.vectors[int(ids[j]):int(ids[j]) + 1])[0]
+            t = sum(a * b for a, b in zip(mq[:768].tolist(), v_q[q][:768].tolist())) % p
+            assert (t if t < p // 2 else t - p) == int(sc[j]), (q, j)
+            checked += 1
+    line = dict(metric="BN flat plain GPU drop-in queries/s + p50/p99 ms, one query per launch",
+                unit="queries/s", n_gpus=1, value=legs["verify_on"]["queries_per_s"],
+                higher_is_better=True, dtype="u64 (F_p, p = 2^61-1)",
+                data="synthetic: seeded 1024-component Gaussian mixture, d=768, unit-norm",
+                config=dict(workload=f"bn flat N={m} d=768 k={args.k}, one query per launch "
+                                     "sequence, verification off and on"),
+                product_alone=dict(ms=round(prod_ms, 4), bytes=prod_bytes,
+                                   gbs=round(prod_bytes / prod_ms / 1e6, 1),
+                                   frac=round(prod_bytes / prod_ms / 1e6 / HBM_PEAK_GBS, 4),
+                                   kernel="k_gemv_chunks"),
+                hbm_ceiling_qps=round(HBM_PEAK_GBS * 1e9 / prod_bytes, 1),
+                legs=legs, zero_noise_scores_checked=checked,
+                setup_s=round(time.perf_counter() - t_all, 1))
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threads):
     """The reference CPU path (BnTmIvfScorer::score, ivf.rs:746-822) as the C
     oracle, one query per thread, on a bounded sample of the workload's
@@ -382,11 +577,32 @@ def main():
                     help="skip the Tiptoe / EMVP measurements")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain-copy calibration dispatch (0 = skip)")
+    ap.add_argument("--served-tenants", type=int, default=0,
+                    help="run only the multi-tenant served configuration with this many "
+                         "tenant sub-indices (0 = the default workload)")
+    ap.add_argument("--tenant-rows", type=int, default=550_000)
+    ap.add_argument("--tenant-nlist", type=int, default=512)
+    ap.add_argument("--tenant-qps", default="1000,2000,4000,8000")
+    ap.add_argument("--tenant-plain-queries", type=int, default=200)
+    ap.add_argument("--flat-n", type=int, default=0,
+                    help="run only the plain flat drop-in configuration over this many rows "
+                         "(0 = the default workload)")
+    ap.add_argument("--flat-queries", type=int, default=1000)
+    ap.add_argument("--flat-product-calls", type=int, default=50)
+    ap.add_argument("--tenant-slots", default="1,2,4",
+                    help="slot counts the multi-tenant served levels are run with")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.served_tenants > 0:
+        served_tenants(args)
+        return
+    if args.flat_n > 0:
+        flat_plain(args)
+        return
 
     import svs
     from svs import data

@@ -1207,7 +1207,9 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
                            d_cand_ptr, (const uint64_t*)d_venc, (const uint64_t*)sl.r.ptr, r_check,
                            (const uint32_t*)d_vseed, v.flags, (uint64_t*)sl.partials.ptr,
                            d_rng_flag);
-        hipLaunchKernelGGL(k_verify_finalize, dim3((ni1 + 255) / 256), dim3(256), 0, st, d_items,
+        hipLaunchKernelGGL(k_verify_finalize,
+                           dim3((uint32_t)std::min<uint64_t>(ni1, (uint64_t)ctx->n_cu * 8)),
+                           dim3(VF_THREADS), 0, st, d_items,
                            v.n_items, d_chunk_ptr, v.flags, (const uint64_t*)sl.partials.ptr,
                            d_item_verdict);
         ctx->launches += 2;
@@ -2025,7 +2027,7 @@ extern "C" int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r,
                        (const uint64_t*)ctx->r.ptr, (const uint64_t*)nullptr,
                        (const uint32_t*)ctx->q_vseed.ptr, (const uint8_t*)ctx->flags.ptr,
                        (uint64_t*)ctx->partials.ptr, d_flag);
-    hipLaunchKernelGGL(k_verify_finalize, dim3(1), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(k_verify_finalize, dim3(1), dim3(VF_THREADS), 0, ctx->stream,
                        (const Item*)ctx->items.ptr, 1u, pre, (const uint8_t*)ctx->flags.ptr,
                        (const uint64_t*)ctx->partials.ptr, d_verdict);
     ctx->launches += 2;

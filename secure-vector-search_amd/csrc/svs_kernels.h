@@ -181,25 +181,48 @@ k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ item
     }
 }
 
-// Per item: sum chunk partials, verdict = first trial whose sums differ.
-__global__ void k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
-                                  const uint32_t* __restrict__ chunk_ptr,
-                                  const uint8_t* __restrict__ item_verify,
-                                  const uint64_t* __restrict__ partials,
-                                  int32_t* __restrict__ item_verdict) {
-    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= n_items) return;
-    if (!item_verify[it] || items[it].m == 0) {
-        item_verdict[it] = -1;
-        return;
+// Per item, one block: the threads stride over the item's chunk partials,
+// then a wave and a block reduction mod p; verdict = first trial whose sums
+// differ. A flat list of 1M rows has 62 500 chunks, so the sum is spread
+// over the block rather than walked by one thread.
+constexpr int VF_THREADS = 256;
+__global__ void __launch_bounds__(VF_THREADS)
+k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
+                  const uint32_t* __restrict__ chunk_ptr,
+                  const uint8_t* __restrict__ item_verify,
+                  const uint64_t* __restrict__ partials,
+                  int32_t* __restrict__ item_verdict) {
+    __shared__ uint64_t red[VF_THREADS / WAVE][2 * LAMBDA];
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        if (!item_verify[it] || items[it].m == 0) {  // uniform over the block
+            if (threadIdx.x == 0) item_verdict[it] = -1;
+            continue;
+        }
+        uint64_t s[2 * LAMBDA] = {0, 0, 0, 0, 0, 0};
+        const uint32_t c1 = chunk_ptr[it + 1];
+        for (uint32_t c = chunk_ptr[it] + threadIdx.x; c < c1; c += blockDim.x)
+#pragma unroll
+            for (int k = 0; k < 2 * LAMBDA; ++k)
+                s[k] = add_mod(s[k], partials[(size_t)c * 2 * LAMBDA + k]);
+#pragma unroll
+        for (int k = 0; k < 2 * LAMBDA; ++k) s[k] = wave_sum_mod(s[k]);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 2 * LAMBDA; ++k) red[wave][k] = s[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t[2 * LAMBDA] = {0, 0, 0, 0, 0, 0};
+            for (uint32_t w = 0; w < VF_THREADS / WAVE; ++w)
+#pragma unroll
+                for (int k = 0; k < 2 * LAMBDA; ++k) t[k] = add_mod(t[k], red[w][k]);
+            int32_t verdict = -1;
+            for (int k = LAMBDA - 1; k >= 0; --k)
+                if (t[k] != t[LAMBDA + k]) verdict = k;
+            item_verdict[it] = verdict;
+        }
+        __syncthreads();
     }
-    uint64_t s[2 * LAMBDA] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t c = chunk_ptr[it]; c < chunk_ptr[it + 1]; ++c)
-        for (int k = 0; k < 2 * LAMBDA; ++k) s[k] = add_mod(s[k], partials[(size_t)c * 2 * LAMBDA + k]);
-    int32_t verdict = -1;
-    for (int t = LAMBDA - 1; t >= 0; --t)
-        if (s[t] != s[LAMBDA + t]) verdict = t;
-    item_verdict[it] = verdict;
 }
 
 // ================================================================== decode
