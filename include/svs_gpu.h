@@ -195,6 +195,17 @@ int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint64_t* g_host
 int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint64_t* v_enc_host,
                const uint8_t rng_seed[32], uint32_t trials, int32_t* failed_trial);
 
+/* Client query encoding on the device (encode_query, crypto.rs:508-553):
+ * for each query, G = 128 sample_fp draws and T = sample_sparse_pm1 from
+ * ChaCha20Rng(q_seed), v_enc = v_q + L G + T. v_q: n_queries x n canonical
+ * (quantise_vector's output, zero-padded); q_seeds: n_queries x 32 bytes.
+ * Outputs v_enc (n_queries x n), g (n_queries x n1) and the dense T
+ * (n_queries x n, entries -1 / 0 / +1). The batch pipeline runs the same
+ * kernels; this entry exposes them. */
+int svs_encode_query(svs_index* idx, uint32_t n_queries, const uint64_t* v_q_host,
+                     const uint8_t* q_seeds_host, uint64_t* v_enc_host, uint64_t* g_host,
+                     int8_t* t_host);
+
 /* Full client decode (decode_scores, crypto.rs:901-940): Mv from r, the
  * per-query mask (g, sparse T) and v_enc. */
 int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint64_t* v_enc_host,

@@ -1149,7 +1149,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
     SVS_TRY(prof_mark(sl, 1, true));
     SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc1));
     if (v.cap_chunks && kernel == KERNEL_VALU) {
-        hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, v.cap_chunks)), dim3(ROW_THREADS), 0,
+        hipLaunchKernelGGL((k_gemv_chunks<N, 0>), dim3(row_grid(ctx, v.cap_chunks)), dim3(ROW_THREADS), 0,
                            st, d_lists, d_items, v.n_items, d_chunk_ptr, d_cand_ptr,
                            (const uint64_t*)d_venc, (uint64_t*)sl.r.ptr);
         ctx->launches++;
@@ -1230,6 +1230,17 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         const bool tiles = kernel == KERNEL_MFMA && v.cap_chunks;
         const uint32_t tblocks = tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ);
         const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
+        if (!tiles && v.cap_chunks) {
+            // one query per list: the dense terms stream the row-major
+            // [AL | H] like the server product streams M_enc
+            SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
+            hipLaunchKernelGGL((k_gemv_chunks<2 * N1, 1>), dim3(row_grid(ctx, v.cap_chunks)),
+                               dim3(ROW_THREADS), 0, st, d_lists, d_items, v.n_items,
+                               d_chunk_ptr, d_cand_ptr, (const uint64_t*)d_gl,
+                               (uint64_t*)sl.dense.ptr);
+            ctx->launches++;
+            dense = (const uint64_t*)sl.dense.ptr;
+        }
         if (tiles) {
             SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
             uint64_t* qc_gl = (uint64_t*)sl.q_corr.ptr + nq;
@@ -1252,9 +1263,10 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
                                 (int64_t*)sl.scores.ptr);
             ctx->launches += 2;
         } else {
-            const uint32_t blocks = (uint32_t)std::min<uint64_t>((v.cap_cands + 255) / 256,
-                                                                 (uint64_t)ctx->n_cu * 32);
-            hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(256), 0, st, d_lists, d_items,
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>(
+                (v.cap_cands + DC_THREADS - 1) / DC_THREADS, (uint64_t)ctx->n_cu * 32);
+            hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(DC_THREADS), 0, st, d_lists,
+                               d_items,
                                v.n_items, d_cand_ptr, (const uint64_t*)sl.r.ptr,
                                (const uint64_t*)d_gl, (const uint64_t*)d_venc,
                                (const uint16_t*)d_tlist, (const uint32_t*)d_tcnt, dense,
@@ -1941,7 +1953,7 @@ extern "C" int svs_compute_products_dev(svs_index* idx, uint32_t list, const uin
         ctx->cached_plan_chunks = pl.n_chunks;
     }
     const uint32_t* pre = (const uint32_t*)ctx->prefixes1.ptr;
-    hipLaunchKernelGGL(k_gemv_chunks, dim3(row_grid(ctx, ctx->cached_plan_chunks)),
+    hipLaunchKernelGGL((k_gemv_chunks<N, 0>), dim3(row_grid(ctx, ctx->cached_plan_chunks)),
                        dim3(ROW_THREADS), 0, ctx->stream, (const ListDev*)idx->lists_dev,
                        (const Item*)ctx->items1.ptr, 1u, pre, pre + 2, v_enc_dev, r_dev);
     ctx->launches++;
@@ -2040,6 +2052,52 @@ extern "C" int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r,
     return SVS_OK;
 }
 
+extern "C" int svs_encode_query(svs_index* idx, uint32_t n_queries, const uint64_t* v_q,
+                                const uint8_t* q_seeds, uint64_t* v_enc, uint64_t* g, int8_t* t) {
+    if (!idx || (n_queries && (!v_q || !q_seeds || !v_enc || !g || !t)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (n_queries == 0) return SVS_OK;
+    SVS_TRY(check_canonical(v_q, (size_t)n_queries * N, "v_q"));
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nq = n_queries;
+    SVS_TRY(ctx->q_seeds.ensure(ctx, 32 * nq));
+    SVS_TRY(ctx->q_vq.ensure(ctx, sizeof(uint64_t) * N * nq));
+    SVS_TRY(ctx->q_gl.ensure(ctx, sizeof(uint64_t) * 2 * N1 * nq));
+    SVS_TRY(ctx->q_tdense.ensure(ctx, (size_t)N * nq));
+    SVS_TRY(ctx->q_tlist.ensure(ctx, sizeof(uint16_t) * N * nq));
+    SVS_TRY(ctx->q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
+    SVS_TRY(ctx->q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
+    SVS_TRY(ctx->q_venc.ensure(ctx, sizeof(uint64_t) * N * nq));
+    SVS_TRY(ctx->tmp_c.ensure(ctx, 64));
+    hipStream_t st = ctx->stream;
+    uint32_t* d_flag = (uint32_t*)ctx->tmp_c.ptr;
+    HIP_TRY(hipMemcpyAsync(ctx->q_seeds.ptr, q_seeds, 32 * nq, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->q_vq.ptr, v_q, sizeof(uint64_t) * N * nq, hipMemcpyHostToDevice,
+                           st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d_flag, 1u, 0u);
+    hipLaunchKernelGGL(k_encode_keystream, dim3(n_queries), dim3(256), 0, st,
+                       (const uint8_t*)ctx->q_seeds.ptr, (uint64_t*)ctx->q_gl.ptr,
+                       (int8_t*)ctx->q_tdense.ptr, (uint16_t*)ctx->q_tlist.ptr,
+                       (uint32_t*)ctx->q_tcnt.ptr, (uint32_t*)ctx->q_vseed.ptr, d_flag);
+    hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, n_queries), dim3(256), 0, st,
+                       (const uint64_t*)idx->lt_dev, (const uint64_t*)ctx->q_gl.ptr,
+                       (const int8_t*)ctx->q_tdense.ptr, (const uint64_t*)ctx->q_vq.ptr,
+                       (uint64_t*)ctx->q_venc.ptr);
+    ctx->launches += 3;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(v_enc, ctx->q_venc.ptr, sizeof(uint64_t) * N * nq,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpy2DAsync(g, sizeof(uint64_t) * N1, ctx->q_gl.ptr, sizeof(uint64_t) * 2 * N1,
+                             sizeof(uint64_t) * N1, nq, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(t, ctx->q_tdense.ptr, (size_t)N * nq, hipMemcpyDeviceToHost, st));
+    uint32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return check_rng_flag(flag);
+}
+
 extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, const uint64_t* v_enc,
                           const uint64_t* g, const uint32_t* t_idx, const int8_t* t_sign,
                           uint32_t t_nnz, uint64_t* mv) {
@@ -2080,7 +2138,8 @@ extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, cons
     hipLaunchKernelGGL(k_lt, dim3(1), dim3(N1 * LT_PARTS), 0, ctx->stream, (const uint64_t*)idx->l_dev,
                        (const uint64_t*)ctx->q_venc.ptr, (uint64_t*)ctx->q_gl.ptr);
     const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
-    hipLaunchKernelGGL(k_decode_cands, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_decode_cands, dim3((m + DC_THREADS - 1) / DC_THREADS), dim3(DC_THREADS), 0,
+                       ctx->stream,
                        (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr, 1u, pre + 2,
                        (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_gl.ptr,
                        (const uint64_t*)ctx->q_venc.ptr, (const uint16_t*)ctx->q_tlist.ptr,

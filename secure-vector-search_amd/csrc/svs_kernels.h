@@ -41,15 +41,20 @@ __device__ __forceinline__ int64_t fp_to_signed(uint64_t s) {
 }
 
 // ================================================================ products
-// Server product over item chunks: r[cand_ptr[it] + row] = M_enc[row] . v_enc[q].
-// One wave per chunk; lane l owns columns 128 j + 2 l + {0,1}, j = 0..7, so a
-// row is streamed as eight fully coalesced 1 KiB loads. v_enc lives in
-// registers for the chunk. Each lane accumulates 16 products (< 2^126) in a
-// lazy 128-bit accumulator, reduces once, and the wave sums mod p.
+// Row streaming over item chunks: out[cand_ptr[it] + row] = A[row] . x[q]
+// with A = M_enc (KE = 1024, the server product) or the row-major [AL | H]
+// (KE = 256, the dense decode terms AL g + H lt against x = [g ; lt]).
+// One wave per chunk; lane l owns columns 128 j + 2 l + {0,1}, j < KE/128,
+// so a row is streamed as KE/128 fully coalesced 1 KiB loads. x lives in
+// registers for the chunk. Each lane accumulates at most 16 products
+// (< 2^126) in a lazy 128-bit accumulator, reduces once, and the wave sums
+// mod p.
+template <int KE, int SRC>
 __global__ void __launch_bounds__(ROW_THREADS)
 k_gemv_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ items, uint32_t n_items,
               const uint32_t* __restrict__ chunk_ptr, const uint32_t* __restrict__ cand_ptr,
-              const uint64_t* __restrict__ venc, uint64_t* __restrict__ r_out) {
+              const uint64_t* __restrict__ xvec, uint64_t* __restrict__ r_out) {
+    constexpr int J = KE / (2 * WAVE);
     const uint32_t lane = threadIdx.x % WAVE;
     const uint32_t waves_per_block = blockDim.x / WAVE;
     const uint32_t n_chunks = chunk_ptr[n_items];
@@ -60,18 +65,19 @@ k_gemv_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ items,
         const ListDev L = lists[item.list];
         const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
         const uint32_t rows = min((uint32_t)CHUNK_ROWS, item.m - row0);
-        const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(venc + (size_t)item.q * N);
-        ulonglong2 v[8];
+        const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(xvec + (size_t)item.q * KE);
+        ulonglong2 v[J];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = vq[j * WAVE + lane];
+        for (int j = 0; j < J; ++j) v[j] = vq[j * WAVE + lane];
         uint64_t* r = r_out + cand_ptr[it] + row0;
+        const uint64_t* base = SRC == 0 ? L.menc : L.alh;
         for (uint32_t i = 0; i < rows; ++i) {
             const ulonglong2* row =
-                reinterpret_cast<const ulonglong2*>(L.menc + (size_t)(row0 + i) * N);
+                reinterpret_cast<const ulonglong2*>(base + (size_t)(row0 + i) * KE);
             Acc128 acc;
             acc.zero();
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < J; ++j) {
                 ulonglong2 a = row[j * WAVE + lane];
                 acc.mac(a.x, v[j].x);
                 acc.mac(a.y, v[j].y);
@@ -230,17 +236,33 @@ k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
 //   Mv = r - (AL g + M T) - (H lt + S v_enc),  lt = L^T v_enc.
 // Column-major client state keeps every access coalesced across a wave:
 // AL^T / H^T rows, the gathered columns M_plain^T[t_e], and S sliced-ELL.
-// Writes the signed score (fp_to_signed, crypto.rs:128-135).
-__global__ void __launch_bounds__(256)
+// S v_enc gathers 128 random entries of the query's v_enc per row, so the
+// block keeps the v_enc of its first candidate's query in LDS (a gather
+// from LDS instead of up to 64 cache lines per load); candidates of another
+// query in the same block read theirs from memory. Writes the signed score
+// (fp_to_signed, crypto.rs:128-135).
+constexpr int DC_THREADS = 256;
+__global__ void __launch_bounds__(DC_THREADS)
 k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items,
                uint32_t n_items, const uint32_t* __restrict__ cand_ptr,
                const uint64_t* __restrict__ r, const uint64_t* __restrict__ gl /* q x 256 */,
                const uint64_t* __restrict__ venc, const uint16_t* __restrict__ t_list,
                const uint32_t* __restrict__ t_cnt, const uint64_t* __restrict__ dense_in,
                int64_t* __restrict__ scores, uint64_t* __restrict__ mv_out) {
+    __shared__ uint64_t v_lds[N];
     const uint32_t total = cand_ptr[n_items];
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += gridDim.x * blockDim.x) {
+    uint32_t q_lds = 0xFFFFFFFFu;  // block-uniform: the query held in v_lds
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t q0 = items[find_seg(cand_ptr, n_items, base)].q;
+        if (q0 != q_lds) {
+            __syncthreads();  // earlier readers of v_lds are done
+            for (uint32_t k = threadIdx.x; k < (uint32_t)N; k += blockDim.x)
+                v_lds[k] = venc[(size_t)q0 * N + k];
+            __syncthreads();
+            q_lds = q0;
+        }
+        const uint32_t g = base + threadIdx.x;
+        if (g >= total) continue;
         const uint32_t it = find_seg(cand_ptr, n_items, g);
         const Item item = items[it];
         const ListDev L = lists[item.list];
@@ -263,32 +285,58 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
                 }
             }
         }
-        // M T: gathered columns of M_plain (sparse +-1 mask T of the query).
+        // M T: gathered columns of M_plain (sparse +-1 mask T of the query),
+        // eight columns in flight.
         uint64_t mt = 0;
         const uint16_t* tq = t_list + (size_t)item.q * N;
         const uint32_t tn = t_cnt[item.q];
-        for (uint32_t e = 0; e < tn; ++e) {
-            const uint16_t te = tq[e];
-            const uint32_t col = te & 0x3FF;
-            uint64_t x;
-            if (L.mp_t32) {
-                const int32_t s = L.mp_t32[(size_t)col * m + i];
-                x = s >= 0 ? (uint64_t)s : P - (uint64_t)(-(int64_t)s);
-            } else {
-                x = L.mp_t64[(size_t)col * m + i];
+#pragma unroll 1
+        for (uint32_t e0 = 0; e0 < tn; e0 += 8) {
+            uint64_t x[8];
+            uint32_t neg = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                x[j] = 0;
+                if (e0 + j < tn) {
+                    const uint16_t te = tq[e0 + j];
+                    const uint32_t c = te & 0x3FF;
+                    if (L.mp_t32) {
+                        const int32_t v = L.mp_t32[(size_t)c * m + i];
+                        x[j] = v >= 0 ? (uint64_t)v : P - (uint64_t)(-(int64_t)v);
+                    } else {
+                        x[j] = L.mp_t64[(size_t)c * m + i];
+                    }
+                    neg |= (uint32_t)(te >> 15) << j;
+                }
             }
-            mt = (te & 0x8000) ? sub_mod(mt, x) : add_mod(mt, x);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) mt = ((neg >> j) & 1) ? sub_mod(mt, x[j]) : add_mod(mt, x[j]);
         }
-        // S v_enc from the row's sliced-ELL entries.
+        // S v_enc from the row's sliced-ELL entries (a row's entries end at
+        // its first padding entry), eight entries in flight.
+        const bool own = item.q == q_lds;
         const uint64_t* vq = venc + (size_t)item.q * N;
         const uint32_t slice = i / SLICE, lane = i % SLICE;
         const uint32_t s0 = L.s_slice[slice], s1 = L.s_slice[slice + 1];
         uint64_t sv = 0;
-        for (uint32_t e = s0 + lane; e < s1; e += SLICE) {
-            const uint16_t se = L.s_ell[e];
-            if (se == S_PAD) break;
-            const uint64_t x = vq[se & 0x3FF];
-            sv = (se & 0x8000) ? sub_mod(sv, x) : add_mod(sv, x);
+        bool done = false;
+#pragma unroll 1
+        for (uint32_t e0 = s0 + lane; e0 < s1 && !done; e0 += 8 * SLICE) {
+            uint16_t se[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t e = e0 + j * SLICE;
+                se[j] = e < s1 ? L.s_ell[e] : S_PAD;
+            }
+            uint64_t x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                x[j] = se[j] == S_PAD ? 0 : (own ? v_lds[se[j] & 0x3FF] : vq[se[j] & 0x3FF]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                done |= se[j] == S_PAD;
+                if (!done) sv = (se[j] & 0x8000) ? sub_mod(sv, x[j]) : add_mod(sv, x[j]);
+            }
         }
         const uint64_t corr = add_mod(add_mod(dense, mt), sv);
         const uint64_t mv = sub_mod(r[g], corr);
@@ -319,7 +367,7 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
        const int64_t* __restrict__ scores, const int32_t* __restrict__ q_status, uint32_t k,
        uint32_t* __restrict__ out_ids, int64_t* __restrict__ out_scores,
        uint32_t* __restrict__ out_count) {
-    __shared__ Cand lst[THREADS][KT];
+    __shared__ Cand lst[KT][THREADS];  // [position][thread]: lanes hit distinct banks
     __shared__ int cnt[THREADS];
     __shared__ int head[THREADS];
     __shared__ int red_idx[THREADS];
@@ -334,13 +382,13 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
     int n = 0;
     for (uint32_t g = c0 + tid; g < c1; g += THREADS) {
         Cand c{scores[g], g - c0, 0};
-        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
+        if (n == (int)k && !cand_better(c, lst[n - 1][tid])) continue;
         int pos = n < (int)k ? n : (int)k - 1;
-        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
-            lst[tid][pos] = lst[tid][pos - 1];
+        while (pos > 0 && cand_better(c, lst[pos - 1][tid])) {
+            lst[pos][tid] = lst[pos - 1][tid];
             --pos;
         }
-        lst[tid][pos] = c;
+        lst[pos][tid] = c;
         if (n < (int)k) ++n;
     }
     cnt[tid] = n;
@@ -354,14 +402,14 @@ k_topk(const Item* __restrict__ items, const uint32_t* __restrict__ item_ptr,
         for (int s = THREADS / 2; s > 0; s >>= 1) {
             if (tid < s) {
                 int a = red_idx[tid], b = red_idx[tid + s];
-                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                if (a < 0 || (b >= 0 && cand_better(lst[head[b]][b], lst[head[a]][a])))
                     red_idx[tid] = b;
             }
             __syncthreads();
         }
         if (tid == 0) {
             const int w = red_idx[0];
-            const Cand c = lst[w][head[w]];
+            const Cand c = lst[head[w]][w];
             head[w]++;
             // map candidate position back to (item, row) -> global id
             const uint32_t g = c0 + c.pos;
@@ -388,7 +436,7 @@ __global__ void __launch_bounds__(THREADS)
 k_topk_seg(const uint32_t* __restrict__ seg_ptr, const uint64_t* __restrict__ mv,
            const uint32_t* __restrict__ ids, uint32_t k, uint32_t* __restrict__ out_ids,
            int64_t* __restrict__ out_scores, uint32_t* __restrict__ out_count) {
-    __shared__ Cand lst[THREADS][KT];
+    __shared__ Cand lst[KT][THREADS];  // [position][thread]: lanes hit distinct banks
     __shared__ int cnt[THREADS];
     __shared__ int head[THREADS];
     __shared__ int red_idx[THREADS];
@@ -398,13 +446,13 @@ k_topk_seg(const uint32_t* __restrict__ seg_ptr, const uint64_t* __restrict__ mv
     int n = 0;
     for (uint32_t g = c0 + tid; g < c1; g += THREADS) {
         Cand c{fp_to_signed(mv[g]), g - c0, 0};
-        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
+        if (n == (int)k && !cand_better(c, lst[n - 1][tid])) continue;
         int pos = n < (int)k ? n : (int)k - 1;
-        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
-            lst[tid][pos] = lst[tid][pos - 1];
+        while (pos > 0 && cand_better(c, lst[pos - 1][tid])) {
+            lst[pos][tid] = lst[pos - 1][tid];
             --pos;
         }
-        lst[tid][pos] = c;
+        lst[pos][tid] = c;
         if (n < (int)k) ++n;
     }
     cnt[tid] = n;
@@ -417,14 +465,14 @@ k_topk_seg(const uint32_t* __restrict__ seg_ptr, const uint64_t* __restrict__ mv
         for (int s = THREADS / 2; s > 0; s >>= 1) {
             if (tid < s) {
                 int a = red_idx[tid], b = red_idx[tid + s];
-                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                if (a < 0 || (b >= 0 && cand_better(lst[head[b]][b], lst[head[a]][a])))
                     red_idx[tid] = b;
             }
             __syncthreads();
         }
         if (tid == 0) {
             const int w = red_idx[0];
-            const Cand c = lst[w][head[w]];
+            const Cand c = lst[head[w]][w];
             head[w]++;
             out_ids[(size_t)q * k + outi] = ids[c0 + c.pos];
             out_scores[(size_t)q * k + outi] = c.score;
@@ -502,7 +550,7 @@ k_topk_final(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
              const int32_t* __restrict__ q_status, uint32_t k, uint32_t* __restrict__ out_ids,
              int64_t* __restrict__ out_scores, uint32_t* __restrict__ out_count) {
     constexpr int KT = TOPK_LANE;
-    __shared__ Cand lst[THREADS][KT];
+    __shared__ Cand lst[KT][THREADS];  // [position][thread]: lanes hit distinct banks
     __shared__ int cnt[THREADS];
     __shared__ int head[THREADS];
     __shared__ int red_idx[THREADS];
@@ -515,18 +563,32 @@ k_topk_final(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
     const uint32_t i0 = item_ptr[q], i1 = item_ptr[q + 1];
     const uint32_t c0 = cand_ptr[i0];
     const uint32_t e0 = qchunk_ptr[q] * k, e1 = qchunk_ptr[q + 1] * k;
+    // A query with many chunks (a flat list of 1M rows has 977) gives each
+    // thread dozens of entries: they are loaded eight at a time before the
+    // insertions, so the loads overlap.
     int n = 0;
-    for (uint32_t e = e0 + tid; e < e1; e += THREADS) {
-        const Cand c = part[e];
-        if (c.pos == 0xFFFFFFFFu) continue;
-        if (n == (int)k && !cand_better(c, lst[tid][n - 1])) continue;
-        int pos = n < (int)k ? n : (int)k - 1;
-        while (pos > 0 && cand_better(c, lst[tid][pos - 1])) {
-            lst[tid][pos] = lst[tid][pos - 1];
-            --pos;
+    constexpr int LB = 8;
+#pragma unroll 1
+    for (uint32_t eb = e0 + tid; eb < e1; eb += LB * THREADS) {
+        Cand cs[LB];
+#pragma unroll
+        for (int j = 0; j < LB; ++j) {
+            const uint32_t e = eb + j * THREADS;
+            cs[j] = e < e1 ? part[e] : Cand{0, 0xFFFFFFFFu, 0};
         }
-        lst[tid][pos] = c;
-        if (n < (int)k) ++n;
+#pragma unroll
+        for (int j = 0; j < LB; ++j) {
+            const Cand c = cs[j];
+            if (c.pos == 0xFFFFFFFFu) continue;
+            if (n == (int)k && !cand_better(c, lst[n - 1][tid])) continue;
+            int pos = n < (int)k ? n : (int)k - 1;
+            while (pos > 0 && cand_better(c, lst[pos - 1][tid])) {
+                lst[pos][tid] = lst[pos - 1][tid];
+                --pos;
+            }
+            lst[pos][tid] = c;
+            if (n < (int)k) ++n;
+        }
     }
     cnt[tid] = n;
     head[tid] = 0;
@@ -538,14 +600,14 @@ k_topk_final(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
         for (int s = THREADS / 2; s > 0; s >>= 1) {
             if (tid < s) {
                 int a = red_idx[tid], b = red_idx[tid + s];
-                if (a < 0 || (b >= 0 && cand_better(lst[b][head[b]], lst[a][head[a]])))
+                if (a < 0 || (b >= 0 && cand_better(lst[head[b]][b], lst[head[a]][a])))
                     red_idx[tid] = b;
             }
             __syncthreads();
         }
         if (tid == 0) {
             const int w = red_idx[0];
-            const Cand c = lst[w][head[w]];
+            const Cand c = lst[head[w]][w];
             head[w]++;
             const uint32_t g = c0 + c.pos;
             uint32_t lo = i0, hi = i1;
@@ -698,13 +760,24 @@ k_topk_large(const Item* __restrict__ items, const uint32_t* __restrict__ item_p
 // enough for G plus a T that selects every entry) is generated in parallel
 // into LDS. G's draws sit at words 2k, 2k+1 unless a draw is rejected
 // (probability 2^-61 each; then one thread walks G sequentially). T's walk
-// (each visited word advances 1, or 2 when it is below the threshold) runs
-// on wave 0 over 64-word windows: a ballot gives the window's "below" mask
-// and the scalar walk records which words it visits; all threads then place
-// entry j = (visited words before) at once.
+// visits a word and, when that word is below the threshold, skips the next
+// one (its sign). Whether word p is visited is a two-state automaton
+// (visited / skipped) run left to right, and its per-word transitions
+// compose associatively: each thread composes the transitions of its
+// ENC_PER words, an exclusive block scan of the compositions gives every
+// thread the state at its first word, and a block scan of (visited,
+// selected) counts gives each visited word its entry index j and each
+// selected one its place in the sparse list.
 constexpr int ENC_BLOCKS = 160;
 constexpr int ENC_WORDS = ENC_BLOCKS * 16;  // 2560 >= 2 * 128 + 2 * 1024 + 2
-constexpr int ENC_WIN = ENC_WORDS / 64;
+constexpr int ENC_PER = (ENC_WORDS + 255) / 256;
+
+// Transitions of the T walk's automaton as 2 bits: bit x = next state from
+// state x (0 = visited, 1 = skipped). outer o inner: inner first.
+__device__ __forceinline__ uint32_t walk_comp(uint32_t outer, uint32_t inner) {
+    return ((outer >> (inner & 1)) & 1) | (((outer >> ((inner >> 1) & 1)) & 1) << 1);
+}
+constexpr uint32_t WALK_ID = 2u;  // identity: 0 -> 0, 1 -> 1
 
 __global__ void __launch_bounds__(256)
 k_encode_keystream(const uint8_t* __restrict__ q_seeds, uint64_t* __restrict__ g_out /* q x 256 (first 128) */,
@@ -712,17 +785,15 @@ k_encode_keystream(const uint8_t* __restrict__ q_seeds, uint64_t* __restrict__ g
                    uint16_t* __restrict__ t_list /* q x 1024 */, uint32_t* __restrict__ t_cnt,
                    uint32_t* __restrict__ vseed_words, uint32_t* __restrict__ rng_flag) {
     __shared__ uint32_t ks[ENC_WORDS];
-    __shared__ uint64_t vis[ENC_WIN], selv[ENC_WIN];
-    __shared__ uint32_t vis_base[ENC_WIN], sel_base[ENC_WIN];
     __shared__ uint32_t key[8];
-    __shared__ uint32_t sh_tpos, sh_wend, sh_nsel;
+    __shared__ uint32_t sc_f[4], sc_c[4];
+    __shared__ uint32_t sh_tpos, sh_nsel;
     const uint32_t q = blockIdx.x;
     const int tid = threadIdx.x, lane = tid % WAVE;
     const uint8_t* seed = q_seeds + (size_t)q * 32;
     if (tid < 8)
         key[tid] = (uint32_t)seed[4 * tid] | ((uint32_t)seed[4 * tid + 1] << 8) |
                    ((uint32_t)seed[4 * tid + 2] << 16) | ((uint32_t)seed[4 * tid + 3] << 24);
-    if (tid < ENC_WIN) vis[tid] = selv[tid] = 0;
     __syncthreads();
     if (tid < ENC_BLOCKS) {
         uint32_t k[8], out[16];
@@ -771,59 +842,106 @@ k_encode_keystream(const uint8_t* __restrict__ q_seeds, uint64_t* __restrict__ g
         sh_tpos = pos;
     }
     __syncthreads();
-    // T walk on wave 0
-    if (tid < WAVE) {
-        const uint32_t tpos = sh_tpos;
-        uint32_t w = tpos / 64, x = tpos % 64, j = 0, nsel = 0;
-        bool overflow = false;
-        while (j < (uint32_t)N) {
-            if (w >= (uint32_t)ENC_WIN) {
-                overflow = true;
-                break;
-            }
-            const uint64_t below = __ballot(ks[64 * w + lane] < SPARSE_THRESHOLD);
-            uint64_t v = 0;
-            const uint32_t jb = j;
-            while (x < 64 && j < (uint32_t)N) {
-                v |= 1ull << x;
-                ++j;
-                x += 1 + (uint32_t)((below >> x) & 1);
-            }
-            // a selected word needs its sign word inside the buffer
-            if (64 * w + x > (uint32_t)ENC_WORDS) overflow = true;
-            if (lane == 0) {
-                vis[w] = v;
-                selv[w] = v & below;
-                vis_base[w] = jb;
-                sel_base[w] = nsel;
-            }
-            nsel += (uint32_t)__popcll(v & below);
-            x -= 64;
-            ++w;
-        }
-        if (lane == 0) {
-            sh_wend = w;
-            sh_nsel = nsel;
-            if (overflow) atomicOr(rng_flag, 2u);
-        }
+    // T: the walk from word tpos. Word p's transition: from "visited" to
+    // "skipped" when p is below the threshold (p >= tpos), else to
+    // "visited"; from "skipped" always to "visited". Before tpos every
+    // transition leads to "visited", so the walk starts visited at tpos.
+    const uint32_t tpos = sh_tpos;
+    const uint32_t p0 = (uint32_t)tid * ENC_PER, wv = tid / WAVE;
+    auto below = [&](uint32_t p) -> bool {
+        return p >= tpos && p < (uint32_t)ENC_WORDS && ks[p] < SPARSE_THRESHOLD;
+    };
+    uint32_t F = WALK_ID;
+#pragma unroll
+    for (int a = 0; a < ENC_PER; ++a) F = walk_comp(below(p0 + a) ? 1u : 0u, F);
+    uint32_t I = F;  // inclusive scan of the compositions within the wave
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t o = __shfl_up(I, off, WAVE);
+        if (lane >= off) I = walk_comp(I, o);
     }
+    uint32_t X = __shfl_up(I, 1, WAVE);
+    if (lane == 0) X = WALK_ID;
+    if (lane == WAVE - 1) sc_f[wv] = I;
     __syncthreads();
-    // place T entries
-    const uint32_t w0 = sh_tpos / 64, w1 = sh_wend;
-    for (uint32_t p = 64 * w0 + tid; p < 64 * w1; p += blockDim.x) {
-        const uint32_t w = p / 64, b = p % 64;
-        const uint64_t below_mask = (1ull << b) - 1;
-        if (!((vis[w] >> b) & 1)) continue;
-        const uint32_t j = vis_base[w] + (uint32_t)__popcll(vis[w] & below_mask);
+    uint32_t E = WALK_ID;
+    for (uint32_t w = 0; w < wv; ++w) E = walk_comp(sc_f[w], E);
+    E = walk_comp(X, E);
+    const uint32_t st0 = E & 1;  // the state at word p0 (from any state at word 0)
+    // visited and selected words of this thread
+    uint32_t st = st0, nv = 0, ns = 0;
+#pragma unroll
+    for (int a = 0; a < ENC_PER; ++a) {
+        const uint32_t p = p0 + a;
+        const bool vis = st == 0 && p >= tpos && p < (uint32_t)ENC_WORDS;
+        const bool sel = vis && below(p);
+        nv += vis;
+        ns += sel;
+        st = sel ? 1u : 0u;
+    }
+    // exclusive scan of the packed counts (each below 2^16)
+    const uint32_t c = nv | (ns << 16);
+    uint32_t ci = c;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t o = __shfl_up(ci, off, WAVE);
+        if (lane >= off) ci += o;
+    }
+    if (lane == WAVE - 1) sc_c[wv] = ci;
+    if (tid == 0) sh_nsel = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t cb = ci - c, tot = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+        if (w < wv) cb += sc_c[w];
+        tot += sc_c[w];
+    }
+    // place the entries j < n of this thread's visited words
+    uint32_t j = cb & 0xFFFFu, e = cb >> 16;
+    bool overflow = false;
+    st = st0;
+#pragma unroll
+    for (int a = 0; a < ENC_PER; ++a) {
+        const uint32_t p = p0 + a;
+        const bool vis = st == 0 && p >= tpos && p < (uint32_t)ENC_WORDS;
+        const bool sel = vis && below(p);
+        st = sel ? 1u : 0u;
+        if (!vis || j >= (uint32_t)N) continue;
         int8_t v = 0;
-        if ((selv[w] >> b) & 1) {
+        if (sel) {
             v = p + 1 < (uint32_t)ENC_WORDS && (ks[p + 1] & 1u) == 0 ? -1 : 1;
-            const uint32_t e = sel_base[w] + (uint32_t)__popcll(selv[w] & below_mask);
             t_list[(size_t)q * N + e] = (uint16_t)(j | (v < 0 ? 0x8000 : 0));
+            ++e;
+            // a selected word needs its sign word inside the buffer
+            if (p + 1 >= (uint32_t)ENC_WORDS) overflow = true;
         }
         t_dense[(size_t)q * N + j] = v;
+        if (j == (uint32_t)N - 1) sh_nsel = e;
+        ++j;
     }
+    if ((tot & 0xFFFFu) < (uint32_t)N) overflow = true;  // the buffer ran out first
+    if (overflow) atomicOr(rng_flag, 2u);
+    __syncthreads();
+    if (tid == 0 && sh_nsel == 0xFFFFFFFFu) sh_nsel = tot >> 16;
+    __syncthreads();
     if (tid == 0) t_cnt[q] = sh_nsel;
+}
+
+// sum_{k < 32} a[k * stride] * b[k] mod p for canonical values (32 products
+// stay below 2^127). The strided loads go out sixteen at a time: one query's
+// encode is a handful of blocks, so its time is load latency, not bandwidth.
+__device__ __forceinline__ uint64_t dot32_strided(const uint64_t* __restrict__ a, size_t stride,
+                                                  const uint64_t* b) {
+    Acc128 acc;
+    acc.zero();
+#pragma unroll 1
+    for (int h = 0; h < 32; h += 16) {
+        uint64_t x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = a[(size_t)(h + j) * stride];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc.mac(x[j], b[h + j]);
+    }
+    return acc.reduce();
 }
 
 // v_enc = L G + T + v (crypto.rs:533-551). Block = (query, 256 columns);
@@ -837,16 +955,10 @@ k_encode_venc(const uint64_t* __restrict__ lt_mat /* 128 x 1024 */,
     if (threadIdx.x < N1) gs[threadIdx.x] = g[(size_t)q * 2 * N1 + threadIdx.x];
     __syncthreads();
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    Acc128 acc;
-    acc.zero();
     uint64_t lg = 0;
-    for (int k = 0; k < N1; ++k) {
-        acc.mac(lt_mat[(size_t)k * N + j], gs[k]);
-        if ((k & 31) == 31) {
-            lg = add_mod(lg, acc.reduce());
-            acc.zero();
-        }
-    }
+#pragma unroll 1
+    for (int k0 = 0; k0 < N1; k0 += 32)
+        lg = add_mod(lg, dot32_strided(lt_mat + (size_t)k0 * N + j, N, gs + k0));
     const int8_t t = t_dense[(size_t)q * N + j];
     if (t == 1) lg = add_mod(lg, 1);
     else if (t == -1) lg = sub_mod(lg, 1);
@@ -868,18 +980,11 @@ k_lt(const uint64_t* __restrict__ l_mat, const uint64_t* __restrict__ venc,
     const uint32_t k = threadIdx.x % N1, pt = threadIdx.x / N1;
     for (int j = threadIdx.x; j < N; j += N1 * LT_PARTS) vs[j] = venc[(size_t)q * N + j];
     __syncthreads();
-    constexpr int ROWS = N / LT_PARTS;  // 128 products: fold after 32 as elsewhere
-    Acc128 acc;
-    acc.zero();
+    constexpr int ROWS = N / LT_PARTS;
     uint64_t s = 0;
-    for (int jj = 0; jj < ROWS; ++jj) {
-        const int j = pt * ROWS + jj;
-        acc.mac(vs[j], l_mat[(size_t)j * N1 + k]);
-        if ((jj & 31) == 31) {
-            s = add_mod(s, acc.reduce());
-            acc.zero();
-        }
-    }
+#pragma unroll 1
+    for (int j0 = pt * ROWS; j0 < (int)(pt + 1) * ROWS; j0 += 32)
+        s = add_mod(s, dot32_strided(l_mat + (size_t)j0 * N1 + k, N1, vs + j0));
     part[pt][k] = s;
     __syncthreads();
     if (pt == 0) {

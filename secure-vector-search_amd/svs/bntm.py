@@ -413,6 +413,18 @@ class GpuIndex:
                                       ptr(out)))
         return None if rc == L.SVS_NONE else out[:m]
 
+    def encode_query(self, v_q, q_seeds):
+        """Device encode_query (crypto.rs:508-553) with this index's L:
+        (v_enc, g, t) for each query, t dense in {-1, 0, 1}."""
+        v = u64(v_q).reshape(-1, N)
+        nq = v.shape[0]
+        seeds = np.ascontiguousarray(q_seeds, np.uint8).reshape(nq, 32)
+        v_enc = np.zeros((nq, N), np.uint64)
+        g = np.zeros((nq, N1), np.uint64)
+        t = np.zeros((nq, N), np.int8)
+        check(L.lib().svs_encode_query(self.h, nq, ptr(v), ptr(seeds), ptr(v_enc), ptr(g), ptr(t)))
+        return v_enc, g, t
+
     def _batch(self, v_q, q_seeds, probe_ptr, probe_lists, verify, k):
         v_q = u64(v_q).reshape(-1, N)
         nq = v_q.shape[0]

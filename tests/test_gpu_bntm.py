@@ -258,6 +258,32 @@ def test_decode_matches_oracle(ctx, l_sub):
         assert [int(x) for x in dense] == want_dense
 
 
+def test_device_encode_matches_oracle(ctx, l_sub):
+    """The device encode (the batch pipeline's keystream, G draws, sparse T
+    walk and v_enc = v + L G + T) equals encode_query (crypto.rs:508-553)
+    for many seeds. The scores cannot show this: decode removes G and T
+    exactly whatever they are (zero noise)."""
+    import svs
+    rng = np.random.default_rng(2027)
+    idx = svs.GpuIndex(ctx, 1, l_sub)
+    nq = 96
+    v_q = _field(rng, (nq, 1024))
+    v_q[:, 768:] = 0
+    v_q[0] = 0
+    seeds = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    seeds[1] = 0
+    seeds[2] = 255
+    v_enc, g, t = idx.encode_query(v_q, seeds)
+    for q in range(nq):
+        w_enc, w_g, w_idx, w_sign = O.encode_query(v_q[q], l_sub, bytes(seeds[q]))
+        dense = np.zeros(1024, np.int8)
+        dense[w_idx] = w_sign
+        assert np.array_equal(g[q], w_g), f"g of query {q}"
+        assert np.array_equal(t[q], dense), f"T of query {q}"
+        assert np.array_equal(v_enc[q], w_enc), f"v_enc of query {q}"
+    idx.close()
+
+
 def test_decode_with_wide_plaintext_rows(ctx, l_sub):
     """M_plain outside the 32-bit signed range uses the 64-bit column store."""
     import svs
