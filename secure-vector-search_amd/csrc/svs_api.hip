@@ -363,6 +363,7 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
     if (rc == SVS_OK) rc = take((void**)&L.d.s_rowterm, sizeof(uint64_t) * m);
     if (rc != SVS_OK) free_list(ctx, L);
     L.d.m = m;
+    L.d.mp_cols = N;
     return rc;
 }
 
@@ -409,6 +410,20 @@ static bool fits_i24(const uint64_t* mp, size_t count) {
         if (!(x < (uint64_t)MP8_LIMIT || x > P - (uint64_t)MP8_LIMIT)) return false;
     }
     return true;
+}
+
+// The smallest multiple of 64 beyond which every row of M_plain is zero:
+// quantise_vector zero-pads d < n dimensions (crypto.rs:105-122), and the
+// decode skips M T over those columns.
+static uint32_t plain_cols(const uint64_t* mp, uint32_t m) {
+    uint32_t w = 0;
+    for (uint32_t r = 0; r < m && w < (uint32_t)N; ++r) {
+        const uint64_t* row = mp + (size_t)r * N;
+        uint32_t c = N;
+        while (c > w && row[c - 1] == 0) --c;
+        w = c;
+    }
+    return (w + 63) / 64 * 64;
 }
 
 static bool fits_i32(const uint64_t* mp, size_t count) {
@@ -521,6 +536,7 @@ extern "C" int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m,
     const bool mp32 = m_plain && fits_i32(m_plain, (size_t)m * N);
     const bool planes = mp32 && fits_i24(m_plain, (size_t)m * N);
     SVS_TRY(list_alloc(idx, list, m, mp32, m_plain != nullptr, planes));
+    if (m_plain) idx->lists[list].d.mp_cols = plain_cols(m_plain, m);
     ListHost& L = idx->lists[list];
     if (m > 0) {
         HIP_TRY(hipMemcpy((void*)L.d.menc, m_enc, sizeof(uint64_t) * (size_t)m * N,
@@ -553,6 +569,7 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
     const bool planes = mp32 && fits_i24(m_plain, (size_t)m * N);
     SVS_TRY(list_alloc(idx, list, m, mp32, true, planes));
     ListHost& L = idx->lists[list];
+    L.d.mp_cols = plain_cols(m_plain, m);
     if (m > 0) {
         DevBuf& tm = ctx->tmp_a;
         DevBuf& th = ctx->tmp_b;
@@ -1155,7 +1172,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         ctx->launches++;
         HIP_TRY(hipGetLastError());
     } else if (v.cap_chunks) {
-        SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+        SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * (3 + MT_BLOCKS) * nq));
         hipLaunchKernelGGL(k_query_corr<N>, dim3(nq), dim3(WAVE), 0, st, (const uint64_t*)d_venc,
                            (uint64_t*)sl.q_corr.ptr);
         if (kernel == KERNEL_MFMA_WAVE) {
@@ -1317,7 +1334,7 @@ int svs::view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k)
     SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc1));
     SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc1));
     SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
-    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * (3 + MT_BLOCKS) * nq));
     SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA *
                                         std::max<uint64_t>(v.cap_chunks, 1)));
     SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
@@ -1432,7 +1449,7 @@ int svs::reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
     SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
     SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
     SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
-    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * 3 * nq));
+    SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * (3 + MT_BLOCKS) * nq));
     SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc));
     SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc));
     SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc));

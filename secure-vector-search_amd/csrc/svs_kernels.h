@@ -297,9 +297,10 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 x[j] = 0;
-                if (e0 + j < tn) {
-                    const uint16_t te = tq[e0 + j];
-                    const uint32_t c = te & 0x3FF;
+                const uint16_t te = e0 + j < tn ? tq[e0 + j] : 0;
+                const uint32_t c = te & 0x3FF;
+                // M_plain is zero from column mp_cols on
+                if (e0 + j < tn && c < L.mp_cols) {
                     if (L.mp_t32) {
                         const int32_t v = L.mp_t32[(size_t)c * m + i];
                         x[j] = v >= 0 ? (uint64_t)v : P - (uint64_t)(-(int64_t)v);
@@ -1634,27 +1635,34 @@ __global__ void k_make_s8(const uint32_t* __restrict__ s_slice, const uint16_t* 
 }
 
 // Per query: the signed byte planes of v_enc (q x 8 x 1024; plane b holds
-// byte b of each element, ^ 0x80) and MP8_OFFSET * sum_k T[k] (mod p).
+// byte b of each element, ^ 0x80) and MP8_OFFSET * sum_{k < 64 b} T[k]
+// (mod p) for b = 0..MT_BLOCKS (q x (MT_BLOCKS + 1)): M T's offset term over
+// the first 64 b columns, the span a list's nonzero plaintext covers.
 __global__ void __launch_bounds__(256)
 k_decode_prep(const uint64_t* __restrict__ venc, const int8_t* __restrict__ t_dense,
               int8_t* __restrict__ vplanes, uint64_t* __restrict__ mt_qterm) {
-    __shared__ int red[256];
+    __shared__ int blk[MT_BLOCKS];
     const uint32_t q = blockIdx.x;
-    int tsum = 0;
+    if (threadIdx.x < MT_BLOCKS) blk[threadIdx.x] = 0;
+    __syncthreads();
     for (int k = threadIdx.x; k < N; k += 256) {
         const uint64_t v = venc[(size_t)q * N + k];
 #pragma unroll
         for (int b = 0; b < 8; ++b)
             vplanes[((size_t)q * 8 + b) * N + k] = (int8_t)(((v >> (8 * b)) & 0xFF) ^ 0x80);
-        tsum += t_dense[(size_t)q * N + k];
+        const int t = t_dense[(size_t)q * N + k];
+        if (t) atomicAdd(&blk[k / 64], t);
     }
-    red[threadIdx.x] = tsum;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t pre = 0;
+        uint64_t* out = mt_qterm + (size_t)q * (MT_BLOCKS + 1);
+        out[0] = 0;
+        for (int b = 0; b < MT_BLOCKS; ++b) {
+            pre += blk[b];
+            out[b + 1] = mul_mod((uint64_t)MP8_OFFSET, i64_to_fp(pre));
+        }
     }
-    if (threadIdx.x == 0) mt_qterm[q] = mul_mod((uint64_t)MP8_OFFSET, i64_to_fp(red[0]));
 }
 
 __device__ __forceinline__ v4i load16b(const int8_t* p) {
@@ -1693,6 +1701,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
         const ListDev L = lists[tile.list];
         const uint32_t m = L.m;
         const int gn = (int)tile.gn;
+        // M T runs over the K-steps of the list's nonzero plaintext columns
+        const int kw = (int)(L.mp_cols / 64);
         // B sources; a column without a query reads query 0's data and is
         // masked to zero (loads stay unconditional).
         const uint32_t q0 = items[gitems[tile.g0]].q;
@@ -1751,7 +1761,7 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                     er[rt][h] = r[g];
                     ed[rt][h] = dense[g];
                     es[rt][h] = L.s_rowterm[ok ? grow : 0];
-                    eq[rt][h] = mt_qterm[items[it].q];
+                    eq[rt][h] = mt_qterm[(size_t)items[it].q * (MT_BLOCKS + 1) + kw];
                 }
             // Two-slot register ring over the K-steps: slot d's MFMAs run,
             // then slot d is reloaded for the step two ahead (the tail
@@ -1760,11 +1770,14 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             // loop stays rolled so the compiler keeps that order.
             v4i ra[2][RTW][4], rb[2][1 + NCT];
             auto issue = [&](int slot, int kc) {
+                // past the nonzero plaintext the plane loads repeat the last
+                // step's addresses (cache hits, results unused)
+                const int kp = kc < kw ? kc : (kw > 0 ? kw - 1 : 0);
 #pragma unroll
                 for (int rt = 0; rt < RTW; ++rt) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a)
-                        ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kc);
+                        ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kp);
                     ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
                 }
                 rb[slot][0] = load16b(bt + 64 * kc);
@@ -1777,13 +1790,15 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
                 for (int slot = 0; slot < 2; ++slot) {
-                    const v4i b_t = rb[slot][0] & tmask;
+                    if (kc0 + slot < kw) {
+                        const v4i b_t = rb[slot][0] & tmask;
 #pragma unroll
-                    for (int rt = 0; rt < RTW; ++rt)
+                        for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                        for (int a = 0; a < 3; ++a)
-                            amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                ra[slot][rt][a], b_t, amt[rt][a], 0, 0, 0);
+                            for (int a = 0; a < 3; ++a)
+                                amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    ra[slot][rt][a], b_t, amt[rt][a], 0, 0, 0);
+                    }
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) {
                         const v4i b_v = rb[slot][1 + ct] & vmask[ct];

@@ -28,12 +28,15 @@ struct ListDev {
     const int8_t* s8;        // m x 1024: S as dense int8 (+-1 / 0)
     const uint64_t* s_rowterm;  // m: 128 * 0x0101010101010101 * sum_k S[row][k] mod p
     uint32_t m;
-    uint32_t pad;
+    uint32_t mp_cols;        // M_plain is zero in columns >= mp_cols (a multiple of 64)
 };
 
 // M_plain = sum_a 256^a s_a + MP8_OFFSET for the planes of mp8 (|M| < 2^23):
 // 128 * (1 + 256 + 65536) - 2^23.
 constexpr int64_t MP8_OFFSET = 32896;
+// Per query, MP8_OFFSET * sum_{k < 64 b} T[k] (mod p) for b = 0..MT_BLOCKS:
+// the offset term of M T over a list's nonzero plaintext columns.
+constexpr int MT_BLOCKS = 1024 / 64;
 constexpr int64_t MP8_LIMIT = 1 << 23;
 
 // One MFMA work tile: MT_ROWS rows of a list times up to MT_QG items

@@ -487,6 +487,56 @@ def test_score_batch_decode_planes_and_fallback(ctx, l_sub):
                 assert scores[q].tolist() == want[q][1].tolist(), f"mode {mode} query {q}"
 
 
+def test_decode_skips_zero_plaintext_columns(ctx, l_sub):
+    """Lists whose plaintext is zero beyond some column (d < n: quantised
+    vectors are zero-padded) decode M T over their nonzero columns only;
+    lists of widths 64, 100 (-> 128), 768 and 1024, alone and in one batch,
+    in every mode, equal the oracle. One row of the 100-wide list carries
+    its last entry at column 99 and nothing else."""
+    import svs
+    rng = np.random.default_rng(2031)
+    m = 90
+
+    def lift(v):
+        v = np.asarray(v, np.int64)
+        return np.array([int(x) % P for x in v], np.uint64)
+
+    mps = []
+    for w in (64, 100, 768, 1024):
+        x = np.zeros((m, 1024), np.int64)
+        x[:, :w] = rng.integers(-(1 << 20), 1 << 20, size=(m, w))
+        if w == 100:
+            x[3, :] = 0
+            x[3, 99] = -5
+        mps.append(lift(x.reshape(-1)))
+    clusters = [O.build_cluster(mp, m, l_sub, bytes([70 + i] * 32)) for i, mp in enumerate(mps)]
+    idx = svs.GpuIndex(ctx, len(clusters), l_sub)
+    for i, cl in enumerate(clusters):
+        idx.upload_list(i, cl, ids=1000 * i + np.arange(m, dtype=np.uint32))
+    nq, k = 6, 10
+    v_q = lift(rng.integers(-(1 << 20), 1 << 20, size=(nq, 1024)).reshape(-1)).reshape(nq, 1024)
+    seeds = rng.integers(0, 256, size=(nq, 32), dtype=np.uint8)
+    for probes in ([[0]] * nq, [[1]] * nq, [[0, 1, 2, 3], [1], [3, 1], [2, 0], [1, 2], [3]]):
+        pp = np.concatenate([[0], np.cumsum([len(p) for p in probes])]).astype(np.uint32)
+        pl = np.concatenate(probes).astype(np.uint32)
+        want = [_oracle_probe_scores(clusters, probes[q], v_q[q], bytes(seeds[q]), l_sub, k)
+                for q in range(nq)]
+        for mode in (0, 1, 2):
+            say("plaintext widths probes", probes, "mode", mode)
+            ids, scores, count, status = idx.score_batch(v_q, seeds, pp, pl, None, k, mode=mode)
+            for q in range(nq):
+                assert status[q] == 0 and count[q] == k
+                assert ids[q].tolist() == want[q][0].tolist(), f"mode {mode} query {q}"
+                assert scores[q].tolist() == want[q][1].tolist(), f"mode {mode} query {q}"
+    # the standalone one-list decode
+    v_enc, g, t_idx, t_sign = O.encode_query(v_q[0], l_sub, bytes(seeds[0]))
+    for i, cl in enumerate(clusters):
+        r = O.matvec(cl["m_enc"], m, 1024, v_enc)
+        want = O.decode_scores(r, m, cl["a_l"], cl["m_plain"], cl["h"], cl["s_rowptr"],
+                               cl["s_idx"], cl["s_sign"], l_sub, v_enc, g, t_idx, t_sign)
+        assert np.array_equal(idx.decode(i, r, v_enc, g, t_idx, t_sign), want), f"list {i}"
+
+
 def test_topk_ties_across_chunks(ctx):
     """Duplicate rows give equal scores; the top-k keeps the earliest
     positions (score desc, position asc) across 1024-candidate chunks, for
