@@ -335,7 +335,7 @@ static void build_s_ell(uint32_t m, const uint32_t* rowptr, const uint32_t* sidx
     }
 }
 
-// Allocates and fills the list's device state except menc / alh_t content.
+// Allocates and fills the list's device state except menc / alh content.
 static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool want_mp,
                       bool planes) {
     svs_ctx* ctx = idx->ctx;
@@ -349,7 +349,6 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
         return rc;
     };
     int rc = take((void**)&L.d.menc, sizeof(uint64_t) * (size_t)m * N);
-    if (rc == SVS_OK) rc = take((void**)&L.d.alh_t, sizeof(uint64_t) * (size_t)m * 2 * N1);
     if (rc == SVS_OK && want_mp) {
         if (mp32) rc = take((void**)&L.d.mp_t32, sizeof(int32_t) * (size_t)m * N);
         else rc = take((void**)&L.d.mp_t64, sizeof(uint64_t) * (size_t)m * N);
@@ -434,23 +433,19 @@ static bool fits_i32(const uint64_t* mp, size_t count) {
     return true;
 }
 
-// Derived device state shared by both upload paths, once menc and alh_t are
+// Derived device state shared by both upload paths, once menc and alh are
 // in place: row-major [AL | H] for the MFMA decode, and the row terms of the
 // MFMA identity for M_enc and for [AL | H].
 static int finish_list(svs_ctx* ctx, ListHost& L) {
     if (!L.m) return SVS_OK;
     const uint32_t m = L.m;
-    dim3 tg((m + 31) / 32, (2 * N1) / 32);
-    hipLaunchKernelGGL(k_transpose_u64, tg, dim3(32, 8), 0, ctx->stream,
-                       (const uint64_t*)L.d.alh_t, (uint32_t)(2 * N1), m, (uint64_t*)L.d.alh,
-                       (uint32_t)(2 * N1));
     const uint32_t blocks = std::min<uint32_t>((m + 3) / 4, (uint32_t)ctx->n_cu * 32);
     hipLaunchKernelGGL(k_row_corr<N>, dim3(blocks), dim3(256), 0, ctx->stream,
                        (const uint64_t*)L.d.menc, m, (uint64_t*)L.d.rowcorr);
     hipLaunchKernelGGL(k_row_corr<2 * N1>, dim3(blocks), dim3(256), 0, ctx->stream,
                        (const uint64_t*)L.d.alh, m, (uint64_t*)L.d.alh_rowcorr);
     HIP_TRY(hipGetLastError());
-    ctx->launches += 3;
+    ctx->launches += 2;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SVS_OK;
 }
@@ -541,8 +536,13 @@ extern "C" int svs_index_upload_list(svs_index* idx, uint32_t list, uint32_t m,
     if (m > 0) {
         HIP_TRY(hipMemcpy((void*)L.d.menc, m_enc, sizeof(uint64_t) * (size_t)m * N,
                           hipMemcpyHostToDevice));
-        SVS_TRY(upload_transposed(ctx, a_l, m, N1, (uint64_t*)L.d.alh_t, m));
-        SVS_TRY(upload_transposed(ctx, h, m, N1, (uint64_t*)L.d.alh_t + (size_t)N1 * m, m));
+        // [AL | H] row-major: AL into columns 0..127, H into 128..255
+        HIP_TRY(hipMemcpy2D((void*)L.d.alh, sizeof(uint64_t) * 2 * N1, a_l,
+                            sizeof(uint64_t) * N1, sizeof(uint64_t) * N1, m,
+                            hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy2D((void*)(L.d.alh + N1), sizeof(uint64_t) * 2 * N1, h,
+                            sizeof(uint64_t) * N1, sizeof(uint64_t) * N1, m,
+                            hipMemcpyHostToDevice));
         SVS_TRY(finish_list(ctx, L));
         if (m_plain) SVS_TRY(upload_mplain(ctx, L, m_plain));
         SVS_TRY(list_upload_s_ids(idx, list, s_rowptr, s_idx, s_sign, ids));
@@ -598,14 +598,11 @@ extern "C" int svs_index_build_list(svs_index* idx, uint32_t list, uint32_t m,
                            (uint64_t*)L.d.menc);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_encrypt_al, dim3((m + 7) / 8), dim3(128), 0, ctx->stream, d_m,
-                           (const uint64_t*)idx->l_dev, m, (uint64_t*)L.d.alh_t);
+                           (const uint64_t*)idx->l_dev, m, (uint64_t*)L.d.alh);
         HIP_TRY(hipGetLastError());
-        dim3 tg((N1 + 31) / 32, (m + 31) / 32);
-        if (tg.y > 65535) return fail(SVS_ERR_ARG, "list of %u rows too large", m);
-        hipLaunchKernelGGL(k_transpose_u64, tg, dim3(32, 8), 0, ctx->stream,
-                           (const uint64_t*)th.ptr, m, (uint32_t)N1,
-                           (uint64_t*)L.d.alh_t + (size_t)N1 * m, m);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpy2DAsync((void*)(L.d.alh + N1), sizeof(uint64_t) * 2 * N1, th.ptr,
+                                 sizeof(uint64_t) * N1, sizeof(uint64_t) * N1, m,
+                                 hipMemcpyDeviceToDevice, ctx->stream));
         if (L.d.mp_t32) {
             dim3 mg(N / 32, (m + 31) / 32);
             hipLaunchKernelGGL(k_transpose_mp32, mg, dim3(32, 8), 0, ctx->stream, d_m, m,
@@ -638,15 +635,12 @@ extern "C" int svs_index_download_list(svs_index* idx, uint32_t list, uint64_t* 
     if (!m) return SVS_OK;
     if (m_enc)
         HIP_TRY(hipMemcpy(m_enc, L.d.menc, sizeof(uint64_t) * m * N, hipMemcpyDeviceToHost));
-    if (a_l || h) {
-        std::vector<uint64_t> t(m * 2 * N1);
-        HIP_TRY(hipMemcpy(t.data(), L.d.alh_t, sizeof(uint64_t) * t.size(), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < m; ++i)
-            for (int k = 0; k < N1; ++k) {
-                if (a_l) a_l[i * N1 + k] = t[(size_t)k * m + i];
-                if (h) h[i * N1 + k] = t[(size_t)(N1 + k) * m + i];
-            }
-    }
+    if (a_l)
+        HIP_TRY(hipMemcpy2D(a_l, sizeof(uint64_t) * N1, L.d.alh, sizeof(uint64_t) * 2 * N1,
+                            sizeof(uint64_t) * N1, m, hipMemcpyDeviceToHost));
+    if (h)
+        HIP_TRY(hipMemcpy2D(h, sizeof(uint64_t) * N1, L.d.alh + N1, sizeof(uint64_t) * 2 * N1,
+                            sizeof(uint64_t) * N1, m, hipMemcpyDeviceToHost));
     return SVS_OK;
 }
 
@@ -1282,12 +1276,12 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         } else {
             const uint32_t blocks = (uint32_t)std::min<uint64_t>(
                 (v.cap_cands + DC_THREADS - 1) / DC_THREADS, (uint64_t)ctx->n_cu * 32);
+            if (!dense) return fail(SVS_ERR_STATE, "decode without its dense terms");
             hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(DC_THREADS), 0, st, d_lists,
-                               d_items,
-                               v.n_items, d_cand_ptr, (const uint64_t*)sl.r.ptr,
-                               (const uint64_t*)d_gl, (const uint64_t*)d_venc,
-                               (const uint16_t*)d_tlist, (const uint32_t*)d_tcnt, dense,
-                               (int64_t*)sl.scores.ptr, (uint64_t*)nullptr);
+                               d_items, v.n_items, d_cand_ptr, (const uint64_t*)sl.r.ptr,
+                               (const uint64_t*)d_venc, (const uint16_t*)d_tlist,
+                               (const uint32_t*)d_tcnt, dense, (int64_t*)sl.scores.ptr,
+                               (uint64_t*)nullptr);
             ctx->launches++;
         }
         HIP_TRY(hipGetLastError());
@@ -2154,15 +2148,20 @@ extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, cons
                            ctx->stream));
     hipLaunchKernelGGL(k_lt, dim3(1), dim3(N1 * LT_PARTS), 0, ctx->stream, (const uint64_t*)idx->l_dev,
                        (const uint64_t*)ctx->q_venc.ptr, (uint64_t*)ctx->q_gl.ptr);
+    SVS_TRY(ctx->dense.ensure(ctx, sizeof(uint64_t) * m));
+    hipLaunchKernelGGL(k_dense_rows, dim3(std::max<uint32_t>(1, std::min<uint32_t>((m + 3) / 4,
+                                                                        ctx->n_cu * 16))),
+                       dim3(256), 0, ctx->stream, idx->lists[list].d.alh, m,
+                       (const uint64_t*)ctx->q_gl.ptr, (uint64_t*)ctx->dense.ptr);
     const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
     hipLaunchKernelGGL(k_decode_cands, dim3((m + DC_THREADS - 1) / DC_THREADS), dim3(DC_THREADS), 0,
                        ctx->stream,
                        (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr, 1u, pre + 2,
-                       (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_gl.ptr,
-                       (const uint64_t*)ctx->q_venc.ptr, (const uint16_t*)ctx->q_tlist.ptr,
-                       (const uint32_t*)ctx->q_tcnt.ptr, (const uint64_t*)nullptr,
-                       (int64_t*)nullptr, (uint64_t*)ctx->scores.ptr);
-    ctx->launches += 2;
+                       (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_venc.ptr,
+                       (const uint16_t*)ctx->q_tlist.ptr, (const uint32_t*)ctx->q_tcnt.ptr,
+                       (const uint64_t*)ctx->dense.ptr, (int64_t*)nullptr,
+                       (uint64_t*)ctx->scores.ptr);
+    ctx->launches += 3;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(mv, ctx->scores.ptr, sizeof(uint64_t) * m, hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -2190,8 +2189,9 @@ extern "C" int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint6
                            ctx->stream));
     HIP_TRY(hipMemcpyAsync((uint64_t*)ctx->q_gl.ptr + N1, lt_venc, sizeof(uint64_t) * N1,
                            hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_dense_terms, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const uint64_t*)idx->lists[list].d.alh_t, m,
+    hipLaunchKernelGGL(k_dense_rows, dim3(std::max<uint32_t>(1, std::min<uint32_t>((m + 3) / 4,
+                                                                        ctx->n_cu * 16))),
+                       dim3(256), 0, ctx->stream, idx->lists[list].d.alh, m,
                        (const uint64_t*)ctx->q_gl.ptr, (uint64_t*)ctx->r.ptr);
     ctx->launches++;
     HIP_TRY(hipGetLastError());

@@ -233,10 +233,10 @@ k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
 
 // ================================================================== decode
 // Client decode (crypto.rs:901-940), one thread per candidate row:
-//   Mv = r - (AL g + M T) - (H lt + S v_enc),  lt = L^T v_enc.
+//   Mv = r - (AL g + M T) - (H lt + S v_enc),  lt = L^T v_enc,
+// with the dense term AL g + H lt computed before (indexed like r).
 // Column-major client state keeps every access coalesced across a wave:
-// AL^T / H^T rows, the gathered columns M_plain^T[t_e], and S sliced-ELL.
-// S v_enc gathers 128 random entries of the query's v_enc per row, so the
+// the gathered columns M_plain^T[t_e] and S sliced-ELL. S v_enc gathers 128 random entries of the query's v_enc per row, so the
 // block keeps the v_enc of its first candidate's query in LDS (a gather
 // from LDS instead of up to 64 cache lines per load); candidates of another
 // query in the same block read theirs from memory. Writes the signed score
@@ -245,10 +245,10 @@ constexpr int DC_THREADS = 256;
 __global__ void __launch_bounds__(DC_THREADS)
 k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items,
                uint32_t n_items, const uint32_t* __restrict__ cand_ptr,
-               const uint64_t* __restrict__ r, const uint64_t* __restrict__ gl /* q x 256 */,
-               const uint64_t* __restrict__ venc, const uint16_t* __restrict__ t_list,
-               const uint32_t* __restrict__ t_cnt, const uint64_t* __restrict__ dense_in,
-               int64_t* __restrict__ scores, uint64_t* __restrict__ mv_out) {
+               const uint64_t* __restrict__ r, const uint64_t* __restrict__ venc,
+               const uint16_t* __restrict__ t_list, const uint32_t* __restrict__ t_cnt,
+               const uint64_t* __restrict__ dense, int64_t* __restrict__ scores,
+               uint64_t* __restrict__ mv_out) {
     __shared__ uint64_t v_lds[N];
     const uint32_t total = cand_ptr[n_items];
     uint32_t q_lds = 0xFFFFFFFFu;  // block-uniform: the query held in v_lds
@@ -268,23 +268,6 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
         const ListDev L = lists[item.list];
         const uint32_t i = g - cand_ptr[it];
         const uint32_t m = item.m;
-        // AL g + H lt as one 256-long dot product against [g ; lt], unless
-        // the batch already computed it (dense_in, indexed like r).
-        uint64_t dense = 0;
-        if (dense_in) {
-            dense = dense_in[g];
-        } else {
-            const uint64_t* glq = gl + (size_t)item.q * 2 * N1;
-            Acc128 acc;
-            acc.zero();
-            for (int k = 0; k < 2 * N1; ++k) {
-                acc.mac(L.alh_t[(size_t)k * m + i], glq[k]);
-                if ((k & 31) == 31) {
-                    dense = add_mod(dense, acc.reduce());
-                    acc.zero();
-                }
-            }
-        }
         // M T: gathered columns of M_plain (sparse +-1 mask T of the query),
         // eight columns in flight.
         uint64_t mt = 0;
@@ -339,7 +322,7 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
                 if (!done) sv = (se[j] & 0x8000) ? sub_mod(sv, x[j]) : add_mod(sv, x[j]);
             }
         }
-        const uint64_t corr = add_mod(add_mod(dense, mt), sv);
+        const uint64_t corr = add_mod(add_mod(dense[g], mt), sv);
         const uint64_t mv = sub_mod(r[g], corr);
         if (scores) scores[g] = fp_to_signed(mv);
         if (mv_out) mv_out[g] = mv;
@@ -1032,11 +1015,12 @@ k_encrypt_menc(const uint64_t* __restrict__ mplain, const uint64_t* __restrict__
     menc[(size_t)i * N + j] = add_mod(mplain[(size_t)i * N + j], mp);
 }
 
-// AL = M L (crypto.rs:457-462), written transposed into alh_t rows 0..127:
-// block = 8 rows x 128 k; M rows in LDS, L read coalesced across k.
+// AL = M L (crypto.rs:457-462), written into columns 0..127 of the
+// row-major [AL | H] (m x 256): block = 8 rows x 128 k; M rows in LDS, L
+// read coalesced across k.
 __global__ void __launch_bounds__(128)
 k_encrypt_al(const uint64_t* __restrict__ mplain, const uint64_t* __restrict__ l_mat, uint32_t m,
-             uint64_t* __restrict__ alh_t) {
+             uint64_t* __restrict__ alh) {
     constexpr int RB = 8;
     __shared__ uint64_t rows[RB][N];
     const uint32_t i0 = blockIdx.x * RB;
@@ -1064,7 +1048,7 @@ k_encrypt_al(const uint64_t* __restrict__ mplain, const uint64_t* __restrict__ l
         }
     }
     for (int r = 0; r < RB; ++r)
-        if (i0 + r < m) alh_t[(size_t)k * m + i0 + r] = s[r];
+        if (i0 + r < m) alh[(size_t)(i0 + r) * 2 * N1 + k] = s[r];
 }
 
 // Generic tiled transpose of a rows x cols u64 matrix into cols x rows,
@@ -1103,24 +1087,27 @@ __global__ void k_transpose_mp32(const uint64_t* __restrict__ src, uint32_t rows
     }
 }
 
-// AL g + H lt for one list (decode_dense_terms): thread per row over the
-// column-major [AL^T ; H^T] block.
+// AL g + H lt for one list (decode_dense_terms): a wave per row of the
+// row-major [AL | H], lane l holding columns 2l, 2l + 1, 128 + 2l, 129 + 2l,
+// [g ; lt] in registers; four products per lane, one reduction, a wave sum.
 __global__ void __launch_bounds__(256)
-k_dense_terms(const uint64_t* __restrict__ alh_t, uint32_t m, const uint64_t* __restrict__ gl,
-              uint64_t* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    Acc128 acc;
-    acc.zero();
-    uint64_t s = 0;
-    for (int k = 0; k < 2 * N1; ++k) {
-        acc.mac(alh_t[(size_t)k * m + i], gl[k]);
-        if ((k & 31) == 31) {
-            s = add_mod(s, acc.reduce());
-            acc.zero();
-        }
+k_dense_rows(const uint64_t* __restrict__ alh, uint32_t m, const uint64_t* __restrict__ gl,
+             uint64_t* __restrict__ out) {
+    const uint32_t lane = threadIdx.x % WAVE, wpb = blockDim.x / WAVE;
+    const ulonglong2 x0 = reinterpret_cast<const ulonglong2*>(gl)[lane];
+    const ulonglong2 x1 = reinterpret_cast<const ulonglong2*>(gl)[WAVE + lane];
+    for (uint32_t i = blockIdx.x * wpb + threadIdx.x / WAVE; i < m; i += gridDim.x * wpb) {
+        const ulonglong2* row = reinterpret_cast<const ulonglong2*>(alh + (size_t)i * 2 * N1);
+        const ulonglong2 a0 = row[lane], a1 = row[WAVE + lane];
+        Acc128 acc;
+        acc.zero();
+        acc.mac(a0.x, x0.x);
+        acc.mac(a0.y, x0.y);
+        acc.mac(a1.x, x1.x);
+        acc.mac(a1.y, x1.y);
+        const uint64_t sum = wave_sum_mod(acc.reduce());
+        if (lane == 0) out[i] = sum;
     }
-    out[i] = s;
 }
 
 // ====================================================== batched product (MFMA)

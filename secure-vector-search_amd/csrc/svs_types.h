@@ -14,14 +14,13 @@ constexpr uint16_t S_PAD = 0xFFFF;      // padding entry in S sliced-ELL
 
 struct ListDev {
     const uint64_t* menc;    // m x 1024, row-major (server state)
-    const uint64_t* alh_t;   // 256 x m: rows 0..127 = AL^T, rows 128..255 = H^T
     const int32_t* mp_t32;   // 1024 x m, signed lift of M_plain (when it fits)
     const uint64_t* mp_t64;  // 1024 x m, canonical M_plain (otherwise)
     const uint32_t* s_slice; // ceil(m/64)+1 offsets (in entries) into s_ell
     const uint16_t* s_ell;   // per slice: width x 64 entries, idx | 0x8000 if sign < 0
     const uint32_t* ids;     // m global vector ids
     const uint64_t* rowcorr; // m: c * sum_k M_enc[row][k] mod p, c = 128 * 0x0101010101010101
-    const uint64_t* alh;     // m x 256, row-major [AL | H] (MFMA decode)
+    const uint64_t* alh;     // m x 256, row-major [AL | H] (the dense decode terms)
     const uint64_t* alh_rowcorr;  // m: c * sum_k alh[row][k] mod p
     // MFMA decode (null when M_plain does not fit the 24-bit planes):
     const int8_t* mp8;       // m x 3 x 1024: byte planes s_a of M_plain + 2^23, each ^ 0x80
