@@ -256,6 +256,19 @@ def served(index, qs, v_q, probes, verify, k, levels, seconds, max_batch, slots,
     return out
 
 
+def served_at_slo(levels, slo_ms, raw_plain_qps, plain_qps):
+    """The highest achieved served rate whose p99 stays under the SLO, and
+    its ratio to the plain one-query paths (raw queries, as served, and
+    pre-routed)."""
+    ok = [x for x in (levels or []) if x["p99_ms"] < slo_ms]
+    if not ok:
+        return None
+    best = max(ok, key=lambda x: x["achieved_qps"])
+    return dict(slo_ms=slo_ms, achieved_qps=best["achieved_qps"], p99_ms=best["p99_ms"],
+                vs_raw_plain=round(best["achieved_qps"] / raw_plain_qps, 2),
+                vs_plain=round(best["achieved_qps"] / plain_qps, 2))
+
+
 def served_mix(index, qs, k, qps, seconds, max_batch, slots, seed=9):
     """One open-loop level with the serving request mix of the reference's
     served configuration: raw queries, nprobe uniform in {8, 16, 32, 64,
@@ -560,7 +573,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="force a block-tile variant (skips the autotuner)")
-    ap.add_argument("--serve-qps", default="10000,14000,18000,22000",
+    ap.add_argument("--serve-qps", default="10000,14000,18000,20000,22000",
                     help="offered rates of the served (open-loop Poisson) levels; '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)
@@ -799,6 +812,7 @@ def main():
                         "note": "f32 queries in host buffers; quantise_vector and probe_route "
                                 "run on the device inside the batch"},
         "served": serve,
+        "served_at_slo": served_at_slo(serve, args.slo_ms, qps_raw_plain, qps_plain),
         "served_mix": mix,
         "served_requests": "routed (v_q + probes)" if args.serve_routed else
                            "raw f32 queries, quantised and routed on the device",
