@@ -178,7 +178,11 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
     }
     const bool dplan = cfg.mode != SVS_MODE_PLAIN && ctx->planner &&
                        device_plan_ok(idx, n_items) && rows < 0xFFFFFFFFull;
-    const int kernel = cfg.mode == SVS_MODE_BATCHED ? 1 : 0;  // MFMA tiles or VALU
+    // MFMA tiles, or the VALU kernels: a batch of one query has no rows to
+    // share between queries, and the row-streaming product with the
+    // per-candidate decode runs one query 10-20 % faster than the tiles
+    // (nprobe >= 16 on 550k-row lists; equal at nprobe 8)
+    const int kernel = cfg.mode == SVS_MODE_BATCHED && nq > 1 ? 1 : 0;
     int variant = 0, hkernel = 0;
     const uint32_t d = idx->cent_d;
     if (raw && !dplan)
