@@ -602,7 +602,7 @@ def main():
                          "(0 = the default workload)")
     ap.add_argument("--flat-queries", type=int, default=1000)
     ap.add_argument("--flat-product-calls", type=int, default=50)
-    ap.add_argument("--tenant-slots", default="1,2,4",
+    ap.add_argument("--tenant-slots", default="1,2",
                     help="slot counts the multi-tenant served levels are run with")
     args = ap.parse_args()
     if args.gpus != 1:
