@@ -814,6 +814,7 @@ struct TileVariant {
     uint32_t regs;  // 1: B windows built in registers (k_fp_gemm_vb), 0: in LDS
     uint32_t vpf;   // query-word ring depth (LDS kernel)
 };
+// (variant 8 also builds two steps of B windows per block barrier)
 static constexpr TileVariant kTileVariants[SVS_TILE_VARIANTS] = {
     {8, 8, 1, 0, 8}, {8, 2, 2, 0, 2}, {16, 4, 1, 0, 4}, {4, 4, 2, 0, 4}, {16, 2, 1, 0, 2},
     {8, 4, 1, 1, 4}, {8, 2, 1, 1, 2}, {16, 2, 1, 1, 2}, {8, 4, 1, 0, 8}, {16, 4, 1, 0, 8}};
@@ -856,10 +857,10 @@ struct TileArgs {
     const uint32_t* cand_ptr;
 };
 
-template <int KE, int SRC, int QG, int PF, int OCC, int VPF = PF>
+template <int KE, int SRC, int QG, int PF, int OCC, int VPF = PF, int SB = 1>
 static void launch_gemm_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
                           const uint64_t* vecs, const uint64_t* qc, uint64_t* out) {
-    hipLaunchKernelGGL((k_fp_gemm_tiles<KE, SRC, QG, PF, OCC, VPF>), dim3(blocks), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_fp_gemm_tiles<KE, SRC, QG, PF, OCC, VPF, SB>), dim3(blocks), dim3(256), 0, st,
                        a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
 }
 
@@ -881,7 +882,7 @@ static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& 
         case 5: launch_gemm_vb<KE, SRC, 8, 4>(st, blocks, a, vecs, qc, out); break;
         case 6: launch_gemm_vb<KE, SRC, 8, 2>(st, blocks, a, vecs, qc, out); break;
         case 7: launch_gemm_vb<KE, SRC, 16, 2>(st, blocks, a, vecs, qc, out); break;
-        case 8: launch_gemm_v<KE, SRC, 8, 4, 1, 8>(st, blocks, a, vecs, qc, out); break;
+        case 8: launch_gemm_v<KE, SRC, 8, 4, 1, 8, 2>(st, blocks, a, vecs, qc, out); break;
         case 9: launch_gemm_v<KE, SRC, 16, 4, 1, 8>(st, blocks, a, vecs, qc, out); break;
         case 3: launch_gemm_v<KE, SRC, 4, 4, 2>(st, blocks, a, vecs, qc, out); break;
         default: launch_gemm_v<KE, SRC, 8, 8, 1>(st, blocks, a, vecs, qc, out); break;

@@ -1303,7 +1303,7 @@ __device__ __forceinline__ uint32_t xcd_worker() {
 // current step's MFMAs run. OCC is the waves per SIMD the register budget
 // is sized for (1: deep ring, 2: shallow ring with a second wave to switch
 // to); the host picks a (QG, PF, OCC) variant by timing them.
-template <int KE, int SRC, int QG, int PF, int OCC, int VPF = 8>
+template <int KE, int SRC, int QG, int PF, int OCC, int VPF = 8, int SB = 1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                 const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
@@ -1313,12 +1313,14 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
     constexpr int STEPS = KE / 8;  // 64 K'-bytes (8 elements) per step
     static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
     static_assert(STEPS % PF == 0 && PF % 2 == 0 && STEPS % VPF == 0 && VPF % PF == 0 &&
-                      VPF % 2 == 0,
+                      VPF % 2 == 0 && (SB == 1 || SB == 2) && VPF % (2 * SB) == 0 &&
+                      VPF > SB,
                   "rings must tile the steps");
     // [buf][query][column][element], each column row padded from 8 to 10
     // words so that the 16 columns of a ds_read_b128 pass start on distinct
     // bank groups
-    __shared__ uint64_t bw[2][QG][16][10];
+    // SB steps per buffer: a block barrier every SB steps
+    __shared__ uint64_t bw[2 * SB][QG][16][10];
     __shared__ int32_t dsm[PB_WAVES][4][16][17];
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
@@ -1367,7 +1369,10 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
         }
 #pragma unroll
         for (int e = 0; e < VPF; ++e) vr[e] = gload_u64(pv + 8 * e);
-        if (producer) produce(0, vr[0]);
+        if (producer) {
+            produce(0, vr[0]);
+            if (SB == 2) produce(1, vr[1]);
+        }
         __syncthreads();
         v4i acc[PB_RT][QG];
 #pragma unroll
@@ -1391,9 +1396,10 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 // All B windows are read first (one LDS wait per step; the
                 // rows of absent queries hold stale data and are not used).
                 v4i b[QG];
+                const int bs = SB == 1 ? (u & 1) : (u & 3);  // buffer slot of this step
 #pragma unroll
                 for (int j = 0; j < QG; ++j)
-                    b[j] = *reinterpret_cast<const v4i*>(&bw[u & 1][j][col][2 * grp]);
+                    b[j] = *reinterpret_cast<const v4i*>(&bw[bs][j][col][2 * grp]);
 #pragma unroll
                 for (int j = 0; j < QG; ++j) {
                     if (j < gn) {
@@ -1405,9 +1411,16 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
                 }
                 // (after the last step this fills the idle buffer with a
                 // repeat of the final step; nothing reads it)
-                if (producer) produce((u + 1) & 1, vr[(u + 1) % VPF]);
-                vr[u % VPF] = gload_u64(pv + 8 * vnxt);
-                __syncthreads();
+                if (SB == 1) {
+                    if (producer) produce((u + 1) & 1, vr[(u + 1) % VPF]);
+                    vr[u % VPF] = gload_u64(pv + 8 * vnxt);
+                    __syncthreads();
+                } else {
+                    // step u writes step u + 2 into the other pair's slot
+                    if (producer) produce((u + 2) & 3, vr[(u + 2) % VPF]);
+                    vr[u % VPF] = gload_u64(pv + 8 * vnxt);
+                    if (u & 1) __syncthreads();
+                }
             }
         }
         // Epilogue, four (row tile, query) pairs per round (see k_gemv_mfma).
