@@ -1590,13 +1590,14 @@ k_fp_gemm_vb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
 
 // ------------------------------------------------------------ MFMA decode
 // Signed integers to F_p.
-__device__ __forceinline__ uint64_t i64_to_fp(int64_t x) {
-    return x >= 0 ? (uint64_t)x % P : sub_mod(0, (uint64_t)(-x) % P);
+// x mod p for any u64: x = hi 2^61 + lo with hi < 8 and 2^61 = 1 (mod p),
+// so lo + hi < 2^61 + 8 needs at most one subtraction.
+__device__ __forceinline__ uint64_t u64_mod_p(uint64_t x) {
+    const uint64_t r = (x & P) + (x >> 61);
+    return r >= P ? r - P : r;
 }
-__device__ __forceinline__ uint64_t i128_to_fp(__int128 x) {
-    // |x| < 2^75 for every caller; adding p * 2^14 (> 2^75) keeps it positive.
-    const unsigned __int128 u = (unsigned __int128)(x + ((__int128)P << 14));
-    return reduce128((uint64_t)u, (uint64_t)(u >> 64));
+__device__ __forceinline__ uint64_t i64_to_fp(int64_t x) {
+    return x >= 0 ? u64_mod_p((uint64_t)x) : sub_mod(0, u64_mod_p(0ull - (uint64_t)x));
 }
 
 // M_plain rows (canonical, row-major m x 1024) -> the 24-bit byte planes of
@@ -1834,13 +1835,21 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                         const int64_t mt = (int64_t)dsm[wave][0][row][j] +
                                            (int64_t)dsm[wave][1][row][j] * 256 +
                                            (int64_t)dsm[wave][2][row][j] * 65536;
-                        __int128 sv = 0;
+                        // S v = sum_b 256^b D_b with |D_b| <= 128 * 1024: two
+                        // halves of four planes each fit in 64 bits, and the
+                        // high half enters as 2^32 times itself
+                        int64_t sv_lo = 0, sv_hi = 0;
 #pragma unroll
-                        for (int b = 0; b < 8; ++b)
-                            sv += (__int128)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
-                                  (__int128)(1ull << (8 * b));
+                        for (int b = 0; b < 4; ++b) {
+                            sv_lo += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
+                                     (int64_t)(1ll << (8 * b));
+                            sv_hi += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + 4 + b] *
+                                     (int64_t)(1ll << (8 * b));
+                        }
+                        const uint64_t sv =
+                            add_mod(i64_to_fp(sv_lo), mul_pow2(i64_to_fp(sv_hi), 32));
                         uint64_t corr = add_mod(i64_to_fp(mt), eq[rt][h]);
-                        corr = add_mod(corr, add_mod(i128_to_fp(sv), es[rt][h]));
+                        corr = add_mod(corr, add_mod(sv, es[rt][h]));
                         corr = add_mod(corr, ed[rt][h]);
                         scores[eg[rt][h]] = fp_to_signed(sub_mod(er[rt][h], corr));
                     }
