@@ -687,6 +687,26 @@ def main():
     # batch 0 of the timed region again, for the bit-exactness check below
     run_batches_dev(index, dq, probes, verify, args.k, args.batch, 1, offset=off)
     res_b = [dq.results(min(args.batch, v_q.shape[0] - off))]
+    # the product at the served batch sizes (same queries, batches of 64):
+    # fewer queries per list, so the tiles stream rows with less MFMA work
+    by_batch = {}
+    for nb in sorted({64, args.batch}):
+        if nb > args.batch:
+            continue
+        nbat = max(1, min(args.steps * args.batch // nb, (v_q.shape[0] - off) // nb))
+        run_batches_dev(index, dq, probes, verify, args.k, nb, 1, offset=off)
+        ctx.profile(True, True)
+        lat_b = run_batches_dev(index, dq, probes, verify, args.k, nb, nbat, offset=off)
+        pb = ctx.profile(False, False)
+        ms_b, calls_b = pb["product"]
+        gbs = pb["gemv_bytes"] / (ms_b * 1e-3) / 1e9 if ms_b else 0.0
+        by_batch[str(nb)] = dict(queries_per_s=round(nb * nbat / sum(lat_b), 1),
+                                 product_ms=round(ms_b / max(calls_b, 1), 4),
+                                 algorithmic_bytes_per_launch=int(pb["gemv_bytes"] /
+                                                                  max(calls_b, 1)),
+                                 achieved_gbs=round(gbs, 1),
+                                 frac=round(gbs / HBM_PEAK_GBS, 4))
+    log(f"product by batch size: {by_batch}")
     # the same batches through host buffers (PCIe-inclusive rate)
     lat_h, _ = run_batches(index, v_q, seeds, probes, verify, args.k, args.batch, args.steps, 1,
                            offset=off)
@@ -822,6 +842,7 @@ def main():
         "tile_variant": variant,
         "autotune": tune,
         "roofline": roofline,
+        "roofline_by_batch": by_batch,
         "schemes": other,
         "copy_probe": copy,
         "cpu_baseline": cpu,
