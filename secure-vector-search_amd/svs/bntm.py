@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import atexit
 import ctypes as C
+import math
 import os
 import time
 import weakref
@@ -724,7 +725,8 @@ class BnTmIvfScorer:
     def communication_cost(self, handle: BnTmIvfHandle, k: int) -> CommunicationCost:
         """ivf.rs:928-970."""
         mean_m = float(handle.sizes.mean()) if handle.sizes.size else 0.0
-        avg = int(round(mean_m * FIELD_BYTES))
+        # f64::round: half away from zero (Python's round is half to even)
+        avg = int(math.floor(mean_m * FIELD_BYTES + 0.5))
         return CommunicationCost(N * FIELD_BYTES, handle.nprobe * avg, avg,
                                  int(handle.sizes.sum()) * N * FIELD_BYTES, 0, 0,
                                  handle.nprobe * int(mean_m) * N * FIELD_BYTES)

@@ -376,6 +376,36 @@ def test_ivf_batch_equals_sequential(ivf_setup):
         assert one == batch[i]
 
 
+def test_verification_overhead_in_expected_band(ctx, ivf_setup):
+    """Verification does real work and no more than 20x the server product
+    (crypto.rs:1482-1523), from the stage events on the library's stream,
+    in both kernel families; with verification off its stage is a no-op."""
+    import svs
+    s = ivf_setup
+    idx = s["handle"].index
+    qs = s["qs"]
+    nq = qs.shape[0]
+    probes = svs.probe_route(s["centres"], qs, 4)
+    pp = np.arange(0, nq * 4 + 1, 4, dtype=np.uint32)
+    v_q = svs.quantise(qs)
+    seeds = np.arange(nq * 32, dtype=np.uint64).astype(np.uint8).reshape(nq, 32)
+    for mode in (0, 1):
+        for ver in (1, 0):
+            flags = np.full(nq, ver, np.uint8)
+            idx.score_batch(v_q, seeds, pp, probes.reshape(-1), flags, 10, mode)
+            ctx.profile(True, True)
+            for _ in range(3):
+                idx.score_batch(v_q, seeds, pp, probes.reshape(-1), flags, 10, mode)
+            prof = ctx.profile(False, False)
+            product_ms, verify_ms = prof["product"][0], prof["verify"][0]
+            say("mode", mode, "verify", ver, "product ms", product_ms, "verify ms", verify_ms)
+            assert product_ms > 0
+            if ver:
+                assert 0 < verify_ms <= 20 * product_ms
+            else:
+                assert verify_ms <= product_ms
+
+
 def test_ivf_tampered_server_rejected_with_cluster(ctx, ivf_setup):
     import svs
     s = ivf_setup

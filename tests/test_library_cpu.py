@@ -129,3 +129,34 @@ def test_cache_fingerprint_is_fnv1a_over_length_prefixed_parts():
         assert svs.cache_fingerprint64(parts) == fnv(parts)
     # the length prefix separates ("ab", "") from ("a", "b")
     assert svs.cache_fingerprint64([b"ab", b""]) != svs.cache_fingerprint64([b"a", b"b"])
+
+
+def test_communication_cost_formulas():
+    """communication_cost of the flat and IVF scorers (lib.rs:477-511,
+    ivf.rs:928-970; the reference's test lib.rs:1007-1029): query n*8,
+    response and setup from the list sizes, the effective bytes the
+    product streams per query; the IVF mean rounds half away from zero."""
+    import svs
+    from svs import bntm as B
+    flat = B.BnTmHandle(index=None, l_subspace=None, m=10, dim=32, verification_enabled=False,
+                        quantisation_q=B.Q_DEFAULT)
+    c = svs.BnTmScorer().communication_cost(flat, 5)
+    assert (c.query_bytes, c.response_bytes, c.cluster_response_bytes, c.setup_bytes) == \
+        (1024 * 8, 10 * 8, 10 * 8, 10 * 1024 * 8)
+    assert (c.pre_query_offline_up_bytes, c.pre_query_offline_down_bytes) == (0, 0)
+    assert c.effective_bytes_per_query == 10 * 1024 * 8
+    # 16 lists holding 1 row in total: mean 1/16 row, mean bytes 0.5 -> 1
+    sizes = np.zeros(16, np.int64)
+    sizes[3] = 1
+    ivf = B.BnTmIvfHandle(index=None, ivf=None, l_subspace=None, sizes=sizes, nprobe=4, dim=32,
+                          verification_enabled=False, quantisation_q=B.Q_DEFAULT)
+    c = svs.BnTmIvfScorer().communication_cost(ivf, 10)
+    assert c.cluster_response_bytes == 1 and c.response_bytes == 4
+    assert c.setup_bytes == 1 * 1024 * 8
+    assert c.effective_bytes_per_query == 0  # (mean_m as u64) truncates 0.0625
+    sizes = np.array([100, 60, 20], np.int64)
+    ivf.sizes, ivf.nprobe = sizes, 2
+    c = svs.BnTmIvfScorer().communication_cost(ivf, 10)
+    assert c.cluster_response_bytes == 480 and c.response_bytes == 960
+    assert c.setup_bytes == 180 * 1024 * 8
+    assert c.effective_bytes_per_query == 2 * 60 * 1024 * 8
