@@ -1,7 +1,9 @@
 """Builds the gfx950 backend library in-tree: svs/libsvs_gpu.so.
 
 hipcc cross-compiles for gfx950 without a GPU. Packed-FP32 instruction
-selection is disabled for all device code (build flags).
+selection is disabled for all device code (no kernel needs it), and
+floating-point contraction is off, so the host and device routing sum
+squared differences in the reference's order.
 """
 import os
 import subprocess
