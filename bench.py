@@ -607,7 +607,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
+              "(lists would be bin-packed across GPUs with one top-k exchange per batch; "
+              "only the one-GPU share is built and measured)", file=sys.stderr)
         sys.exit(2)
 
     if args.served_tenants > 0:

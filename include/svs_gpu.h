@@ -7,7 +7,8 @@
  * states scorer-bntm/src/gpu/mod.rs (FlatGpuState, IvfGpuState) and
  * scorer-emvp/src/gpu/mod.rs, plus Tiptoe's server answer
  * (scorer-tiptoe/src/pir/simplepir.rs:191-204), which has no GPU path in the
- * reference. A Rust FFI crate binds these symbols.
+ * reference. A Rust FFI crate binds these symbols as plain extern "C"
+ * declarations; every call returns an int status (svs_last_error for text).
  *
  * Conventions
  *  - Every field element in and out is canonical: a u64 in [0, p), p = 2^61-1.
