@@ -249,6 +249,13 @@ extern "C" int svs_ctx_set_graphs(svs_ctx* ctx, int on) {
 }
 
 // ====================================================================== index
+// Generations key the captured batch graphs: a graph holds the device
+// addresses of its index (L, L^T, the list table, the centroids) and the
+// list count, so a new index, or new centroids, must never replay a graph
+// captured before, even at a reused host address.
+static std::atomic<uint64_t> g_index_gen{1};
+uint64_t svs::next_index_gen() { return g_index_gen.fetch_add(1); }
+
 extern "C" int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* l_host,
                                 svs_index** out) {
     if (!ctx || !out || !l_host) return fail(SVS_ERR_ARG, "null argument");
@@ -259,6 +266,7 @@ extern "C" int svs_index_create(svs_ctx* ctx, uint32_t n_lists, const uint64_t* 
         if (l_host[i] >= P) return fail(SVS_ERR_ARG, "L[%zu] is not canonical", i);
     auto* idx = new svs_index();
     idx->ctx = ctx;
+    idx->gen = next_index_gen();
     idx->lists.resize(n_lists);
     int rc = dev_alloc(ctx, (void**)&idx->l_dev, sizeof(uint64_t) * N * N1);
     if (rc == SVS_OK) rc = dev_alloc(ctx, (void**)&idx->lt_dev, sizeof(uint64_t) * N * N1);
@@ -362,7 +370,9 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
     if (rc == SVS_OK) rc = take((void**)&L.d.s_rowterm, sizeof(uint64_t) * m);
     if (rc != SVS_OK) free_list(ctx, L);
     L.d.m = m;
-    L.d.mp_cols = N;
+    // no M_plain: no column of it may be read (the decode kernels bound
+    // their M_plain gathers by mp_cols)
+    L.d.mp_cols = want_mp ? N : 0;
     return rc;
 }
 
@@ -1615,7 +1625,7 @@ static int batch_device_planned(svs_index* idx, const svs_query_batch* b, int mo
         HIP_TRY(hipMemcpyAsync(sl.q_seeds.ptr, b->q_seeds, (size_t)nq * 32,
                                hipMemcpyDeviceToDevice, sl.stream));
     }
-    const std::vector<uint64_t> key{1, (uint64_t)(uintptr_t)idx, nq, n_items, b->k,
+    const std::vector<uint64_t> key{1, idx->gen, nq, n_items, b->k,
                                     (uint64_t)kernel, (uint64_t)variant, v.any_verify,
                                     v.planes_ok, cap, dev, (uint64_t)(uintptr_t)h.pptr};
     SVS_TRY(run_graphed(ctx, sl, key, [&]() -> int {
@@ -1727,6 +1737,9 @@ extern "C" int svs_index_set_centroids(svs_index* idx, const float* centroids, u
     HIP_TRY(hipMemcpy(p, centroids, sizeof(float) * nl * d, hipMemcpyHostToDevice));
     idx->cent_dev = p;
     idx->cent_d = d;
+    // graphs captured for this index hold the old centroid buffer: a new
+    // generation keys fresh captures
+    idx->gen = next_index_gen();
     return SVS_OK;
 }
 
@@ -1829,7 +1842,7 @@ extern "C" int svs_score_raw_batch(svs_index* idx, const svs_raw_batch* b, int m
             SVS_TRY(view_reserve(ctx, sl, v, b->k));
             uint64_t qbits;
             std::memcpy(&qbits, &q_scale, 8);
-            const std::vector<uint64_t> key{2, (uint64_t)(uintptr_t)idx, nq, n_items, b->k,
+            const std::vector<uint64_t> key{2, idx->gen, nq, n_items, b->k,
                                             (uint64_t)kernel, (uint64_t)variant, v.any_verify,
                                             v.planes_ok, bound, d, qbits,
                                             (uint64_t)(uintptr_t)h.pptr};

@@ -18,6 +18,8 @@ struct svs_ctx;
 namespace svs {
 
 int fail(int code, const char* fmt, ...);
+// A process-unique index generation (see svs_index::gen).
+uint64_t next_index_gen();
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
@@ -171,6 +173,8 @@ struct svs_index {
     // changes; top_rows)
     std::vector<uint64_t> top_rows_prefix;
     bool top_rows_dirty = true;
+    // generation: unique per index and per centroid set (graph cache key)
+    uint64_t gen = 0;
 };
 
 // ---------------------------------------------------------------- batches

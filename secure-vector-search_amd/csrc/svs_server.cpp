@@ -238,7 +238,7 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
             uint64_t qbits;
             std::memcpy(&qbits, &qscale, 8);
             const std::vector<uint64_t> key{
-                3, (uint64_t)(uintptr_t)idx, raw, nq, n_items, k, (uint64_t)kernel,
+                3, idx->gen, raw, nq, n_items, k, (uint64_t)kernel,
                 (uint64_t)variant, v.any_verify, v.planes_ok, cap, qbits,
                 (uint64_t)(uintptr_t)s.h_pptr.p, (uint64_t)(uintptr_t)s.h_plist.p,
                 (uint64_t)(uintptr_t)s.h_ver.p, (uint64_t)(uintptr_t)s.h_x.p,
@@ -433,6 +433,13 @@ extern "C" int svs_server_create(svs_index* const* tenants, uint32_t n_tenants,
         if (!tenants[t]) return fail(SVS_ERR_ARG, "tenant %u is null", t);
         if (ctx && tenants[t]->ctx != ctx) return fail(SVS_ERR_ARG, "tenants on different contexts");
         ctx = tenants[t]->ctx;
+        // every served batch decodes, so every list needs its M_plain (as
+        // svs_score_batch and svs_score_raw_batch require)
+        for (size_t c = 0; c < tenants[t]->lists.size(); ++c) {
+            const ListHost& L = tenants[t]->lists[c];
+            if (L.m && !L.d.mp_t32 && !L.d.mp_t64)
+                return fail(SVS_ERR_STATE, "tenant %u list %zu was uploaded without M_plain", t, c);
+        }
     }
     auto s = std::make_unique<svs_server>();
     s->ctx = ctx;
