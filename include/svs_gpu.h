@@ -196,6 +196,20 @@ int svs_decode_dense_terms(svs_index* idx, uint32_t list, const uint64_t* g_host
 int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r_host, const uint64_t* v_enc_host,
                const uint8_t rng_seed[32], uint32_t trials, int32_t* failed_trial);
 
+/* The same check with r and v_enc in device memory (r_dev: the list's m
+ * rows; v_enc_dev: n canonical u64), calling no hipMalloc, hipFree or other
+ * allocating routine: its scratch (plan, seed words, per-chunk partial sums,
+ * verdict) is a fixed device array of the library, 3 MiB per GPU, held until
+ * the process ends. No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). Lists of up to 1,048,576 rows (larger:
+ * SVS_ERR_ARG; use svs_verify). Calls on one GPU share the scratch: calls
+ * through one context are serialised, but calls that overlap from different
+ * contexts on the same GPU race. Non-canonical v_enc: SVS_ERR_ARG. */
+int svs_verify_noalloc(svs_index* idx, uint32_t list, const uint64_t* r_dev,
+                       const uint64_t* v_enc_dev, const uint8_t rng_seed[32], uint32_t trials,
+                       int32_t* failed_trial);
+
 /* Client query encoding on the device (encode_query, crypto.rs:508-553):
  * for each query, G = 128 sample_fp draws and T = sample_sparse_pm1 from
  * ChaCha20Rng(q_seed), v_enc = v_q + L G + T. v_q: n_queries x n canonical

@@ -173,6 +173,13 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
         delete ctx;
         return fail(SVS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    // the fixed scratch of svs_verify_noalloc on this device
+    const int rs = noalloc_scratch_init(device);
+    if (rs != SVS_OK) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rs;
+    }
     *out = ctx;
     return SVS_OK;
 }
@@ -2072,6 +2079,122 @@ extern "C" int svs_verify(svs_index* idx, uint32_t list, const uint64_t* r,
     int32_t host[3];
     HIP_TRY(hipMemcpyAsync(host, d_verdict, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (host[2]) return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p");
+    *failed_trial = host[0];
+    return SVS_OK;
+}
+
+// ---------------------------------------------- verification without allocation
+// svs_verify_noalloc: the Protocol-2 check of svs_verify for callers that
+// cannot allocate device memory. Its scratch (the one-item plan, the seed
+// words, the per-chunk partial sums, the verdict and error flags) is this
+// file-scope device array; each GPU holds its own copy at its own address,
+// looked up once per device with hipGetSymbolAddress (noalloc_scratch_init,
+// called by svs_ctx_create) and kept in g_scratch_ptr. Host code passes that
+// pointer, never the array's name, to kernels.
+constexpr uint32_t NOALLOC_MAX_ROWS = 1u << 20;
+constexpr size_t NA_ITEM = 0;        // Item
+constexpr size_t NA_PREFIX = 32;     // chunk_ptr[2], cand_ptr[2]
+constexpr size_t NA_FLAG = 64;       // verify flag of the item
+constexpr size_t NA_SEED = 96;       // 8 seed words
+constexpr size_t NA_VERDICT = 128;   // verdict, unused, rng flag, non-canonical v_enc flag
+constexpr size_t NA_PARTIALS = 256;  // 2 LAMBDA u64 per 16-row chunk
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    NA_PARTIALS + sizeof(uint64_t) * 2 * LAMBDA * (NOALLOC_MAX_ROWS / CHUNK_ROWS);
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_BYTES / sizeof(uint64_t)];
+
+static std::mutex g_scratch_mu;
+static std::vector<uint8_t*> g_scratch_ptr;  // per device; null until looked up
+
+int svs::noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return SVS_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SVS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_ptr[device] = (uint8_t*)p;
+    return SVS_OK;
+}
+
+static uint8_t* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+struct SeedWords {
+    uint32_t w[8];
+};
+
+// Writes the one-item plan, the seed words and cleared flags into the
+// scratch from kernel arguments (no host-to-device copy).
+__global__ void k_verify_noalloc_setup(uint8_t* __restrict__ base, uint32_t list, uint32_t m,
+                                       uint32_t chunks, SeedWords seed) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        Item* it = (Item*)(base + NA_ITEM);
+        it->q = 0;
+        it->list = list;
+        it->probe_pos = 0;
+        it->m = m;
+        uint32_t* pre = (uint32_t*)(base + NA_PREFIX);
+        pre[0] = 0;
+        pre[1] = chunks;
+        pre[2] = 0;
+        pre[3] = m;
+        base[NA_FLAG] = 1;
+    }
+    if (t < 8) ((uint32_t*)(base + NA_SEED))[t] = seed.w[t];
+    if (t < 4) ((int32_t*)(base + NA_VERDICT))[t] = 0;
+}
+
+extern "C" int svs_verify_noalloc(svs_index* idx, uint32_t list, const uint64_t* r_dev,
+                                  const uint64_t* v_enc_dev, const uint8_t rng_seed[32],
+                                  uint32_t trials, int32_t* failed_trial) {
+    int rc = list_ok(idx, list);
+    if (rc != SVS_OK) return rc;
+    if (!r_dev || !v_enc_dev || !rng_seed || !failed_trial)
+        return fail(SVS_ERR_ARG, "null argument");
+    if (trials != (uint32_t)LAMBDA)
+        return fail(SVS_ERR_ARG, "trials = %u; Sec128 uses lambda' = %d", trials, LAMBDA);
+    const uint32_t m = idx->lists[list].m;
+    if (m > NOALLOC_MAX_ROWS)
+        return fail(SVS_ERR_ARG,
+                    "list %u has %u rows; svs_verify_noalloc takes lists of up to %u rows "
+                    "(its fixed scratch); use svs_verify",
+                    list, m, NOALLOC_MAX_ROWS);
+    svs_ctx* ctx = idx->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint8_t* base = noalloc_scratch(ctx->device);
+    if (!base) return fail(SVS_ERR_STATE, "no scratch address for device %d", ctx->device);
+    const uint32_t chunks = (m + CHUNK_ROWS - 1) / CHUNK_ROWS;
+    SeedWords sw;
+    seed_to_key(rng_seed, sw.w);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_verify_noalloc_setup, dim3(1), dim3(64), 0, st, base, list, m, chunks,
+                       sw);
+    int32_t* d_verdict = (int32_t*)(base + NA_VERDICT);
+    hipLaunchKernelGGL(k_check_canonical, dim3(4), dim3(256), 0, st, v_enc_dev, (uint64_t)N,
+                       (uint32_t*)(d_verdict + 3));
+    const Item* d_item = (const Item*)(base + NA_ITEM);
+    const uint32_t* pre = (const uint32_t*)(base + NA_PREFIX);
+    const uint8_t* d_flag = base + NA_FLAG;
+    uint64_t* d_partials = (uint64_t*)(base + NA_PARTIALS);
+    hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, chunks)), dim3(ROW_THREADS), 0, st,
+                       (const ListDev*)idx->lists_dev, d_item, 1u, pre, pre + 2, v_enc_dev, r_dev,
+                       (const uint64_t*)nullptr, (const uint32_t*)(base + NA_SEED), d_flag,
+                       d_partials, (uint32_t*)(d_verdict + 2));
+    hipLaunchKernelGGL(k_verify_finalize, dim3(1), dim3(VF_THREADS), 0, st, d_item, 1u, pre,
+                       d_flag, (const uint64_t*)d_partials, d_verdict);
+    ctx->launches += 4;
+    HIP_TRY(hipGetLastError());
+    int32_t host[4];
+    HIP_TRY(hipMemcpyAsync(host, d_verdict, sizeof(host), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (host[3]) return fail(SVS_ERR_ARG, "v_enc holds a non-canonical value (>= p)");
     if (host[2]) return fail(SVS_ERR_RNG, "a Freivalds u-draw equalled p");
     *failed_trial = host[0];
     return SVS_OK;

@@ -20,6 +20,9 @@ namespace svs {
 int fail(int code, const char* fmt, ...);
 // A process-unique index generation (see svs_index::gen).
 uint64_t next_index_gen();
+// Looks up (once per device, with that device current) the device address
+// of svs_verify_noalloc's fixed scratch array.
+int noalloc_scratch_init(int device);
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \

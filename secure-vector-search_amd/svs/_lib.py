@@ -38,7 +38,7 @@ EXPORTS = [
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
     "svs_ctx_set_planner", "svs_index_set_centroids", "svs_route_batch", "svs_score_raw_batch",
     "svs_ctx_set_graphs", "svs_topk", "svs_index_load_ivf_cache", "svs_cache_fingerprint64",
-    "svs_index_get_centroids", "svs_index_get_subspace",
+    "svs_index_get_centroids", "svs_index_get_subspace", "svs_verify_noalloc",
 ]
 TILE_VARIANTS = 10
 ROUTE_MAX_LISTS = 8192  # device routing sorts a query's distances in LDS
@@ -137,12 +137,13 @@ def lib() -> C.CDLL:
             "svs_index_destroy": ([vp], i32),
             "svs_index_upload_list": ([vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
             "svs_index_build_list": ([vp, u32, u32, vp, vp, vp], i32),
-            "svs_index_download_list": ([vp, u32, vp], i32),
+            "svs_index_download_list": ([vp, u32, vp, vp, vp], i32),
             "svs_index_list_rows": ([vp, u32], u32),
             "svs_compute_products": ([vp, u32, vp, vp], i32),
             "svs_compute_products_dev": ([vp, u32, vp, vp], i32),
             "svs_decode_dense_terms": ([vp, u32, vp, vp, vp], i32),
             "svs_verify": ([vp, u32, vp, vp, vp, u32, C.POINTER(C.c_int32)], i32),
+            "svs_verify_noalloc": ([vp, u32, vp, vp, vp, u32, C.POINTER(C.c_int32)], i32),
             "svs_decode": ([vp, u32, vp, vp, vp, vp, vp, u32, vp], i32),
             "svs_score_batch": ([vp, C.POINTER(QueryBatch), i32], i32),
             "svs_score_batch_dev": ([vp, C.POINTER(QueryBatch), i32], i32),

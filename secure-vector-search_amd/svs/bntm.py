@@ -403,6 +403,19 @@ class GpuIndex:
             return -1
         return ft.value
 
+    def verify_noalloc(self, list_id, r_dev_ptr: int, v_enc_dev_ptr: int, rng_seed,
+                       trials=L.LAMBDA) -> int:
+        """svs_verify_noalloc: r and v_enc already in device memory (e.g.
+        DeviceArray.ptr()); no device allocation. -1 on accept, else the
+        first failing trial."""
+        ft = C.c_int32(0)
+        rc = check(L.lib().svs_verify_noalloc(self.h, list_id, C.c_void_p(r_dev_ptr),
+                                              C.c_void_p(v_enc_dev_ptr),
+                                              ptr(seed_array(rng_seed)), trials, C.byref(ft)))
+        if rc == L.SVS_NONE:
+            return -1
+        return ft.value
+
     def decode(self, list_id, r, v_enc, g, t_idx, t_sign):
         m = self.rows(list_id)
         out = np.zeros(max(m, 1), np.uint64)
