@@ -888,6 +888,30 @@ static void launch_gemm_vb(hipStream_t st, uint32_t blocks, const TileArgs& a,
                        a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, vecs, qc, out);
 }
 
+// The batched server product on the byte-plane kernel for the variant's
+// query group (a plan tile holds up to QG queries of one list). The grid is
+// the resident blocks (one per CU and wave slot); units are dealt as
+// described at k_prod_planes.
+template <int QW, int PF, int OCC>
+static void launch_planes_v(svs_ctx* ctx, hipStream_t st, uint64_t cap_tiles, const TileArgs& a,
+                            const int8_t* qplanes, const uint64_t* qc, uint64_t* out) {
+    const uint64_t units = 64 * ((std::max<uint64_t>(cap_tiles, 1) + 3) / 4);
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((units + 3) / 4, (uint64_t)ctx->n_cu * OCC * ctx->tile_grid));
+    hipLaunchKernelGGL((k_prod_planes<QW, PF, OCC>), dim3(blocks), dim3(256), 0, st, a.lists,
+                       a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc, out);
+}
+
+static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tiles,
+                          const TileArgs& a, const int8_t* qplanes, const uint64_t* qc,
+                          uint64_t* out) {
+    switch (kTileVariants[v].qg) {
+        case 16: launch_planes_v<16, 2, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        case 4: launch_planes_v<4, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        default: launch_planes_v<8, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+    }
+}
+
 // The product (SRC 0, KE = n) or dense decode (SRC 1, KE = 2 n1) tiles.
 template <int KE, int SRC>
 static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& a,
@@ -1193,10 +1217,14 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
                                v.gitems, d_items, d_cand_ptr, (const uint64_t*)d_venc,
                                (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
         } else {
-            const uint32_t blocks = tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ);
+            // byte planes of the queries, then the plane-by-plane product
+            SVS_TRY(sl.q_pplanes.ensure(ctx, (size_t)nq * 8 * N));
+            hipLaunchKernelGGL(k_query_planes, dim3(nq), dim3(256), 0, st,
+                               (const uint64_t*)d_venc, (int8_t*)sl.q_pplanes.ptr);
             const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
-            launch_gemm<N, 0>(variant, st, blocks, ta, (const uint64_t*)d_venc,
-                              (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+            launch_planes(ctx, variant, st, v.cap_tiles, ta, (const int8_t*)sl.q_pplanes.ptr,
+                          (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+            ctx->launches++;
         }
         ctx->launches += 2;
         HIP_TRY(hipGetLastError());
@@ -1217,10 +1245,12 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA *
                                             std::max<uint64_t>(v.cap_chunks, 1)));
         // The check's right-hand side sum_i u_t[i] (M_enc[i] . v_enc) needs
-        // the row products again: on the MFMA path they are recomputed by
-        // the product tiles over the verified items only, independently of
-        // the r the server returned; otherwise k_verify_chunks recomputes
-        // them row by row.
+        // the row products again, computed independently of the r the server
+        // returned: on the MFMA path by the window-formulation tiles
+        // (k_fp_gemm_tiles: other operand layouts, other limb products, other
+        // epilogue than the byte-plane kernel that produced r) over the
+        // verified items only; otherwise k_verify_chunks recomputes them row
+        // by row on the VALU.
         const uint64_t* r_check = nullptr;
         if (kernel == KERNEL_MFMA && v.cap_vtiles) {
             SVS_TRY(sl.rv.ensure(ctx, sizeof(uint64_t) * nc1));
@@ -1350,6 +1380,7 @@ int svs::view_reserve(svs_ctx* ctx, svs_slot& sl, const PlanView& v, uint32_t k)
     SVS_TRY(sl.partials.ensure(ctx, sizeof(uint64_t) * 2 * LAMBDA *
                                         std::max<uint64_t>(v.cap_chunks, 1)));
     SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+    SVS_TRY(sl.q_pplanes.ensure(ctx, (size_t)nq * 8 * N));
     SVS_TRY(sl.out.ensure(ctx, (sizeof(uint32_t) + sizeof(int64_t)) * (size_t)nq * k +
                                    sizeof(uint32_t) * nq + 64));
     if (k <= (uint32_t)TOPK_LANE)
@@ -1461,6 +1492,7 @@ int svs::reserve_slot(svs_ctx* ctx, svs_slot& sl, uint32_t nq, uint32_t n_items,
     SVS_TRY(sl.q_tcnt.ensure(ctx, sizeof(uint32_t) * nq));
     SVS_TRY(sl.q_vseed.ensure(ctx, sizeof(uint32_t) * 8 * nq));
     SVS_TRY(sl.q_vplanes.ensure(ctx, (size_t)nq * 8 * N));
+    SVS_TRY(sl.q_pplanes.ensure(ctx, (size_t)nq * 8 * N));
     SVS_TRY(sl.q_corr.ensure(ctx, sizeof(uint64_t) * (3 + MT_BLOCKS) * nq));
     SVS_TRY(sl.r.ensure(ctx, sizeof(uint64_t) * nc));
     SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc));

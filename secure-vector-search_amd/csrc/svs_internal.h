@@ -96,7 +96,8 @@ struct svs_prof {
 struct svs_slot {
     hipStream_t stream = nullptr;
     svs::DevBuf items, prefixes, flags;
-    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes;
+    svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes,
+        q_pplanes;
     svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part, rv;
     // inputs of device-planned batches: probes + per-query flags, raw f32
     // queries, routing distances
@@ -114,7 +115,7 @@ struct svs_slot {
     uint64_t graph_clock = 0, graph_launches = 0, graph_captures = 0;
     std::vector<svs::DevBuf*> slot_scratch() {
         return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
-                &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &r, &scores, &partials, &verdict,
+                &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &q_pplanes, &r, &scores, &partials, &verdict,
                 &out, &mtiles, &q_corr, &dense, &topk_part, &rv, &q_probes, &q_raw,
                 &route_dist};
     }

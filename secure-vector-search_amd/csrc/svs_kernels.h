@@ -1670,6 +1670,251 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
     return *(gv4i*)(p);
 }
 
+// ---------------------------------------------------------------------------
+// Byte plane by byte plane F_p products on int8 MFMA (the batched server
+// product). With signed byte limbs s_a = byte_a - 128 of a row element and
+// t_b of a query element (identity (*) above),
+//   sum_k M'_k V'_k = sum_{a, b < 8} 256^(a+b) C_ab,  C_ab = sum_k s_a[k] t_b[k],
+// |C_ab| <= 1024 * 2^14 = 2^24. One v_mfma_i32_16x16x64_i8 contracts 64
+// elements of plane a of 16 rows (A) with planes 0..7 of two queries (B
+// columns (query jj, plane b)), so every multiply-accumulate is a useful
+// limb product (64 per element pair; the window formulation of
+// k_fp_gemm_tiles spends 128).
+//
+// A comes from the row-major M_enc with no layout change: lane (row r =
+// lane % 16, group g = lane / 16) loads the 16 elements it contributes at
+// step s (elements 64 s + 8 j + 2 g + t, j < 8, t < 2; eight 16-byte loads,
+// 128 contiguous bytes per row and step over the wave's four groups) and
+// transposes them into 8 planes of 16 bytes with v_perm_b32 (eight per 4 x 4
+// byte block). B is the per-batch query-plane array (k_query_planes) in the
+// same element order, so a lane's B operand is one 16-byte load. No LDS, no
+// barrier: each wave streams its own 16-row units, with a PF-step register
+// ring that runs on into the next unit's first steps.
+//
+// Work units: unit u is 16-row block (u / 4) % 16 of plan tile
+// 4 (u / 64) + u % 4 (a 256-row tile of one list and up to QW of its
+// queries); wave w of worker b takes units 4 b + w, + 4 G, ...; units past
+// the list's end are skipped. The four waves of a block thus stream the same
+// 16 rows for four consecutive tiles at once (the query groups of one row
+// block, when the list has more queries than a tile holds); the next row
+// blocks of those tiles run on the next blocks of the same XCD.
+
+// Query planes for k_prod_planes: per query 16 steps x 4 groups x 8 planes x
+// 16 bytes; byte n of (step s, group g, plane b) is byte b of v[e] ^ 0x80
+// with e = 64 s + 8 (n / 2) + 2 g + n % 2.
+__global__ void __launch_bounds__(256)
+k_query_planes(const uint64_t* __restrict__ vecs, int8_t* __restrict__ qplanes) {
+    const uint32_t q = blockIdx.x;
+    for (int e = threadIdx.x; e < N; e += 256) {
+        const uint64_t v = vecs[(size_t)q * N + e];
+        const int s = e / 64, o = e % 64;
+        const int j = o / 8, rem = o % 8, g = rem / 2, n = 2 * j + rem % 2;
+        int8_t* dst = qplanes + (size_t)q * 8 * N + ((s * 4 + g) * 8) * 16 + n;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) dst[16 * b] = (int8_t)(((v >> (8 * b)) & 0xFF) ^ 0x80);
+    }
+}
+
+__device__ __forceinline__ uint32_t bperm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// The 16 elements of a lane's step (loads L[j] = elements 2j, 2j + 1 of its
+// list, low and high dwords) -> 8 planes x 16 bytes, each byte ^ 0x80 (the
+// signed limb). Plane 4h + a, dword i holds byte a of dword h of elements
+// 4i .. 4i + 3.
+__device__ __forceinline__ void planes_of(const v4i (&L)[8], v4i (&pl)[8]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t y0 = (uint32_t)L[2 * i][h] ^ 0x80808080u;
+            const uint32_t y1 = (uint32_t)L[2 * i][2 + h] ^ 0x80808080u;
+            const uint32_t y2 = (uint32_t)L[2 * i + 1][h] ^ 0x80808080u;
+            const uint32_t y3 = (uint32_t)L[2 * i + 1][2 + h] ^ 0x80808080u;
+            const uint32_t p01l = bperm(y1, y0, 0x05010400u), p01h = bperm(y1, y0, 0x07030602u);
+            const uint32_t p23l = bperm(y3, y2, 0x05010400u), p23h = bperm(y3, y2, 0x07030602u);
+            pl[4 * h + 0][i] = (int)bperm(p23l, p01l, 0x05040100u);
+            pl[4 * h + 1][i] = (int)bperm(p23l, p01l, 0x07060302u);
+            pl[4 * h + 2][i] = (int)bperm(p23h, p01h, 0x05040100u);
+            pl[4 * h + 3][i] = (int)bperm(p23h, p01h, 0x07060302u);
+        }
+}
+
+template <int QW, int PF, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
+k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+              const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
+              const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+              const int8_t* __restrict__ qplanes, const uint64_t* __restrict__ qcorr,
+              uint64_t* __restrict__ out) {
+    constexpr int KS = N / 64;  // 16 steps of 64 elements
+    constexpr int NB = QW / 2;  // B tiles of two queries
+    constexpr uint32_t QBYTES = 8 * N;
+    static_assert(QW == 4 || QW == 8 || QW == 16, "query group");
+    static_assert(KS % PF == 0 && PF >= 1, "the ring must tile the steps");
+    // the wave index as a wave-uniform value: unit indices, tile fields and
+    // the per-unit metadata then live in scalar registers and come from
+    // scalar loads
+    const int lane = threadIdx.x % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
+    const uint32_t n_tiles = *n_tiles_p;
+    const uint32_t n_units = 64u * ((n_tiles + 3) / 4);
+    const uint32_t ustride = 4u * gridDim.x;
+
+    // the first non-empty unit at or after x (stepping by ustride), or n_units
+    struct Unit {
+        uint32_t u, m, row0;
+        MTile tile;
+        const uint64_t* menc;
+        const uint64_t* rowcorr;
+    };
+    auto find = [&](uint32_t x) -> Unit {
+        Unit r;
+        r.u = n_units;
+        for (; x < n_units; x += ustride) {
+            // unit x: 16-row block (x / 4) % 16 of tile 4 (x / 64) + x % 4
+            const uint32_t ti = ((x >> 6) << 2) | (x & 3u);
+            if (ti >= n_tiles) continue;
+            const MTile t = tiles[ti];
+            const uint32_t m = lists[t.list].m;
+            const uint32_t row0 = t.row0 + 16u * ((x >> 2) & 15u);
+            if (row0 < m) {
+                r.u = x;
+                r.m = m;
+                r.row0 = row0;
+                r.tile = t;
+                r.menc = lists[t.list].menc;
+                r.rowcorr = lists[t.list].rowcorr;
+                return r;
+            }
+        }
+        return r;
+    };
+    // this lane's A row pointer and B byte offsets for a unit. The query
+    // indices are read with wave-uniform addresses (scalar loads): a vector
+    // load here would make its wait drain the ring of A loads in flight.
+    auto lane_ptrs = [&](const Unit& x, const uint64_t*& ap, uint32_t (&qb)[NB]) {
+        const uint32_t row = min(x.row0 + (uint32_t)col, x.m - 1);
+        ap = x.menc + (size_t)row * N + 2 * g;
+        const uint32_t last = x.tile.gn - 1;
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            const uint32_t q0 =
+                items[gitems[x.tile.g0 + min((uint32_t)(2 * jb), last)]].q;
+            const uint32_t q1 =
+                items[gitems[x.tile.g0 + min((uint32_t)(2 * jb + 1), last)]].q;
+            qb[jb] = (jj ? q1 : q0) * QBYTES + (uint32_t)(g * 8 + bpl) * 16u;
+        }
+    };
+    v4i ra[PF][8], rb[PF][NB];
+    auto issue = [&](int slot, const uint64_t* ap, const uint32_t (&qb)[NB], int s) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ra[slot][j] = load16(ap + 64 * s + 8 * j);
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) rb[slot][jb] = load16b(qplanes + qb[jb] + 512u * s);
+    };
+
+    Unit cur = find(4u * xcd_worker() + wave);
+    if (cur.u >= n_units) return;
+    const uint64_t* a_cur;
+    uint32_t qb_cur[NB];
+    lane_ptrs(cur, a_cur, qb_cur);
+#pragma unroll
+    for (int d = 0; d < PF; ++d) issue(d, a_cur, qb_cur, d);
+    for (;;) {
+        const Unit nxt = find(cur.u + ustride);
+        const bool has_next = nxt.u < n_units;
+        const uint64_t* a_nxt = a_cur;
+        uint32_t qb_nxt[NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) qb_nxt[jb] = qb_cur[jb];
+        if (has_next) lane_ptrs(nxt, a_nxt, qb_nxt);
+        const int gn = (int)cur.tile.gn;
+        // epilogue input, loaded ahead of the steps: this lane writes row
+        // wrow (after the plane reduction) for query 2 jb + jj
+        const uint32_t wrow = cur.row0 + 4u * g + (uint32_t)(bpl & 3);
+        const uint64_t rcorr = cur.rowcorr[min(wrow, cur.m - 1)];
+        // keep this load here: sunk to its use it would be the youngest
+        // load at the epilogue, whose wait would then drain the ring
+        __builtin_amdgcn_sched_barrier(0);
+        v4i acc[8][NB];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) acc[a][jb] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int slot = s % PF;
+            v4i pl[8];
+            planes_of(ra[slot], pl);
+            v4i bq[NB];
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) bq[jb] = rb[slot][jb];
+            // refill the slot: this unit's step s + PF, or the next unit's
+            // first steps (the last unit re-loads its own final step)
+            if (s + PF < KS) {
+                issue(slot, a_cur, qb_cur, s + PF);
+            } else {
+                issue(slot, has_next ? a_nxt : a_cur, has_next ? qb_nxt : qb_cur,
+                      has_next ? s + PF - KS : KS - 1);
+            }
+            // every B tile, present or not (absent columns repeat the last
+            // query and are not written): a branch around MFMAs splits the
+            // step into blocks whose waits drain the ring
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb)
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(pl[a], bq[jb], acc[a][jb],
+                                                                         0, 0, 0);
+            // one step per scheduling region: the scheduler would otherwise
+            // sink each step's loads next to their use PF steps later
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Epilogue per B tile: lane (g, jj, b) holds C_ab for rows 4 g + i;
+        // it weights them by 256^(a+b) mod p, then the eight lanes of its
+        // (g, jj) reduce-scatter over b, leaving row 4 g + (b & 3) whole.
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            uint64_t v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t lo = (int64_t)acc[0][jb][i] + ((int64_t)acc[1][jb][i] << 8) +
+                                   ((int64_t)acc[2][jb][i] << 16) + ((int64_t)acc[3][jb][i] << 24);
+                const int64_t hi = (int64_t)acc[4][jb][i] + ((int64_t)acc[5][jb][i] << 8) +
+                                   ((int64_t)acc[6][jb][i] << 16) + ((int64_t)acc[7][jb][i] << 24);
+                const uint64_t x = add_mod(i64_to_fp(lo), mul_pow2(i64_to_fp(hi), 32));
+                v[i] = mul_pow2(x, 8 * bpl);
+            }
+            const bool b1 = (bpl & 2) != 0, b0 = (bpl & 1) != 0;
+            uint64_t k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
+            const uint64_t s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
+            k0 = add_mod(k0, shfl_xor_u64(s0, 2));
+            k1 = add_mod(k1, shfl_xor_u64(s1, 2));
+            uint64_t kp = b0 ? k1 : k0;
+            const uint64_t sd = b0 ? k0 : k1;
+            kp = add_mod(kp, shfl_xor_u64(sd, 1));
+            kp = add_mod(kp, shfl_xor_u64(kp, 4));
+            // the tile's two queries of this B tile (scalar loads)
+            const uint32_t last = cur.tile.gn - 1;
+            const uint32_t it0 = gitems[cur.tile.g0 + min((uint32_t)(2 * jb), last)];
+            const uint32_t it1 = gitems[cur.tile.g0 + min((uint32_t)(2 * jb + 1), last)];
+            const uint32_t cb0 = cand_ptr[it0], cb1 = cand_ptr[it1];
+            const uint64_t qt0 = qcorr[items[it0].q], qt1 = qcorr[items[it1].q];
+            const int j = 2 * jb + jj;
+            if (bpl < 4 && j < gn && wrow < cur.m)
+                out[(jj ? cb1 : cb0) + wrow] = add_mod(add_mod(kp, rcorr), jj ? qt1 : qt0);
+        }
+        if (!has_next) break;
+        cur = nxt;
+        a_cur = a_nxt;
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) qb_cur[jb] = qb_nxt[jb];
+    }
+}
+
 // Block-cooperative MFMA decode over the product's tiles (crypto.rs:901-940):
 //   Mv = r - (AL g + H lt) - M T - S v_enc (mod p), score = fp_to_signed(Mv).
 // The dense term comes in precomputed (indexed like r). M T and S v_enc are
