@@ -58,6 +58,7 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.or_domain_separate.argtypes = [u8p, C.c_char_p, sz, u8p]
     L.or_sparse_threshold.restype = C.c_uint32
     L.or_quantise_vector.argtypes = [f32p, sz, sz, C.c_uint64, u64p]
+    L.or_quantise_rows.argtypes = [f32p, sz, sz, sz, C.c_uint64, u64p]
     L.or_fp_to_signed.restype = C.c_int64
     L.or_fp_to_signed.argtypes = [C.c_uint64]
     L.or_generate_subspace.argtypes = [u8p, sz, sz, u64p]
@@ -141,6 +142,16 @@ def quantise_vector(v, n=N, q=1 << 20) -> np.ndarray:
     out = np.zeros(n, np.uint64)
     lib().or_quantise_vector(v, v.size, n, q, out)
     return out
+
+
+def quantise_rows(vectors, n=N, q=1 << 20) -> np.ndarray:
+    """quantise_vector of every row of a (rows, d) float32 matrix."""
+    v = np.ascontiguousarray(vectors, np.float32)
+    rows, d = v.shape
+    out = np.zeros((max(rows, 1), n), np.uint64)
+    if rows:
+        lib().or_quantise_rows(v.reshape(-1), rows, d, n, q, out.reshape(-1))
+    return out[:rows]
 
 
 def fp_to_signed(s: int) -> int:

@@ -171,6 +171,12 @@ void or_quantise_vector(const float* v, size_t d, size_t n, uint64_t q, uint64_t
     }
 }
 
+/* quantise_vector over `rows` row-major vectors of d floats each. */
+void or_quantise_rows(const float* v, size_t rows, size_t d, size_t n, uint64_t q,
+                      uint64_t* out) {
+    for (size_t r = 0; r < rows; ++r) or_quantise_vector(v + r * d, d, n, q, out + r * n);
+}
+
 int64_t or_fp_to_signed(uint64_t s) {
     return s < OR_P / 2 ? (int64_t)s : (int64_t)s - (int64_t)OR_P;
 }

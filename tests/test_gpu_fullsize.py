@@ -1,6 +1,7 @@
 """Full-size workloads through size-independent properties.
 
-* BN-IVF at N = 1,000,000, d = 768, 1024 Zipf(1.1) lists, nprobe = 32: the
+* BN-IVF at N = 1,000,000, d = 768, 1024 Zipf(1.1) lists, nprobe 32 (and 8,
+  128, and a 512-query batch of mixed nprobe): the
   zero-noise invariant of decode (crypto.rs:1232-1266) — every returned
   score equals fp_to_signed(M_q . v_q) recomputed from the plaintext
   quantised vectors — plus the batch == per-query invariant
@@ -89,6 +90,63 @@ def test_fullsize_ivf_zero_noise_and_batch_invariants(ctx, big):
     for q in (3, 128, 200):
         for i in range(10):
             assert scores[q, i] == _plain_score(corpus, v_q[q], int(ids[q, i])), (q, i)
+
+
+def _zero_noise(corpus, v_q, ids, scores, count, queries):
+    for q in queries:
+        for i in range(int(count[q])):
+            assert scores[q, i] == _plain_score(corpus, v_q[q], int(ids[q, i])), (q, i)
+
+
+@pytest.mark.parametrize("nprobe", [8, 128])
+def test_fullsize_ivf_nprobe_8_and_128(ctx, big, nprobe):
+    """The other nprobe of the batched configuration at N = 1M: a raw batch
+    of 256 queries (half verified) against the host router's probe lists,
+    the plain one-query path and the zero-noise invariant."""
+    import svs
+    from svs import data
+    idx, corpus = big["idx"], big["corpus"]
+    qs = data.make_queries(corpus, 256, seed=90 + nprobe)
+    rng = np.random.default_rng(91 + nprobe)
+    seeds = rng.integers(0, 256, (256, 32), dtype=np.uint8)
+    verify = (rng.random(256) < 0.5).astype(np.uint8)
+    say("raw batch of 256, nprobe", nprobe)
+    ids, scores, count, status, probes = idx.score_raw_batch(qs, seeds, nprobe, verify, 10)
+    assert (status == 0).all() and (count == 10).all()
+    hp = svs.probe_route(corpus.centres, qs, nprobe)
+    assert np.array_equal(probes, hp.reshape(-1))
+    v_q = svs.quantise(qs)
+    for q in (0, 101, 255):
+        say("plain query", q)
+        p_out = idx.score_batch(v_q[q:q + 1], seeds[q:q + 1],
+                                np.array([0, nprobe], np.uint32), hp[q], verify[q:q + 1], 10, 0)
+        assert p_out[0][0].tolist() == ids[q].tolist()
+        assert p_out[1][0].tolist() == scores[q].tolist()
+    _zero_noise(corpus, v_q, ids, scores, count, (5, 77, 200))
+
+
+def test_fullsize_ivf_mixed_nprobe_batch_512(ctx, big):
+    """One 512-query batch with per-query nprobe in {8, 32, 128} (the
+    batched configuration's mixed-nprobe launch): every query equals itself
+    scored alone, and the zero-noise invariant holds."""
+    import svs
+    from svs import data
+    idx, corpus = big["idx"], big["corpus"]
+    rng = np.random.default_rng(92)
+    qs = data.make_queries(corpus, 512, seed=93)
+    npr = rng.choice([8, 32, 128], 512).astype(np.uint32)
+    seeds = rng.integers(0, 256, (512, 32), dtype=np.uint8)
+    verify = (rng.random(512) < 0.5).astype(np.uint8)
+    say("mixed-nprobe raw batch of 512")
+    ids, scores, count, status, probes = idx.score_raw_batch(qs, seeds, npr, verify, 10)
+    assert (status == 0).all() and (count == 10).all()
+    assert probes.size == int(npr.sum())
+    for q in (0, 17, 300, 511):
+        one = idx.score_raw_batch(qs[q:q + 1], seeds[q:q + 1], int(npr[q]), verify[q:q + 1],
+                                  10, mode=0)
+        assert one[0][0].tolist() == ids[q].tolist(), q
+        assert one[1][0].tolist() == scores[q].tolist(), q
+    _zero_noise(corpus, svs.quantise(qs), ids, scores, count, (3, 250, 499))
 
 
 def test_fullsize_products_of_sampled_rows(ctx, big):

@@ -50,6 +50,33 @@ def test_stream_conventions_cross_implementation(golden):
         assert O.domain_separate(seed, b"protocol2").tobytes().hex() == st["protocol2"]
 
 
+def test_chacha20rng_stream_against_rfc7539(golden):
+    """ChaCha20Rng::from_seed (rand_chacha 0.10, the reference's workspace
+    dependency) takes the 32-byte seed as the RFC 7539 key and emits block 0,
+    1, 2, ... of the zero-nonce stream, 16 u32 words per block; next_u64 is
+    (word 2i+1 << 32) | word 2i (rand_core's BlockRng), and random::<u8>()
+    is next_u32() truncated (rand's StandardUniform). The same construction
+    as rand_chacha's own test_chacha_true_values_{a,b,c}: the RFC A.1 keys
+    used as seeds, earlier blocks skipped. Here it goes through the oracle's
+    sample_fp (crypto.rs:155-162) and derive_h_seed (ivf.rs:258-264), so
+    seed byte order, block sequencing and word order are all checked
+    against the published keystreams."""
+    for tv in golden["chacha20_rfc7539_a1"]:
+        seed = bytes.fromhex(tv["key"])
+        words = struct.unpack("<16I", bytes.fromhex(tv["keystream"]))
+        c = tv["counter"]
+        draws = O.u_draws(seed, 8 * (c + 1))[8 * c:]
+        want = [((words[2 * i + 1] << 32) | words[2 * i]) & P for i in range(8)]
+        assert all(w < P for w in want)  # no rejection inside these blocks
+        assert draws == want, tv["key"]
+    # derive_h_seed(0, 0) seeds ChaCha20Rng with 32 zero bytes: its 32 bytes
+    # are the low bytes of the zero key's blocks 0 and 1
+    blocks = {tv["counter"]: struct.unpack("<16I", bytes.fromhex(tv["keystream"]))
+              for tv in golden["chacha20_rfc7539_a1"] if int(tv["key"], 16) == 0}
+    want = bytes(w & 0xFF for w in blocks[0] + blocks[1])
+    assert O.derive_h_seed(0, 0).tobytes() == want
+
+
 # --------------------------------------------------------------------- field
 def test_field_kat(golden):
     for row in golden["field_kat"]:
