@@ -453,6 +453,25 @@ int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const uint64_t* q
  * the resident-input path, without the partials' trip over PCIe. */
 int svs_emvp_compute_products_dev(svs_emvp* e, uint32_t n_queries, const uint64_t* q_hat_dev,
                                   uint64_t* partials_dev);
+/* decode_scores (scorer-emvp/src/crypto.rs:435-453; the reference runs it
+ * on the client CPU, scorer-emvp/src/lib.rs:549,567): for each query q and
+ * row i, scores[q][i] = decode_field(sum_si partials[q][si * m + i] -
+ * R[i] . q_hat[q]) as f32, R[i] replayed on the device from row_rng(r_seed,
+ * i) (crypto.rs:100-107). q_hat and partials must be canonical (checked:
+ * SVS_ERR_ARG). Host buffers; SVS_NONE for an empty cluster. */
+int svs_emvp_decode_scores(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                           const uint64_t* q_hat_host, const uint64_t* partials_host,
+                           float* scores_host);
+/* Same with device pointers (not checked), asynchronous on the context
+ * stream. */
+int svs_emvp_decode_scores_dev(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                               const uint64_t* q_hat_dev, const uint64_t* partials_dev,
+                               float* scores_dev);
+/* compute_products then decode_scores for n_queries queries of this cluster
+ * (EmvpScorer's per-cluster work, scorer-emvp/src/lib.rs:536-567): q_hat in,
+ * f32 scores (n_queries x m) out; the partials never leave the device. */
+int svs_emvp_score(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                   const uint64_t* q_hat_host, float* scores_host);
 
 /* ---- Tiptoe SimplePIR answer (same kernel family) --------------------------
  * Resident database M_db (rows x cols, row-major) of u32 (elem_bytes 4) or

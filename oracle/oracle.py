@@ -26,6 +26,7 @@ u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -86,11 +87,25 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.or_emvp_fp_reduce.argtypes = [C.c_uint64, C.c_uint64]
     L.or_emvp_compute_products.argtypes = [u64p, sz, u64p, u64p]
     L.or_emvp_compute_products_batch.argtypes = [u64p, sz, u64p, sz, u64p]
+    L.or_emvp_quantize.restype = C.c_uint64
+    L.or_emvp_quantize.argtypes = [C.c_float]
+    L.or_emvp_decode_field.restype = C.c_float
+    L.or_emvp_decode_field.argtypes = [C.c_uint64]
+    L.or_emvp_generate_d.argtypes = [u8p, u64p]
+    L.or_emvp_encrypt_cluster.argtypes = [u64p, u8p, f32p, sz, sz, u64p]
+    L.or_emvp_encode_query.argtypes = [u64p, f64p, u64p, u64p]
+    L.or_emvp_decode_scores.argtypes = [sz, u8p, u64p, u64p, f32p]
     L.or_tiptoe_server_answer_u64.argtypes = [u64p, sz, sz, u64p, u64p]
     L.or_tiptoe_server_answer_u32.argtypes = [u32p, sz, sz, u32p, u32p]
     L.or_bntm_flat_score.restype = C.c_int
     L.or_bntm_flat_score.argtypes = [u64p, u64p, u64p, u64p, u32p, u32p, i8p, u64p, sz, u64p,
                                      u8p, C.c_int, sz, u32p, i64p, C.POINTER(C.c_size_t)]
+    L.or_has_avx512f.restype = C.c_int
+    L.or_matvec_avx512.restype = C.c_int
+    L.or_matvec_avx512.argtypes = [u64p, sz, sz, u64p, u64p]
+    L.or_ivf_rows_mt.restype = C.c_int
+    L.or_ivf_rows_mt.argtypes = [C.c_void_p, sz, sz, sz, u64p, u64p, u64p, u32p, i8p, sz,
+                                 C.c_int, C.c_int, i64p]
     return L
 
 
@@ -294,6 +309,53 @@ def emvp_compute_products_batch(m_hat, m, q_hats) -> np.ndarray:
     return out.reshape(q_hats.shape[0], EMVP_S * max(m, 1))[:, : EMVP_S * m]
 
 
+EMVP_N, EMVP_S, EMVP_B, EMVP_ELL0, EMVP_K = 1292, 76, 17, 1024, 268
+
+
+def emvp_generate_d(key_seed) -> np.ndarray:
+    """generate_d (scorer-emvp/src/crypto.rs:72-97), ell0 x n."""
+    d = np.zeros(EMVP_ELL0 * EMVP_N, np.uint64)
+    lib().or_emvp_generate_d(seed32(key_seed), d)
+    return d.reshape(EMVP_ELL0, EMVP_N)
+
+
+def emvp_encrypt_cluster(d_mat, r_seed, vectors) -> np.ndarray:
+    """encrypt_cluster (crypto.rs:124-163): m x n M_hat."""
+    v = np.ascontiguousarray(vectors, np.float32)
+    m, d = v.shape
+    out = np.zeros(max(m, 1) * EMVP_N, np.uint64)
+    if m:
+        lib().or_emvp_encrypt_cluster(_u64(d_mat).reshape(-1), seed32(r_seed), v.reshape(-1), m,
+                                      d, out)
+    return out[: m * EMVP_N].reshape(m, EMVP_N)
+
+
+def emvp_encode_query(d_mat, query, u) -> np.ndarray:
+    """encode_query (crypto.rs:171-190): q_hat = (q - D'u | u)."""
+    out = np.zeros(EMVP_N, np.uint64)
+    lib().or_emvp_encode_query(_u64(d_mat).reshape(-1), np.ascontiguousarray(query, np.float64),
+                               _u64(u), out)
+    return out
+
+
+def emvp_decode_scores(m, r_seed, q_hat, partials) -> np.ndarray:
+    """decode_scores (crypto.rs:435-453): f32 score per row."""
+    out = np.zeros(max(m, 1), np.float32)
+    if m:
+        lib().or_emvp_decode_scores(m, seed32(r_seed), _u64(q_hat), _u64(partials).reshape(-1),
+                                    out)
+    return out[:m]
+
+
+def emvp_quantize(v) -> int:
+    """quantize (scorer-emvp/src/crypto.rs:47-52)."""
+    return lib().or_emvp_quantize(float(v))
+
+
+def emvp_decode_field(v: int) -> float:
+    return lib().or_emvp_decode_field(v)
+
+
 def tiptoe_answer_u64(m_db, r, c, ct) -> np.ndarray:
     out = np.zeros(r, np.uint64)
     lib().or_tiptoe_server_answer_u64(_u64(m_db).reshape(-1), r, c, _u64(ct), out)
@@ -379,3 +441,69 @@ class VerificationFailedOracle(Exception):
     def __init__(self, cluster, trial):
         super().__init__(f"cluster {cluster} trial {trial}")
         self.cluster, self.trial = cluster, trial
+
+
+# ------------------------------------------------------- timed CPU baseline
+def has_avx512f() -> bool:
+    return bool(lib().or_has_avx512f())
+
+
+def matvec_avx512(mat, rows, cols, vec) -> np.ndarray:
+    """mersenne_matvec_avx512 (crypto.rs:643-739); None without AVX-512F."""
+    out = np.zeros(max(rows, 1), np.uint64)
+    if rows and lib().or_matvec_avx512(_u64(mat).reshape(-1), rows, cols, _u64(vec), out) != 0:
+        return None
+    return out[:rows]
+
+
+class _ClusterView(C.Structure):
+    _fields_ = [("m_enc", C.c_void_p), ("a_l", C.c_void_p), ("m_plain", C.c_void_p),
+                ("h", C.c_void_p), ("s_rowptr", C.c_void_p), ("s_idx", C.c_void_p),
+                ("s_sign", C.c_void_p), ("m", C.c_uint64)]
+
+
+class CpuIvf:
+    """BnTmIvfScorer::score (ivf.rs:746-822) as the reference runs it in its
+    closed-loop B=1 protocol (eval.rs:2338-2344): one query at a time; route,
+    quantise and encode on one thread, then compute_products and
+    decode_scores over the probed lists with the rows split across
+    `nthreads` threads (rayon's row-parallel matvec, crypto.rs:609-625).
+    Verification off. `clusters` maps list id -> cluster dict (the keys of
+    build_cluster); lists are viewed, not copied."""
+
+    def __init__(self, clusters, ids_per_list, centroids, l):
+        self.clusters, self.ids, self.l = clusters, ids_per_list, _u64(l)
+        self.centroids = np.ascontiguousarray(centroids, np.float32)
+        self._views = {}
+
+    def _view(self, ci):
+        v = self._views.get(ci)
+        if v is None:
+            cl = self.clusters[ci]
+            keep = [_u64(cl["m_enc"]), _u64(cl["a_l"]), _u64(cl["m_plain"]), _u64(cl["h"]),
+                    np.ascontiguousarray(cl["s_rowptr"], np.uint32),
+                    np.ascontiguousarray(cl["s_idx"], np.uint32) if cl["s_idx"].size
+                    else np.zeros(1, np.uint32),
+                    np.ascontiguousarray(cl["s_sign"], np.int8) if cl["s_sign"].size
+                    else np.zeros(1, np.int8)]
+            v = (_ClusterView(*[a.ctypes.data for a in keep], cl["m"]), keep)
+            self._views[ci] = v
+        return v[0]
+
+    def score(self, query, nprobe, q_seed, k, nthreads, simd=False):
+        probes = probe_route(self.centroids, query, nprobe)
+        v_q = quantise_vector(query)
+        v_enc, g, t_idx, t_sign = encode_query(v_q, self.l, q_seed)
+        lt = matvec_transpose(self.l, N, N1, v_enc)
+        live = [ci for ci in probes.tolist() if self.clusters[ci]["m"] > 0]
+        arr = (_ClusterView * max(len(live), 1))(*[self._view(ci) for ci in live])
+        total = sum(int(self.clusters[ci]["m"]) for ci in live)
+        scores = np.zeros(max(total, 1), np.int64)
+        rc = lib().or_ivf_rows_mt(C.addressof(arr), len(live), N, N1, v_enc, g, lt,
+                                  t_idx if t_idx.size else np.zeros(1, np.uint32),
+                                  t_sign if t_sign.size else np.zeros(1, np.int8), t_idx.size,
+                                  int(nthreads), int(bool(simd)), scores)
+        if rc != 0:
+            raise RuntimeError("AVX-512F is not available on this CPU")
+        ids = np.concatenate([self.ids[ci] for ci in live]) if live else np.zeros(0, np.uint32)
+        return topk(ids.astype(np.uint32), scores[:total], k)

@@ -533,6 +533,109 @@ void or_emvp_compute_products_batch(const uint64_t* m_hat, size_t m, const uint6
 }
 
 /* ------------------------------------------------------------------------
+ * EMVP client side (scorer-emvp/src/crypto.rs). SEC128: ell0 = 1024,
+ * k = 268 (n - ell0), Q = 2^20 (params.rs).
+ * ---------------------------------------------------------------------- */
+#define EMVP_ELL0 1024
+#define EMVP_K 268
+#define EMVP_Q 1048576.0
+
+static uint64_t emvp_neg(uint64_t a) { return a == 0 ? 0 : OR_P - a; } /* :38-41 */
+
+/* quantize, crypto.rs:47-52: round(v * Q) as i64 (f64 round half away from
+ * zero, saturating `as`), then ((vi % p) + p) % p with Rust's truncating %. */
+uint64_t or_emvp_quantize(float v) {
+    double rd = round((double)v * EMVP_Q);
+    int64_t vi = rd != rd ? 0
+                 : rd >= 9223372036854775808.0 ? INT64_MAX
+                 : rd <= -9223372036854775808.0 ? INT64_MIN
+                 : (int64_t)rd;
+    int64_t p = (int64_t)OR_P;
+    return (uint64_t)(((vi % p) + p) % p);
+}
+
+/* decode_field, crypto.rs:54-62: signed lift (v > p/2 is negative), then
+ * (signed as f64 / (Q as f64 * Q as f64)) as f32. */
+float or_emvp_decode_field(uint64_t v) {
+    int64_t s = v > OR_P / 2 ? (int64_t)v - (int64_t)OR_P : (int64_t)v;
+    return (float)((double)s / (EMVP_Q * EMVP_Q));
+}
+
+/* next_u64 % P of a ChaCha20Rng (rng.random::<u64>() % P). */
+static uint64_t emvp_draw(or_rng* r) { return or_next_u64(r) % OR_P; }
+
+/* generate_d, crypto.rs:72-97: D = [I_ell0 | D'] (ell0 x n, row-major); D'
+ * row by row from ChaCha20Rng(key_seed). */
+void or_emvp_generate_d(const uint8_t key_seed[32], uint64_t* d) {
+    memset(d, 0, sizeof(uint64_t) * EMVP_ELL0 * EMVP_N);
+    for (size_t i = 0; i < EMVP_ELL0; ++i) d[i * EMVP_N + i] = 1;
+    or_rng r;
+    or_rng_seed(&r, key_seed);
+    for (size_t i = 0; i < EMVP_ELL0; ++i)
+        for (size_t j = 0; j < EMVP_K; ++j) d[i * EMVP_N + EMVP_ELL0 + j] = emvp_draw(&r);
+}
+
+/* row_rng, crypto.rs:100-107: r_seed with its first 8 bytes xor the row
+ * index (u64 LE). */
+static void emvp_row_rng(or_rng* r, const uint8_t r_seed[32], uint64_t row) {
+    uint8_t seed[32];
+    memcpy(seed, r_seed, 32);
+    for (int i = 0; i < 8; ++i) seed[i] ^= (uint8_t)(row >> (8 * i));
+    or_rng_seed(r, seed);
+}
+
+/* encrypt_cluster, crypto.rs:124-163: M_hat[i][j] = fp_add(fold_l fp_add(acc,
+ * fp_mul(M0[i][l], D[l][j])), R[i][j]) with M0 = quantize(vectors[i]) zero-
+ * padded to ell0 (d floats per row, the first min(d, ell0) used) and R[i]
+ * drawn from row_rng(r_seed, i) in column order. */
+void or_emvp_encrypt_cluster(const uint64_t* d_mat, const uint8_t r_seed[32], const float* vecs,
+                             size_t m, size_t d, uint64_t* m_hat) {
+    uint64_t* m0 = (uint64_t*)malloc(sizeof(uint64_t) * EMVP_ELL0);
+    size_t take = d < EMVP_ELL0 ? d : EMVP_ELL0;
+    for (size_t i = 0; i < m; ++i) {
+        for (size_t l = 0; l < EMVP_ELL0; ++l) m0[l] = l < take ? or_emvp_quantize(vecs[i * d + l]) : 0;
+        or_rng r;
+        emvp_row_rng(&r, r_seed, (uint64_t)i);
+        for (size_t j = 0; j < EMVP_N; ++j) {
+            uint64_t dot = 0;
+            for (size_t l = 0; l < EMVP_ELL0; ++l)
+                dot = or_add_mod(dot, emvp_mul(m0[l], d_mat[l * EMVP_N + j]));
+            m_hat[i * EMVP_N + j] = or_add_mod(dot, emvp_draw(&r));
+        }
+    }
+    free(m0);
+}
+
+/* encode_query, crypto.rs:171-190: q_hat = (q - D' u | u) with
+ * q[i] = quantize(query[i] as f32). */
+void or_emvp_encode_query(const uint64_t* d_mat, const double* query, const uint64_t* u,
+                          uint64_t* q_hat) {
+    for (size_t i = 0; i < EMVP_ELL0; ++i) {
+        uint64_t dp_u = 0;
+        for (size_t j = 0; j < EMVP_K; ++j)
+            dp_u = or_add_mod(dp_u, emvp_mul(d_mat[i * EMVP_N + EMVP_ELL0 + j], u[j]));
+        q_hat[i] = or_add_mod(or_emvp_quantize((float)query[i]), emvp_neg(dp_u));
+    }
+    for (size_t j = 0; j < EMVP_K; ++j) q_hat[EMVP_ELL0 + j] = u[j];
+}
+
+/* decode_scores, crypto.rs:435-453: score[i] = decode_field(fp_add(
+ * sum_si partials[si*m + i], fp_neg(R[i] . q_hat))), R[i] replayed from
+ * row_rng(r_seed, i). */
+void or_emvp_decode_scores(size_t m, const uint8_t r_seed[32], const uint64_t* q_hat,
+                           const uint64_t* partials, float* scores) {
+    for (size_t i = 0; i < m; ++i) {
+        uint64_t block_sum = 0;
+        for (size_t si = 0; si < EMVP_S; ++si) block_sum = or_add_mod(block_sum, partials[si * m + i]);
+        or_rng r;
+        emvp_row_rng(&r, r_seed, (uint64_t)i);
+        uint64_t r_dot = 0;
+        for (size_t j = 0; j < EMVP_N; ++j) r_dot = or_add_mod(r_dot, emvp_mul(emvp_draw(&r), q_hat[j]));
+        scores[i] = or_emvp_decode_field(or_add_mod(block_sum, emvp_neg(r_dot)));
+    }
+}
+
+/* ------------------------------------------------------------------------
  * Tiptoe SimplePIR server answer, scorer-tiptoe/src/pir/simplepir.rs:191-204:
  * ans[i] = wrapping sum_j m_db[i*c + j] * ct[j] (mod 2^64). The mod-2^32
  * variant is the same sum truncated (2^32 divides 2^64).

@@ -9,9 +9,13 @@
  *
  * Every function cites the reference file:line it restates. Pinning status
  * (what checks each function against something other than itself) is listed
- * per function below; functions marked "parity unpinned" depend on the
- * rand_chacha 0.10 / rand 0.10 word-consumption conventions, which no
- * reference test pins (no Rust toolchain here to run the reference).
+ * per function below. The ChaCha20Rng stream (rand_chacha 0.10, a
+ * dependency the reference does not vendor) is pinned against the RFC 7539
+ * A.1 keystreams the way rand_chacha's own tests use them: seed = key,
+ * blocks in counter order, u32 words in order. next_u64 taking the low word
+ * first (rand_core's BlockRng) and random::<u8>() = next_u32() truncated
+ * (rand's StandardUniform) restate those crates' documented behaviour; no
+ * reference file holds a ChaCha20Rng output to pin them further.
  */
 #ifndef SVS_ORACLE_H
 #define SVS_ORACLE_H
@@ -116,6 +120,19 @@ void or_emvp_compute_products(const uint64_t* m_hat, size_t m, const uint64_t* q
                               uint64_t* partials /* s x m */);
 void or_emvp_compute_products_batch(const uint64_t* m_hat, size_t m, const uint64_t* q_hats,
                                     size_t nq, uint64_t* partials /* nq x s x m */);
+/* ---- EMVP client side (scorer-emvp/src/crypto.rs:47-62,72-190,435-453) --
+ * Pinned by the reference's encode_decode_single_vector (crypto.rs:509-542)
+ * and by the exact round trip decode(compute(encrypt(M0), encode(q))) =
+ * fl32(signed(M0 . q mod p) / 2^40). */
+uint64_t or_emvp_quantize(float v);
+float or_emvp_decode_field(uint64_t v);
+void or_emvp_generate_d(const uint8_t key_seed[32], uint64_t* d);
+void or_emvp_encrypt_cluster(const uint64_t* d_mat, const uint8_t r_seed[32], const float* vecs,
+                             size_t m, size_t d, uint64_t* m_hat);
+void or_emvp_encode_query(const uint64_t* d_mat, const double* query, const uint64_t* u,
+                          uint64_t* q_hat);
+void or_emvp_decode_scores(size_t m, const uint8_t r_seed[32], const uint64_t* q_hat,
+                           const uint64_t* partials, float* scores);
 
 /* ---- Tiptoe SimplePIR answer (scorer-tiptoe/src/pir/simplepir.rs:191-204) */
 void or_tiptoe_server_answer_u64(const uint64_t* m_db, size_t r, size_t c, const uint64_t* ct,
@@ -137,6 +154,29 @@ int or_bntm_flat_score(const uint64_t* m_enc, const uint64_t* a_l, const uint64_
  * nthreads, mirroring rayon's row parallelism in crypto.rs:614-625). */
 void or_matvec_mt(const uint64_t* mat, size_t rows, size_t cols, const uint64_t* vec,
                   uint64_t* out, int nthreads);
+
+/* ---- CPU baseline (svs_oracle_simd.c) ----------------------------------- */
+typedef struct {
+    const uint64_t* m_enc;    /* m x n */
+    const uint64_t* a_l;      /* m x n1 */
+    const uint64_t* m_plain;  /* m x n */
+    const uint64_t* h;        /* m x n1 */
+    const uint32_t* s_rowptr; /* m + 1 */
+    const uint32_t* s_idx;
+    const int8_t* s_sign;
+    uint64_t m;
+} or_cluster;
+
+int or_has_avx512f(void);
+/* mersenne_matvec_avx512, crypto.rs:643-739 (-1 without AVX-512F). */
+int or_matvec_avx512(const uint64_t* mat, size_t rows, size_t cols, const uint64_t* vec,
+                     uint64_t* out);
+/* compute_products + decode_scores of one query over its probed lists,
+ * rows split across nthreads (ivf.rs:746-822, crypto.rs:609-625). */
+int or_ivf_rows_mt(const or_cluster* cl, size_t ncl, size_t n, size_t n1, const uint64_t* v_enc,
+                   const uint64_t* g, const uint64_t* lt, const uint32_t* t_idx,
+                   const int8_t* t_sign, size_t t_nnz, int nthreads, int simd,
+                   int64_t* scores_out);
 
 #ifdef __cplusplus
 }

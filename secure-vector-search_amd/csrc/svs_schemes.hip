@@ -1,7 +1,9 @@
 // svs_schemes.hip — the two other schemes on the BN kernel family:
 //   * EMVP's score GEMV (block partials over F_p), scorer-emvp/src/crypto.rs
 //     :200-222 / :369-415, replacing scorer-emvp/src/gpu/mod.rs and its
-//     emvp_compute_products kernel (scorer-emvp/src/gpu/kernels.cu:90-110);
+//     emvp_compute_products kernel (scorer-emvp/src/gpu/kernels.cu:90-110),
+//     and EMVP's client decode (crypto.rs:435-453), which has no GPU path in
+//     the reference;
 //   * Tiptoe's SimplePIR server answer, scorer-tiptoe/src/pir/simplepir.rs
 //     :191-204, in the mod-2^32 regime (a u32 database) and mod 2^64.
 #include <hip/hip_runtime.h>
@@ -72,6 +74,82 @@ k_emvp(const uint64_t* __restrict__ m_hat, uint32_t m, const uint64_t* __restric
 #pragma unroll
             for (int l = 0; l < EMVP_B; ++l) acc.mac(a[l], qb[l]);
             partials[((size_t)q * EMVP_S + si) * m + i] = acc.reduce();
+        }
+    }
+}
+
+// ============================================================ EMVP decode
+// decode_scores (scorer-emvp/src/crypto.rs:435-453) on the device: for row i
+// the client replays R[i] from row_rng(r_seed, i) (r_seed with its first 8
+// bytes xor i, crypto.rs:100-107): R[i][j] = next_u64 % p, the u64 draw j
+// being keystream words 2j (low) and 2j + 1 (high), i.e. block j / 8 of the
+// row's ChaCha20 stream. score = decode_field(sum_si partials[si m + i] -
+// R[i] . q_hat). One wave per row: lane l generates blocks l, l + 64 and
+// l + 128 (162 blocks cover the 1292 draws), keeps its 24 draws in registers
+// for all of the batch's queries, multiplies them into a 128-bit lazy sum
+// (24 products of canonical values stay below 2^127) and adds its one or two
+// block partials; an add_mod butterfly over the wave gives the row's value.
+// The stream generation is VALU work (about 1,000 integer ops a block), so
+// the kernel is bound by the ALUs, not HBM: a batch of queries shares it.
+constexpr int EMVP_BLOCKS = (2 * EMVP_N + 15) / 16;  // 162
+constexpr int ED_T = (EMVP_BLOCKS + TWAVE - 1) / TWAVE;  // 3 blocks per lane
+static_assert(EMVP_S <= 2 * TWAVE, "two block partials per lane");
+
+__device__ __forceinline__ uint64_t mod_p_u64(uint64_t x) {
+    const uint64_t r = (x & P) + (x >> 61);  // <= 2^61 + 6
+    return r >= P ? r - P : r;
+}
+
+// decode_field (crypto.rs:54-62): signed lift, / Q^2 in f64, then f32.
+__device__ __forceinline__ float emvp_decode_field(uint64_t v) {
+    const int64_t s = v > P / 2 ? (int64_t)v - (int64_t)P : (int64_t)v;
+    return (float)((double)s / (1048576.0 * 1048576.0));
+}
+
+__global__ void __launch_bounds__(256)
+k_emvp_decode(uint32_t m, uint4 seed_lo, uint4 seed_hi, const uint64_t* __restrict__ q_hat,
+              uint32_t nq, const uint64_t* __restrict__ partials, float* __restrict__ scores) {
+    const uint32_t lane = threadIdx.x % TWAVE;
+    const uint32_t wpb = blockDim.x / TWAVE;
+    for (uint32_t i = blockIdx.x * wpb + threadIdx.x / TWAVE; i < m; i += gridDim.x * wpb) {
+        // row_rng: the row index (u64 LE) xor the seed's first 8 bytes
+        const uint32_t key[8] = {seed_lo.x ^ i, seed_lo.y, seed_lo.z, seed_lo.w,
+                                 seed_hi.x,     seed_hi.y, seed_hi.z, seed_hi.w};
+        uint64_t r[ED_T][8];
+#pragma unroll
+        for (int t = 0; t < ED_T; ++t) {
+            const uint32_t b = lane + TWAVE * t;
+            uint32_t w[16];
+            chacha20_block(key, b, w);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool live = b < EMVP_BLOCKS && 8 * b + e < (uint32_t)EMVP_N;
+                r[t][e] = live ? mod_p_u64((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : 0;
+            }
+        }
+        for (uint32_t q = 0; q < nq; ++q) {
+            const uint64_t* qh = q_hat + (size_t)q * EMVP_N;
+            Acc128 acc;
+            acc.zero();
+#pragma unroll
+            for (int t = 0; t < ED_T; ++t) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint32_t j = 8 * (lane + TWAVE * t) + e;
+                    if (j < (uint32_t)EMVP_N) acc.mac(r[t][e], qh[j]);
+                }
+            }
+            const uint64_t* pq = partials + (size_t)q * EMVP_S * m + i;
+            uint64_t bs = lane < (uint32_t)EMVP_S ? pq[(size_t)lane * m] : 0;
+            if (lane + TWAVE < (uint32_t)EMVP_S) bs = add_mod(bs, pq[(size_t)(lane + TWAVE) * m]);
+            uint64_t v = sub_mod(bs, acc.reduce());
+#pragma unroll
+            for (int off = TWAVE / 2; off >= 1; off >>= 1) {
+                const uint64_t o = ((uint64_t)__shfl_xor((uint32_t)(v >> 32), off, TWAVE) << 32) |
+                                   (uint32_t)__shfl_xor((uint32_t)v, off, TWAVE);
+                v = add_mod(v, o);
+            }
+            if (lane == 0) scores[(size_t)q * m + i] = emvp_decode_field(v);
         }
     }
 }
@@ -236,7 +314,7 @@ struct svs_emvp {
     svs_ctx* ctx = nullptr;
     uint32_t m = 0;
     uint64_t* m_hat = nullptr;
-    DevBuf q, out;
+    DevBuf q, out, sc;
 };
 
 struct svs_tiptoe {
@@ -285,6 +363,7 @@ extern "C" int svs_emvp_destroy(svs_emvp* e) {
         dev_free(ctx, e->m_hat, sizeof(uint64_t) * (size_t)e->m * EMVP_N);
         e->q.release(ctx);
         e->out.release(ctx);
+        e->sc.release(ctx);
         delete e;
     }
     ctx_release(ctx);
@@ -334,6 +413,90 @@ extern "C" int svs_emvp_compute_products(svs_emvp* e, uint32_t n_queries, const 
 }
 
 // ------------------------------------------------------------------ Tiptoe
+static int emvp_decode_launch(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                              const uint64_t* q_dev, const uint64_t* partials_dev,
+                              float* scores_dev) {
+    svs_ctx* ctx = e->ctx;
+    uint32_t k[8];
+    seed_to_key(r_seed, k);
+    const uint4 lo = make_uint4(k[0], k[1], k[2], k[3]), hi = make_uint4(k[4], k[5], k[6], k[7]);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)e->m + 3) / 4,
+                                                         (uint64_t)ctx->n_cu * 16);
+    hipLaunchKernelGGL(k_emvp_decode, dim3(blocks), dim3(256), 0, ctx->stream, e->m, lo, hi,
+                       q_dev, n_queries, partials_dev, scores_dev);
+    HIP_TRY(hipGetLastError());
+    ctx->launches++;
+    return SVS_OK;
+}
+
+extern "C" int svs_emvp_decode_scores_dev(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                                          const uint64_t* q_hat_dev, const uint64_t* partials_dev,
+                                          float* scores_dev) {
+    if (!e || !r_seed || (n_queries && (!q_hat_dev || !partials_dev || !scores_dev)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (e->m == 0 || n_queries == 0) return e->m == 0 ? SVS_NONE : SVS_OK;
+    svs_ctx* ctx = e->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return emvp_decode_launch(e, n_queries, r_seed, q_hat_dev, partials_dev, scores_dev);
+}
+
+static int check_canonical(const uint64_t* a, size_t n, const char* what) {
+    for (size_t i = 0; i < n; ++i)
+        if (a[i] >= P) return fail(SVS_ERR_ARG, "%s[%zu] is not canonical", what, i);
+    return SVS_OK;
+}
+
+extern "C" int svs_emvp_decode_scores(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                                      const uint64_t* q_hat, const uint64_t* partials,
+                                      float* scores) {
+    if (!e || !r_seed || (n_queries && (!q_hat || !partials || !scores)))
+        return fail(SVS_ERR_ARG, "null argument");
+    if (e->m == 0 || n_queries == 0) return e->m == 0 ? SVS_NONE : SVS_OK;
+    const size_t nq_n = (size_t)n_queries * EMVP_N, np_ = (size_t)n_queries * EMVP_S * e->m;
+    SVS_TRY(check_canonical(q_hat, nq_n, "q_hat"));
+    SVS_TRY(check_canonical(partials, np_, "partials"));
+    svs_ctx* ctx = e->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(e->q.ensure(ctx, sizeof(uint64_t) * nq_n));
+    SVS_TRY(e->out.ensure(ctx, sizeof(uint64_t) * np_));
+    SVS_TRY(e->sc.ensure(ctx, sizeof(float) * (size_t)n_queries * e->m));
+    HIP_TRY(hipMemcpyAsync(e->q.ptr, q_hat, sizeof(uint64_t) * nq_n, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(e->out.ptr, partials, sizeof(uint64_t) * np_, hipMemcpyHostToDevice,
+                           ctx->stream));
+    SVS_TRY(emvp_decode_launch(e, n_queries, r_seed, (const uint64_t*)e->q.ptr,
+                               (const uint64_t*)e->out.ptr, (float*)e->sc.ptr));
+    HIP_TRY(hipMemcpyAsync(scores, e->sc.ptr, sizeof(float) * (size_t)n_queries * e->m,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
+extern "C" int svs_emvp_score(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
+                              const uint64_t* q_hat, float* scores) {
+    if (!e || !r_seed || (n_queries && (!q_hat || !scores))) return fail(SVS_ERR_ARG, "null argument");
+    if (e->m == 0 || n_queries == 0) return e->m == 0 ? SVS_NONE : SVS_OK;
+    const size_t nq_n = (size_t)n_queries * EMVP_N, np_ = (size_t)n_queries * EMVP_S * e->m;
+    SVS_TRY(check_canonical(q_hat, nq_n, "q_hat"));
+    svs_ctx* ctx = e->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    SVS_TRY(e->q.ensure(ctx, sizeof(uint64_t) * nq_n));
+    SVS_TRY(e->out.ensure(ctx, sizeof(uint64_t) * np_));
+    SVS_TRY(e->sc.ensure(ctx, sizeof(float) * (size_t)n_queries * e->m));
+    HIP_TRY(hipMemcpyAsync(e->q.ptr, q_hat, sizeof(uint64_t) * nq_n, hipMemcpyHostToDevice,
+                           ctx->stream));
+    SVS_TRY(emvp_launch(e, n_queries, (const uint64_t*)e->q.ptr, (uint64_t*)e->out.ptr));
+    SVS_TRY(emvp_decode_launch(e, n_queries, r_seed, (const uint64_t*)e->q.ptr,
+                               (const uint64_t*)e->out.ptr, (float*)e->sc.ptr));
+    HIP_TRY(hipMemcpyAsync(scores, e->sc.ptr, sizeof(float) * (size_t)n_queries * e->m,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SVS_OK;
+}
+
 extern "C" int svs_tiptoe_create(svs_ctx* ctx, uint32_t rows, uint32_t cols, int elem_bytes,
                                  const void* m_db, svs_tiptoe** out) {
     if (!ctx || !out || !m_db || rows == 0 || cols == 0) return fail(SVS_ERR_ARG, "bad argument");

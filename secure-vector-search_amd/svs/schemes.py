@@ -3,7 +3,8 @@
 EmvpCluster mirrors the EMVP GPU state (scorer-emvp/src/gpu/mod.rs:217-380):
 a resident M_hat and compute_products / compute_products_batch
 (scorer-emvp/src/crypto.rs:200-222, 369-415) returning the s x m block-major
-partials. TiptoeDatabase mirrors server_answer
+partials, plus the client's decode_scores (crypto.rs:435-453) on the device
+and the two fused (score_batch). TiptoeDatabase mirrors server_answer
 (scorer-tiptoe/src/pir/simplepir.rs:191-204) over a resident database, in the
 mod-2^32 regime (u32 database) and the reference's mod-2^64 regime.
 """
@@ -18,6 +19,14 @@ from ._lib import check, ptr, u64
 from .bntm import Context, _live_indexes, default_context
 
 EMVP_N, EMVP_S, EMVP_B = 1292, 76, 17
+
+
+def _seed(r_seed) -> np.ndarray:
+    a = np.frombuffer(bytes(r_seed), np.uint8).copy() if not isinstance(r_seed, np.ndarray) \
+        else np.ascontiguousarray(r_seed, np.uint8)
+    if a.size != 32:
+        raise ValueError("r_seed must be 32 bytes")
+    return a
 
 
 class EmvpCluster:
@@ -48,6 +57,38 @@ class EmvpCluster:
         rc = check(L.lib().svs_emvp_compute_products_dev(self.h, n_queries, C.c_void_p(q_hat_ptr),
                                                          C.c_void_p(partials_ptr)))
         return rc != L.SVS_NONE
+
+    def decode_scores_batch(self, r_seed, q_hats, partials):
+        """decode_scores (scorer-emvp/src/crypto.rs:435-453) on the device:
+        n_queries x m f32 scores; None for an empty cluster."""
+        q = u64(q_hats).reshape(-1, EMVP_N)
+        pa = u64(partials).reshape(q.shape[0], -1)
+        if pa.shape[1] != EMVP_S * self.m:
+            raise ValueError(f"partials must be n_queries x {EMVP_S * self.m}")
+        out = np.zeros((q.shape[0], max(self.m, 1)), np.float32)
+        seed = _seed(r_seed)
+        rc = check(L.lib().svs_emvp_decode_scores(self.h, q.shape[0], ptr(seed), ptr(q), ptr(pa),
+                                                  ptr(out)))
+        return None if rc == L.SVS_NONE else out[:, : self.m]
+
+    def decode_scores_dev(self, n_queries: int, r_seed, q_hat_ptr: int, partials_ptr: int,
+                          scores_ptr: int):
+        """Device pointers, asynchronous on the context stream."""
+        seed = _seed(r_seed)
+        rc = check(L.lib().svs_emvp_decode_scores_dev(self.h, n_queries, ptr(seed),
+                                                      C.c_void_p(q_hat_ptr),
+                                                      C.c_void_p(partials_ptr),
+                                                      C.c_void_p(scores_ptr)))
+        return rc != L.SVS_NONE
+
+    def score_batch(self, r_seed, q_hats):
+        """compute_products + decode_scores on the device (the partials stay
+        there): n_queries x m f32 scores; None for an empty cluster."""
+        q = u64(q_hats).reshape(-1, EMVP_N)
+        out = np.zeros((q.shape[0], max(self.m, 1)), np.float32)
+        seed = _seed(r_seed)
+        rc = check(L.lib().svs_emvp_score(self.h, q.shape[0], ptr(seed), ptr(q), ptr(out)))
+        return None if rc == L.SVS_NONE else out[:, : self.m]
 
     def close(self):
         if getattr(self, "h", None):

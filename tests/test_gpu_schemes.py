@@ -1,6 +1,6 @@
-"""GPU parity of EMVP's score GEMV and Tiptoe's SimplePIR answer against the
-oracle (bit-exact). Mirrors scorer-emvp/src/crypto.rs:463-507 and
-scorer-tiptoe/src/pir/simplepir.rs:293-376."""
+"""GPU parity of EMVP's score GEMV and client decode and Tiptoe's SimplePIR
+answer against the oracle (bit-exact). Mirrors scorer-emvp/src/crypto.rs
+:463-542 and scorer-tiptoe/src/pir/simplepir.rs:293-376."""
 import sys
 
 import numpy as np
@@ -43,6 +43,113 @@ def test_emvp_partials_match_oracle(ctx, golden):
         want = O.emvp_compute_products_batch(mh, m, qs)
         assert np.array_equal(got, want)
     assert EmvpCluster(np.zeros((0, 1292), np.uint64), ctx).compute_products(q_hat) is None
+
+
+def _emvp_plain_scores(vecs, queries):
+    """fl32(signed(M0 . q mod p) / 2^40) with M0, q quantised (params.rs Q =
+    2^20): what decode must return when R cancels exactly."""
+    out = []
+    for q in queries:
+        qq = [int(O.emvp_quantize(np.float32(x))) for x in q]
+        row = []
+        for v in vecs:
+            t = sum(int(O.emvp_quantize(x)) * y for x, y in zip(v, qq)) % P
+            t = t - P if t > P // 2 else t
+            row.append(np.float32(np.float64(t) / 2.0 ** 40))
+        out.append(row)
+    return np.array(out, np.float32)
+
+
+def test_emvp_decode_roundtrip_exact(ctx):
+    """Encrypt with the oracle (generate_d, encrypt_cluster), encode with a
+    random mask u, then score on the device (product + decode, partials kept
+    on the device): every score equals the plaintext product, exactly. Also
+    the reference's encode_decode_single_vector (crypto.rs:509-542)."""
+    from svs.schemes import EmvpCluster
+    rng = np.random.default_rng(31)
+    d = O.emvp_generate_d(bytes(32))
+    m = 24
+    vecs = rng.standard_normal((m, 1024)).astype(np.float32)
+    vecs /= np.linalg.norm(vecs, axis=1, keepdims=True)
+    r_seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+    say("emvp round trip: encrypting", m, "rows with the oracle")
+    m_hat = O.emvp_encrypt_cluster(d, r_seed, vecs)
+    queries = rng.standard_normal((3, 1024))
+    queries /= np.linalg.norm(queries, axis=1, keepdims=True)
+    q_hats = np.stack([O.emvp_encode_query(d, q, rng.integers(0, P, 268, dtype=np.uint64))
+                       for q in queries])
+    cl = EmvpCluster(m_hat, ctx)
+    got = cl.score_batch(r_seed, q_hats)
+    want = _emvp_plain_scores(vecs, queries)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    for j in range(3):
+        parts = O.emvp_compute_products(m_hat, m, q_hats[j])
+        assert np.array_equal(O.emvp_decode_scores(m, r_seed, q_hats[j], parts), want[j])
+    cl.close()
+    # the reference's single-vector test: key [0; 32], r_seed [1; 32], u = 0
+    dim = 1024
+    v = np.array([(i / dim) * 0.01 for i in range(dim)], np.float32)
+    q = np.array([((dim - i) / dim) * 0.01 for i in range(dim)], np.float64)
+    m_hat1 = O.emvp_encrypt_cluster(d, bytes([1] * 32), v[None, :])
+    q_hat1 = O.emvp_encode_query(d, q, np.zeros(268, np.uint64))
+    one = EmvpCluster(m_hat1, ctx)
+    dec = one.score_batch(bytes([1] * 32), q_hat1[None, :])[0, 0]
+    ip = np.float32(0)
+    for a, b in zip(v, q):
+        ip = np.float32(ip + a * np.float32(b))
+    assert abs((dec - ip) / max(abs(ip), 1e-9)) < 1e-3
+    one.close()
+
+
+def test_emvp_decode_20000_rows_matches_oracle(ctx):
+    """decode_scores on the device at 20,000 x 1292 against the oracle's
+    decode_scores (crypto.rs:435-453), bit for bit (f32 bits): fused with
+    the product, from host partials (with all-(p-1) and zero partials), and
+    from device pointers; an empty cluster and non-canonical inputs."""
+    from concurrent.futures import ThreadPoolExecutor
+    import svs
+    from svs import _lib as L
+    from svs.schemes import EmvpCluster
+    rng = np.random.default_rng(32)
+    m, nq = 20000, 4
+    mh = rng.integers(0, P, size=(m, 1292), dtype=np.uint64)
+    mh[m - 1, :] = P - 1
+    qs = rng.integers(0, P, size=(nq, 1292), dtype=np.uint64)
+    qs[nq - 1] = P - 1
+    r_seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+    cl = EmvpCluster(mh, ctx)
+    say("emvp decode: device, m =", m)
+    got = cl.score_batch(r_seed, qs)
+    parts = cl.compute_products_batch(qs)
+    parts[1, :m] = P - 1           # block 0 of every row at its maximum
+    parts[2, 5 * m:6 * m] = 0
+    with ThreadPoolExecutor(nq) as ex:
+        want_fused = list(ex.map(
+            lambda j: O.emvp_decode_scores(m, r_seed, qs[j],
+                                           O.emvp_compute_products(mh, m, qs[j])), range(nq)))
+        want_host = list(ex.map(lambda j: O.emvp_decode_scores(m, r_seed, qs[j], parts[j]),
+                                range(nq)))
+    say("emvp decode: oracle done")
+    for j in range(nq):
+        assert np.array_equal(got[j].view(np.uint32), want_fused[j].view(np.uint32)), j
+    host = cl.decode_scores_batch(r_seed, qs, parts)
+    for j in range(nq):
+        assert np.array_equal(host[j].view(np.uint32), want_host[j].view(np.uint32)), j
+    qd = svs.DeviceArray.from_host(ctx, qs)
+    pd = svs.DeviceArray.from_host(ctx, parts)
+    sd = svs.DeviceArray(ctx, (nq, m), np.float32)
+    assert cl.decode_scores_dev(nq, r_seed, qd.ptr(), pd.ptr(), sd.ptr())
+    assert np.array_equal(sd.to_host().view(np.uint32), host.view(np.uint32))
+    for a in (qd, pd, sd):
+        a.close()
+    bad = parts.copy()
+    bad[0, 3] = P
+    with pytest.raises(L.GpuError):
+        cl.decode_scores_batch(r_seed, qs, bad)
+    cl.close()
+    empty = EmvpCluster(np.zeros((0, 1292), np.uint64), ctx)
+    assert empty.score_batch(r_seed, qs[:1]) is None
+    empty.close()
 
 
 def test_emvp_dev_grid_stride_matches_oracle(ctx):

@@ -162,6 +162,66 @@ def test_matvec_mt_matches_scalar():
     assert np.array_equal(O.matvec_mt(mat, 257, 1024, vec, 4), O.matvec(mat, 257, 1024, vec))
 
 
+@pytest.mark.skipif(not O.has_avx512f(), reason="this CPU lacks AVX-512F")
+def test_matvec_avx512_matches_scalar(golden):
+    """mersenne_matvec_avx512 (crypto.rs:643-739) equals the scalar matvec
+    at the reference gate's shapes (crypto.rs:1184-1198), at the all-(p-1)
+    extreme (every lane partial and fold at its maximum) and against the
+    golden big-int sums; cols % 8 != 0 takes the scalar row."""
+    rng = np.random.default_rng(0xC0DE)
+    for rows, cols in ((100, 512), (100, 1024), (3, 8), (5, 64), (7, 12)):
+        mat = _field(rng, (rows, cols))
+        vec = _field(rng, cols)
+        assert np.array_equal(O.matvec_avx512(mat, rows, cols, vec),
+                              O.matvec(mat, rows, cols, vec)), (rows, cols)
+    c = golden["matvec_all_pm1"]
+    mat = np.full((c["rows"], c["cols"]), P - 1, np.uint64)
+    vec = np.full(c["cols"], P - 1, np.uint64)
+    assert all(str(int(x)) == c["out"] for x in O.matvec_avx512(mat, c["rows"], c["cols"], vec))
+    for case in golden["matvec"]:
+        r = np.random.default_rng(case["seed"])
+        mat = _field(r, (case["rows"], case["cols"]))
+        vec = _field(r, case["cols"])
+        got = O.matvec_avx512(mat, case["rows"], case["cols"], vec)
+        assert [str(int(x)) for x in got] == case["out"]
+    # one-hot vectors pick columns: a wrong lane recombination would not
+    mat = _field(rng, (4, 1024))
+    for col in (0, 7, 8, 1023):
+        v = np.zeros(1024, np.uint64)
+        v[col] = 1
+        assert np.array_equal(O.matvec_avx512(mat, 4, 1024, v), mat[:, col])
+
+
+@pytest.mark.parametrize("simd", [False, True])
+def test_cpu_ivf_rows_mt_equals_ivf_score(simd):
+    """The timed CPU path (closed-loop B=1, rows across threads) returns
+    exactly BnTmIvfScorer::score's hits, including empty and one-row lists,
+    lists smaller than the thread count and ties in probe order."""
+    if simd and not O.has_avx512f():
+        pytest.skip("this CPU lacks AVX-512F")
+    rng = np.random.default_rng(77)
+    l = O.generate_subspace(bytes([3] * 32))
+    sizes = [0, 1, 5, 40, 129, 7]
+    lists, clusters, nxt = [], [], 0
+    for ci, m in enumerate(sizes):
+        vecs = rng.standard_normal((m, 768)).astype(np.float32) / 28.0
+        mq = O.quantise_rows(vecs) if m else np.zeros((0, O.N), np.uint64)
+        clusters.append(O.build_cluster(mq.reshape(-1), m, l, O.derive_h_seed(5, ci)))
+        lists.append(np.arange(nxt, nxt + m, dtype=np.uint32))
+        nxt += m
+    centroids = rng.standard_normal((len(sizes), 768)).astype(np.float32)
+    cpu = O.CpuIvf(clusters, lists, centroids, l)
+    for q in range(4):
+        query = rng.standard_normal(768).astype(np.float32) / 28.0
+        seed = bytes([q + 1] * 32)
+        for nprobe, k in ((6, 10), (3, 4), (1, 2)):
+            want = O.ivf_score(clusters, lists, centroids, l, query, nprobe, seed, False, k)
+            for threads in (1, 3, 8):
+                got = cpu.score(query, nprobe, seed, k, threads, simd)
+                assert got[0].tolist() == want[0].tolist(), (q, nprobe, threads)
+                assert got[1].tolist() == want[1].tolist(), (q, nprobe, threads)
+
+
 # ------------------------------------------------------ protocol invariants
 def _toy_cluster(seed_byte, m, n1=128):
     rng = np.random.default_rng(seed_byte)
@@ -313,6 +373,54 @@ def test_emvp_batch_matches_per_query():
 
 
 # ------------------------------------------------------------------- Tiptoe
+def test_emvp_client_roundtrip_exact_and_reference_smoke():
+    """EMVP client side (scorer-emvp/src/crypto.rs): decode(compute(encrypt(
+    M0), encode(q, u))) = fl32(signed(M0 . q mod p) / 2^40) for random masks
+    u (the D' u and R terms cancel exactly); D = [I | D'] puts the identity
+    where encode_query assumes it; and the reference's own
+    encode_decode_single_vector (crypto.rs:509-542)."""
+    rng = np.random.default_rng(41)
+    d = O.emvp_generate_d(bytes([5] * 32))
+    assert np.array_equal(d[:, :1024], np.eye(1024, dtype=np.uint64))
+    assert (d[:, 1024:] < P).all() and len(set(d[:, 1024:].reshape(-1).tolist())) > 270000
+    m = 3
+    vecs = rng.standard_normal((m, 768)).astype(np.float32) / 28.0  # d < ell0: zero-padded
+    r_seed = bytes(range(32))
+    m_hat = O.emvp_encrypt_cluster(d, r_seed, vecs)
+    for t in range(2):
+        q = rng.standard_normal(1024) / 32.0
+        u = rng.integers(0, P, 268, dtype=np.uint64)
+        q_hat = O.emvp_encode_query(d, q, u)
+        assert np.array_equal(q_hat[1024:], u)
+        got = O.emvp_decode_scores(m, r_seed, q_hat, O.emvp_compute_products(m_hat, m, q_hat))
+        for i in range(m):
+            t_ = sum(O.emvp_quantize(vecs[i][l]) * O.emvp_quantize(np.float32(q[l]))
+                     for l in range(768)) % P
+            t_ = t_ - P if t_ > P // 2 else t_
+            assert got[i] == np.float32(np.float64(t_) / 2.0 ** 40), (t, i)
+    # q = 0 with a random mask: the true score is 0, and only the right
+    # r_seed cancels R . q_hat
+    q_hat = O.emvp_encode_query(d, np.zeros(1024), rng.integers(0, P, 268, dtype=np.uint64))
+    parts = O.emvp_compute_products(m_hat, m, q_hat)
+    assert (O.emvp_decode_scores(m, r_seed, q_hat, parts) == 0).all()
+    assert (O.emvp_decode_scores(m, bytes(32), q_hat, parts) != 0).all()
+    dim = 1024
+    v = np.array([(i / dim) * 0.01 for i in range(dim)], np.float32)
+    qq = np.array([((dim - i) / dim) * 0.01 for i in range(dim)], np.float64)
+    d0 = O.emvp_generate_d(bytes(32))
+    mh = O.emvp_encrypt_cluster(d0, bytes([1] * 32), v[None, :])
+    qh = O.emvp_encode_query(d0, qq, np.zeros(268, np.uint64))
+    dec = O.emvp_decode_scores(1, bytes([1] * 32), qh, O.emvp_compute_products(mh, 1, qh))[0]
+    ip = np.float32(0)
+    for a, b in zip(v, qq):
+        ip = np.float32(ip + a * np.float32(b))
+    assert abs((dec - ip) / max(abs(ip), 1e-9)) < 1e-3
+    # quantize and decode_field at their edges (crypto.rs:47-62)
+    assert O.emvp_quantize(0.5) == 524288 and O.emvp_quantize(-0.5) == P - 524288
+    assert O.emvp_decode_field(P - 1) == np.float32(-1.0 / 2 ** 40)
+    assert O.emvp_decode_field(P // 2) == np.float32(float(P // 2) / 2 ** 40)
+
+
 def test_tiptoe_kat(golden):
     t = golden["tiptoe_kat"]
     m = np.array(t["m"], np.uint64)
