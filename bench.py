@@ -229,19 +229,28 @@ def schemes(ctx, seconds=1.0):
     return out
 
 
+def traffic_profile(args):
+    """The committed counter summary of this workload, if there is one."""
+    if not (args.n == 1_000_000 and args.nprobe == 32 and args.batch == 256 and
+            args.nlist == 1024):
+        return None
+    tag = "bn_ivf_kmeans_1M_nprobe32_b256" if args.lists == "kmeans" else \
+        "bn_ivf_1M_nprobe32_b256"
+    return f"profiles/hbm_traffic_{tag}.json"
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-probe factors), when
     it was taken on this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_bn_ivf_1M_nprobe32_b256.json")
-    if not (args.n == 1_000_000 and args.nprobe == 32 and args.batch == 256 and
-            args.nlist == 1024 and os.path.exists(path)):
+    rel = traffic_profile(args)
+    if rel is None or not os.path.exists(os.path.join(ROOT, rel)):
         return None, None
-    with open(path) as f:
+    with open(os.path.join(ROOT, rel)) as f:
         k = json.load(f)["kernels"].get(kernel)
     if not k:
         return None, None
-    return k["traffic_bytes"], "profiles/hbm_traffic_bn_ivf_1M_nprobe32_b256.json"
+    return k["traffic_bytes"], rel
 
 
 def served(index, qs, v_q, probes, verify, k, levels, seconds, max_batch, slots, pool, raw,
@@ -559,7 +568,8 @@ def host_cpu():
     return dict(model=model, nproc=os.cpu_count(), usable_cpus=usable, avx512f=O.has_avx512f())
 
 
-def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threads, upload_seed=7):
+def cpu_baseline(index, corpus, centroids, lists, l_sub, qs, nprobe, k, budget_s, max_threads,
+                 upload_seed=7):
     """The reference CPU path on this host, timed as the reference times it:
     the closed-loop B=1 protocol (eval.rs:2338-2344: one query, then the
     next) of BnTmIvfScorer::score (ivf.rs:746-822), verification off, with
@@ -575,7 +585,7 @@ def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threa
     host = host_cpu()
     threads = max(1, min(max_threads, host["usable_cpus"]))
     pool = min(qs.shape[0], 256)
-    probes = svs.probe_route(corpus.centres, qs[:pool], nprobe)
+    probes = svs.probe_route(centroids, qs[:pool], nprobe)
     state = {}
     lock = threading.Lock()
 
@@ -594,7 +604,7 @@ def cpu_baseline(index, corpus, lists, l_sub, qs, nprobe, k, budget_s, max_threa
         ts = [threading.Thread(target=load, args=(c,)) for c in todo[i:i + threads]]
         [t.start() for t in ts]
         [t.join() for t in ts]
-    cpu = O.CpuIvf(state, lists, corpus.centres, l_sub)
+    cpu = O.CpuIvf(state, lists, centroids, l_sub)
     seeds = [bytes([qi % 256]) * 32 for qi in range(pool)]
 
     def closed_loop(simd, seconds):
@@ -660,6 +670,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nlist", type=int, default=1024)
     ap.add_argument("--nprobe", type=int, default=32)
+    ap.add_argument("--lists", choices=("kmeans", "mixture"), default="kmeans",
+                    help="IVF lists: a seeded k-means of the corpus, or its mixture components")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--plain-queries", type=int, default=1000)
@@ -671,7 +683,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="force a block-tile variant (skips the autotuner)")
-    ap.add_argument("--serve-qps", default="12000,16000,20000,22000,24000,26000",
+    ap.add_argument("--serve-qps", default="16000,20000,24000,28000,32000,36000",
                     help="offered rates of the served (open-loop Poisson) levels; '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)
@@ -727,14 +739,19 @@ def main():
     t_all = time.perf_counter()
     log(f"corpus N={args.n} nlist={args.nlist}")
     corpus = data.make_corpus(args.n, 768, components=args.nlist, zipf_s=1.1, seed=42)
-    lists = data.ivf_lists(corpus)
+    t0 = time.perf_counter()
+    if args.lists == "kmeans":
+        centroids, lists = data.kmeans_lists(corpus, args.nlist, seed=42, max_iter=25)
+    else:
+        centroids, lists = corpus.centres, data.ivf_lists(corpus)
+    kmeans_s = time.perf_counter() - t0
     sizes = np.array([x.size for x in lists])
-    log(f"lists: max {sizes.max()} min {sizes.min()} mean {sizes.mean():.1f} "
-        f"({time.perf_counter() - t_all:.1f} s)")
+    log(f"{args.lists} lists in {kmeans_s:.1f} s: max {sizes.max()} min {sizes.min()} "
+        f"mean {sizes.mean():.1f} empty {(sizes == 0).sum()} ({time.perf_counter() - t_all:.1f} s)")
 
     ctx = svs.Context(0)
     scorer = svs.BnTmIvfScorer(ctx)
-    cfg = svs.BnTmIvfConfig(index=svs.IvfIndex(corpus.centres, lists), nprobe=args.nprobe,
+    cfg = svs.BnTmIvfConfig(index=svs.IvfIndex(centroids, lists), nprobe=args.nprobe,
                             key_seed=bytes([42] * 32), upload_seed=7)
     t0 = time.perf_counter()
     handle, _ = scorer.upload_cluster(cfg, corpus.vectors)
@@ -743,14 +760,19 @@ def main():
     total_q = (args.warmup + args.steps) * args.batch
     qs = data.make_queries(corpus, max(total_q, args.warmup * args.batch + args.plain_queries),
                            seed=4242)
-    probes = svs.probe_route(corpus.centres, qs, args.nprobe)
+    probes = svs.probe_route(centroids, qs, args.nprobe)
     v_q = svs.quantise(qs)
     rng = np.random.default_rng(7)
     seeds = rng.integers(0, 256, size=(qs.shape[0], 32), dtype=np.uint8)
     verify = (rng.random(qs.shape[0]) < args.verify_frac).astype(np.uint8)
     index = handle.index
     probed_rows = np.array([sizes[p].sum() for p in probes])
-    log(f"probed rows per query: mean {probed_rows.mean():.0f} p50 {np.median(probed_rows):.0f}")
+    probed_by_nprobe = {}
+    for npb in (8, 32, 128):
+        pr = sizes[svs.probe_route(centroids, qs[:2048], npb)].sum(axis=1)
+        probed_by_nprobe[str(npb)] = dict(mean=round(float(pr.mean()), 1),
+                                          p50=float(np.median(pr)), max=int(pr.max()))
+    log(f"probed rows per query: {probed_by_nprobe}")
 
     # block-tile variant for this batch size: timed on the first warm-up batch
     tune = None
@@ -918,7 +940,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(index, corpus, lists, handle.l_subspace, qs[off:], args.nprobe, args.k,
+        cpu = cpu_baseline(index, corpus, centroids, lists, handle.l_subspace, qs[off:],
+                           args.nprobe, args.k,
                            args.cpu_seconds, args.cpu_threads)
     dq_close = [dq.v, dq.s, dq.ids, dq.sc, dq.cnt, dq.st]
     for a in dq_close:
@@ -943,12 +966,18 @@ def main():
         "vs_baseline": None,
         "dtype": "u64 (F_p, p = 2^61-1)",
         "data": "synthetic: seeded 1024-component Gaussian mixture (Zipf 1.1 weights), d=768, "
-                "unit-norm; lists = mixture components",
+                "unit-norm; " + ("lists = seeded k-means (k-means++ init, Lloyd)"
+                                 if args.lists == "kmeans" else "lists = mixture components"),
         "config": {"workload": f"bn-ivf N={args.n} d=768 nlist={args.nlist} nprobe={args.nprobe} "
                                f"k={args.k}, served: open-loop Poisson arrivals of raw f32 "
                                f"queries, continuous batching up to {args.serve_max_batch}, "
                                f"p99 < {args.slo_ms:g} ms",
                    "nprobe": args.nprobe, "mean_probed_rows": round(float(probed_rows.mean()), 1),
+                   "lists": args.lists, "list_sizes": dict(max=int(sizes.max()),
+                                                            min=int(sizes.min()),
+                                                            p50=float(np.median(sizes)),
+                                                            empty=int((sizes == 0).sum())),
+                   "probed_rows_by_nprobe": probed_by_nprobe, "kmeans_s": round(kmeans_s, 1),
                    "closed_loop_batch": args.batch, "verify_frac": args.verify_frac},
         "value_is": "served_at_slo.achieved_qps: the highest served rate (over the offered "
                     "levels and slot counts) whose p99 latency, scheduled arrival to result, "

@@ -111,6 +111,41 @@ def test_synthetic_lists_partition_the_corpus():
         assert np.all(c.component[members] == ci)
 
 
+def test_kmeans_lists_mirror_the_reference_tests():
+    """The seeded k-means fixture against kmeans.rs's own tests (140-200):
+    deterministic under the seed; every point assigned; three well-separated
+    groups get one centroid each. Then its lists: a partition of the ids,
+    ascending within a list, every vector in the list of its nearest
+    centroid (brute force, float64), and a centroid equal to its list's mean
+    after Lloyd converged."""
+    from svs import data
+    rng = np.random.default_rng(11)
+    x = rng.random((400, 8), dtype=np.float32)
+    c1 = data.kmeans(x, 12, seed=99, max_iter=20)
+    c2 = data.kmeans(x, 12, seed=99, max_iter=20)
+    assert np.array_equal(c1, c2)
+    a = data._nearest(x, c1)
+    assert a.shape == (400,) and (a < 12).all()
+    pts = np.array([[base + i * 0.01, i * 0.01] for base in (0.0, 100.0, 200.0)
+                    for i in range(30)], np.float32)
+    c3 = data.kmeans(pts, 3, seed=1, max_iter=25)
+    assert sorted(int(round(c[0] / 100.0)) for c in c3) == [0, 1, 2]
+    corpus = data.make_corpus(6000, components=48, zipf_s=1.1, seed=5)
+    cents, lists = data.kmeans_lists(corpus, 40, seed=3, max_iter=50, train_sample=6000)
+    allids = np.concatenate(lists)
+    assert np.array_equal(np.sort(allids), np.arange(6000))
+    d = ((corpus.vectors.astype(np.float64)[:, None, :] -
+          cents.astype(np.float64)[None, :, :]) ** 2).sum(axis=2)
+    best = d.min(axis=1)
+    for ci, members in enumerate(lists):
+        assert np.all(np.diff(members.astype(np.int64)) > 0)
+        # nearest up to float32 rounding of the distance expansion
+        assert np.all(d[members, ci] <= best[members] + 1e-4)
+        if members.size and not np.array_equal(cents[ci], 0):
+            mean = corpus.vectors[members].astype(np.float64).mean(axis=0)
+            assert np.allclose(cents[ci], mean, atol=1e-5)
+
+
 def test_cache_fingerprint_is_fnv1a_over_length_prefixed_parts():
     """fingerprint64 (scorer-core/src/cache.rs:22-37) restated from the
     FNV-1a definition: offset basis 0xcbf29ce484222325, prime 0x100000001b3,
