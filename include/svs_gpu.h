@@ -473,6 +473,39 @@ int svs_emvp_decode_scores_dev(svs_emvp* e, uint32_t n_queries, const uint8_t* r
 int svs_emvp_score(svs_emvp* e, uint32_t n_queries, const uint8_t* r_seed,
                    const uint64_t* q_hat_host, float* scores_host);
 
+/* ---- Tiptoe BFV hint compression -------------------------------------------
+ * decompose_hint + apply_hint (scorer-tiptoe/src/bfv/compress.rs:146-190,
+ * 241-266; the reference runs both on the CPU through the fhe crate) with
+ * the limb plaintexts resident in HBM. BFV at tiptoe_text_params' sizes
+ * (scorer-tiptoe/src/bfv/mod.rs:50-70): N = 2048, t = 65537, one 38-bit
+ * modulus q = the largest prime below 2^38 that is 1 mod 4096
+ * (274877820929). Polynomials are in the negacyclic NTT domain of
+ * psi = g^((q-1)/4096), g the smallest primitive root mod q, natural order
+ * in and bit-reversed order out (slot i of the transform of a is
+ * sum_k a_k psi^((2 bitrev(i) + 1) k)); query tokens must be encrypted in
+ * that domain. */
+typedef struct svs_tiptoe_hint svs_tiptoe_hint;
+/* The hint H (r x n_lwe u64, row-major) split into its top 8 four-bit limbs,
+ * 2048-row chunks and columns, each limb polynomial transformed on the
+ * device: 8 x ceil(r / 2048) x n_lwe x 2048 u64 resident. */
+int svs_tiptoe_hint_create(svs_ctx* ctx, uint32_t r, uint32_t n_lwe, const uint64_t* hint_host,
+                           svs_tiptoe_hint** out);
+int svs_tiptoe_hint_destroy(svs_tiptoe_hint* h);
+/* The modulus q, the chunk count ceil(r / 2048) and the resident bytes. */
+int svs_tiptoe_hint_info(const svs_tiptoe_hint* h, uint64_t* q, uint32_t* chunks,
+                         uint64_t* stored_bytes);
+/* apply_hint for n_queries query tokens. ct: n_queries x n_lwe x 2 x 2048
+ * (ciphertext j of a token, its two polynomials, NTT domain, every value
+ * below q: SVS_ERR_ARG otherwise); out: n_queries x 8 x chunks x 2 x 2048,
+ * out[b][c] = sum_j ct[j] (x) pt[b][c][j] slot by slot mod q, limb b being
+ * bits 60 - 4b .. 63 - 4b. Host buffers. */
+int svs_tiptoe_apply_hint(svs_tiptoe_hint* h, uint32_t n_queries, const uint64_t* ct_host,
+                          uint64_t* out_host);
+/* Same with device pointers (not checked), asynchronous on the context
+ * stream. */
+int svs_tiptoe_apply_hint_dev(svs_tiptoe_hint* h, uint32_t n_queries, const uint64_t* ct_dev,
+                              uint64_t* out_dev);
+
 /* ---- Tiptoe SimplePIR answer (same kernel family) --------------------------
  * Resident database M_db (rows x cols, row-major) of u32 (elem_bytes 4) or
  * u64 (elem_bytes 8) entries. A u32 database whose entries are all < 128 is

@@ -100,6 +100,23 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.or_bntm_flat_score.restype = C.c_int
     L.or_bntm_flat_score.argtypes = [u64p, u64p, u64p, u64p, u32p, u32p, i8p, u64p, sz, u64p,
                                      u8p, C.c_int, sz, u32p, i64p, C.POINTER(C.c_size_t)]
+    L.or_bfv_modulus.restype = C.c_uint64
+    L.or_bfv_psi.restype = C.c_uint64
+    L.or_bfv_ntt.argtypes = [u64p]
+    L.or_bfv_intt.argtypes = [u64p]
+    L.or_bfv_negacyclic_naive.argtypes = [u64p, u64p, u64p]
+    L.or_bfv_keygen.argtypes = [u8p, u64p]
+    L.or_bfv_encrypt.argtypes = [u64p, u64p, u8p, u64p, u64p]
+    L.or_bfv_decrypt.argtypes = [u64p, u64p, u64p, u64p]
+    L.or_tiptoe_get_chunk.restype = C.c_uint64
+    L.or_tiptoe_get_chunk.argtypes = [C.c_uint64, sz]
+    L.or_tiptoe_from_modulo_t.restype = C.c_int64
+    L.or_tiptoe_from_modulo_t.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_tiptoe_decompose_hint.argtypes = [u64p, sz, sz, u64p]
+    L.or_tiptoe_encrypt_secret.restype = C.c_int
+    L.or_tiptoe_encrypt_secret.argtypes = [u8p, sz, u64p, u8p, u64p]
+    L.or_tiptoe_apply_hint.argtypes = [u64p, sz, sz, u64p, u64p]
+    L.or_tiptoe_recover_h_s.argtypes = [u64p, sz, u64p, u64p]
     L.or_has_avx512f.restype = C.c_int
     L.or_matvec_avx512.restype = C.c_int
     L.or_matvec_avx512.argtypes = [u64p, sz, sz, u64p, u64p]
@@ -507,3 +524,90 @@ class CpuIvf:
             raise RuntimeError("AVX-512F is not available on this CPU")
         ids = np.concatenate([self.ids[ci] for ci in live]) if live else np.zeros(0, np.uint32)
         return topk(ids.astype(np.uint32), scores[:total], k)
+
+
+# ------------------------------------------------- Tiptoe BFV compression
+BFV_N, BFV_T, TIPTOE_LIMBS = 2048, 65537, 8
+
+
+def bfv_modulus() -> int:
+    return int(lib().or_bfv_modulus())
+
+
+def bfv_ntt(a) -> np.ndarray:
+    x = _u64(a).copy()
+    lib().or_bfv_ntt(x)
+    return x
+
+
+def bfv_intt(a) -> np.ndarray:
+    x = _u64(a).copy()
+    lib().or_bfv_intt(x)
+    return x
+
+
+def bfv_negacyclic_naive(a, b) -> np.ndarray:
+    out = np.zeros(BFV_N, np.uint64)
+    lib().or_bfv_negacyclic_naive(_u64(a), _u64(b), out)
+    return out
+
+
+def bfv_keygen(seed) -> np.ndarray:
+    s = np.zeros(BFV_N, np.uint64)
+    lib().or_bfv_keygen(seed32(seed), s)
+    return s
+
+
+def bfv_encrypt(sk_ntt, m, seed):
+    c0, c1 = np.zeros(BFV_N, np.uint64), np.zeros(BFV_N, np.uint64)
+    lib().or_bfv_encrypt(_u64(sk_ntt), _u64(m), seed32(seed), c0, c1)
+    return c0, c1
+
+
+def bfv_decrypt(sk_ntt, c0, c1) -> np.ndarray:
+    m = np.zeros(BFV_N, np.uint64)
+    lib().or_bfv_decrypt(_u64(sk_ntt), _u64(c0), _u64(c1), m)
+    return m
+
+
+def tiptoe_get_chunk(v, idx) -> int:
+    return int(lib().or_tiptoe_get_chunk(int(v), idx))
+
+
+def tiptoe_from_modulo_t(t, v) -> int:
+    return int(lib().or_tiptoe_from_modulo_t(t, v))
+
+
+def tiptoe_decompose_hint(hint, r, n_lwe) -> np.ndarray:
+    """decompose_hint (compress.rs:146-190): (8, chunks, n_lwe, N) plaintexts
+    in the NTT domain."""
+    chunks = (r + BFV_N - 1) // BFV_N
+    pt = np.zeros(TIPTOE_LIMBS * chunks * n_lwe * BFV_N, np.uint64)
+    lib().or_tiptoe_decompose_hint(_u64(hint).reshape(-1), r, n_lwe, pt)
+    return pt.reshape(TIPTOE_LIMBS, chunks, n_lwe, BFV_N)
+
+
+def tiptoe_encrypt_secret(s, sk_ntt, seed) -> np.ndarray:
+    """encrypt_secret_under_bfv (compress.rs:201-227): (n_lwe, 2, N)."""
+    s = np.ascontiguousarray(s, np.uint8)
+    ct = np.zeros(s.size * 2 * BFV_N, np.uint64)
+    rc = lib().or_tiptoe_encrypt_secret(s, s.size, _u64(sk_ntt), seed32(seed), ct)
+    if rc != 0:
+        raise ValueError(f"LWE secret value out of {{0,1,2}} at index {-1 - rc}")
+    return ct.reshape(s.size, 2, BFV_N)
+
+
+def tiptoe_apply_hint(pt, ct) -> np.ndarray:
+    """apply_hint (compress.rs:241-266): (8, chunks, 2, N)."""
+    pt = _u64(pt)
+    limbs, chunks, n_lwe, _ = pt.shape
+    out = np.zeros(limbs * chunks * 2 * BFV_N, np.uint64)
+    lib().or_tiptoe_apply_hint(pt.reshape(-1), chunks, n_lwe, _u64(ct).reshape(-1), out)
+    return out.reshape(limbs, chunks, 2, BFV_N)
+
+
+def tiptoe_recover_h_s(answers, r, sk_ntt) -> np.ndarray:
+    """recover_h_s (compress.rs:277-318): H s mod 2^64, r values."""
+    out = np.zeros(max(r, 1), np.uint64)
+    lib().or_tiptoe_recover_h_s(_u64(answers).reshape(-1), r, _u64(sk_ntt), out)
+    return out[:r]

@@ -178,6 +178,26 @@ int or_ivf_rows_mt(const or_cluster* cl, size_t ncl, size_t n, size_t n1, const 
                    const int8_t* t_sign, size_t t_nnz, int nthreads, int simd,
                    int64_t* scores_out);
 
+/* ---- Tiptoe BFV compression (svs_oracle_bfv.c; compress.rs) ------------ */
+uint64_t or_bfv_modulus(void);
+uint64_t or_bfv_psi(void);
+void or_bfv_ntt(uint64_t* a);
+void or_bfv_intt(uint64_t* a);
+void or_bfv_negacyclic_naive(const uint64_t* a, const uint64_t* b, uint64_t* out);
+void or_bfv_keygen(const uint8_t seed[32], uint64_t* s_ntt);
+void or_bfv_encrypt(const uint64_t* s_ntt, const uint64_t* m, const uint8_t seed[32],
+                    uint64_t* c0, uint64_t* c1);
+void or_bfv_decrypt(const uint64_t* s_ntt, const uint64_t* c0, const uint64_t* c1, uint64_t* m);
+uint64_t or_tiptoe_get_chunk(uint64_t v, size_t idx);
+int64_t or_tiptoe_from_modulo_t(uint64_t t, uint64_t v);
+void or_tiptoe_decompose_hint(const uint64_t* hint, size_t r, size_t n_lwe, uint64_t* pt);
+int or_tiptoe_encrypt_secret(const uint8_t* s, size_t n_lwe, const uint64_t* sk_ntt,
+                             const uint8_t seed[32], uint64_t* ct);
+void or_tiptoe_apply_hint(const uint64_t* pt, size_t chunks, size_t n_lwe, const uint64_t* ct,
+                          uint64_t* out);
+void or_tiptoe_recover_h_s(const uint64_t* answers, size_t r, const uint64_t* sk_ntt,
+                           uint64_t* h_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "svs", "libsvs_gpu.so")
-SOURCES = ["svs_api.hip", "svs_schemes.hip", "svs_host.cpp", "svs_server.cpp", "svs_cache.cpp"]
+SOURCES = ["svs_api.hip", "svs_schemes.hip", "svs_bfv.hip", "svs_host.cpp", "svs_server.cpp",
+           "svs_cache.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-ffp-contract=off",

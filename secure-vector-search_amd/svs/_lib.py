@@ -30,7 +30,9 @@ EXPORTS = [
     "svs_debug_inject_fault", "svs_ctx_profile", "svs_compute_products_batch",
     "svs_emvp_create", "svs_emvp_destroy", "svs_emvp_compute_products",
     "svs_emvp_compute_products_dev", "svs_emvp_decode_scores", "svs_emvp_decode_scores_dev",
-    "svs_emvp_score", "svs_encode_query", "svs_tiptoe_create",
+    "svs_emvp_score", "svs_encode_query", "svs_tiptoe_hint_create", "svs_tiptoe_hint_destroy",
+    "svs_tiptoe_hint_info", "svs_tiptoe_apply_hint", "svs_tiptoe_apply_hint_dev",
+    "svs_tiptoe_create",
     "svs_tiptoe_destroy", "svs_tiptoe_stored_bytes", "svs_tiptoe_answer_u32",
     "svs_tiptoe_answer_u32_dev", "svs_tiptoe_answer_u64", "svs_ctx_copy_probe",
     "svs_ctx_set_tile_variant", "svs_ctx_get_tile_variant", "svs_autotune",
@@ -167,6 +169,11 @@ def lib() -> C.CDLL:
             "svs_emvp_decode_scores": ([vp, u32, vp, vp, vp, vp], i32),
             "svs_emvp_decode_scores_dev": ([vp, u32, vp, vp, vp, vp], i32),
             "svs_emvp_score": ([vp, u32, vp, vp, vp], i32),
+            "svs_tiptoe_hint_create": ([vp, u32, u32, vp, C.POINTER(vp)], i32),
+            "svs_tiptoe_hint_destroy": ([vp], i32),
+            "svs_tiptoe_hint_info": ([vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u64)], i32),
+            "svs_tiptoe_apply_hint": ([vp, u32, vp, vp], i32),
+            "svs_tiptoe_apply_hint_dev": ([vp, u32, vp, vp], i32),
             "svs_encode_query": ([vp, u32, vp, vp, vp, vp, vp], i32),
             "svs_tiptoe_create": ([vp, u32, u32, i32, vp, C.POINTER(vp)], i32),
             "svs_tiptoe_destroy": ([vp], i32),

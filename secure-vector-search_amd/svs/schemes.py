@@ -7,6 +7,7 @@ partials, plus the client's decode_scores (crypto.rs:435-453) on the device
 and the two fused (score_batch). TiptoeDatabase mirrors server_answer
 (scorer-tiptoe/src/pir/simplepir.rs:191-204) over a resident database, in the
 mod-2^32 regime (u32 database) and the reference's mod-2^64 regime.
+TiptoeHint holds the BFV-compressed hint (compress.rs) and runs apply_hint.
 """
 from __future__ import annotations
 
@@ -136,6 +137,54 @@ class TiptoeDatabase:
     def close(self):
         if getattr(self, "h", None):
             L.lib().svs_tiptoe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TiptoeHint:
+    """Tiptoe's compressed hint (scorer-tiptoe/src/bfv/compress.rs
+    CompressedHint) resident on the device: decompose_hint (:146-190) at
+    construction, apply_hint (:241-266) per batch of query tokens. Tokens
+    and answers are in the negacyclic NTT domain mod q (see svs_gpu.h)."""
+
+    N, LIMBS = 2048, 8
+
+    def __init__(self, hint, r: int, n_lwe: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        a = u64(hint).reshape(-1)
+        if a.size != r * n_lwe:
+            raise ValueError(f"hint length {a.size} != r * n_lwe = {r * n_lwe}")
+        self.r, self.n_lwe = r, n_lwe
+        h = C.c_void_p()
+        check(L.lib().svs_tiptoe_hint_create(self.ctx.h, r, n_lwe, ptr(a), C.byref(h)))
+        self.h = h
+        q, ch, nb = C.c_uint64(0), C.c_uint32(0), C.c_uint64(0)
+        check(L.lib().svs_tiptoe_hint_info(self.h, C.byref(q), C.byref(ch), C.byref(nb)))
+        self.q, self.chunks, self.stored_bytes = q.value, ch.value, nb.value
+        _live_indexes.add(self)
+
+    def apply_hint(self, ct):
+        """ct: (n_lwe, 2, N) for one token or (nq, n_lwe, 2, N); returns
+        (8, chunks, 2, N) per token."""
+        c = u64(ct)
+        one = c.ndim == 3
+        c = c.reshape(-1, self.n_lwe, 2, self.N)
+        out = np.zeros((c.shape[0], self.LIMBS, self.chunks, 2, self.N), np.uint64)
+        check(L.lib().svs_tiptoe_apply_hint(self.h, c.shape[0], ptr(c), ptr(out)))
+        return out[0] if one else out
+
+    def apply_hint_dev(self, n_queries: int, ct_ptr: int, out_ptr: int):
+        check(L.lib().svs_tiptoe_apply_hint_dev(self.h, n_queries, C.c_void_p(ct_ptr),
+                                                 C.c_void_p(out_ptr)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().svs_tiptoe_hint_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -227,3 +227,83 @@ def test_tiptoe_wide_entries_use_u32_store(ctx):
     assert db.stored() == (40 * 100 * 4, 4)
     want = np.stack([O.tiptoe_answer_u32(mdb, 40, 100, c) for c in ct])
     assert np.array_equal(db.server_answer(ct), want)
+
+
+# ------------------------------------------------ Tiptoe BFV compression
+def test_tiptoe_apply_hint_matches_oracle(ctx):
+    """decompose_hint + apply_hint on the device (compress.rs:146-190,
+    241-266) against the oracle, bit for bit in the NTT domain: a ragged
+    second row chunk, random full-width hint entries and random tokens
+    (1 and 5 per call: the 4-token pass and its remainder)."""
+    from svs.schemes import TiptoeHint
+    from svs import _lib as L
+    rng = np.random.default_rng(61)
+    r, n_lwe = 2048 + 37, 24
+    hint = rng.integers(0, 2 ** 64, size=(r, n_lwe), dtype=np.uint64)
+    hint[0, :] = 2 ** 64 - 1
+    th = TiptoeHint(hint, r, n_lwe, ctx)
+    q = O.bfv_modulus()
+    assert th.q == q and th.chunks == 2 and th.stored_bytes == 8 * 2 * n_lwe * 2048 * 8
+    pt = O.tiptoe_decompose_hint(hint, r, n_lwe)
+    for nq in (1, 5):
+        say("tiptoe apply_hint vs oracle, tokens =", nq)
+        ct = rng.integers(0, q, size=(nq, n_lwe, 2, 2048), dtype=np.uint64)
+        ct[0, 0, :, :] = q - 1
+        got = th.apply_hint(ct)
+        for i in range(nq):
+            assert np.array_equal(got[i], O.tiptoe_apply_hint(pt, ct[i])), i
+    bad = rng.integers(0, q, size=(n_lwe, 2, 2048), dtype=np.uint64)
+    bad[3, 1, 7] = q
+    with pytest.raises(L.GpuError):
+        th.apply_hint(bad)
+    th.close()
+
+
+def test_tiptoe_apply_hint_recovers_h_times_s(ctx):
+    """The reference's apply_hint_recovers_top_32_bits_of_h_times_s_small
+    (compress.rs:445-476) with the device doing decompose_hint and
+    apply_hint: r = 4, n_lwe = 2048, H[k] = ((17 k + 3) & 0xFFFF) << 32, a
+    ternary LWE secret encrypted under BFV; decrypting the 8 answers
+    recovers H s exactly."""
+    from svs.schemes import TiptoeHint
+    rng = np.random.default_rng(62)
+    r, n_lwe = 4, 2048
+    hint = ((np.arange(r * n_lwe, dtype=np.uint64) * np.uint64(17) + np.uint64(3))
+            & np.uint64(0xFFFF)) << np.uint64(32)
+    s = rng.integers(0, 3, n_lwe).astype(np.uint8)
+    sk = O.bfv_keygen(bytes([21] * 32))
+    say("tiptoe: encrypting the secret (oracle)")
+    ct = O.tiptoe_encrypt_secret(s, sk, bytes([22] * 32))
+    th = TiptoeHint(hint, r, n_lwe, ctx)
+    got = O.tiptoe_recover_h_s(th.apply_hint(ct), r, sk)
+    h = hint.reshape(r, n_lwe)
+    want = [sum(int(h[i, j]) * int(s[j]) for j in range(n_lwe)) % 2 ** 64 for i in range(r)]
+    assert got.tolist() == want
+    th.close()
+
+
+def test_tiptoe_apply_hint_full_size_32768x2048(ctx):
+    """The SimplePIR hint of the 32768 x 8192 database (r = 32768,
+    n_lwe = 2048: 16 chunks, 4 GiB of limb plaintexts): entries with random
+    top halves and zero low halves, one BFV-encrypted ternary secret; every
+    one of the 32768 recovered values equals H s mod 2^64 (the limb budget
+    2048 x 2 x 15 < t and the noise margin hold at full size)."""
+    from svs.schemes import TiptoeHint
+    rng = np.random.default_rng(63)
+    r, n_lwe = 32768, 2048
+    hint = rng.integers(0, 2 ** 32, size=(r, n_lwe), dtype=np.uint64) << np.uint64(32)
+    s = rng.integers(0, 3, n_lwe).astype(np.uint8)
+    sk = O.bfv_keygen(bytes([31] * 32))
+    ct = O.tiptoe_encrypt_secret(s, sk, bytes([32] * 32))
+    say("tiptoe full size: decompose on the device")
+    th = TiptoeHint(hint, r, n_lwe, ctx)
+    assert th.chunks == 16 and th.stored_bytes == 8 * 16 * 2048 * 2048 * 8
+    say("tiptoe full size: apply_hint")
+    ans = th.apply_hint(ct)
+    th.close()
+    got = O.tiptoe_recover_h_s(ans, r, sk)
+    want = np.zeros(r, np.uint64)
+    s64 = s.astype(np.uint64)
+    for a in range(0, r, 4096):
+        want[a:a + 4096] = (hint[a:a + 4096] * s64[None, :]).sum(axis=1, dtype=np.uint64)
+    assert np.array_equal(got, want)

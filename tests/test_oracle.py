@@ -421,6 +421,128 @@ def test_emvp_client_roundtrip_exact_and_reference_smoke():
     assert O.emvp_decode_field(P // 2) == np.float32(float(P // 2) / 2 ** 40)
 
 
+# ------------------------------------------------ Tiptoe BFV compression
+def _is_prime(n):
+    if n < 2:
+        return False
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d, r = d // 2, r + 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_bfv_modulus_ntt_and_roundtrip():
+    """The oracle's BFV at tiptoe_text_params' sizes (bfv/mod.rs:50-70,
+    N = 2048, t = 65537, one 38-bit modulus): q is the largest prime below
+    2^38 that is 1 mod 2N; psi is a primitive 2N-th root; the NTT inverts
+    and turns pointwise products into negacyclic products (against the
+    schoolbook product); encryption decrypts, and a plaintext x ciphertext
+    product decrypts to the negacyclic product mod t."""
+    q = O.bfv_modulus()
+    assert q < 2 ** 38 and q.bit_length() == 38 and (q - 1) % 4096 == 0 and _is_prime(q)
+    assert not any(_is_prime(c) for c in range(q + 4096, 2 ** 38, 4096))
+    psi = int(O.lib().or_bfv_psi())
+    assert pow(psi, 2048, q) == q - 1
+    rng = np.random.default_rng(51)
+    a = rng.integers(0, q, 2048, dtype=np.uint64)
+    b = rng.integers(0, q, 2048, dtype=np.uint64)
+    A, B = O.bfv_ntt(a), O.bfv_ntt(b)
+    assert np.array_equal(O.bfv_intt(A), a)
+    prod = np.array([int(x) * int(y) % q for x, y in zip(A.tolist(), B.tolist())], np.uint64)
+    assert np.array_equal(O.bfv_intt(prod), O.bfv_negacyclic_naive(a, b))
+    one = np.zeros(2048, np.uint64)
+    one[0] = 1
+    assert (O.bfv_ntt(one) == 1).all()  # the constant 1 is 1 in every slot
+    sk = O.bfv_keygen(bytes(range(32)))
+    m = rng.integers(0, 65537, 2048, dtype=np.uint64)
+    c0, c1 = O.bfv_encrypt(sk, m, bytes([3] * 32))
+    assert np.array_equal(O.bfv_decrypt(sk, c0, c1), m)
+    # m x p for a small plaintext p (limb-sized coefficients)
+    p = rng.integers(0, 16, 2048, dtype=np.uint64)
+    m2 = np.zeros(2048, np.uint64)
+    m2[0], m2[5] = 2, 1
+    c0, c1 = O.bfv_encrypt(sk, m2, bytes([4] * 32))
+    P_ = O.bfv_ntt(p)
+    mul = lambda c: np.array([int(x) * int(y) % q for x, y in zip(c.tolist(), P_.tolist())],
+                             np.uint64)
+    want = [0] * 2048
+    for i in range(2048):
+        for j, mv in ((0, 2), (5, 1)):
+            k = i + j
+            if k < 2048:
+                want[k] += mv * int(p[i])
+            else:
+                want[k - 2048] -= mv * int(p[i])
+    assert O.bfv_decrypt(sk, mul(c0), mul(c1)).tolist() == [w % 65537 for w in want]
+
+
+def test_tiptoe_limbs_and_signed_lift():
+    """compress.rs:343-370: get_chunk walks the 4-bit limbs low to high;
+    from_modulo_p lifts (t/2, t) to negatives."""
+    v = 0xFEDCBA9876543210
+    assert [O.tiptoe_get_chunk(v, i) for i in range(16)] == list(range(16))
+    assert O.tiptoe_get_chunk(0, 15) == 0 and O.tiptoe_get_chunk(2 ** 64 - 1, 0) == 15
+    t = 65537
+    assert O.tiptoe_from_modulo_t(t, 0) == 0 and O.tiptoe_from_modulo_t(t, t // 2) == t // 2
+    assert O.tiptoe_from_modulo_t(t, t // 2 + 1) == -(t // 2)
+    assert O.tiptoe_from_modulo_t(t, t - 1) == -1
+
+
+def test_tiptoe_decompose_hint_recombines_top_32_bits():
+    """compress.rs:372-429 on the oracle: the inverse NTT of each limb
+    plaintext holds that limb of its column, and the 8 limbs recombine to
+    entry & 0xFFFF_FFFF_0000_0000."""
+    hint = np.array([0xFEDCBA9876543210, 0x0123456789ABCDEF, 0x00000000FFFFFFFF,
+                     0xFFFFFFFF00000000, 0xDEADBEEFCAFEBABE, 0x1111222233334444,
+                     0xAAAA5555AAAA5555, 0, 2 ** 64 - 1, 1, 0x8000000000000000,
+                     0x0FFFFFFFFFFFFFFF], np.uint64)
+    r, n_lwe = 3, 4
+    pt = O.tiptoe_decompose_hint(hint, r, n_lwe)
+    assert pt.shape == (8, 1, n_lwe, 2048)
+    h = hint.reshape(r, n_lwe)
+    for col in range(n_lwe):
+        coeffs = [O.bfv_intt(pt[b, 0, col]) for b in range(8)]
+        for row in range(r):
+            re = 0
+            for b in range(8):
+                assert int(coeffs[b][row]) == O.tiptoe_get_chunk(h[row, col], 15 - b)
+                re |= int(coeffs[b][row]) << (4 * (15 - b))
+            assert re == int(h[row, col]) & 0xFFFFFFFF00000000
+        assert all((c[r:] == 0).all() for c in coeffs)
+
+
+def test_tiptoe_apply_hint_recovers_top_32_bits_of_h_times_s():
+    """compress.rs:445-476 on the oracle (n_lwe 512 here; 2048 in the GPU
+    test): H with zero low halves, s in {0,1,2}; decrypting apply_hint's
+    output recovers H s exactly; an out-of-range secret is refused
+    (compress.rs:211-216)."""
+    rng = np.random.default_rng(52)
+    r, n_lwe = 4, 512
+    hint = ((np.arange(r * n_lwe, dtype=np.uint64) * 17 + 3) & 0xFFFF) << np.uint64(32)
+    s = rng.integers(0, 3, n_lwe).astype(np.uint8)
+    sk = O.bfv_keygen(bytes([8] * 32))
+    ct = O.tiptoe_encrypt_secret(s, sk, bytes([9] * 32))
+    ans = O.tiptoe_apply_hint(O.tiptoe_decompose_hint(hint, r, n_lwe), ct)
+    got = O.tiptoe_recover_h_s(ans, r, sk)
+    h = hint.reshape(r, n_lwe)
+    want = [sum(int(h[i, j]) * int(s[j]) for j in range(n_lwe)) % 2 ** 64 for i in range(r)]
+    assert got.tolist() == want
+    bad = s.copy()
+    bad[7] = 3
+    with pytest.raises(ValueError):
+        O.tiptoe_encrypt_secret(bad, sk, bytes(32))
+
+
 def test_tiptoe_kat(golden):
     t = golden["tiptoe_kat"]
     m = np.array(t["m"], np.uint64)
