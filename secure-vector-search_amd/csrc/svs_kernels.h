@@ -1292,6 +1292,45 @@ __device__ __forceinline__ uint32_t xcd_worker() {
     return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
 }
 
+// The XCD's share of a plan's tile list. Tiles run list-major, row block,
+// query group innermost; a popular list's row block has more query groups
+// than one XCD takes per round of a contiguous dealing, so its tiles would
+// land on two to four XCDs and each would read the rows from HBM into its
+// own L2. Instead, with a grid that is a multiple of 8, XCD x takes tiles
+// [lo, hi): x nt / 8 and (x + 1) nt / 8, each moved forward to the start
+// of a row block (a tile whose list or first row differs from its
+// predecessor's), so every query group of a row block runs on one XCD; its
+// G / 8 blocks are workers 0 .. G / 8 - 1 of the share. Other grids: one
+// share of all tiles, worker = block.
+struct XcdShare {
+    uint32_t lo, hi, worker, workers;
+};
+
+__device__ __forceinline__ XcdShare xcd_share(const MTile* __restrict__ tiles, uint32_t nt) {
+    const uint32_t b = blockIdx.x, g = gridDim.x;
+    XcdShare s;
+    if (g & 7) {
+        s.lo = 0;
+        s.hi = nt;
+        s.worker = b;
+        s.workers = g;
+        return s;
+    }
+    auto cut = [&](uint32_t k) -> uint32_t {
+        uint32_t i = (uint32_t)(((uint64_t)nt * k) >> 3);
+        while (i > 0 && i < nt && tiles[i - 1].list == tiles[i].list &&
+               tiles[i - 1].row0 == tiles[i].row0)
+            ++i;
+        return i;
+    };
+    const uint32_t x = b & 7;
+    s.lo = cut(x);
+    s.hi = cut(x + 1);
+    s.worker = b >> 3;
+    s.workers = g >> 3;
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // Block-cooperative F_p GEMM tiles on int8 MFMA (same identity (*) as
 // k_gemv_mfma). A work tile is PB_ROWS rows of one list's row-major KE-wide
@@ -1330,8 +1369,8 @@ k_fp_gemm_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ til
     constexpr int CPT = QG / 2;    // columns per producer thread
     const int pj = tid / TPQ, pw = tid % TPQ;
     const int pe = pw / (16 / CPT), pd0 = (pw % (16 / CPT)) * CPT;
-    const uint32_t n_tiles = *n_tiles_p;
-    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+    for (uint32_t t = sh.lo + sh.worker; t < sh.hi; t += sh.workers) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
         const uint32_t m = L.m;
@@ -1492,8 +1531,8 @@ k_fp_gemm_vb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
         sel0 |= (uint32_t)(b0 >= 0 && b0 <= 7 ? b0 : 0x0C) << (8 * a);
         sel1 |= (uint32_t)(b1 >= 0 && b1 <= 7 ? b1 : 0x0C) << (8 * a);
     }
-    const uint32_t n_tiles = *n_tiles_p;
-    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+    for (uint32_t t = sh.lo + sh.worker; t < sh.hi; t += sh.workers) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
         const uint32_t m = L.m;
@@ -1691,13 +1730,15 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 // barrier: each wave streams its own 16-row units, with a PF-step register
 // ring that runs on into the next unit's first steps.
 //
-// Work units: unit u is 16-row block (u / 4) % 16 of plan tile
-// 4 (u / 64) + u % 4 (a 256-row tile of one list and up to QW of its
-// queries); wave w of worker b takes units 4 b + w, + 4 G, ...; units past
-// the list's end are skipped. The four waves of a block thus stream the same
-// 16 rows for four consecutive tiles at once (the query groups of one row
-// block, when the list has more queries than a tile holds); the next row
-// blocks of those tiles run on the next blocks of the same XCD.
+// Work units: within its XCD's share of the tile list (xcd_share), unit u
+// is 16-row block (u / 4) % 16 of tile lo + 4 (u / 64) + u % 4 (a 256-row
+// tile of one list and up to QW of its queries); wave w of worker b takes
+// units 4 b + w, + 4 G/8, ...; units past the list's end are skipped. The
+// four waves of a block thus stream the same 16 rows for four consecutive
+// tiles at once (the query groups of one row block, when the list has more
+// queries than a tile holds); the next row blocks of those tiles run on the
+// next blocks of the same XCD, and all of a row block's query groups stay
+// on that XCD.
 
 // Query planes for k_prod_planes: per query 16 steps x 4 groups x 8 planes x
 // 16 bytes; byte n of (step s, group g, plane b) is byte b of v[e] ^ 0x80
@@ -1759,9 +1800,9 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     const int lane = threadIdx.x % WAVE;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
-    const uint32_t n_tiles = *n_tiles_p;
-    const uint32_t n_units = 64u * ((n_tiles + 3) / 4);
-    const uint32_t ustride = 4u * gridDim.x;
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+    const uint32_t n_units = 64u * ((sh.hi - sh.lo + 3) / 4);
+    const uint32_t ustride = 4u * sh.workers;
 
     // the first non-empty unit at or after x (stepping by ustride), or n_units
     struct Unit {
@@ -1774,9 +1815,9 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         Unit r;
         r.u = n_units;
         for (; x < n_units; x += ustride) {
-            // unit x: 16-row block (x / 4) % 16 of tile 4 (x / 64) + x % 4
-            const uint32_t ti = ((x >> 6) << 2) | (x & 3u);
-            if (ti >= n_tiles) continue;
+            // unit x: 16-row block (x / 4) % 16 of tile lo + 4 (x / 64) + x % 4
+            const uint32_t ti = sh.lo + (((x >> 6) << 2) | (x & 3u));
+            if (ti >= sh.hi) continue;
             const MTile t = tiles[ti];
             const uint32_t m = lists[t.list].m;
             const uint32_t row0 = t.row0 + 16u * ((x >> 2) & 15u);
@@ -1816,7 +1857,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         for (int jb = 0; jb < NB; ++jb) rb[slot][jb] = load16b(qplanes + qb[jb] + 512u * s);
     };
 
-    Unit cur = find(4u * xcd_worker() + wave);
+    Unit cur = find(4u * sh.worker + wave);
     if (cur.u >= n_units) return;
     const uint64_t* a_cur;
     uint32_t qb_cur[NB];
@@ -1941,8 +1982,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int grp = lane / 16, col = lane % 16;
-    const uint32_t n_tiles = *n_tiles_p;
-    for (uint32_t t = xcd_worker(); t < n_tiles; t += gridDim.x) {
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+    for (uint32_t t = sh.lo + sh.worker; t < sh.hi; t += sh.workers) {
         const MTile tile = tiles[t];
         const ListDev L = lists[tile.list];
         const uint32_t m = L.m;
