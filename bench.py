@@ -452,9 +452,11 @@ def served_tenants(args, ctx=None):
         alone[str(npb)] = row
     log(f"one query alone by nprobe: {alone}")
     levels = []
-    for slots in [int(x) for x in args.tenant_slots.split(",") if x]:
-        with svs.Server([h.index for h in handles], max_batch=args.serve_max_batch,
-                        slots=slots) as srv:
+    configs = [(int(x), int(c)) for c in args.tenant_cross.split(",") if c
+               for x in args.tenant_slots.split(",") if x]
+    for slots, cross in configs:
+        with svs.Server([h.index for h in handles], max_batch=args.tenant_max_batch,
+                        slots=slots, cross_tenant=bool(cross)) as srv:
             srv.run_poisson([req(i) for i in range(2048)], qps=2000, seed=3)  # sizes the slots
             for qps in [float(x) for x in args.tenant_qps.split(",") if x]:
                 n = max(64, int(qps * args.serve_seconds))
@@ -464,7 +466,8 @@ def served_tenants(args, ctx=None):
                 s1 = srv.stats()
                 lat = [r.latency_ms for r in res]
                 nb = s1["batches"] - s0["batches"]
-                lvl = dict(slots=slots, offered_qps=qps, achieved_qps=round(n / wall, 1),
+                lvl = dict(slots=slots, cross_tenant=bool(cross), offered_qps=qps,
+                           achieved_qps=round(n / wall, 1),
                            requests=n, p50_ms=round(pct(lat, 50), 3),
                            p99_ms=round(pct(lat, 99), 3), p999_ms=round(pct(lat, 99.9), 3),
                            mean_batch=round((s1["batched_queries"] - s0["batched_queries"]) /
@@ -483,7 +486,7 @@ def served_tenants(args, ctx=None):
                                      f"{args.tenant_rows} rows (N={rows}) d=768 nlist="
                                      f"{args.tenant_nlist} per tenant, raw queries, nprobe "
                                      "uniform in {8,16,32,64,128}, 50% verified, k="
-                                     f"{args.k}", max_batch=args.serve_max_batch),
+                                     f"{args.k}", max_batch=args.tenant_max_batch),
                 resident_bytes=st["resident_bytes"], plain_qps=round(plain_qps, 1),
                 alone_by_nprobe=alone,
                 levels=levels, slo_ms=args.slo_ms,
@@ -751,6 +754,10 @@ def main():
     ap.add_argument("--tenant-plain-queries", type=int, default=200)
     ap.add_argument("--tenant-slots", default="1,2",
                     help="slot counts the multi-tenant served levels are run with")
+    ap.add_argument("--tenant-cross", default="0",
+                    help="batching of the multi-tenant levels: 0 = a batch holds one tenant, "
+                         "1 = batches across tenants")
+    ap.add_argument("--tenant-max-batch", type=int, default=512)
     ap.add_argument("--flat-n", type=int, default=1_000_000,
                     help="rows of the plain flat drop-in configuration")
     ap.add_argument("--flat-only", action="store_true",

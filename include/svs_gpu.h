@@ -365,7 +365,8 @@ int svs_autotune(svs_index* idx, const svs_query_batch* batch_host, uint32_t rep
  * chunk handed to it (eval.rs:2428-2433). A server keeps one or more
  * resident indexes (tenants: each its own keys and lists, all on one
  * context) and a queue of requests. A dispatcher thread packs the pending
- * requests of one tenant into a batch whenever one of `slots` device
+ * requests (of every tenant, or of one tenant: cross_tenant) into a batch
+ * whenever one of `slots` device
  * scratch/stream slots is free, admitting requests while the batch's device
  * scratch estimate stays within slot_bytes (at least one request), and runs
  * the batch asynchronously; batches on different slots overlap. Results are
@@ -379,6 +380,14 @@ typedef struct {
                              context budget, at least 1 GiB); a request routed
                              on the device counts its nprobe largest lists */
     int mode;             /* SVS_MODE_BATCHED or SVS_MODE_BATCHED_VALU */
+    uint32_t cross_tenant;  /* with several tenants: 1 = a batch takes the
+                               pending requests of every tenant (each query
+                               encoded with its tenant's keys, routed on its
+                               tenant's centroids, scored on its tenant's
+                               lists, through one table of all tenants'
+                               lists); 0 (the better default when tenants'
+                               queries share their lists) = a batch holds one
+                               tenant */
 } svs_server_config;
 typedef struct {
     uint32_t tenant;         /* position in the tenants array */

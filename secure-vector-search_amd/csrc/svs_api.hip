@@ -1113,6 +1113,7 @@ int svs::route_setup(svs_index* idx, svs_slot& sl, uint32_t nq) {
     svs_ctx* ctx = idx->ctx;
     SVS_TRY(sl.q_vq.ensure(ctx, sizeof(uint64_t) * (size_t)nq * N));
     SVS_TRY(sl.q_raw.ensure(ctx, sizeof(float) * (size_t)nq * idx->cent_d));
+    // (a cross-tenant table's list count bounds every tenant's)
     SVS_TRY(sl.route_dist.ensure(ctx, sizeof(float) * (size_t)nq * idx->lists.size()));
     return SVS_OK;
 }
@@ -1120,22 +1121,37 @@ int svs::route_setup(svs_index* idx, svs_slot& sl, uint32_t nq) {
 int svs::route_launch(svs_index* idx, svs_slot& sl, uint32_t nq, uint32_t n_items,
                       double q_scale) {
     svs_ctx* ctx = idx->ctx;
-    const uint32_t nl = (uint32_t)idx->lists.size(), d = idx->cent_d;
+    const uint32_t d = idx->cent_d;
     hipStream_t st = sl.stream;
     const float* x = (const float*)sl.q_raw.ptr;
     hipLaunchKernelGGL(k_quantise, dim3((uint32_t)std::min<uint64_t>(((uint64_t)nq * N + 255) / 256,
                                                                       (uint64_t)ctx->n_cu * 8)),
                        dim3(256), 0, st, x, nq, d, q_scale, (uint64_t*)sl.q_vq.ptr);
-    hipLaunchKernelGGL(k_route_dist, dim3((nq + RT_Q - 1) / RT_Q, (nl + RT_C - 1) / RT_C),
-                       dim3(256), 0, st, x, nq, (const float*)idx->cent_dev, nl, d,
-                       (float*)sl.route_dist.ptr);
-    uint32_t p2 = 1;
-    while (p2 < nl) p2 <<= 1;
+    ctx->launches++;
+    // per tenant run of queries: its centroids, its list ids shifted to
+    // where its lists sit in the batch's list table
     uint32_t* d_probes = (uint32_t*)sl.q_probes.ptr;
-    hipLaunchKernelGGL(k_route_select, dim3(nq), dim3(1024), sizeof(uint64_t) * p2, st,
-                       (const float*)sl.route_dist.ptr, nl, (const uint32_t*)(d_probes + n_items),
-                       d_probes);
-    ctx->launches += 3;
+    const TenantSeg whole{0, nq, 0, idx};
+    const size_t nseg = sl.segs.empty() ? 1 : sl.segs.size();
+    size_t dist_off = 0;
+    for (size_t si = 0; si < nseg; ++si) {
+        const TenantSeg& sg = sl.segs.empty() ? whole : sl.segs[si];
+        if (sg.nq == 0) continue;
+        const uint32_t nl = (uint32_t)sg.idx->lists.size();
+        if (!sg.idx->cent_dev || sg.idx->cent_d != d)
+            return fail(SVS_ERR_STATE, "routing needs centroids of dimension %u", d);
+        float* dist = (float*)sl.route_dist.ptr + dist_off;
+        hipLaunchKernelGGL(k_route_dist, dim3((sg.nq + RT_Q - 1) / RT_Q, (nl + RT_C - 1) / RT_C),
+                           dim3(256), 0, st, x + (size_t)sg.q0 * d, sg.nq,
+                           (const float*)sg.idx->cent_dev, nl, d, dist);
+        uint32_t p2 = 1;
+        while (p2 < nl) p2 <<= 1;
+        hipLaunchKernelGGL(k_route_select, dim3(sg.nq), dim3(1024), sizeof(uint64_t) * p2, st,
+                           (const float*)dist, nl,
+                           (const uint32_t*)(d_probes + n_items + sg.q0), sg.list_base, d_probes);
+        dist_off += (size_t)sg.nq * nl;
+        ctx->launches += 2;
+    }
     HIP_TRY(hipGetLastError());
     return SVS_OK;
 }
@@ -1189,12 +1205,28 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         hipLaunchKernelGGL(k_encode_keystream, dim3(nq), dim3(256), 0, st,
                            (const uint8_t*)sl.q_seeds.ptr, d_gl, d_tdense, d_tlist, d_tcnt,
                            d_vseed, d_rng_flag);
-        hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, nq), dim3(256), 0, st,
-                           (const uint64_t*)idx->lt_dev, (const uint64_t*)d_gl,
-                           (const int8_t*)d_tdense, (const uint64_t*)sl.q_vq.ptr, d_venc);
-        hipLaunchKernelGGL(k_lt, dim3(nq), dim3(N1 * LT_PARTS), 0, st, (const uint64_t*)idx->l_dev,
-                           (const uint64_t*)d_venc, d_gl);
-        ctx->launches += 3;
+        // v_enc = L G + T + v and L^T v_enc with each query's own keys: one
+        // segment for a batch of one index, one per tenant otherwise
+        const TenantSeg whole{0, nq, 0, idx};
+        const size_t nseg = sl.segs.empty() ? 1 : sl.segs.size();
+        for (size_t si = 0; si < nseg; ++si) {
+            const TenantSeg& sg = sl.segs.empty() ? whole : sl.segs[si];
+            if (sg.nq == 0) continue;
+            if (!sg.idx->lt_dev || !sg.idx->l_dev)
+                return fail(SVS_ERR_STATE, "encoding needs an index with keys");
+            hipLaunchKernelGGL(k_encode_venc, dim3(N / 256, sg.nq), dim3(256), 0, st,
+                               (const uint64_t*)sg.idx->lt_dev,
+                               (const uint64_t*)(d_gl + (size_t)sg.q0 * 2 * N1),
+                               (const int8_t*)(d_tdense + (size_t)sg.q0 * N),
+                               (const uint64_t*)sl.q_vq.ptr + (size_t)sg.q0 * N,
+                               d_venc + (size_t)sg.q0 * N);
+            hipLaunchKernelGGL(k_lt, dim3(sg.nq), dim3(N1 * LT_PARTS), 0, st,
+                               (const uint64_t*)sg.idx->l_dev,
+                               (const uint64_t*)(d_venc + (size_t)sg.q0 * N),
+                               d_gl + (size_t)sg.q0 * 2 * N1);
+            ctx->launches += 2;
+        }
+        ctx->launches += 1;
         HIP_TRY(hipGetLastError());
     }
     SVS_TRY(prof_mark(sl, 0, false));

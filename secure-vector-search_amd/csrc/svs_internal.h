@@ -93,8 +93,20 @@ struct svs_prof {
 // Per-batch device scratch and the stream a batch runs on. The context is
 // itself the default slot (used by every synchronous entry point); a server
 // owns further slots so that batches overlap on the device.
+// One tenant's run of queries in a cross-tenant batch (queries q0 ..
+// q0 + nq - 1 of the batch, sorted by tenant): encoding uses the tenant's
+// keys, routing its centroids, and its lists sit at list_base.. in the
+// batch's combined list table.
+struct TenantSeg {
+    uint32_t q0, nq, list_base;
+    const svs_index* idx;
+};
+
 struct svs_slot {
     hipStream_t stream = nullptr;
+    // set for a cross-tenant batch (a server over several tenants); empty for
+    // a batch of one index
+    std::vector<TenantSeg> segs;
     svs::DevBuf items, prefixes, flags;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes,
         q_pplanes;
@@ -179,6 +191,9 @@ struct svs_index {
     bool top_rows_dirty = true;
     // generation: unique per index and per centroid set (graph cache key)
     uint64_t gen = 0;
+    // a combined list table over other indexes' lists (a cross-tenant
+    // server's): its lists are not owned, it has no keys of its own
+    bool view = false;
 };
 
 // ---------------------------------------------------------------- batches

@@ -108,11 +108,12 @@ __device__ __forceinline__ uint32_t f32_total_key(float f) {
 
 // Per query (one block of 1024 threads): sorts (total_key(dist), centroid)
 // for all nc <= RT_SELECT_MAX centroids in LDS (bitonic) and writes the
-// first probe_ptr[q+1] - probe_ptr[q] centroids to probes[probe_ptr[q]..].
+// first probe_ptr[q+1] - probe_ptr[q] centroids to probes[probe_ptr[q]..],
+// each plus list_base (the tenant's first list in a cross-tenant table).
 // Unused slots of the padded sort hold all-ones keys, which sort last.
 __global__ void __launch_bounds__(1024)
 k_route_select(const float* __restrict__ dist, uint32_t nc, const uint32_t* __restrict__ probe_ptr,
-               uint32_t* __restrict__ probes) {
+               uint32_t list_base, uint32_t* __restrict__ probes) {
     extern __shared__ uint64_t keys[];
     const uint32_t q = blockIdx.x;
     const uint32_t np = probe_ptr[q + 1] - probe_ptr[q];
@@ -140,7 +141,7 @@ k_route_select(const float* __restrict__ dist, uint32_t nc, const uint32_t* __re
         }
     }
     uint32_t* out = probes + probe_ptr[q];
-    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) out[i] = (uint32_t)keys[i];
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) out[i] = list_base + (uint32_t)keys[i];
 }
 
 }  // namespace svs

@@ -115,6 +115,10 @@ struct svs_server {
     // per tenant: sum of its n largest list sizes (the row bound of a
     // routed request with nprobe n); indexes do not change while serving
     std::vector<std::vector<uint64_t>> top_rows;
+    // cross-tenant batching: every tenant's lists in one table (a view
+    // index, no keys of its own) and each tenant's first list in it
+    svs_index* combined = nullptr;
+    std::vector<uint32_t> list_base;
     svs_server_config cfg{};
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
@@ -160,8 +164,22 @@ uint64_t svs_server::batch_bytes(uint64_t rows, uint32_t nq, uint32_t k) const {
 
 void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
     const uint32_t nq = (uint32_t)batch.size();
-    svs_index* idx = tenants[batch[0]->tenant];
+    // one tenant's index, or the combined list table of a cross-tenant
+    // batch (requests sorted by tenant: one segment per tenant)
+    svs_index* idx = combined ? combined : tenants[batch[0]->tenant];
     const bool raw = batch[0]->raw;  // batches are of one kind
+    s.sl.segs.clear();
+    std::vector<uint64_t> seg_key;
+    if (combined) {
+        for (uint32_t q = 0; q < nq;) {
+            const uint32_t t = batch[q]->tenant;
+            uint32_t e = q;
+            while (e < nq && batch[e]->tenant == t) ++e;
+            s.sl.segs.push_back(TenantSeg{q, e - q, list_base[t], tenants[t]});
+            seg_key.push_back(((uint64_t)t << 32) | (e - q));
+            q = e;
+        }
+    }
     uint32_t k = 0;
     for (Request* r : batch) k = std::max(k, r->k);
     int rc = SVS_OK;
@@ -172,8 +190,9 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
         if (raw) {
             rows += top_rows[r->tenant][r->nprobe];
         } else {
+            const svs_index* ti = tenants[r->tenant];
             for (uint32_t c : r->probes)
-                if (c < idx->lists.size()) rows += idx->lists[c].m;
+                if (c < ti->lists.size()) rows += ti->lists[c].m;
         }
     }
     const bool dplan = cfg.mode != SVS_MODE_PLAIN && ctx->planner &&
@@ -203,7 +222,17 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
         for (uint32_t q = 0; q < nq; ++q) {
             const Request* r = batch[q];
             const uint32_t np = raw ? r->nprobe : (uint32_t)r->probes.size();
-            if (!raw) std::copy(r->probes.begin(), r->probes.end(), s.h_plist.p + s.h_pptr.p[q]);
+            if (!raw && !combined) {
+                std::copy(r->probes.begin(), r->probes.end(), s.h_plist.p + s.h_pptr.p[q]);
+            } else if (!raw) {
+                // a tenant's list c sits at list_base + c of the combined
+                // table; ids past the tenant's lists stay out of range
+                const uint32_t nl = (uint32_t)tenants[r->tenant]->lists.size();
+                const uint32_t total = (uint32_t)combined->lists.size();
+                uint32_t* dst = s.h_plist.p + s.h_pptr.p[q];
+                for (size_t i = 0; i < r->probes.size(); ++i)
+                    dst[i] = r->probes[i] < nl ? list_base[r->tenant] + r->probes[i] : total;
+            }
             s.h_pptr.p[q + 1] = s.h_pptr.p[q] + np;
             s.h_ver.p[q] = r->verify;
             std::memcpy(s.h_seeds.p + (size_t)q * 32, r->seed, 32);
@@ -237,7 +266,7 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
             if (rc == SVS_OK) rc = view_reserve(ctx, s.sl, v, k);
             uint64_t qbits;
             std::memcpy(&qbits, &qscale, 8);
-            const std::vector<uint64_t> key{
+            std::vector<uint64_t> key{
                 3, idx->gen, raw, nq, n_items, k, (uint64_t)kernel,
                 (uint64_t)variant, v.any_verify, v.planes_ok, cap, qbits,
                 (uint64_t)(uintptr_t)s.h_pptr.p, (uint64_t)(uintptr_t)s.h_plist.p,
@@ -245,7 +274,8 @@ void svs_server::launch(Slot& s, std::vector<Request*>& batch) {
                 (uint64_t)(uintptr_t)s.h_vq.p, (uint64_t)(uintptr_t)s.h_seeds.p,
                 (uint64_t)(uintptr_t)s.h_ids.p, (uint64_t)(uintptr_t)s.h_scores.p,
                 (uint64_t)(uintptr_t)s.h_count.p, (uint64_t)(uintptr_t)s.h_status.p,
-                (uint64_t)(uintptr_t)s.h_flag.p};
+                (uint64_t)(uintptr_t)s.h_flag.p, (uint64_t)seg_key.size()};
+            key.insert(key.end(), seg_key.begin(), seg_key.end());
             if (rc == SVS_OK)
                 rc = run_graphed(ctx, s.sl, key, [&]() -> int {
                     SVS_TRY(plan_stage(s.sl, v, s.h_pptr.p, raw ? nullptr : s.h_plist.p,
@@ -373,14 +403,18 @@ void svs_server::run() {
             for (auto& sp : slots) any_busy |= sp->busy;
             if (stop && pending.empty() && !any_busy) return;
             if (free_slot && !pending.empty()) {
-                // FIFO over requests; a batch holds one tenant's requests
+                // FIFO over requests; a batch holds requests of one kind (raw
+                // with one quantisation scale, or routed) and, unless the
+                // server batches across tenants, of one tenant
                 const uint32_t tenant = pending.front()->tenant;
                 const bool raw = pending.front()->raw;
+                const uint64_t qq = pending.front()->qq;
                 uint64_t rows = 0;
                 uint32_t items = 0, kmax = 0;
                 for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch;) {
                     Request* r = *it;
-                    if (r->tenant != tenant || r->raw != raw) {
+                    if ((!combined && r->tenant != tenant) || r->raw != raw ||
+                        (raw && r->qq != qq)) {
                         ++it;
                         continue;
                     }
@@ -415,7 +449,15 @@ void svs_server::run() {
                 continue;
             }
         }
-        if (!batch.empty()) launch(*free_slot, batch);
+        if (!batch.empty()) {
+            // a cross-tenant batch runs in tenant segments (arrival order
+            // kept within a tenant)
+            if (combined)
+                std::stable_sort(batch.begin(), batch.end(), [](const Request* a, const Request* b) {
+                    return a->tenant < b->tenant;
+                });
+            launch(*free_slot, batch);
+        }
     }
 }
 
@@ -451,6 +493,51 @@ extern "C" int svs_server_create(svs_index* const* tenants, uint32_t n_tenants,
             for (size_t n = 0; n < tr.size(); ++n) tr[n] = top_rows(t, (uint32_t)n);
             s->top_rows.push_back(std::move(tr));
         }
+    }
+    if (cfg->cross_tenant && n_tenants > 1) {
+        // one list table over every tenant's lists (device copies of their
+        // list descriptors; the lists stay the tenants')
+        uint32_t d = 0;
+        for (svs_index* t : s->tenants)
+            if (t->cent_dev) {
+                if (d && t->cent_d != d)
+                    return fail(SVS_ERR_ARG, "cross-tenant batching needs one centroid dimension");
+                d = t->cent_d;
+            }
+        auto* u = new svs_index();
+        u->ctx = ctx;
+        u->view = true;
+        u->gen = next_index_gen();
+        u->cent_d = d;
+        for (svs_index* t : s->tenants) {
+            s->list_base.push_back((uint32_t)u->lists.size());
+            for (const ListHost& L : t->lists) {
+                ListHost c;
+                c.m = L.m;
+                c.d = L.d;
+                u->lists.push_back(c);
+            }
+        }
+        if (u->lists.size() >= PLAN_MAX_LISTS) {
+            delete u;
+            return fail(SVS_ERR_ARG, "cross-tenant batching takes up to %u lists in all",
+                        PLAN_MAX_LISTS - 1);
+        }
+        std::lock_guard<std::mutex> g(ctx->mu);
+        HIP_TRY(hipSetDevice(ctx->device));
+        int rc = dev_alloc(ctx, (void**)&u->lists_dev, sizeof(ListDev) * u->lists.size());
+        for (size_t t = 0; rc == SVS_OK && t < s->tenants.size(); ++t)
+            if (!s->tenants[t]->lists.empty() &&
+                hipMemcpy(u->lists_dev + s->list_base[t], s->tenants[t]->lists_dev,
+                          sizeof(ListDev) * s->tenants[t]->lists.size(),
+                          hipMemcpyDeviceToDevice) != hipSuccess)
+                rc = fail(SVS_ERR_HIP, "copying list descriptors failed");
+        if (rc != SVS_OK) {
+            if (u->lists_dev) dev_free(ctx, u->lists_dev, sizeof(ListDev) * u->lists.size());
+            delete u;
+            return rc;
+        }
+        s->combined = u;
     }
     s->cfg = *cfg;
     if (s->cfg.slot_bytes == 0)
@@ -492,6 +579,7 @@ extern "C" int svs_server_destroy(svs_server* s) {
                 for (auto& e : sp->sl.prof.ev) hipEventDestroy(e);
             hipEventDestroy(sp->done_ev);
             hipStreamDestroy(sp->sl.stream);
+            sp->sl.segs.clear();
             sp->h_vq.release();
             sp->h_seeds.release();
             sp->h_pptr.release();
@@ -504,6 +592,11 @@ extern "C" int svs_server_destroy(svs_server* s) {
             sp->h_scores.release();
             sp->h_status.release();
         }
+    }
+    if (s->combined) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        dev_free(ctx, s->combined->lists_dev, sizeof(ListDev) * s->combined->lists.size());
+        delete s->combined;
     }
     for (auto& kv : s->by_ticket) delete kv.second;
     delete s;

@@ -97,7 +97,7 @@ class RawBatch(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("slots", C.c_uint32), ("slot_bytes", C.c_uint64),
-                ("mode", C.c_int)]
+                ("mode", C.c_int), ("cross_tenant", C.c_uint32)]
 
 
 class Request(C.Structure):

@@ -62,11 +62,11 @@ def _any_request(rq, keep):
 
 class Server:
     def __init__(self, tenants, max_batch: int = 256, slots: int = 2, slot_bytes: int = 0,
-                 mode: int = L.MODE_BATCHED):
+                 mode: int = L.MODE_BATCHED, cross_tenant: bool = False):
         tenants = [tenants] if isinstance(tenants, GpuIndex) else list(tenants)
         self._tenants = tenants  # the indexes must outlive the server
         arr = (C.c_void_p * len(tenants))(*[t.h for t in tenants])
-        cfg = L.ServerConfig(max_batch, slots, slot_bytes, mode)
+        cfg = L.ServerConfig(max_batch, slots, slot_bytes, mode, 1 if cross_tenant else 0)
         h = C.c_void_p()
         check(L.lib().svs_server_create(arr, len(tenants), C.byref(cfg), C.byref(h)))
         self.h = h

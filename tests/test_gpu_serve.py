@@ -242,3 +242,53 @@ def test_served_raw_queries_equal_alone(ctx, tenant_a, tenant_b):
             assert res.scores.tolist() == scores.tolist(), f"request {i}"
         else:
             _check(res, t, rq, f"request {i}")
+
+
+@pytest.mark.parametrize("cross", [True, False])
+def test_served_cross_tenant_batches(ctx, tenant_a, tenant_b, cross):
+    """Requests of two tenants queued together: with cross-tenant batching a
+    batch mixes them (each query encoded with its tenant's keys, routed on
+    its tenant's centroids and scored on its tenant's lists through the
+    combined list table), without it a batch holds one tenant; either way
+    every result equals the same request alone. Probe ids past a tenant's
+    own lists (which would be the other tenant's lists in the combined
+    table) score nothing, as alone."""
+    import svs
+    rng = np.random.default_rng(21 if cross else 22)
+    reqs = []
+    for i in range(64):
+        t, ti = (tenant_a, 0) if i % 2 else (tenant_b, 1)
+        if i % 3 == 0:
+            qi = int(rng.integers(0, t["qs"].shape[0]))
+            rq = dict(tenant=ti, query=np.asarray(t["qs"][qi], np.float32),
+                      nprobe=int(rng.choice([1, 3, 8])), q_seed=rng.integers(0, 256, 32, np.uint8),
+                      verify=bool(rng.integers(0, 2)), k=int(rng.choice([1, 10, 16])))
+        else:
+            rq = _requests(t, rng, 1, tenant=ti)[0]
+            if i % 5 == 0:  # one probe past the tenant's own lists
+                n_own = len(t["lists"])
+                rq["probes"] = np.append(rq["probes"], np.uint32(n_own + 2))
+        reqs.append(rq)
+    with svs.Server([tenant_a["handle"].index, tenant_b["handle"].index], max_batch=32,
+                    slots=1, cross_tenant=cross) as srv:
+        tickets = []
+        for rq in reqs:
+            if rq.get("query") is not None:
+                tickets.append(srv.submit_raw(rq["query"], rq["q_seed"], rq["nprobe"],
+                                              rq["verify"], rq["k"], tenant=rq["tenant"]))
+            else:
+                tickets.append(srv.submit(rq["v_q"], rq["q_seed"], rq["probes"], rq["verify"],
+                                          rq["k"], tenant=rq["tenant"]))
+        results = [srv.wait(tk) for tk in tickets]
+        st = srv.stats()
+    say("cross-tenant" if cross else "one tenant per batch", st)
+    assert st["completed"] == len(reqs)
+    for i, (res, rq) in enumerate(zip(results, reqs)):
+        t = tenant_a if rq["tenant"] == 0 else tenant_b
+        if rq.get("query") is not None:
+            ids, scores, status = _alone_raw(t, rq)
+            assert res.status == status, f"request {i}"
+            assert res.ids.tolist() == ids.tolist(), f"request {i}"
+            assert res.scores.tolist() == scores.tolist(), f"request {i}"
+        else:
+            _check(res, t, rq, f"request {i}")
