@@ -166,6 +166,7 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
     if (const char* g = getenv("SVS_TILE_GRID")) ctx->tile_grid = std::max(1, std::min(8, atoi(g)));
+    if (const char* g = getenv("SVS_PLANE_MAP")) ctx->plane_map = atoi(g) != 0;
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -892,23 +893,33 @@ static void launch_gemm_vb(hipStream_t st, uint32_t blocks, const TileArgs& a,
 // query group (a plan tile holds up to QG queries of one list). The grid is
 // the resident blocks (one per CU and wave slot); units are dealt as
 // described at k_prod_planes.
-template <int QW, int PF, int OCC>
+template <int KE, int SRC, int QW, int PF, int OCC>
 static void launch_planes_v(svs_ctx* ctx, hipStream_t st, uint64_t cap_tiles, const TileArgs& a,
                             const int8_t* qplanes, const uint64_t* qc, uint64_t* out) {
     const uint64_t units = 64 * ((std::max<uint64_t>(cap_tiles, 1) + 3) / 4);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((units + 3) / 4, (uint64_t)ctx->n_cu * OCC * ctx->tile_grid));
-    hipLaunchKernelGGL((k_prod_planes<QW, PF, OCC>), dim3(blocks), dim3(256), 0, st, a.lists,
-                       a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc, out);
+    if (ctx->plane_map)
+        hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC, 1>), dim3(blocks), dim3(256), 0,
+                           st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc,
+                           out);
+    else
+        hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC, 0>), dim3(blocks), dim3(256), 0,
+                           st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc,
+                           out);
 }
 
+// The plane product over the plan's tiles: SRC 0 the server product
+// (KE = n over M_enc), SRC 1 the dense decode term (KE = 2 n1 over [AL | H]).
+template <int KE, int SRC>
 static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tiles,
                           const TileArgs& a, const int8_t* qplanes, const uint64_t* qc,
                           uint64_t* out) {
+    constexpr int PF16 = KE / 64 >= 4 ? 2 : 1;
     switch (kTileVariants[v].qg) {
-        case 16: launch_planes_v<16, 2, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
-        case 4: launch_planes_v<4, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
-        default: launch_planes_v<8, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        case 16: launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        case 4: launch_planes_v<KE, SRC, 4, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        default: launch_planes_v<KE, SRC, 8, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
     }
 }
 
@@ -1251,11 +1262,12 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
         } else {
             // byte planes of the queries, then the plane-by-plane product
             SVS_TRY(sl.q_pplanes.ensure(ctx, (size_t)nq * 8 * N));
-            hipLaunchKernelGGL(k_query_planes, dim3(nq), dim3(256), 0, st,
+            hipLaunchKernelGGL(k_query_planes<N>, dim3(nq), dim3(256), 0, st,
                                (const uint64_t*)d_venc, (int8_t*)sl.q_pplanes.ptr);
             const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
-            launch_planes(ctx, variant, st, v.cap_tiles, ta, (const int8_t*)sl.q_pplanes.ptr,
-                          (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+            launch_planes<N, 0>(ctx, variant, st, v.cap_tiles, ta,
+                                (const int8_t*)sl.q_pplanes.ptr, (const uint64_t*)sl.q_corr.ptr,
+                                (uint64_t*)sl.r.ptr);
             ctx->launches++;
         }
         ctx->launches += 2;
@@ -1293,11 +1305,21 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             ctx->launches++;
             r_check = (const uint64_t*)sl.rv.ptr;
         }
-        hipLaunchKernelGGL(k_verify_chunks, dim3(row_grid(ctx, std::max<uint64_t>(v.cap_chunks, 1))),
-                           dim3(ROW_THREADS), 0, st, d_lists, d_items, v.n_items, d_chunk_ptr,
-                           d_cand_ptr, (const uint64_t*)d_venc, (const uint64_t*)sl.r.ptr, r_check,
-                           (const uint32_t*)d_vseed, v.flags, (uint64_t*)sl.partials.ptr,
-                           d_rng_flag);
+        if (r_check)
+            hipLaunchKernelGGL(k_verify_rows,
+                               dim3(row_grid(ctx, std::max<uint64_t>(
+                                                      (v.cap_chunks + VR_GROUP - 1) / VR_GROUP, 1))),
+                               dim3(ROW_THREADS), 0, st, d_items, v.n_items, d_chunk_ptr,
+                               d_cand_ptr, (const uint64_t*)sl.r.ptr, r_check,
+                               (const uint32_t*)d_vseed, v.flags, (uint64_t*)sl.partials.ptr,
+                               d_rng_flag);
+        else
+            hipLaunchKernelGGL(k_verify_chunks,
+                               dim3(row_grid(ctx, std::max<uint64_t>(v.cap_chunks, 1))),
+                               dim3(ROW_THREADS), 0, st, d_lists, d_items, v.n_items, d_chunk_ptr,
+                               d_cand_ptr, (const uint64_t*)d_venc, (const uint64_t*)sl.r.ptr,
+                               r_check, (const uint32_t*)d_vseed, v.flags,
+                               (uint64_t*)sl.partials.ptr, d_rng_flag);
         hipLaunchKernelGGL(k_verify_finalize,
                            dim3((uint32_t)std::min<uint64_t>(ni1, (uint64_t)ctx->n_cu * 8)),
                            dim3(VF_THREADS), 0, st, d_items,
@@ -1314,12 +1336,12 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
     SVS_TRY(sl.scores.ensure(ctx, sizeof(int64_t) * nc1));
     SVS_TRY(prof_mark(sl, 3, true));
     if (v.cap_cands) {
-        // The dense decode terms AL G + H L^T v_enc: the same MFMA tiles over
-        // the row-major [AL | H] when the batch runs on block tiles, else
-        // inside the per-candidate decode kernel.
+        // The dense decode terms AL G + H L^T v_enc: block tiles over the
+        // row-major [AL | H] on the product's tiles when the batch runs on
+        // block tiles, else a row-streaming pass before the per-candidate
+        // decode kernel.
         const uint64_t* dense = nullptr;
         const bool tiles = kernel == KERNEL_MFMA && v.cap_chunks;
-        const uint32_t tblocks = tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ);
         const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
         if (!tiles && v.cap_chunks) {
             // one query per list: the dense terms stream the row-major
@@ -1333,12 +1355,18 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             dense = (const uint64_t*)sl.dense.ptr;
         }
         if (tiles) {
+            // the dense terms AL G + H L^T v_enc on the window-formulation
+            // tiles over [AL | H] (K = 256: four 64-element steps are too few
+            // for the plane kernel's per-unit set-up to pay, measured 0.1-0.2
+            // ms slower per 256-query batch)
             SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
             uint64_t* qc_gl = (uint64_t*)sl.q_corr.ptr + nq;
             hipLaunchKernelGGL(k_query_corr<2 * N1>, dim3(nq), dim3(WAVE), 0, st,
                                (const uint64_t*)d_gl, qc_gl);
-            launch_gemm<2 * N1, 1>(variant, st, tblocks, ta, (const uint64_t*)d_gl,
-                                   (const uint64_t*)qc_gl, (uint64_t*)sl.dense.ptr);
+            launch_gemm<2 * N1, 1>(variant, st,
+                                   tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ), ta,
+                                   (const uint64_t*)d_gl, (const uint64_t*)qc_gl,
+                                   (uint64_t*)sl.dense.ptr);
             ctx->launches += 2;
             dense = (const uint64_t*)sl.dense.ptr;
         }

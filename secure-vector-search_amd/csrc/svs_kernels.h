@@ -138,6 +138,38 @@ k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ item
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (r_check) {
+            // batches: the row products come recomputed; lane 16 t + i takes
+            // trial t of the chunk's row i, then each trial's 16 lanes sum
+            // their u r and u (M v) terms (add_mod butterfly)
+            static_assert(LAMBDA * CHUNK_ROWS <= WAVE, "one lane per (trial, row)");
+            const uint32_t t = lane / CHUNK_ROWS, i = lane % CHUNK_ROWS;
+            uint64_t ur = 0, zv = 0;
+            bool rej = false;
+            if (t < LAMBDA && i < rows) {
+                const uint64_t fb = (2ull * ((uint64_t)t * m + row0)) / 16;
+                const uint64_t wpos = 2ull * ((uint64_t)t * m + row0 + i) - fb * 16;
+                const uint64_t raw = (uint64_t)words[wave][t][wpos] |
+                                     ((uint64_t)words[wave][t][wpos + 1] << 32);
+                const uint64_t u = raw & P;
+                rej = u == P;
+                const size_t at = (size_t)cand_ptr[it] + row0 + i;
+                ur = mul_mod(u, r_claim[at]);
+                zv = mul_mod(u, r_check[at]);
+            }
+#pragma unroll
+            for (int off = CHUNK_ROWS / 2; off >= 1; off >>= 1) {
+                ur = add_mod(ur, shfl_xor_u64(ur, off));
+                zv = add_mod(zv, shfl_xor_u64(zv, off));
+            }
+            if (t < LAMBDA && i == 0) {
+                partials[(size_t)ch * 2 * LAMBDA + t] = ur;
+                partials[(size_t)ch * 2 * LAMBDA + LAMBDA + t] = zv;
+            }
+            if (rej) atomicOr(rng_flag, 1u);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
         const ulonglong2* vq = reinterpret_cast<const ulonglong2*>(venc + (size_t)item.q * N);
         ulonglong2 v[8];
         if (!r_check) {
@@ -182,6 +214,92 @@ k_verify_chunks(const ListDev* __restrict__ lists, const Item* __restrict__ item
             partials[(size_t)ch * 2 * LAMBDA + lane] = ur.reduce();
             partials[(size_t)ch * 2 * LAMBDA + LAMBDA + lane] = zv.reduce();
             if (rejected) atomicOr(rng_flag, 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Batched verification, the row products recomputed (r_check): a wave takes
+// VR_GROUP consecutive chunks. Its lanes first generate the 3 x 3 ChaCha20
+// blocks that cover each chunk's u draws (lane 9 c + 3 t + b: chunk c, trial
+// t, block b; draw t m + i is keystream words 2 (t m + i), + 1), so 63 of 64
+// lanes run the stream where one chunk at a time would keep 9; then, chunk by
+// chunk, lane 16 t + i takes trial t of row i and each trial's 16 lanes sum
+// u r and u (M v) (add_mod butterfly) into the chunk's partials, as
+// k_verify_chunks writes them.
+constexpr uint32_t VR_GROUP = 7;
+__global__ void __launch_bounds__(ROW_THREADS)
+k_verify_rows(const Item* __restrict__ items, uint32_t n_items,
+              const uint32_t* __restrict__ chunk_ptr, const uint32_t* __restrict__ cand_ptr,
+              const uint64_t* __restrict__ r_claim, const uint64_t* __restrict__ r_check,
+              const uint32_t* __restrict__ vseed_words, const uint8_t* __restrict__ item_verify,
+              uint64_t* __restrict__ partials, uint32_t* __restrict__ rng_flag) {
+    static_assert(VR_GROUP * 3 * LAMBDA <= WAVE && LAMBDA * CHUNK_ROWS <= WAVE, "lane roles");
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t wave = threadIdx.x / WAVE;
+    const uint32_t wpb = blockDim.x / WAVE;
+    __shared__ uint32_t words[ROW_THREADS / WAVE][VR_GROUP][LAMBDA][48];
+    const uint32_t n_chunks = chunk_ptr[n_items];
+    const uint32_t n_groups = (n_chunks + VR_GROUP - 1) / VR_GROUP;
+    for (uint32_t gi = blockIdx.x * wpb + wave; gi < n_groups; gi += gridDim.x * wpb) {
+        const uint32_t ch0 = gi * VR_GROUP;
+        if (lane < VR_GROUP * 3 * LAMBDA) {
+            const uint32_t c = lane / (3 * LAMBDA), t = (lane / 3) % LAMBDA, b = lane % 3;
+            const uint32_t ch = ch0 + c;
+            if (ch < n_chunks) {
+                const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+                if (item_verify[it]) {
+                    const Item item = items[it];
+                    const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
+                    const uint64_t fb = (2ull * ((uint64_t)t * item.m + row0)) / 16;
+                    uint32_t key[8];
+#pragma unroll
+                    for (int w = 0; w < 8; ++w) key[w] = vseed_words[(size_t)item.q * 8 + w];
+                    uint32_t out[16];
+                    chacha20_block(key, fb + b, out);
+#pragma unroll
+                    for (int w = 0; w < 16; ++w) words[wave][c][t][b * 16 + w] = out[w];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t c = 0; c < VR_GROUP; ++c) {
+            const uint32_t ch = ch0 + c;
+            if (ch >= n_chunks) break;
+            const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+            if (!item_verify[it]) {
+                if (lane < 2 * LAMBDA) partials[(size_t)ch * 2 * LAMBDA + lane] = 0;
+                continue;
+            }
+            const Item item = items[it];
+            const uint32_t m = item.m;
+            const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
+            const uint32_t rows = min((uint32_t)CHUNK_ROWS, m - row0);
+            const uint32_t t = lane / CHUNK_ROWS, i = lane % CHUNK_ROWS;
+            uint64_t ur = 0, zv = 0;
+            bool rej = false;
+            if (t < LAMBDA && i < rows) {
+                const uint64_t fb = (2ull * ((uint64_t)t * m + row0)) / 16;
+                const uint64_t wpos = 2ull * ((uint64_t)t * m + row0 + i) - fb * 16;
+                const uint64_t raw = (uint64_t)words[wave][c][t][wpos] |
+                                     ((uint64_t)words[wave][c][t][wpos + 1] << 32);
+                const uint64_t u = raw & P;
+                rej = u == P;
+                const size_t at = (size_t)cand_ptr[it] + row0 + i;
+                ur = mul_mod(u, r_claim[at]);
+                zv = mul_mod(u, r_check[at]);
+            }
+#pragma unroll
+            for (int off = CHUNK_ROWS / 2; off >= 1; off >>= 1) {
+                ur = add_mod(ur, shfl_xor_u64(ur, off));
+                zv = add_mod(zv, shfl_xor_u64(zv, off));
+            }
+            if (t < LAMBDA && i == 0) {
+                partials[(size_t)ch * 2 * LAMBDA + t] = ur;
+                partials[(size_t)ch * 2 * LAMBDA + LAMBDA + t] = zv;
+            }
+            if (rej) atomicOr(rng_flag, 1u);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -1711,7 +1829,10 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 
 // ---------------------------------------------------------------------------
 // Byte plane by byte plane F_p products on int8 MFMA (the batched server
-// product). With signed byte limbs s_a = byte_a - 128 of a row element and
+// product, SRC 0: M_enc rows of KE = 1024 against v_enc; SRC 1 takes the
+// [AL | H] rows of KE = 256 against [G ; L^T v_enc], which the window tiles
+// do faster at so short a K).
+// With signed byte limbs s_a = byte_a - 128 of a row element and
 // t_b of a query element (identity (*) above),
 //   sum_k M'_k V'_k = sum_{a, b < 8} 256^(a+b) C_ab,  C_ab = sum_k s_a[k] t_b[k],
 // |C_ab| <= 1024 * 2^14 = 2^24. One v_mfma_i32_16x16x64_i8 contracts 64
@@ -1743,14 +1864,15 @@ __device__ __forceinline__ v4i load16b(const int8_t* p) {
 // Query planes for k_prod_planes: per query 16 steps x 4 groups x 8 planes x
 // 16 bytes; byte n of (step s, group g, plane b) is byte b of v[e] ^ 0x80
 // with e = 64 s + 8 (n / 2) + 2 g + n % 2.
+template <int KE>
 __global__ void __launch_bounds__(256)
 k_query_planes(const uint64_t* __restrict__ vecs, int8_t* __restrict__ qplanes) {
     const uint32_t q = blockIdx.x;
-    for (int e = threadIdx.x; e < N; e += 256) {
-        const uint64_t v = vecs[(size_t)q * N + e];
+    for (int e = threadIdx.x; e < KE; e += 256) {
+        const uint64_t v = vecs[(size_t)q * KE + e];
         const int s = e / 64, o = e % 64;
         const int j = o / 8, rem = o % 8, g = rem / 2, n = 2 * j + rem % 2;
-        int8_t* dst = qplanes + (size_t)q * 8 * N + ((s * 4 + g) * 8) * 16 + n;
+        int8_t* dst = qplanes + (size_t)q * 8 * KE + ((s * 4 + g) * 8) * 16 + n;
 #pragma unroll
         for (int b = 0; b < 8; ++b) dst[16 * b] = (int8_t)(((v >> (8 * b)) & 0xFF) ^ 0x80);
     }
@@ -1782,16 +1904,16 @@ __device__ __forceinline__ void planes_of(const v4i (&L)[8], v4i (&pl)[8]) {
         }
 }
 
-template <int QW, int PF, int OCC>
+template <int KE, int SRC, int QW, int PF, int OCC, int MAP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
               const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
               const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
               const int8_t* __restrict__ qplanes, const uint64_t* __restrict__ qcorr,
               uint64_t* __restrict__ out) {
-    constexpr int KS = N / 64;  // 16 steps of 64 elements
-    constexpr int NB = QW / 2;  // B tiles of two queries
-    constexpr uint32_t QBYTES = 8 * N;
+    constexpr int KS = KE / 64;  // steps of 64 elements (16 over M_enc, 4 over [AL | H])
+    constexpr int NB = QW / 2;   // B tiles of two queries
+    constexpr uint32_t QBYTES = 8 * KE;
     static_assert(QW == 4 || QW == 8 || QW == 16, "query group");
     static_assert(KS % PF == 0 && PF >= 1, "the ring must tile the steps");
     // the wave index as a wave-uniform value: unit indices, tile fields and
@@ -1801,7 +1923,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
     const XcdShare sh = xcd_share(tiles, *n_tiles_p);
-    const uint32_t n_units = 64u * ((sh.hi - sh.lo + 3) / 4);
+    const uint32_t n_units = MAP ? 16u * (sh.hi - sh.lo) : 64u * ((sh.hi - sh.lo + 3) / 4);
     const uint32_t ustride = 4u * sh.workers;
 
     // the first non-empty unit at or after x (stepping by ustride), or n_units
@@ -1815,19 +1937,22 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         Unit r;
         r.u = n_units;
         for (; x < n_units; x += ustride) {
-            // unit x: 16-row block (x / 4) % 16 of tile lo + 4 (x / 64) + x % 4
-            const uint32_t ti = sh.lo + (((x >> 6) << 2) | (x & 3u));
+            // MAP 0, unit x: 16-row block (x / 4) % 16 of tile lo + 4 (x / 64)
+            // + x % 4 (a block's waves: one row block, four query groups);
+            // MAP 1: 16-row block x % 16 of tile lo + x / 16 (a block's
+            // waves: four row blocks, one query group)
+            const uint32_t ti = MAP ? sh.lo + (x >> 4) : sh.lo + (((x >> 6) << 2) | (x & 3u));
             if (ti >= sh.hi) continue;
             const MTile t = tiles[ti];
             const uint32_t m = lists[t.list].m;
-            const uint32_t row0 = t.row0 + 16u * ((x >> 2) & 15u);
+            const uint32_t row0 = t.row0 + 16u * (MAP ? (x & 15u) : ((x >> 2) & 15u));
             if (row0 < m) {
                 r.u = x;
                 r.m = m;
                 r.row0 = row0;
                 r.tile = t;
-                r.menc = lists[t.list].menc;
-                r.rowcorr = lists[t.list].rowcorr;
+                r.menc = SRC ? lists[t.list].alh : lists[t.list].menc;
+                r.rowcorr = SRC ? lists[t.list].alh_rowcorr : lists[t.list].rowcorr;
                 return r;
             }
         }
@@ -1838,7 +1963,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     // load here would make its wait drain the ring of A loads in flight.
     auto lane_ptrs = [&](const Unit& x, const uint64_t*& ap, uint32_t (&qb)[NB]) {
         const uint32_t row = min(x.row0 + (uint32_t)col, x.m - 1);
-        ap = x.menc + (size_t)row * N + 2 * g;
+        ap = x.menc + (size_t)row * KE + 2 * g;
         const uint32_t last = x.tile.gn - 1;
 #pragma unroll
         for (int jb = 0; jb < NB; ++jb) {
