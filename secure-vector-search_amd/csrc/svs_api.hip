@@ -166,7 +166,6 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
     if (const char* g = getenv("SVS_TILE_GRID")) ctx->tile_grid = std::max(1, std::min(8, atoi(g)));
-    if (const char* g = getenv("SVS_PLANE_MAP")) ctx->plane_map = atoi(g) != 0;
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -899,14 +898,8 @@ static void launch_planes_v(svs_ctx* ctx, hipStream_t st, uint64_t cap_tiles, co
     const uint64_t units = 64 * ((std::max<uint64_t>(cap_tiles, 1) + 3) / 4);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((units + 3) / 4, (uint64_t)ctx->n_cu * OCC * ctx->tile_grid));
-    if (ctx->plane_map)
-        hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC, 1>), dim3(blocks), dim3(256), 0,
-                           st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc,
-                           out);
-    else
-        hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC, 0>), dim3(blocks), dim3(256), 0,
-                           st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc,
-                           out);
+    hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC>), dim3(blocks), dim3(256), 0, st,
+                       a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc, out);
 }
 
 // The plane product over the plan's tiles: SRC 0 the server product

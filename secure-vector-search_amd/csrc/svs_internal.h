@@ -161,10 +161,6 @@ struct svs_ctx : svs_slot {
     int planner = 1;
     // persistent tile grids: resident blocks per CU times this
     int tile_grid = 1;
-    // work-unit order of the plane product (SVS_PLANE_MAP): 0 = a block's
-    // waves take one row block of four query groups, 1 = four row blocks
-    // of one query group
-    int plane_map = 0;
     // device-planned batches replay captured HIP graphs (1) or launch (0)
     int graphs = 1;
     // pinned staging of caller inputs for graphed batches

@@ -1904,7 +1904,7 @@ __device__ __forceinline__ void planes_of(const v4i (&L)[8], v4i (&pl)[8]) {
         }
 }
 
-template <int KE, int SRC, int QW, int PF, int OCC, int MAP>
+template <int KE, int SRC, int QW, int PF, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
               const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
@@ -1923,7 +1923,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
     const XcdShare sh = xcd_share(tiles, *n_tiles_p);
-    const uint32_t n_units = MAP ? 16u * (sh.hi - sh.lo) : 64u * ((sh.hi - sh.lo + 3) / 4);
+    const uint32_t n_units = 64u * ((sh.hi - sh.lo + 3) / 4);
     const uint32_t ustride = 4u * sh.workers;
 
     // the first non-empty unit at or after x (stepping by ustride), or n_units
@@ -1937,15 +1937,15 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         Unit r;
         r.u = n_units;
         for (; x < n_units; x += ustride) {
-            // MAP 0, unit x: 16-row block (x / 4) % 16 of tile lo + 4 (x / 64)
-            // + x % 4 (a block's waves: one row block, four query groups);
-            // MAP 1: 16-row block x % 16 of tile lo + x / 16 (a block's
-            // waves: four row blocks, one query group)
-            const uint32_t ti = MAP ? sh.lo + (x >> 4) : sh.lo + (((x >> 6) << 2) | (x & 3u));
+            // unit x: 16-row block (x / 4) % 16 of tile lo + 4 (x / 64) + x % 4
+            // (a block's waves: one row block, four query groups; the other
+            // order, four row blocks of one query group so that the waves
+            // share the query planes, read 5 % less and ran 3 % slower)
+            const uint32_t ti = sh.lo + (((x >> 6) << 2) | (x & 3u));
             if (ti >= sh.hi) continue;
             const MTile t = tiles[ti];
             const uint32_t m = lists[t.list].m;
-            const uint32_t row0 = t.row0 + 16u * (MAP ? (x & 15u) : ((x >> 2) & 15u));
+            const uint32_t row0 = t.row0 + 16u * ((x >> 2) & 15u);
             if (row0 < m) {
                 r.u = x;
                 r.m = m;
