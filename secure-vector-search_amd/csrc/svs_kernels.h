@@ -1975,12 +1975,23 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         }
     };
     v4i ra[PF][8], rb[PF][NB];
-    auto issue = [&](int slot, const uint64_t* ap, const uint32_t (&qb)[NB], int s) {
+    auto issue_a = [&](int slot, const uint64_t* ap, int s) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) ra[slot][j] = load16(ap + 64 * s + 8 * j);
+    };
+    auto issue_b = [&](int slot, const uint32_t* qb, int s) {
 #pragma unroll
         for (int jb = 0; jb < NB; ++jb) rb[slot][jb] = load16b(qplanes + qb[jb] + 512u * s);
     };
+    auto issue = [&](int slot, const uint64_t* ap, const uint32_t (&qb)[NB], int s) {
+        issue_a(slot, ap, s);
+        issue_b(slot, qb, s);
+    };
+    // 16 queries per wave: the MFMAs read the B operands from the ring slot
+    // and the slot is refilled after them (no copy of B, and the A slot's
+    // registers hold the planes meanwhile), which keeps the wave within 512
+    // registers; narrower waves copy B and refill the whole slot first
+    constexpr bool LATE_B = QW >= 16;
 
     Unit cur = find(4u * sh.worker + wave);
     if (cur.u >= n_units) return;
@@ -2015,26 +2026,37 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             const int slot = s % PF;
             v4i pl[8];
             planes_of(ra[slot], pl);
-            v4i bq[NB];
-#pragma unroll
-            for (int jb = 0; jb < NB; ++jb) bq[jb] = rb[slot][jb];
-            // refill the slot: this unit's step s + PF, or the next unit's
-            // first steps (the last unit re-loads its own final step)
-            if (s + PF < KS) {
-                issue(slot, a_cur, qb_cur, s + PF);
-            } else {
-                issue(slot, has_next ? a_nxt : a_cur, has_next ? qb_nxt : qb_cur,
-                      has_next ? s + PF - KS : KS - 1);
-            }
+            // the slot's next contents: this unit's step s + PF, or the next
+            // unit's first steps (the last unit re-loads its own final step)
+            const bool in_unit = s + PF < KS;
+            const uint64_t* ap = in_unit ? a_cur : (has_next ? a_nxt : a_cur);
+            const uint32_t* qbp = in_unit ? qb_cur : (has_next ? qb_nxt : qb_cur);
+            const int sn = in_unit ? s + PF : (has_next ? s + PF - KS : KS - 1);
             // every B tile, present or not (absent columns repeat the last
             // query and are not written): a branch around MFMAs splits the
             // step into blocks whose waits drain the ring
+            if constexpr (LATE_B) {
 #pragma unroll
-            for (int jb = 0; jb < NB; ++jb)
+                for (int jb = 0; jb < NB; ++jb)
 #pragma unroll
-                for (int a = 0; a < 8; ++a)
-                    acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(pl[a], bq[jb], acc[a][jb],
-                                                                         0, 0, 0);
+                    for (int a = 0; a < 8; ++a)
+                        acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            pl[a], rb[slot][jb], acc[a][jb], 0, 0, 0);
+                issue_a(slot, ap, sn);
+                issue_b(slot, qbp, sn);
+            } else {
+                v4i bq[NB];
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) bq[jb] = rb[slot][jb];
+                issue_a(slot, ap, sn);
+                issue_b(slot, qbp, sn);
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb)
+#pragma unroll
+                    for (int a = 0; a < 8; ++a)
+                        acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            pl[a], bq[jb], acc[a][jb], 0, 0, 0);
+            }
             // one step per scheduling region: the scheduler would otherwise
             // sink each step's loads next to their use PF steps later
             __builtin_amdgcn_sched_barrier(0);
