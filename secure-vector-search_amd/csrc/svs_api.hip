@@ -166,6 +166,8 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
     if (const char* g = getenv("SVS_TILE_GRID")) ctx->tile_grid = std::max(1, std::min(8, atoi(g)));
+    if (const char* g = getenv("SVS_PROD_SPLIT")) ctx->prod_split = atoi(g) ? 1 : 0;
+    if (const char* g = getenv("SVS_PROD_SPLIT_PF")) ctx->prod_split_pf = atoi(g);
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -894,12 +896,14 @@ static void launch_gemm_vb(hipStream_t st, uint32_t blocks, const TileArgs& a,
 // described at k_prod_planes.
 template <int KE, int SRC, int QW, int PF, int OCC>
 static void launch_planes_v(svs_ctx* ctx, hipStream_t st, uint64_t cap_tiles, const TileArgs& a,
-                            const int8_t* qplanes, const uint64_t* qc, uint64_t* out) {
+                            const uint32_t* tsel, const int8_t* qplanes, const uint64_t* qc,
+                            uint64_t* out) {
     const uint64_t units = 64 * ((std::max<uint64_t>(cap_tiles, 1) + 3) / 4);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((units + 3) / 4, (uint64_t)ctx->n_cu * OCC * ctx->tile_grid));
     hipLaunchKernelGGL((k_prod_planes<KE, SRC, QW, PF, OCC>), dim3(blocks), dim3(256), 0, st,
-                       a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, qplanes, qc, out);
+                       a.lists, a.tiles, a.nt, tsel, a.gitems, a.items, a.cand_ptr, qplanes, qc,
+                       out);
 }
 
 // The plane product over the plan's tiles: SRC 0 the server product
@@ -910,9 +914,15 @@ static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tile
                           uint64_t* out) {
     constexpr int PF16 = KE / 64 >= 4 ? 2 : 1;
     switch (kTileVariants[v].qg) {
-        case 16: launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
-        case 4: launch_planes_v<KE, SRC, 4, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
-        default: launch_planes_v<KE, SRC, 8, 4, 1>(ctx, st, cap_tiles, a, qplanes, qc, out); break;
+        case 16:
+            launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);
+            break;
+        case 4:
+            launch_planes_v<KE, SRC, 4, 4, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);
+            break;
+        default:
+            launch_planes_v<KE, SRC, 8, 4, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);
+            break;
     }
 }
 
@@ -1258,9 +1268,38 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             hipLaunchKernelGGL(k_query_planes<N>, dim3(nq), dim3(256), 0, st,
                                (const uint64_t*)d_venc, (int8_t*)sl.q_pplanes.ptr);
             const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
-            launch_planes<N, 0>(ctx, variant, st, v.cap_tiles, ta,
-                                (const int8_t*)sl.q_pplanes.ptr, (const uint64_t*)sl.q_corr.ptr,
-                                (uint64_t*)sl.r.ptr);
+            if (ctx->prod_split && kTileVariants[variant].qg == 16) {
+                // row blocks of one query group: k_prod_planes over the
+                // selected tiles; of several: k_prod_shared over quads
+                const uint64_t cap = std::max<uint64_t>(v.cap_tiles, 1);
+                SVS_TRY(sl.split.ensure(ctx, sizeof(uint32_t) * (4 * cap + 2)));
+                uint32_t* stile = (uint32_t*)sl.split.ptr;
+                uint32_t* quads = stile + cap;
+                uint32_t* hdr = const_cast<uint32_t*>(v.hdr);
+                hipLaunchKernelGGL(k_plan_split, dim3(1), dim3(PLAN_THREADS), 0, st, v.tiles,
+                                   hdr, stile, quads, quads + cap);
+                const TileArgs ts{d_lists, v.tiles, hdr + PH_NS, v.gitems, d_items, d_cand_ptr};
+                launch_planes_v<N, 0, 16, 2, 1>(ctx, st, cap, ts, stile,
+                                                (const int8_t*)sl.q_pplanes.ptr,
+                                                (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+                const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>(16 * cap, (uint64_t)ctx->n_cu * ctx->tile_grid));
+                auto shared = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, d_lists, v.tiles,
+                                       (const uint32_t*)quads, (const uint32_t*)(hdr + PH_NQ),
+                                       v.gitems, d_items, d_cand_ptr,
+                                       (const int8_t*)sl.q_pplanes.ptr,
+                                       (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+                };
+                // ring depth: steps of 40 KB of LDS per block in flight
+                if (ctx->prod_split_pf == 2) shared(k_prod_shared<2>);
+                else shared(k_prod_shared<4>);
+                ctx->launches += 2;
+            } else {
+                launch_planes<N, 0>(ctx, variant, st, v.cap_tiles, ta,
+                                    (const int8_t*)sl.q_pplanes.ptr,
+                                    (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+            }
             ctx->launches++;
         }
         ctx->launches += 2;
@@ -1457,6 +1496,8 @@ void svs::release_graphs(svs_slot& sl) {
 int svs::run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
                      const std::function<int()>& enqueue) {
     if (!ctx->graphs || sl.prof.on || ctx->fault_list >= 0) return enqueue();
+    key.push_back((uint64_t)ctx->prod_split);
+    key.push_back((uint64_t)ctx->prod_split_pf);
     for (DevBuf* b : sl.slot_scratch()) {
         key.push_back((uint64_t)(uintptr_t)b->ptr);
         key.push_back(b->cap);
@@ -1809,6 +1850,13 @@ extern "C" int svs_ctx_set_planner(svs_ctx* ctx, int on_device) {
     if (!ctx) return fail(SVS_ERR_ARG, "null context");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->planner = on_device ? 1 : 0;
+    return SVS_OK;
+}
+
+extern "C" int svs_ctx_set_product_split(svs_ctx* ctx, int on) {
+    if (!ctx) return fail(SVS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->prod_split = on ? 1 : 0;
     return SVS_OK;
 }
 

@@ -65,6 +65,38 @@ SVS_HD uint64_t mul_pow2(uint64_t x, int s) {
     return r == P ? 0 : r;
 }
 
+// Byte-plane fold of the plane product kernels: with c[a] = C_ab, the
+// contraction of signed byte plane a of a row with signed byte plane b of a
+// query (|c[a]| <= 1024 * 128 * 128 = 2^24), returns
+//   2^(8b) * sum_a 2^(8a) c[a]  (mod p)   as a value in [0, p]
+// (p itself stands for 0), rot = 8b in 0..56. Exact: lo and hi (the a < 4
+// and a >= 4 halves) stay below 2^49 in magnitude; hi 2^32 = h1 2^61 +
+// h0 2^32 with hi = h1 2^29 + h0 is h1 + h0 2^32 (mod p), so z = lo +
+// h0 2^32 + h1 lies in (-2^50, 2^62); z + p for negative z, one Mersenne
+// fold and a 61-bit rotation finish it. Eight such values sum to < 2^64.
+SVS_HD uint64_t plane_fold(int32_t c0, int32_t c1, int32_t c2, int32_t c3, int32_t c4,
+                           int32_t c5, int32_t c6, int32_t c7, int rot) {
+    const int64_t lo = (int64_t)c0 + (int64_t)c1 * 256 + (int64_t)c2 * 65536 +
+                       (int64_t)c3 * 16777216;
+    const int64_t hi = (int64_t)c4 + (int64_t)c5 * 256 + (int64_t)c6 * 65536 +
+                       (int64_t)c7 * 16777216;
+    const int64_t h1 = hi >> 29;  // arithmetic: floor(hi / 2^29)
+    const uint64_t h0 = (uint64_t)hi & ((1ull << 29) - 1);
+    const int64_t z = lo + (int64_t)(h0 << 32) + h1;
+    const uint64_t zu = z < 0 ? (uint64_t)(z + (int64_t)P) : (uint64_t)z;
+    const uint64_t r = (zu & P) + (zu >> 61);
+    return rot ? ((r << rot) & P) | (r >> (61 - rot)) : r;
+}
+
+// A sum of up to eight plane_fold values plus two canonical terms, as a
+// canonical field element.
+SVS_HD uint64_t plane_finish(uint64_t folded_sum, uint64_t t1, uint64_t t2) {
+    uint64_t k = (folded_sum & P) + (folded_sum >> 61);  // <= p + 7
+    k += t1 + t2;                                         // < 2^63
+    k = (k & P) + (k >> 61);                              // <= p + 3
+    return k >= P ? k - P : k;
+}
+
 // quantise_vector's element map (crypto.rs:105-122): round(x * q) with f64
 // round-half-away-from-zero, then into [0, p) with negatives as p - (-k mod p).
 // The f64 -> i64 conversion follows Rust's `as` (NaN -> 0, out of range

@@ -109,7 +109,7 @@ struct svs_slot {
     std::vector<TenantSeg> segs;
     svs::DevBuf items, prefixes, flags;
     svs::DevBuf q_vq, q_seeds, q_venc, q_gl, q_tdense, q_tlist, q_tcnt, q_vseed, q_vplanes,
-        q_pplanes;
+        q_pplanes, split;
     svs::DevBuf r, scores, partials, verdict, out, mtiles, q_corr, dense, topk_part, rv;
     // inputs of device-planned batches: probes + per-query flags, raw f32
     // queries, routing distances
@@ -127,7 +127,8 @@ struct svs_slot {
     uint64_t graph_clock = 0, graph_launches = 0, graph_captures = 0;
     std::vector<svs::DevBuf*> slot_scratch() {
         return {&items, &prefixes, &flags, &q_vq, &q_seeds, &q_venc, &q_gl, &q_tdense,
-                &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &q_pplanes, &r, &scores, &partials, &verdict,
+                &q_tlist, &q_tcnt, &q_vseed, &q_vplanes, &q_pplanes, &split, &r,
+                &scores, &partials, &verdict,
                 &out, &mtiles, &q_corr, &dense, &topk_part, &rv, &q_probes, &q_raw,
                 &route_dist};
     }
@@ -161,6 +162,11 @@ struct svs_ctx : svs_slot {
     int planner = 1;
     // persistent tile grids: resident blocks per CU times this
     int tile_grid = 1;
+    // the batched product splits its tiles (16-query groups): row blocks of
+    // one group on k_prod_planes, of several on k_prod_shared (1), or runs
+    // every tile on k_prod_planes (0, the default: measured faster so far)
+    int prod_split = 0;
+    int prod_split_pf = 4;  // k_prod_shared's ring depth in steps: 2 or 4
     // device-planned batches replay captured HIP graphs (1) or launch (0)
     int graphs = 1;
     // pinned staging of caller inputs for graphed batches

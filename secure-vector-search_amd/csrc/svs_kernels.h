@@ -1424,7 +1424,10 @@ struct XcdShare {
     uint32_t lo, hi, worker, workers;
 };
 
-__device__ __forceinline__ XcdShare xcd_share(const MTile* __restrict__ tiles, uint32_t nt) {
+// tsel: when not null, the tile list is tiles[tsel[0 .. nt)] (a selection in
+// plan order)
+__device__ __forceinline__ XcdShare xcd_share(const MTile* __restrict__ tiles, uint32_t nt,
+                                              const uint32_t* __restrict__ tsel = nullptr) {
     const uint32_t b = blockIdx.x, g = gridDim.x;
     XcdShare s;
     if (g & 7) {
@@ -1434,10 +1437,10 @@ __device__ __forceinline__ XcdShare xcd_share(const MTile* __restrict__ tiles, u
         s.workers = g;
         return s;
     }
+    auto at = [&](uint32_t i) -> const MTile& { return tiles[tsel ? tsel[i] : i]; };
     auto cut = [&](uint32_t k) -> uint32_t {
         uint32_t i = (uint32_t)(((uint64_t)nt * k) >> 3);
-        while (i > 0 && i < nt && tiles[i - 1].list == tiles[i].list &&
-               tiles[i - 1].row0 == tiles[i].row0)
+        while (i > 0 && i < nt && at(i - 1).list == at(i).list && at(i - 1).row0 == at(i).row0)
             ++i;
         return i;
     };
@@ -1904,10 +1907,58 @@ __device__ __forceinline__ void planes_of(const v4i (&L)[8], v4i (&pl)[8]) {
         }
 }
 
+// Epilogue of the plane kernels for one 16-row unit: lane (g, jj, b) holds
+// C_ab of rows 4 g + i for query 2 jb + jj of each B tile jb; it folds them
+// into 2^(8b) sum_a 2^(8a) C_ab (mod p, plane_fold), then the eight lanes
+// of its (g, jj) reduce-scatter over b with plain 64-bit adds (eight folded
+// values stay below 2^64), leaving row 4 g + (b & 3) whole, and lanes b < 4
+// write it with the row term c sum M and the query term c sum V - n c^2 of
+// identity (*) (plane_finish). wrow = row0 + 4 g + (b & 3); absent queries
+// (j >= gn) and rows past m are not written, nor anything when !write.
+template <int NB>
+__device__ __forceinline__ void planes_epilogue(const v4i (&acc)[8][NB], int jj, int bpl,
+                                                uint32_t wrow, uint32_t m, uint64_t rcorr,
+                                                const MTile& tile, bool write,
+                                                const uint32_t* __restrict__ gitems,
+                                                const Item* __restrict__ items,
+                                                const uint32_t* __restrict__ cand_ptr,
+                                                const uint64_t* __restrict__ qcorr,
+                                                uint64_t* __restrict__ out) {
+    const int gn = (int)tile.gn;
+#pragma unroll
+    for (int jb = 0; jb < NB; ++jb) {
+        uint64_t v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v[i] = plane_fold(acc[0][jb][i], acc[1][jb][i], acc[2][jb][i], acc[3][jb][i],
+                              acc[4][jb][i], acc[5][jb][i], acc[6][jb][i], acc[7][jb][i],
+                              8 * bpl);
+        const bool b1 = (bpl & 2) != 0, b0 = (bpl & 1) != 0;
+        uint64_t k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
+        const uint64_t s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
+        k0 += shfl_xor_u64(s0, 2);
+        k1 += shfl_xor_u64(s1, 2);
+        uint64_t kp = b0 ? k1 : k0;
+        const uint64_t sd = b0 ? k0 : k1;
+        kp += shfl_xor_u64(sd, 1);
+        kp += shfl_xor_u64(kp, 4);
+        // the tile's two queries of this B tile (scalar loads)
+        const uint32_t last = tile.gn - 1;
+        const uint32_t it0 = gitems[tile.g0 + min((uint32_t)(2 * jb), last)];
+        const uint32_t it1 = gitems[tile.g0 + min((uint32_t)(2 * jb + 1), last)];
+        const uint32_t cb0 = cand_ptr[it0], cb1 = cand_ptr[it1];
+        const uint64_t qt0 = qcorr[items[it0].q], qt1 = qcorr[items[it1].q];
+        const int j = 2 * jb + jj;
+        if (write && bpl < 4 && j < gn && wrow < m)
+            out[(jj ? cb1 : cb0) + wrow] = plane_finish(kp, rcorr, jj ? qt1 : qt0);
+    }
+}
+
 template <int KE, int SRC, int QW, int PF, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
-              const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
+              const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ tsel,
+              const uint32_t* __restrict__ gitems,
               const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
               const int8_t* __restrict__ qplanes, const uint64_t* __restrict__ qcorr,
               uint64_t* __restrict__ out) {
@@ -1922,7 +1973,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     const int lane = threadIdx.x % WAVE;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
-    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p, tsel);
     const uint32_t n_units = 64u * ((sh.hi - sh.lo + 3) / 4);
     const uint32_t ustride = 4u * sh.workers;
 
@@ -1943,7 +1994,7 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             // share the query planes, read 5 % less and ran 3 % slower)
             const uint32_t ti = sh.lo + (((x >> 6) << 2) | (x & 3u));
             if (ti >= sh.hi) continue;
-            const MTile t = tiles[ti];
+            const MTile t = tiles[tsel ? tsel[ti] : ti];
             const uint32_t m = lists[t.list].m;
             const uint32_t row0 = t.row0 + 16u * ((x >> 2) & 15u);
             if (row0 < m) {
@@ -2008,11 +2059,10 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
 #pragma unroll
         for (int jb = 0; jb < NB; ++jb) qb_nxt[jb] = qb_cur[jb];
         if (has_next) lane_ptrs(nxt, a_nxt, qb_nxt);
-        const int gn = (int)cur.tile.gn;
         // epilogue input, loaded ahead of the steps: this lane writes row
         // wrow (after the plane reduction) for query 2 jb + jj
         const uint32_t wrow = cur.row0 + 4u * g + (uint32_t)(bpl & 3);
-        const uint64_t rcorr = cur.rowcorr[min(wrow, cur.m - 1)];
+        const uint64_t rcorr = gload_u64(cur.rowcorr + min(wrow, cur.m - 1));
         // keep this load here: sunk to its use it would be the youngest
         // load at the epilogue, whose wait would then drain the ring
         __builtin_amdgcn_sched_barrier(0);
@@ -2061,46 +2111,287 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             // sink each step's loads next to their use PF steps later
             __builtin_amdgcn_sched_barrier(0);
         }
-        // Epilogue per B tile: lane (g, jj, b) holds C_ab for rows 4 g + i;
-        // it weights them by 256^(a+b) mod p, then the eight lanes of its
-        // (g, jj) reduce-scatter over b, leaving row 4 g + (b & 3) whole.
-#pragma unroll
-        for (int jb = 0; jb < NB; ++jb) {
-            uint64_t v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t lo = (int64_t)acc[0][jb][i] + ((int64_t)acc[1][jb][i] << 8) +
-                                   ((int64_t)acc[2][jb][i] << 16) + ((int64_t)acc[3][jb][i] << 24);
-                const int64_t hi = (int64_t)acc[4][jb][i] + ((int64_t)acc[5][jb][i] << 8) +
-                                   ((int64_t)acc[6][jb][i] << 16) + ((int64_t)acc[7][jb][i] << 24);
-                const uint64_t x = add_mod(i64_to_fp(lo), mul_pow2(i64_to_fp(hi), 32));
-                v[i] = mul_pow2(x, 8 * bpl);
-            }
-            const bool b1 = (bpl & 2) != 0, b0 = (bpl & 1) != 0;
-            uint64_t k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
-            const uint64_t s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
-            k0 = add_mod(k0, shfl_xor_u64(s0, 2));
-            k1 = add_mod(k1, shfl_xor_u64(s1, 2));
-            uint64_t kp = b0 ? k1 : k0;
-            const uint64_t sd = b0 ? k0 : k1;
-            kp = add_mod(kp, shfl_xor_u64(sd, 1));
-            kp = add_mod(kp, shfl_xor_u64(kp, 4));
-            // the tile's two queries of this B tile (scalar loads)
-            const uint32_t last = cur.tile.gn - 1;
-            const uint32_t it0 = gitems[cur.tile.g0 + min((uint32_t)(2 * jb), last)];
-            const uint32_t it1 = gitems[cur.tile.g0 + min((uint32_t)(2 * jb + 1), last)];
-            const uint32_t cb0 = cand_ptr[it0], cb1 = cand_ptr[it1];
-            const uint64_t qt0 = qcorr[items[it0].q], qt1 = qcorr[items[it1].q];
-            const int j = 2 * jb + jj;
-            if (bpl < 4 && j < gn && wrow < cur.m)
-                out[(jj ? cb1 : cb0) + wrow] = add_mod(add_mod(kp, rcorr), jj ? qt1 : qt0);
-        }
+        planes_epilogue<NB>(acc, jj, bpl, wrow, cur.m, rcorr, cur.tile, true, gitems, items,
+                            cand_ptr, qcorr, out);
         if (!has_next) break;
         cur = nxt;
         a_cur = a_nxt;
 #pragma unroll
         for (int jb = 0; jb < NB; ++jb) qb_cur[jb] = qb_nxt[jb];
     }
+}
+
+// Calls f(ic<i>{}) for i = 0 .. N - 1 (i usable as a constant inside f).
+template <int I>
+struct ic {
+    static constexpr int value = I;
+};
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(ic<I>{});
+        static_for<N, I + 1>(static_cast<F&&>(f));
+    }
+}
+
+// LDS DMA: lane i's 16 bytes land at LDS byte address m0 + 16 i
+// (global_load_lds_dwordx4; m0 = a wave-uniform base plus a constant,
+// which the instruction reads one wait state after it is written; m0 is a
+// reserved register that the compiler sets itself before any use). Written
+// as asm so that the compiler does not track the LDS writes: it would
+// otherwise wait for every DMA in flight before any LDS read. Nothing comes
+// back into registers; the caller waits for the load counter and takes a
+// block barrier before reading the data. dma16 takes a 64-bit address per
+// lane, dma16s a uniform base and a 32-bit offset per lane.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+template <uint32_t OFF>
+__device__ __forceinline__ void dma16(const void* g, uint32_t lds_base) {
+    asm volatile("s_add_u32 m0, %0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_base), "v"(g), "i"(OFF)
+                 : "memory", "scc");
+}
+template <uint32_t OFF>
+__device__ __forceinline__ void dma16s(const void* base, uint32_t vofs, uint32_t lds_base) {
+    asm volatile("s_add_u32 m0, %0, %3\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2"
+                 :
+                 : "s"(lds_base), "v"(vofs), "s"(base), "i"(OFF)
+                 : "memory", "scc");
+}
+
+// ---------------------------------------------------------------------------
+// The batched server product for row blocks that several query groups probe
+// (k_prod_planes takes the row blocks of one group): the same byte plane by
+// byte plane contraction, with each row read from HBM once for up to 64
+// queries. A work unit is 16 rows of a 256-row block and a quad of up to
+// four of its query groups (plan tiles of 16 queries, k_plan_split); wave w
+// of the block takes group w of the quad, and all four share the rows.
+//
+// Operands move by LDS DMA (global_load_lds_dwordx4: lane i's 16 bytes land
+// at the instruction's LDS base + 16 i), so no registers hold loads in
+// flight. Per 64-element step: the rows, 8 KB in eight 1 KB slots, slot j
+// lane 16 kg + r holding elements 8 j + 2 kg, +1 of row r (the element
+// order of k_prod_planes and k_query_planes), each wave issuing two slots
+// (64 contiguous bytes per row and instruction); and each wave's own query
+// planes, eight 1 KB B tiles. Readers take slot j or B tile jb at lane
+// offset 16 l: no bank conflicts. Each wave turns its 8 row slots into the
+// 8 byte planes in registers (planes_of) and runs its 64 MFMAs.
+//
+// A ring of PF steps: iteration t waits for its own DMAs of step t (the
+// load counter completes in issue order; ten DMAs per wave and step), takes
+// a block barrier (all rows of step t in LDS, and every wave done with step
+// t - 1), reads its operands, issues step t + PF - 1 into the slot of step
+// t - 1 and runs the MFMAs. The ring runs on into the next unit. A wave past
+// the quad's groups repeats the last group and writes nothing.
+//
+// Units: within the XCD's share of the quads (cut at row blocks), unit u is
+// 16-row block u % 16 of quad lo + u / 16; block b takes units worker,
+// worker + workers, ...; units past the list's end are skipped.
+template <int PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+              const uint32_t* __restrict__ quads, const uint32_t* __restrict__ n_quads_p,
+              const uint32_t* __restrict__ gitems, const Item* __restrict__ items,
+              const uint32_t* __restrict__ cand_ptr, const int8_t* __restrict__ qplanes,
+              const uint64_t* __restrict__ qcorr, uint64_t* __restrict__ out) {
+    constexpr int KE = 1024, KS = KE / 64, NB = 8;
+    constexpr uint32_t QBYTES = 8 * KE;
+    constexpr int DMAS = 2 + NB;  // per wave and step
+    static_assert(PF >= 2 && KS % PF == 0 && (PF - 2) * DMAS <= 63, "ring depth");
+    __shared__ v4i arow[PF][8][WAVE];
+    __shared__ v4i bq[PF][PB_WAVES][NB][WAVE];
+    const int lane = threadIdx.x % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
+    const uint32_t nq = *n_quads_p;
+    // the XCD's share of the quads, cut where a row block starts
+    uint32_t lo = 0, hi = nq, worker = blockIdx.x, workers = gridDim.x;
+    auto quad_tile = [&](uint32_t i) -> const MTile& { return tiles[quads[i] >> 3]; };
+    if ((gridDim.x & 7) == 0) {
+        auto cut = [&](uint32_t k) -> uint32_t {
+            uint32_t i = (uint32_t)(((uint64_t)nq * k) >> 3);
+            while (i > 0 && i < nq && quad_tile(i - 1).list == quad_tile(i).list &&
+                   quad_tile(i - 1).row0 == quad_tile(i).row0)
+                ++i;
+            return i;
+        };
+        lo = cut(blockIdx.x & 7);
+        hi = cut((blockIdx.x & 7) + 1);
+        worker = blockIdx.x >> 3;
+        workers = gridDim.x >> 3;
+    }
+    const uint32_t n_units = 16u * (hi - lo);
+
+    struct Unit {
+        uint32_t u, m, row0;
+        MTile tile;  // this wave's group
+        bool idle;   // a wave past the quad's groups
+        const uint64_t* menc;
+        const uint64_t* rowcorr;
+    };
+    auto find = [&](uint32_t x) -> Unit {
+        Unit r;
+        r.u = n_units;
+        for (; x < n_units; x += workers) {
+            const uint32_t qd = quads[lo + x / 16];
+            const uint32_t cnt = qd & 7u;
+            const MTile t0 = tiles[qd >> 3];
+            const uint32_t m = lists[t0.list].m;
+            const uint32_t row0 = t0.row0 + 16u * (x % 16);
+            if (row0 < m) {
+                r.u = x;
+                r.m = m;
+                r.row0 = row0;
+                r.idle = (uint32_t)wave >= cnt;
+                r.tile = tiles[(qd >> 3) + min((uint32_t)wave, cnt - 1)];
+                r.menc = lists[t0.list].menc;
+                r.rowcorr = lists[t0.list].rowcorr;
+                return r;
+            }
+        }
+        return r;
+    };
+    // this lane's row pointer (row lane % 16, elements 2 (lane / 16), +1 of
+    // each 8-element slot) and query-plane offsets
+    auto row_ptr = [&](const Unit& x) -> const uint64_t* {
+        const uint32_t row = min(x.row0 + (uint32_t)col, x.m - 1);
+        return x.menc + (size_t)row * KE + 2 * g;
+    };
+    auto b_ptrs = [&](const Unit& x, uint32_t (&qb)[NB]) {
+        const uint32_t last = x.tile.gn - 1;
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            const uint32_t q0 = items[gitems[x.tile.g0 + min((uint32_t)(2 * jb), last)]].q;
+            const uint32_t q1 = items[gitems[x.tile.g0 + min((uint32_t)(2 * jb + 1), last)]].q;
+            qb[jb] = (jj ? q1 : q0) * QBYTES + (uint32_t)(g * 8 + bpl) * 16u;
+        }
+    };
+    // LDS bases: this wave's row slots and query-plane tiles in ring slot 0.
+    const uint32_t a_lds = lds_addr(&arow[0][2 * wave][0]);
+    const uint32_t b_lds = lds_addr(&bq[0][wave][0][0]);
+    constexpr uint32_t A_SLOT = sizeof(arow[0]), B_SLOT = sizeof(bq[0]);
+    auto issue = [&](auto sl_c, const uint64_t* ap, const uint32_t (&qb)[NB], int s) {
+        constexpr uint32_t sl = decltype(sl_c)::value;
+        // the step offsets pass through empty asm, so that no address of a
+        // later step is formed before its own issue point
+        uint32_t so = (uint32_t)s;
+        asm volatile("" : "+s"(so));
+        const uint64_t* as = ap + 64 * so + 16 * wave;
+        dma16<sl * A_SLOT>(as, a_lds);
+        dma16<sl * A_SLOT + WAVE * 16>(as + 8, a_lds);
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            const uint32_t vo = qb[jb] + 512u * so;
+            switch (jb) {  // the tile's LDS offset as an immediate
+                case 0: dma16s<sl * B_SLOT + 0 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 1: dma16s<sl * B_SLOT + 1 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 2: dma16s<sl * B_SLOT + 2 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 3: dma16s<sl * B_SLOT + 3 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 4: dma16s<sl * B_SLOT + 4 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 5: dma16s<sl * B_SLOT + 5 * WAVE * 16>(qplanes, vo, b_lds); break;
+                case 6: dma16s<sl * B_SLOT + 6 * WAVE * 16>(qplanes, vo, b_lds); break;
+                default: dma16s<sl * B_SLOT + 7 * WAVE * 16>(qplanes, vo, b_lds); break;
+            }
+        }
+    };
+    // Before the block barrier: this wave's DMAs of the step PF - 2 steps
+    // before the newest are done (vmcnt (PF - 2) * DMAS) and its LDS reads
+    // of the previous step too (lgkmcnt 0); expcnt is not waited. The
+    // barrier is a bare s_barrier: a workgroup fence would wait for every
+    // DMA in flight (they write LDS), draining the ring.
+    auto wait_barrier = [](auto vm_c) {
+        constexpr int VM = decltype(vm_c)::value;
+        __builtin_amdgcn_s_waitcnt((VM & 15) | (7 << 4) | (0 << 8) | ((VM >> 4) << 14));
+        asm volatile("s_barrier" ::: "memory");
+    };
+    // operands of a step from ring slot sl: the row slots (turned into
+    // planes by the caller) and this wave's query-plane tiles
+    auto read_step = [&](int sl, v4i (&L)[8], v4i (&B)[NB]) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) L[j] = arow[sl][j][lane];
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) B[jb] = bq[sl][wave][jb][lane];
+    };
+
+    Unit cur = find(worker);
+    if (cur.u >= n_units) return;  // the whole block: the unit is the block's
+    const uint64_t* ap_cur = row_ptr(cur);  // uniform
+    uint32_t qb_cur[NB];
+    b_ptrs(cur, qb_cur);
+    // Software pipeline: iteration s runs the MFMAs of step s on planes
+    // made in iteration s - 1, while it reads and splits step s + 1. Steps
+    // 0 .. PF - 1 go in flight, step 0 is read ahead. Ring slots by step:
+    // PF divides a unit's 16 steps, so step s of every unit is in slot
+    // s % PF.
+    static_for<PF>([&](auto d) { issue(d, ap_cur, qb_cur, (int)decltype(d)::value); });
+    v4i pl[8], Bc[NB];
+    {
+        v4i L[8];
+        wait_barrier(ic<(PF - 1) * DMAS>{});
+        read_step(0, L, Bc);
+        planes_of(L, pl);
+    }
+    for (;;) {
+        const Unit nxt = find(cur.u + workers);
+        const bool has_next = nxt.u < n_units;
+        const uint64_t* ap_nxt = has_next ? row_ptr(nxt) : ap_cur;
+        uint32_t qb_nxt[NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) qb_nxt[jb] = qb_cur[jb];
+        if (has_next) b_ptrs(nxt, qb_nxt);
+        // the row term of this lane's output row (its wait, at the
+        // epilogue, is the one point per unit where the ring drains)
+        const uint32_t wrow = cur.row0 + 4u * g + (uint32_t)(bpl & 3);
+        const uint64_t rcorr = gload_u64(cur.rowcorr + min(wrow, cur.m - 1));
+        v4i acc[8][NB];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) acc[a][jb] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            // step s + 1 (the next unit's step 0 after the last) is in LDS
+            wait_barrier(ic<(PF - 2) * DMAS>{});
+            v4i L[8], Bn[NB];
+            read_step((s + 1) % PF, L, Bn);
+            // step s + PF into the slot step s was read from (every wave is
+            // past that read): this unit's, or the next unit's first steps
+            // (ap_nxt and qb_nxt are this unit's without a next unit, which
+            // re-loads its own final step)
+            {
+                const int sn = s + PF;
+                static_for<PF>([&](auto c) {
+                    if (decltype(c)::value == sn % PF) {
+                        if (sn < KS) issue(c, ap_cur, qb_cur, sn);
+                        else issue(c, ap_nxt, qb_nxt, has_next ? sn - KS : KS - 1);
+                    }
+                });
+            }
+            // MFMAs of step s on the planes in hand, the split of step s + 1
+            // beside them
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb)
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(pl[a], Bc[jb], acc[a][jb],
+                                                                       0, 0, 0);
+            planes_of(L, pl);
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) Bc[jb] = Bn[jb];
+        }
+        planes_epilogue<NB>(acc, jj, bpl, wrow, cur.m, rcorr, cur.tile, !cur.idle, gitems, items,
+                            cand_ptr, qcorr, out);
+        if (!has_next) break;
+        cur = nxt;
+        ap_cur = ap_nxt;
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) qb_cur[jb] = qb_nxt[jb];
+    }
+    // drain the ring before the block's LDS goes away
+    __builtin_amdgcn_s_waitcnt(0);
+    asm volatile("" ::: "memory");
 }
 
 // Block-cooperative MFMA decode over the product's tiles (crypto.rs:901-940):

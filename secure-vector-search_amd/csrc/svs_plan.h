@@ -249,4 +249,68 @@ k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __re
     }
 }
 
+// The product's split of a plan's tiles (tiles list-major, then row block,
+// then query group). A row block that one query group probes (a single
+// tile) goes to the spread list stile, where the waves of a block stream
+// different rows; the tiles of a row block that several groups probe go to
+// quads: (first tile << 3 | tiles), up to four consecutive tiles of one row
+// block, whose groups share each staged row. hdr[PH_NS] and hdr[PH_NQ] get
+// the counts. One block of PLAN_THREADS; tmp: 2 (nt + 1) words of scratch.
+__global__ void __launch_bounds__(PLAN_THREADS)
+k_plan_split(const MTile* __restrict__ tiles, uint32_t* __restrict__ hdr,
+             uint32_t* __restrict__ stile, uint32_t* __restrict__ quads,
+             uint32_t* __restrict__ tmp) {
+    __shared__ uint64_t red[PLAN_THREADS];
+    __shared__ uint32_t wmax[PLAN_THREADS / WAVE];
+    const uint32_t nt = hdr[PH_NT];
+    const uint32_t t = threadIdx.x, T = blockDim.x;
+    auto same = [&](uint32_t i, uint32_t j) {
+        return tiles[i].list == tiles[j].list && tiles[i].row0 == tiles[j].row0;
+    };
+    // each thread walks a contiguous segment; the row block in progress at
+    // its start is the last row-block start of the segments before it
+    const uint32_t seg = (nt + T - 1) / T;
+    const uint32_t b = min(nt, t * seg), e = min(nt, b + seg);
+    uint32_t last = 0;  // tile 0 starts a row block
+    for (uint32_t i = b; i < e; ++i)
+        if (i == 0 || !same(i - 1, i)) last = i;
+    // exclusive max over the earlier threads' values: the row block in
+    // progress at this segment's start (later segments hold later starts)
+    const uint32_t lane = t % WAVE, w = t / WAVE;
+    uint32_t x = last;  // inclusive max within the wave
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, WAVE);
+        if ((int)lane >= off) x = max(x, y);
+    }
+    if (lane == WAVE - 1) wmax[w] = x;
+    __syncthreads();
+    const uint32_t below = __shfl_up(x, 1, WAVE);
+    uint32_t rbs = lane ? below : 0u;
+    for (uint32_t k = 0; k < w; ++k) rbs = max(rbs, wmax[k]);
+    uint32_t* sflag = tmp;
+    uint32_t* qflag = tmp + nt + 1;
+    for (uint32_t i = b; i < e; ++i) {
+        if (i == 0 || !same(i - 1, i)) rbs = i;
+        const bool single = i == rbs && (i + 1 == nt || !same(i, i + 1));
+        sflag[i] = single ? 1u : 0u;
+        qflag[i] = (!single && (i - rbs) % 4 == 0) ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t ns = (uint32_t)plan_scan(sflag, nt, red);
+    const uint32_t nq = (uint32_t)plan_scan(qflag, nt, red);
+    for (uint32_t i = t; i < nt; i += T) {
+        if (sflag[i + 1] != sflag[i]) stile[sflag[i]] = i;
+        if (qflag[i + 1] != qflag[i]) {
+            uint32_t cnt = 1;
+            while (cnt < 4 && i + cnt < nt && same(i, i + cnt)) ++cnt;
+            quads[qflag[i]] = i << 3 | cnt;
+        }
+    }
+    if (t == 0) {
+        hdr[PH_NS] = ns;
+        hdr[PH_NQ] = nq;
+    }
+}
+
 }  // namespace svs

@@ -39,7 +39,8 @@ EXPORTS = [
     "svs_server_create", "svs_server_destroy", "svs_server_submit", "svs_server_wait",
     "svs_server_stats", "svs_server_run_poisson", "svs_score_batch_dev", "svs_dev_alloc",
     "svs_dev_free", "svs_memcpy", "svs_ctx_read_probe", "svs_server_reserve",
-    "svs_ctx_set_planner", "svs_index_set_centroids", "svs_route_batch", "svs_score_raw_batch",
+    "svs_ctx_set_planner", "svs_ctx_set_product_split", "svs_index_set_centroids",
+    "svs_route_batch", "svs_score_raw_batch",
     "svs_ctx_set_graphs", "svs_topk", "svs_index_load_ivf_cache", "svs_cache_fingerprint64",
     "svs_index_get_centroids", "svs_index_get_subspace", "svs_verify_noalloc",
 ]
@@ -197,6 +198,7 @@ def lib() -> C.CDLL:
             "svs_server_run_poisson": ([vp, vp, u32, C.c_double, u64, vp,
                                         C.POINTER(C.c_double)], i32),
             "svs_ctx_set_planner": ([vp, i32], i32),
+            "svs_ctx_set_product_split": ([vp, i32], i32),
             "svs_ctx_set_graphs": ([vp, i32], i32),
             "svs_topk": ([vp, u32, vp, vp, vp, u32, vp, vp, vp], i32),
             "svs_index_load_ivf_cache": ([vp, C.c_char_p, u64, i32, C.POINTER(vp),
