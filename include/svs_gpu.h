@@ -296,13 +296,14 @@ int svs_ctx_set_planner(svs_ctx* ctx, int on_device);
  * identical either way. No reference counterpart. */
 int svs_ctx_set_graphs(svs_ctx* ctx, int on);
 
-/* How the batched server product runs on 16-query tile variants: 1 splits
- * the plan's tiles, so that a 256-row block probed by one query group
- * streams on the per-wave plane kernel and a row block probed by several
- * groups is staged once per quad of groups and shared by them (rows read
- * once for up to 64 queries); 0 (default) runs every tile on the per-wave
- * kernel. Results are identical either way. No reference counterpart (the
- * reference launches one GEMV per probed list, gpu/mod.rs:584-614). */
+/* How the batched server product runs on 16-query tile variants: 1 runs
+ * the staged kernel, whose work units (16 rows x up to four query groups of
+ * a 256-row block, 32 rows x two, or 64 rows x one) stage each step's rows
+ * and query planes once in LDS for the block's four waves, so a row is
+ * read from HBM once for up to 64 queries; 0 (default, measured faster)
+ * runs every tile on the per-wave plane kernel. Results are identical
+ * either way. No reference counterpart (the reference launches one GEMV
+ * per probed list, gpu/mod.rs:584-614). */
 int svs_ctx_set_product_split(svs_ctx* ctx, int on);
 
 /* ---- client routing and quantisation on the device -------------------------

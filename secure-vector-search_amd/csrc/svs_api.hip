@@ -1269,32 +1269,26 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
                                (const uint64_t*)d_venc, (int8_t*)sl.q_pplanes.ptr);
             const TileArgs ta{d_lists, v.tiles, d_nt, v.gitems, d_items, d_cand_ptr};
             if (ctx->prod_split && kTileVariants[variant].qg == 16) {
-                // row blocks of one query group: k_prod_planes over the
-                // selected tiles; of several: k_prod_shared over quads
+                // the staged kernel over work units of 16..64 rows x 1..4
+                // query groups (k_plan_units)
                 const uint64_t cap = std::max<uint64_t>(v.cap_tiles, 1);
-                SVS_TRY(sl.split.ensure(ctx, sizeof(uint32_t) * (4 * cap + 2)));
-                uint32_t* stile = (uint32_t*)sl.split.ptr;
-                uint32_t* quads = stile + cap;
+                SVS_TRY(sl.split.ensure(ctx, sizeof(uint32_t) * (17 * cap + 2)));
+                uint32_t* units = (uint32_t*)sl.split.ptr;
                 uint32_t* hdr = const_cast<uint32_t*>(v.hdr);
-                hipLaunchKernelGGL(k_plan_split, dim3(1), dim3(PLAN_THREADS), 0, st, v.tiles,
-                                   hdr, stile, quads, quads + cap);
-                const TileArgs ts{d_lists, v.tiles, hdr + PH_NS, v.gitems, d_items, d_cand_ptr};
-                launch_planes_v<N, 0, 16, 2, 1>(ctx, st, cap, ts, stile,
-                                                (const int8_t*)sl.q_pplanes.ptr,
-                                                (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
+                hipLaunchKernelGGL(k_plan_units, dim3(1), dim3(PLAN_THREADS), 0, st, d_lists,
+                                   v.tiles, hdr, units, units + 16 * cap);
                 const uint32_t blocks = (uint32_t)std::max<uint64_t>(
                     1, std::min<uint64_t>(16 * cap, (uint64_t)ctx->n_cu * ctx->tile_grid));
-                auto shared = [&](auto kern) {
+                auto staged = [&](auto kern) {
                     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, st, d_lists, v.tiles,
-                                       (const uint32_t*)quads, (const uint32_t*)(hdr + PH_NQ),
+                                       (const uint32_t*)units, (const uint32_t*)(hdr + PH_NU),
                                        v.gitems, d_items, d_cand_ptr,
                                        (const int8_t*)sl.q_pplanes.ptr,
                                        (const uint64_t*)sl.q_corr.ptr, (uint64_t*)sl.r.ptr);
                 };
                 // ring depth: steps of 40 KB of LDS per block in flight
-                if (ctx->prod_split_pf == 2) shared(k_prod_shared<2>);
-                else shared(k_prod_shared<4>);
-                ctx->launches += 2;
+                staged(k_prod_staged<2>);
+                ctx->launches += 1;
             } else {
                 launch_planes<N, 0>(ctx, variant, st, v.cap_tiles, ta,
                                     (const int8_t*)sl.q_pplanes.ptr,

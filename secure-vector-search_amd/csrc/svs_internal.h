@@ -162,11 +162,11 @@ struct svs_ctx : svs_slot {
     int planner = 1;
     // persistent tile grids: resident blocks per CU times this
     int tile_grid = 1;
-    // the batched product splits its tiles (16-query groups): row blocks of
-    // one group on k_prod_planes, of several on k_prod_shared (1), or runs
-    // every tile on k_prod_planes (0, the default: measured faster so far)
+    // the batched product on 16-query tile variants: the staged kernel over
+    // work units that share each staged row and query group (1), or every
+    // tile on k_prod_planes (0, the default: measured faster so far)
     int prod_split = 0;
-    int prod_split_pf = 4;  // k_prod_shared's ring depth in steps: 2 or 4
+    int prod_split_pf = 2;  // k_prod_staged's ring depth in steps: 2 or 4
     // device-planned batches replay captured HIP graphs (1) or launch (0)
     int graphs = 1;
     // pinned staging of caller inputs for graphed batches

@@ -2162,58 +2162,62 @@ __device__ __forceinline__ void dma16s(const void* base, uint32_t vofs, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// The batched server product for row blocks that several query groups probe
-// (k_prod_planes takes the row blocks of one group): the same byte plane by
-// byte plane contraction, with each row read from HBM once for up to 64
-// queries. A work unit is 16 rows of a 256-row block and a quad of up to
-// four of its query groups (plan tiles of 16 queries, k_plan_split); wave w
-// of the block takes group w of the quad, and all four share the rows.
+// The batched server product, staged: the byte plane by byte plane
+// contraction of k_prod_planes with the operands staged in LDS once per
+// block and shared by its four waves, so that a row crosses HBM once for
+// every query of its row block in the unit (up to 64) and a query group's
+// planes once for every row of the unit. Work units (k_plan_units): an
+// entry of c 16-query groups of one 256-row block,
+//   c >= 3: 16 rows x c groups  (wave w: group w; c = 3 leaves wave 3 idle),
+//   c == 2: 32 rows x 2 groups  (wave w: rows 16 (w >> 1), group w & 1),
+//   c == 1: 64 rows x 1 group   (wave w: rows 16 w).
+// Each wave owns 16 rows x 16 queries (64 MFMAs per 64-element step).
+// Per step the unit stages five 8 KB regions: c >= 3 rows in region 0 and
+// group g's planes in 1 + g; c == 2 rows in 0 and 1, groups in 2 and 3;
+// c == 1 rows in 0 .. 3, the group in 4. A row region: eight 1 KB slots,
+// slot j lane 16 kg + r holding elements 8 j + 2 kg, +1 of row r (the
+// element order of k_prod_planes and k_query_planes); a group region: its
+// eight B tiles. Every wave issues ten 1 KB DMAs per step (rows and the
+// planes of its own group, and two repeats for c == 2), reads its row
+// region and its group's region (lane offsets 16 l: no bank conflicts),
+// splits the rows into byte planes in registers and runs its MFMAs.
 //
-// Operands move by LDS DMA (global_load_lds_dwordx4: lane i's 16 bytes land
-// at the instruction's LDS base + 16 i), so no registers hold loads in
-// flight. Per 64-element step: the rows, 8 KB in eight 1 KB slots, slot j
-// lane 16 kg + r holding elements 8 j + 2 kg, +1 of row r (the element
-// order of k_prod_planes and k_query_planes), each wave issuing two slots
-// (64 contiguous bytes per row and instruction); and each wave's own query
-// planes, eight 1 KB B tiles. Readers take slot j or B tile jb at lane
-// offset 16 l: no bank conflicts. Each wave turns its 8 row slots into the
-// 8 byte planes in registers (planes_of) and runs its 64 MFMAs.
+// Operands move by LDS DMA, in a ring of PF steps. Iteration s waits for
+// its own DMAs of step s + 1 (the load counter completes in issue order),
+// takes a block barrier (step s + 1 in LDS for every wave, and every wave
+// done with step s's slot), reads and splits step s + 1, issues step
+// s + PF into step s's slot and runs the MFMAs of step s on the planes
+// split one iteration earlier. The ring runs on into the next unit.
 //
-// A ring of PF steps: iteration t waits for its own DMAs of step t (the
-// load counter completes in issue order; ten DMAs per wave and step), takes
-// a block barrier (all rows of step t in LDS, and every wave done with step
-// t - 1), reads its operands, issues step t + PF - 1 into the slot of step
-// t - 1 and runs the MFMAs. The ring runs on into the next unit. A wave past
-// the quad's groups repeats the last group and writes nothing.
-//
-// Units: within the XCD's share of the quads (cut at row blocks), unit u is
-// 16-row block u % 16 of quad lo + u / 16; block b takes units worker,
-// worker + workers, ...; units past the list's end are skipped.
+// Units: within the XCD's share of the units (cut at row blocks), block b
+// takes units worker, worker + workers, ... . Rows past a list's end are
+// read as its last row and not written.
 template <int PF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
-              const uint32_t* __restrict__ quads, const uint32_t* __restrict__ n_quads_p,
+k_prod_staged(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+              const uint32_t* __restrict__ units, const uint32_t* __restrict__ n_units_p,
               const uint32_t* __restrict__ gitems, const Item* __restrict__ items,
               const uint32_t* __restrict__ cand_ptr, const int8_t* __restrict__ qplanes,
               const uint64_t* __restrict__ qcorr, uint64_t* __restrict__ out) {
     constexpr int KE = 1024, KS = KE / 64, NB = 8;
     constexpr uint32_t QBYTES = 8 * KE;
-    constexpr int DMAS = 2 + NB;  // per wave and step
+    constexpr int DMAS = 10;  // per wave and step
+    constexpr uint32_t REGION = 8 * WAVE * 16;  // 8 KB
+    constexpr uint32_t SLOT = 5 * REGION;       // one step
     static_assert(PF >= 2 && KS % PF == 0 && (PF - 2) * DMAS <= 63, "ring depth");
-    __shared__ v4i arow[PF][8][WAVE];
-    __shared__ v4i bq[PF][PB_WAVES][NB][WAVE];
+    __shared__ v4i ring[PF][5][8][WAVE];
     const int lane = threadIdx.x % WAVE;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
-    const uint32_t nq = *n_quads_p;
-    // the XCD's share of the quads, cut where a row block starts
-    uint32_t lo = 0, hi = nq, worker = blockIdx.x, workers = gridDim.x;
-    auto quad_tile = [&](uint32_t i) -> const MTile& { return tiles[quads[i] >> 3]; };
+    const uint32_t nu = *n_units_p;
+    // the XCD's share of the units, cut where a row block starts
+    uint32_t lo = 0, hi = nu, worker = blockIdx.x, workers = gridDim.x;
+    auto unit_tile = [&](uint32_t i) -> const MTile& { return tiles[units[i] >> 8]; };
     if ((gridDim.x & 7) == 0) {
         auto cut = [&](uint32_t k) -> uint32_t {
-            uint32_t i = (uint32_t)(((uint64_t)nq * k) >> 3);
-            while (i > 0 && i < nq && quad_tile(i - 1).list == quad_tile(i).list &&
-                   quad_tile(i - 1).row0 == quad_tile(i).row0)
+            uint32_t i = (uint32_t)(((uint64_t)nu * k) >> 3);
+            while (i > 0 && i < nu && unit_tile(i - 1).list == unit_tile(i).list &&
+                   unit_tile(i - 1).row0 == unit_tile(i).row0)
                 ++i;
             return i;
         };
@@ -2222,39 +2226,54 @@ k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         worker = blockIdx.x >> 3;
         workers = gridDim.x >> 3;
     }
-    const uint32_t n_units = 16u * (hi - lo);
 
+    // this wave's part of a unit (all wave-uniform)
     struct Unit {
-        uint32_t u, m, row0;
-        MTile tile;  // this wave's group
-        bool idle;   // a wave past the quad's groups
+        uint32_t u;       // index in the share (>= hi - lo: none)
+        uint32_t c;       // groups in the unit's entry
+        uint32_t m, row0; // the list's rows; this wave's first row
+        MTile tile;       // this wave's group
+        bool idle;        // a wave past the entry's groups (c == 3)
+        uint32_t a_reg, b_reg;  // this wave's row and group regions
         const uint64_t* menc;
         const uint64_t* rowcorr;
     };
-    auto find = [&](uint32_t x) -> Unit {
+    auto make = [&](uint32_t x) -> Unit {
         Unit r;
-        r.u = n_units;
-        for (; x < n_units; x += workers) {
-            const uint32_t qd = quads[lo + x / 16];
-            const uint32_t cnt = qd & 7u;
-            const MTile t0 = tiles[qd >> 3];
-            const uint32_t m = lists[t0.list].m;
-            const uint32_t row0 = t0.row0 + 16u * (x % 16);
-            if (row0 < m) {
-                r.u = x;
-                r.m = m;
-                r.row0 = row0;
-                r.idle = (uint32_t)wave >= cnt;
-                r.tile = tiles[(qd >> 3) + min((uint32_t)wave, cnt - 1)];
-                r.menc = lists[t0.list].menc;
-                r.rowcorr = lists[t0.list].rowcorr;
-                return r;
-            }
+        r.u = x;
+        const uint32_t e = units[lo + x];
+        const uint32_t t0 = e >> 8, c = (e >> 4) & 15u, k = e & 15u;
+        const MTile first = tiles[t0];
+        r.c = c;
+        r.m = lists[first.list].m;
+        r.menc = lists[first.list].menc;
+        r.rowcorr = lists[first.list].rowcorr;
+        uint32_t sub, grp;
+        if (c >= 3) {
+            sub = k;
+            grp = min((uint32_t)wave, c - 1);
+            r.a_reg = 0;
+            r.b_reg = 1 + (uint32_t)wave;
+        } else if (c == 2) {
+            sub = 2 * k + ((uint32_t)wave >> 1);
+            grp = (uint32_t)wave & 1u;
+            r.a_reg = (uint32_t)wave >> 1;
+            r.b_reg = 2 + ((uint32_t)wave & 1u);
+        } else {
+            sub = 4 * k + (uint32_t)wave;
+            grp = 0;
+            r.a_reg = (uint32_t)wave;
+            r.b_reg = 4;
         }
+        r.idle = (uint32_t)wave >= c && c >= 3;
+        r.tile = tiles[t0 + grp];
+        // a wave whose 16 rows lie past the list's end re-reads its last
+        // rows (in bounds) and writes nothing (wrow >= m)
+        r.row0 = first.row0 + 16u * sub;
         return r;
     };
-    // this lane's row pointer (row lane % 16, elements 2 (lane / 16), +1 of
-    // each 8-element slot) and query-plane offsets
+    // this lane's row pointer (row lane % 16 of the wave's rows, elements
+    // 2 (lane / 16), +1 of each 8-element slot) and query-plane offsets
     auto row_ptr = [&](const Unit& x) -> const uint64_t* {
         const uint32_t row = min(x.row0 + (uint32_t)col, x.m - 1);
         return x.menc + (size_t)row * KE + 2 * g;
@@ -2268,37 +2287,70 @@ k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             qb[jb] = (jj ? q1 : q0) * QBYTES + (uint32_t)(g * 8 + bpl) * 16u;
         }
     };
-    // LDS bases: this wave's row slots and query-plane tiles in ring slot 0.
-    const uint32_t a_lds = lds_addr(&arow[0][2 * wave][0]);
-    const uint32_t b_lds = lds_addr(&bq[0][wave][0][0]);
-    constexpr uint32_t A_SLOT = sizeof(arow[0]), B_SLOT = sizeof(bq[0]);
-    auto issue = [&](auto sl_c, const uint64_t* ap, const uint32_t (&qb)[NB], int s) {
+    const uint32_t ring_lds = lds_addr(&ring[0][0][0][0]);
+    // Step s of a unit into ring slot sl (a constant): this wave's share of
+    // its row region (slots j0 .. j0 + nj - 1) and of its group's tiles
+    // (jb0 .. jb0 + nb - 1), ten DMAs in all. The step offset passes through
+    // empty asm, so that no address of a later step is formed ahead.
+    auto issue = [&](auto sl_c, const Unit& x, const uint64_t* ap, const uint32_t (&qb)[NB],
+                     int s) {
         constexpr uint32_t sl = decltype(sl_c)::value;
-        // the step offsets pass through empty asm, so that no address of a
-        // later step is formed before its own issue point
         uint32_t so = (uint32_t)s;
         asm volatile("" : "+s"(so));
-        const uint64_t* as = ap + 64 * so + 16 * wave;
-        dma16<sl * A_SLOT>(as, a_lds);
-        dma16<sl * A_SLOT + WAVE * 16>(as + 8, a_lds);
-#pragma unroll
-        for (int jb = 0; jb < NB; ++jb) {
-            const uint32_t vo = qb[jb] + 512u * so;
-            switch (jb) {  // the tile's LDS offset as an immediate
-                case 0: dma16s<sl * B_SLOT + 0 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 1: dma16s<sl * B_SLOT + 1 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 2: dma16s<sl * B_SLOT + 2 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 3: dma16s<sl * B_SLOT + 3 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 4: dma16s<sl * B_SLOT + 4 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 5: dma16s<sl * B_SLOT + 5 * WAVE * 16>(qplanes, vo, b_lds); break;
-                case 6: dma16s<sl * B_SLOT + 6 * WAVE * 16>(qplanes, vo, b_lds); break;
-                default: dma16s<sl * B_SLOT + 7 * WAVE * 16>(qplanes, vo, b_lds); break;
+        const uint64_t* as = ap + 64 * so;
+        const uint32_t a_base = ring_lds + sl * SLOT + x.a_reg * REGION;
+        const uint32_t b_base = ring_lds + sl * SLOT + x.b_reg * REGION;
+        const int8_t* qs = qplanes;
+        auto row_slot = [&](auto j_c) {
+            constexpr uint32_t j = decltype(j_c)::value;
+            dma16<j * WAVE * 16>(as + 8 * j, a_base);
+        };
+        auto b_tile = [&](auto jb_c) {
+            constexpr uint32_t jb = decltype(jb_c)::value;
+            dma16s<jb * WAVE * 16>(qs, qb[jb] + 512u * so, b_base);
+        };
+        if (x.c >= 3) {
+            // rows: slots 2 w, 2 w + 1; planes: the wave's 8 tiles
+            static_for<4>([&](auto h_c) {
+                if (wave == decltype(h_c)::value) {
+                    row_slot(ic<2 * decltype(h_c)::value>{});
+                    row_slot(ic<2 * decltype(h_c)::value + 1>{});
+                }
+            });
+            static_for<NB>(b_tile);
+        } else if (x.c == 2) {
+            // rows: slots 4 (w & 1) .. + 3 of region w >> 1; planes: tiles
+            // 4 (w >> 1) .. + 3 of group w & 1; two repeats of its first
+            // tile keep ten DMAs
+            if (wave & 1) {
+                static_for<4>([&](auto j_c) { row_slot(ic<4 + decltype(j_c)::value>{}); });
+            } else {
+                static_for<4>([&](auto j_c) { row_slot(ic<decltype(j_c)::value>{}); });
             }
+            if (wave >> 1) {
+                static_for<4>([&](auto j_c) { b_tile(ic<4 + decltype(j_c)::value>{}); });
+                b_tile(ic<4>{});
+                b_tile(ic<4>{});
+            } else {
+                static_for<4>([&](auto j_c) { b_tile(ic<decltype(j_c)::value>{}); });
+                b_tile(ic<0>{});
+                b_tile(ic<0>{});
+            }
+        } else {
+            // rows: all 8 slots of the wave's region; planes: tiles 2 w,
+            // 2 w + 1 of the group
+            static_for<8>(row_slot);
+            static_for<4>([&](auto h_c) {
+                if (wave == decltype(h_c)::value) {
+                    b_tile(ic<2 * decltype(h_c)::value>{});
+                    b_tile(ic<2 * decltype(h_c)::value + 1>{});
+                }
+            });
         }
     };
-    // Before the block barrier: this wave's DMAs of the step PF - 2 steps
-    // before the newest are done (vmcnt (PF - 2) * DMAS) and its LDS reads
-    // of the previous step too (lgkmcnt 0); expcnt is not waited. The
+    // Before the block barrier: this wave's DMAs up to the step PF - 2
+    // steps before the newest are done (vmcnt (PF - 2) * DMAS) and its LDS
+    // reads of the previous step too (lgkmcnt 0); expcnt is not waited. The
     // barrier is a bare s_barrier: a workgroup fence would wait for every
     // DMA in flight (they write LDS), draining the ring.
     auto wait_barrier = [](auto vm_c) {
@@ -2306,36 +2358,34 @@ k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         __builtin_amdgcn_s_waitcnt((VM & 15) | (7 << 4) | (0 << 8) | ((VM >> 4) << 14));
         asm volatile("s_barrier" ::: "memory");
     };
-    // operands of a step from ring slot sl: the row slots (turned into
-    // planes by the caller) and this wave's query-plane tiles
-    auto read_step = [&](int sl, v4i (&L)[8], v4i (&B)[NB]) {
+    auto read_step = [&](int sl, const Unit& x, v4i (&L)[8], v4i (&B)[NB]) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) L[j] = arow[sl][j][lane];
+        for (int j = 0; j < 8; ++j) L[j] = ring[sl][x.a_reg][j][lane];
 #pragma unroll
-        for (int jb = 0; jb < NB; ++jb) B[jb] = bq[sl][wave][jb][lane];
+        for (int jb = 0; jb < NB; ++jb) B[jb] = ring[sl][x.b_reg][jb][lane];
     };
 
-    Unit cur = find(worker);
-    if (cur.u >= n_units) return;  // the whole block: the unit is the block's
-    const uint64_t* ap_cur = row_ptr(cur);  // uniform
+    const uint32_t n_share = hi - lo;
+    if (worker >= n_share) return;  // the whole block
+    Unit cur = make(worker);
+    const uint64_t* ap_cur = row_ptr(cur);
     uint32_t qb_cur[NB];
     b_ptrs(cur, qb_cur);
     // Software pipeline: iteration s runs the MFMAs of step s on planes
-    // made in iteration s - 1, while it reads and splits step s + 1. Steps
-    // 0 .. PF - 1 go in flight, step 0 is read ahead. Ring slots by step:
-    // PF divides a unit's 16 steps, so step s of every unit is in slot
-    // s % PF.
-    static_for<PF>([&](auto d) { issue(d, ap_cur, qb_cur, (int)decltype(d)::value); });
+    // split in iteration s - 1. Steps 0 .. PF - 1 go in flight, step 0 is
+    // read ahead. Ring slots by step: PF divides a unit's 16 steps, so step
+    // s of every unit is in slot s % PF.
+    static_for<PF>([&](auto d) { issue(d, cur, ap_cur, qb_cur, (int)decltype(d)::value); });
     v4i pl[8], Bc[NB];
     {
         v4i L[8];
         wait_barrier(ic<(PF - 1) * DMAS>{});
-        read_step(0, L, Bc);
+        read_step(0, cur, L, Bc);
         planes_of(L, pl);
     }
     for (;;) {
-        const Unit nxt = find(cur.u + workers);
-        const bool has_next = nxt.u < n_units;
+        const bool has_next = cur.u + workers < n_share;
+        const Unit nxt = has_next ? make(cur.u + workers) : cur;
         const uint64_t* ap_nxt = has_next ? row_ptr(nxt) : ap_cur;
         uint32_t qb_nxt[NB];
 #pragma unroll
@@ -2355,17 +2405,16 @@ k_prod_shared(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             // step s + 1 (the next unit's step 0 after the last) is in LDS
             wait_barrier(ic<(PF - 2) * DMAS>{});
             v4i L[8], Bn[NB];
-            read_step((s + 1) % PF, L, Bn);
+            read_step((s + 1) % PF, s + 1 < KS ? cur : nxt, L, Bn);
             // step s + PF into the slot step s was read from (every wave is
             // past that read): this unit's, or the next unit's first steps
-            // (ap_nxt and qb_nxt are this unit's without a next unit, which
-            // re-loads its own final step)
+            // (the last unit re-loads its own final step)
             {
                 const int sn = s + PF;
                 static_for<PF>([&](auto c) {
                     if (decltype(c)::value == sn % PF) {
-                        if (sn < KS) issue(c, ap_cur, qb_cur, sn);
-                        else issue(c, ap_nxt, qb_nxt, has_next ? sn - KS : KS - 1);
+                        if (sn < KS) issue(c, cur, ap_cur, qb_cur, sn);
+                        else issue(c, nxt, ap_nxt, qb_nxt, has_next ? sn - KS : KS - 1);
                     }
                 });
             }

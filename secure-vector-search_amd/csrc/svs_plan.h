@@ -249,16 +249,22 @@ k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __re
     }
 }
 
-// The product's split of a plan's tiles (tiles list-major, then row block,
-// then query group). A row block that one query group probes (a single
-// tile) goes to the spread list stile, where the waves of a block stream
-// different rows; the tiles of a row block that several groups probe go to
-// quads: (first tile << 3 | tiles), up to four consecutive tiles of one row
-// block, whose groups share each staged row. hdr[PH_NS] and hdr[PH_NQ] get
-// the counts. One block of PLAN_THREADS; tmp: 2 (nt + 1) words of scratch.
+// Work units of the staged product kernel over a plan's tiles (tiles
+// list-major, then row block, then 16-query group). A row block's groups
+// go out in entries of up to four consecutive groups (full quads, then the
+// rest); an entry of c groups covers the row block in units of
+//   c >= 3: 16 rows x the c groups (one wave per group; c = 3 leaves one
+//           wave idle),
+//   c == 2: 32 rows x the 2 groups (waves: 2 row blocks of 16 x 2 groups),
+//   c == 1: 64 rows x the group (one wave per 16 rows),
+// so that a block's four waves always share what one staging of a step
+// serves (each unit stages 40 KB per 64-element step). units[] =
+// (first tile << 8 | c << 4 | k), k the unit's index within its entry;
+// hdr[PH_NU] = the unit count. One block of PLAN_THREADS; tmp: nt + 1
+// words of scratch.
 __global__ void __launch_bounds__(PLAN_THREADS)
-k_plan_split(const MTile* __restrict__ tiles, uint32_t* __restrict__ hdr,
-             uint32_t* __restrict__ stile, uint32_t* __restrict__ quads,
+k_plan_units(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+             uint32_t* __restrict__ hdr, uint32_t* __restrict__ units,
              uint32_t* __restrict__ tmp) {
     __shared__ uint64_t red[PLAN_THREADS];
     __shared__ uint32_t wmax[PLAN_THREADS / WAVE];
@@ -274,8 +280,8 @@ k_plan_split(const MTile* __restrict__ tiles, uint32_t* __restrict__ hdr,
     uint32_t last = 0;  // tile 0 starts a row block
     for (uint32_t i = b; i < e; ++i)
         if (i == 0 || !same(i - 1, i)) last = i;
-    // exclusive max over the earlier threads' values: the row block in
-    // progress at this segment's start (later segments hold later starts)
+    // exclusive max over the earlier threads' values (later segments hold
+    // later starts)
     const uint32_t lane = t % WAVE, w = t / WAVE;
     uint32_t x = last;  // inclusive max within the wave
 #pragma unroll
@@ -288,29 +294,31 @@ k_plan_split(const MTile* __restrict__ tiles, uint32_t* __restrict__ hdr,
     const uint32_t below = __shfl_up(x, 1, WAVE);
     uint32_t rbs = lane ? below : 0u;
     for (uint32_t k = 0; k < w; ++k) rbs = max(rbs, wmax[k]);
-    uint32_t* sflag = tmp;
-    uint32_t* qflag = tmp + nt + 1;
+    // per tile: the units of the entry it starts (0 if it starts none)
+    uint32_t* cnt = tmp;
     for (uint32_t i = b; i < e; ++i) {
         if (i == 0 || !same(i - 1, i)) rbs = i;
-        const bool single = i == rbs && (i + 1 == nt || !same(i, i + 1));
-        sflag[i] = single ? 1u : 0u;
-        qflag[i] = (!single && (i - rbs) % 4 == 0) ? 1u : 0u;
+        uint32_t n = 0;
+        if ((i - rbs) % 4 == 0) {
+            uint32_t c = 1;
+            while (c < 4 && i + c < nt && same(i, i + c)) ++c;
+            const MTile tl = tiles[i];
+            const uint32_t rows = min(256u, lists[tl.list].m - tl.row0);
+            const uint32_t nsb = (rows + 15) / 16;
+            n = c >= 3 ? nsb : c == 2 ? (nsb + 1) / 2 : (nsb + 3) / 4;
+        }
+        cnt[i] = n;
     }
     __syncthreads();
-    const uint32_t ns = (uint32_t)plan_scan(sflag, nt, red);
-    const uint32_t nq = (uint32_t)plan_scan(qflag, nt, red);
+    const uint32_t nu = (uint32_t)plan_scan(cnt, nt, red);
     for (uint32_t i = t; i < nt; i += T) {
-        if (sflag[i + 1] != sflag[i]) stile[sflag[i]] = i;
-        if (qflag[i + 1] != qflag[i]) {
-            uint32_t cnt = 1;
-            while (cnt < 4 && i + cnt < nt && same(i, i + cnt)) ++cnt;
-            quads[qflag[i]] = i << 3 | cnt;
-        }
+        const uint32_t n = cnt[i + 1] - cnt[i];
+        if (!n) continue;
+        uint32_t c = 1;
+        while (c < 4 && i + c < nt && same(i, i + c)) ++c;
+        for (uint32_t k = 0; k < n; ++k) units[cnt[i] + k] = i << 8 | c << 4 | k;
     }
-    if (t == 0) {
-        hdr[PH_NS] = ns;
-        hdr[PH_NQ] = nq;
-    }
+    if (t == 0) hdr[PH_NU] = nu;
 }
 
 }  // namespace svs

@@ -69,11 +69,8 @@ constexpr uint32_t PLAN_ITEM_BITS = 14;
 constexpr uint32_t PLAN_MAX_ITEMS = 1u << PLAN_ITEM_BITS;
 constexpr uint32_t PLAN_MAX_LISTS = 1u << (32 - PLAN_ITEM_BITS);
 // Header words of a plan in device memory.
-// PH_NS / PH_NQ: the product's split of the tiles (k_plan_split): tiles of
-// row blocks that one query group probes, and quads of query groups.
-enum : uint32_t {
-    PH_NT = 0, PH_NVT = 1, PH_OVERFLOW = 2, PH_NCANDS = 3, PH_NS = 4, PH_NQ = 5, PH_WORDS = 8
-};
+// PH_NU: the staged product's work units (k_plan_units).
+enum : uint32_t { PH_NT = 0, PH_NVT = 1, PH_OVERFLOW = 2, PH_NCANDS = 3, PH_NU = 4, PH_WORDS = 8 };
 // Device routing sorts a query's centroid distances in LDS (64 KiB).
 constexpr int RT_SELECT_MAX = 8192;
 

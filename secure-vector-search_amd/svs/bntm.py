@@ -214,10 +214,10 @@ class Context:
         check(L.lib().svs_ctx_set_planner(self.h, 1 if on_device else 0))
 
     def set_product_split(self, on: bool = True):
-        """Split the batched product's 16-query tiles between the per-wave
-        plane kernel (row blocks of one query group) and the shared-staging
-        kernel (row blocks of several groups); off (the default): the
-        per-wave kernel for every tile. Same results either way."""
+        """Run the batched product of 16-query tile variants on the staged
+        kernel (each step's rows and query planes staged once in LDS per
+        work unit of up to 64 queries); off (the default): the per-wave
+        plane kernel for every tile. Same results either way."""
         check(L.lib().svs_ctx_set_product_split(self.h, 1 if on else 0))
 
     def set_graphs(self, on: bool = True):

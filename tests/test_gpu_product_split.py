@@ -1,11 +1,12 @@
-"""The batched server product with its tiles split between the per-wave
-plane kernel (256-row blocks that one 16-query group probes) and the
-shared-staging kernel (row blocks that several groups probe, in quads of up
-to four groups): compute_products_batch (scorer-bntm/src/crypto.rs:782-828)
-generalised to uneven probe sets, every (query, list) product against the
-oracle's matvec, with the split on and off, on both planners, and through
-replays of the captured batch graph; then whole IVF queries against the
-oracle's BnTmIvfScorer::score (scorer-bntm/src/ivf.rs:746-822)."""
+"""The batched server product on the staged kernel (work units of 16 rows x
+up to four 16-query groups, 32 rows x two groups or 64 rows x one group,
+each staging a step's rows and query planes once in LDS for its four
+waves) against the per-wave plane kernel: compute_products_batch
+(scorer-bntm/src/crypto.rs:782-828) generalised to uneven probe sets, every
+(query, list) product against the oracle's matvec, with the staged product
+on and off and on both planners; then whole IVF queries against the
+oracle's BnTmIvfScorer::score (scorer-bntm/src/ivf.rs:746-822), launched
+and through replays of the captured batch graph."""
 import sys
 
 import numpy as np
@@ -17,8 +18,10 @@ pytestmark = pytest.mark.gpu
 P = O.P
 
 # list sizes (partial 16-row blocks, one and several 256-row blocks) and the
-# number of queries probing each: one group (1, 5, 16), quads of 2, 3 and 4
-# groups (17, 33, 64) and row blocks of several quads (65, 120)
+# number of queries probing each: one group (1, 5, 16: 64-row units), two
+# (17: 32-row units), three and four (33, 64: 16-row units, one wave idle
+# at three) and row blocks of several entries (65: four + one, 120: seven
+# full groups and a partial one)
 SIZES = [300, 17, 256, 1500, 257, 700, 1, 520]
 PROBED = [1, 5, 16, 17, 33, 64, 65, 120]
 
@@ -96,7 +99,7 @@ def test_split_product_matches_oracle(split_setup, variant, planner):
 
 
 def test_split_ivf_queries_match_oracle_and_replays():
-    """Whole IVF queries on a 16-query tile variant with the split product:
+    """Whole IVF queries on a 16-query tile variant with the staged product:
     many queries on few lists (row blocks of several quads), through the
     launched batch and two graph replays, each against the oracle."""
     import svs
