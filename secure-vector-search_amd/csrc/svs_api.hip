@@ -168,6 +168,7 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     if (const char* g = getenv("SVS_TILE_GRID")) ctx->tile_grid = std::max(1, std::min(8, atoi(g)));
     if (const char* g = getenv("SVS_PROD_SPLIT")) ctx->prod_split = atoi(g) ? 1 : 0;
     if (const char* g = getenv("SVS_PROD_SPLIT_PF")) ctx->prod_split_pf = atoi(g);
+    if (const char* g = getenv("SVS_DENSE_PLANES")) ctx->dense_planes = atoi(g) ? 1 : 0;
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1381,18 +1382,29 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             dense = (const uint64_t*)sl.dense.ptr;
         }
         if (tiles) {
-            // the dense terms AL G + H L^T v_enc on the window-formulation
-            // tiles over [AL | H] (K = 256: four 64-element steps are too few
-            // for the plane kernel's per-unit set-up to pay, measured 0.1-0.2
-            // ms slower per 256-query batch)
+            // the dense terms AL G + H L^T v_enc over [AL | H] (K = 256) on
+            // the plane kernel (with the plane_fold epilogue 0.17 ms faster
+            // per 256-query batch than the window-formulation tiles, which
+            // remain as ctx->dense_planes = 0)
             SVS_TRY(sl.dense.ensure(ctx, sizeof(uint64_t) * nc1));
             uint64_t* qc_gl = (uint64_t*)sl.q_corr.ptr + nq;
             hipLaunchKernelGGL(k_query_corr<2 * N1>, dim3(nq), dim3(WAVE), 0, st,
                                (const uint64_t*)d_gl, qc_gl);
-            launch_gemm<2 * N1, 1>(variant, st,
-                                   tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ), ta,
-                                   (const uint64_t*)d_gl, (const uint64_t*)qc_gl,
-                                   (uint64_t*)sl.dense.ptr);
+            if (ctx->dense_planes) {
+                // the plane kernel over [AL | H]; the product's query planes
+                // are spent, so their buffer takes [G ; L^T v_enc]'s
+                hipLaunchKernelGGL(k_query_planes<2 * N1>, dim3(nq), dim3(256), 0, st,
+                                   (const uint64_t*)d_gl, (int8_t*)sl.q_pplanes.ptr);
+                launch_planes<2 * N1, 1>(ctx, variant, st, v.cap_tiles, ta,
+                                         (const int8_t*)sl.q_pplanes.ptr,
+                                         (const uint64_t*)qc_gl, (uint64_t*)sl.dense.ptr);
+                ctx->launches++;
+            } else {
+                launch_gemm<2 * N1, 1>(variant, st,
+                                       tile_blocks(ctx, v.cap_tiles, kTileVariants[variant].occ),
+                                       ta, (const uint64_t*)d_gl, (const uint64_t*)qc_gl,
+                                       (uint64_t*)sl.dense.ptr);
+            }
             ctx->launches += 2;
             dense = (const uint64_t*)sl.dense.ptr;
         }
@@ -1492,6 +1504,7 @@ int svs::run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
     if (!ctx->graphs || sl.prof.on || ctx->fault_list >= 0) return enqueue();
     key.push_back((uint64_t)ctx->prod_split);
     key.push_back((uint64_t)ctx->prod_split_pf);
+    key.push_back((uint64_t)ctx->dense_planes);
     for (DevBuf* b : sl.slot_scratch()) {
         key.push_back((uint64_t)(uintptr_t)b->ptr);
         key.push_back(b->cap);

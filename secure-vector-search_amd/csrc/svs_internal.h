@@ -167,6 +167,9 @@ struct svs_ctx : svs_slot {
     // tile on k_prod_planes (0, the default: measured faster so far)
     int prod_split = 0;
     int prod_split_pf = 2;  // k_prod_staged's ring depth in steps: 2 or 4
+    // the batched dense decode terms on the plane kernel (1, the default:
+    // 0.17 ms faster per 256-query batch) or the window tiles (0)
+    int dense_planes = 1;
     // device-planned batches replay captured HIP graphs (1) or launch (0)
     int graphs = 1;
     // pinned staging of caller inputs for graphed batches
