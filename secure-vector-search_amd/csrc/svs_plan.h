@@ -5,9 +5,10 @@
 // by MFMA work tiles over the items grouped by list. The host builds the same
 // arrays in svs::make_plan; k_plan builds them on the device from the probe
 // lists, so that routing, planning and the whole pipeline run back to back on
-// one stream with no host round trip. The arrays are identical to the host
-// plan's, element for element: items in probe order, gitems = items with rows
-// stably sorted by list, tiles list-major, then row block, then query group.
+// one stream with no host round trip. The arrays match the host plan's:
+// items in probe order, gitems = the items with rows grouped by list (the
+// host orders each list's items by item; the device's counting sort leaves
+// them in no set order), tiles list-major, then row block, then query group.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -198,19 +199,43 @@ k_plan(const ListDev* __restrict__ lists, uint32_t n_lists, const uint32_t* __re
     }
     uint64_t nt = 0, nvt = 0;
     if (!over) {
-        uint32_t p2 = 1;
-        while (p2 < n_items) p2 <<= 1;
-        for (uint32_t i = n_items + threadIdx.x; i < p2; i += T) keys[i] = 0xFFFFFFFFu;
-        __syncthreads();
-        plan_sort(keys, p2);
-        // items with rows: the keys below the all-ones padding
-        uint32_t lo = 0, hi = n_items;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (keys[mid] != 0xFFFFFFFFu) lo = mid + 1;
-            else hi = mid;
+        uint32_t ng;
+        if (n_lists <= PLAN_MAX_ITEMS) {
+            // Counting sort by list: per-list counts (LDS atomics in
+            // scratch, free until the tiles), their scan, then a scatter
+            // with per-list cursors. Items of one list come out in no set
+            // order; a list's tiles then group its queries differently
+            // from run to run, which changes no result (every (query, row)
+            // product is exact and written at the item's own place).
+            for (uint32_t l = threadIdx.x; l <= n_lists; l += T) scratch[l] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n_items; i += T) {
+                const uint32_t k = keys[i];
+                tmp[i] = k;
+                if (k != 0xFFFFFFFFu) atomicAdd(&scratch[k >> PLAN_ITEM_BITS], 1u);
+            }
+            __syncthreads();
+            ng = (uint32_t)plan_scan(scratch, n_lists, red);
+            for (uint32_t i = threadIdx.x; i < n_items; i += T) {
+                const uint32_t k = tmp[i];
+                if (k != 0xFFFFFFFFu) keys[atomicAdd(&scratch[k >> PLAN_ITEM_BITS], 1u)] = k;
+            }
+            __syncthreads();
+        } else {
+            uint32_t p2 = 1;
+            while (p2 < n_items) p2 <<= 1;
+            for (uint32_t i = n_items + threadIdx.x; i < p2; i += T) keys[i] = 0xFFFFFFFFu;
+            __syncthreads();
+            plan_sort(keys, p2);
+            // items with rows: the keys below the all-ones padding
+            uint32_t lo = 0, hi = n_items;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (keys[mid] != 0xFFFFFFFFu) lo = mid + 1;
+                else hi = mid;
+            }
+            ng = lo;
         }
-        const uint32_t ng = lo;
         nt = plan_tiles(keys, ng, items, tile_rows, tile_qg, scratch, gitems, tiles, cap_tiles,
                         red);
         over = nt > cap_tiles;
