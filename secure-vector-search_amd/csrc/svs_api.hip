@@ -169,6 +169,9 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     if (const char* g = getenv("SVS_PROD_SPLIT")) ctx->prod_split = atoi(g) ? 1 : 0;
     if (const char* g = getenv("SVS_PROD_SPLIT_PF")) ctx->prod_split_pf = atoi(g);
     if (const char* g = getenv("SVS_DENSE_PLANES")) ctx->dense_planes = atoi(g) ? 1 : 0;
+    // graphs off for the whole process (profilers that trace kernels have
+    // crashed on the host inside graph capture; the kernels are the same)
+    if (const char* g = getenv("SVS_GRAPHS")) ctx->graphs = atoi(g) ? 1 : 0;
     ctx->budget_bytes = budget_bytes ? budget_bytes : (uint64_t)(free_b * 0.9);
     ctx->hbm_total = total_b;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
