@@ -964,7 +964,7 @@ static void launch_decode_tiles(int v, hipStream_t st, uint32_t blocks, const Ti
                                 const uint64_t* mt_q, const uint64_t* r, const uint64_t* dense,
                                 int64_t* scores) {
     switch (kTileVariants[v].qg) {
-        case 16: launch_decode_v<16, 2>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        case 16: launch_decode_v<16, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
         case 4: launch_decode_v<4, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
         default: launch_decode_v<8, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
     }

@@ -2515,28 +2515,6 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
             }
-            // The epilogue's inputs for this lane's (row, query) pairs are
-            // loaded now, so their latency hides behind the K loop: the
-            // candidate index, r, the dense term, the row and query terms.
-            constexpr int NH = QG / 4;
-            uint32_t eg[RTW][NH];
-            uint64_t er[RTW][NH], ed[RTW][NH], es[RTW][NH], eq[RTW][NH];
-#pragma unroll
-            for (int rt = 0; rt < RTW; ++rt)
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    const int pr = lane + 64 * h;
-                    const int row = pr % 16, j = pr / 16;
-                    const uint32_t grow = prow0 + 16 * rt + row;
-                    const bool ok = j < gn && grow < m;
-                    const uint32_t it = gitems[tile.g0 + (ok ? j : 0)];
-                    const uint32_t g = cand_ptr[it] + (ok ? grow : 0);
-                    eg[rt][h] = ok ? g : 0xFFFFFFFFu;
-                    er[rt][h] = r[g];
-                    ed[rt][h] = dense[g];
-                    es[rt][h] = L.s_rowterm[ok ? grow : 0];
-                    eq[rt][h] = mt_qterm[(size_t)items[it].q * (MT_BLOCKS + 1) + kw];
-                }
             // Two-slot register ring over the K-steps: slot d's MFMAs run,
             // then slot d is reloaded for the step two ahead (the tail
             // re-loads the last step, in bounds), so one step's loads are
@@ -2584,6 +2562,29 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                     issue(slot, min(kc0 + slot + 2, KC - 1));
                 }
             }
+            // The epilogue's inputs for this lane's (row, query) pairs, loaded
+            // all at once after the K loop (whose ring registers are free by
+            // then): the candidate index, r, the dense term, the row and
+            // query terms.
+            constexpr int NH = QG / 4;
+            uint32_t eg[RTW][NH];
+            uint64_t er[RTW][NH], ed[RTW][NH], es[RTW][NH], eq[RTW][NH];
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    const int pr = lane + 64 * h;
+                    const int row = pr % 16, j = pr / 16;
+                    const uint32_t grow = prow0 + 16 * rt + row;
+                    const bool ok = j < gn && grow < m;
+                    const uint32_t it = gitems[tile.g0 + (ok ? j : 0)];
+                    const uint32_t g = cand_ptr[it] + (ok ? grow : 0);
+                    eg[rt][h] = ok ? g : 0xFFFFFFFFu;
+                    er[rt][h] = r[g];
+                    ed[rt][h] = dense[g];
+                    es[rt][h] = L.s_rowterm[ok ? grow : 0];
+                    eq[rt][h] = mt_qterm[(size_t)items[it].q * (MT_BLOCKS + 1) + kw];
+                }
             // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i,
             // column col), then each lane finishes (row, query) pairs lane,
             // lane + 64, ...
