@@ -75,17 +75,20 @@ SVS_HD uint64_t mul_pow2(uint64_t x, int s) {
 // h0 2^32 + h1 lies in (-2^50, 2^62); z + p for negative z, one Mersenne
 // fold and a 61-bit rotation finish it. Eight such values sum to < 2^64.
 SVS_HD uint64_t plane_fold(int32_t c0, int32_t c1, int32_t c2, int32_t c3, int32_t c4,
-                           int32_t c5, int32_t c6, int32_t c7, int rot) {
-    const int64_t lo = (int64_t)c0 + (int64_t)c1 * 256 + (int64_t)c2 * 65536 +
-                       (int64_t)c3 * 16777216;
-    const int64_t hi = (int64_t)c4 + (int64_t)c5 * 256 + (int64_t)c6 * 65536 +
-                       (int64_t)c7 * 16777216;
+                           int32_t c5, int32_t c6, int32_t c7, int rot, int32_t m8 = 256,
+                           int32_t m16 = 65536, int32_t m24 = 16777216) {
+    // m8, m16, m24 = 2^8, 2^16, 2^24 (callers may pass them opaque, so that
+    // each term is one 32 x 32 + 64-bit multiply-add instead of a sign
+    // extension, a 64-bit shift and an add)
+    const int64_t lo = (int64_t)c0 + (int64_t)c1 * m8 + (int64_t)c2 * m16 + (int64_t)c3 * m24;
+    const int64_t hi = (int64_t)c4 + (int64_t)c5 * m8 + (int64_t)c6 * m16 + (int64_t)c7 * m24;
     const int64_t h1 = hi >> 29;  // arithmetic: floor(hi / 2^29)
     const uint64_t h0 = (uint64_t)hi & ((1ull << 29) - 1);
     const int64_t z = lo + (int64_t)(h0 << 32) + h1;
     const uint64_t zu = z < 0 ? (uint64_t)(z + (int64_t)P) : (uint64_t)z;
     const uint64_t r = (zu & P) + (zu >> 61);
-    return rot ? ((r << rot) & P) | (r >> (61 - rot)) : r;
+    // r < 2^61, so the rotation by 0 (r >> 61 = 0) needs no special case
+    return ((r << rot) & P) | (r >> (61 - rot));
 }
 
 // A sum of up to eight plane_fold values plus two canonical terms, as a

@@ -1925,6 +1925,9 @@ __device__ __forceinline__ void planes_epilogue(const v4i (&acc)[8][NB], int jj,
                                                 const uint64_t* __restrict__ qcorr,
                                                 uint64_t* __restrict__ out) {
     const int gn = (int)tile.gn;
+    // the plane weights, opaque to the compiler (see plane_fold)
+    int32_t m8 = 256, m16 = 65536, m24 = 16777216;
+    asm volatile("" : "+s"(m8), "+s"(m16), "+s"(m24));
 #pragma unroll
     for (int jb = 0; jb < NB; ++jb) {
         uint64_t v[4];
@@ -1932,7 +1935,7 @@ __device__ __forceinline__ void planes_epilogue(const v4i (&acc)[8][NB], int jj,
         for (int i = 0; i < 4; ++i)
             v[i] = plane_fold(acc[0][jb][i], acc[1][jb][i], acc[2][jb][i], acc[3][jb][i],
                               acc[4][jb][i], acc[5][jb][i], acc[6][jb][i], acc[7][jb][i],
-                              8 * bpl);
+                              8 * bpl, m8, m16, m24);
         const bool b1 = (bpl & 2) != 0, b0 = (bpl & 1) != 0;
         uint64_t k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
         const uint64_t s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
@@ -1951,6 +1954,9 @@ __device__ __forceinline__ void planes_epilogue(const v4i (&acc)[8][NB], int jj,
         const int j = 2 * jb + jj;
         if (write && bpl < 4 && j < gn && wrow < m)
             out[(jj ? cb1 : cb0) + wrow] = plane_finish(kp, rcorr, jj ? qt1 : qt0);
+        // one B tile at a time: interleaved, the folds of all eight keep
+        // more values live than the 16-query kernel has registers for
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
