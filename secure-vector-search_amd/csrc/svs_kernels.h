@@ -2031,7 +2031,12 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
             qb[jb] = (jj ? q1 : q0) * QBYTES + (uint32_t)(g * 8 + bpl) * 16u;
         }
     };
-    v4i ra[PF][8], rb[PF][NB];
+    // 16 queries per wave: one B slot, refilled with the next step's tiles
+    // right after this step's MFMAs read it and issued ahead of the A
+    // refill (the query planes stay in L2, so a step's lead covers them);
+    // narrower waves keep B in the PF-deep ring beside A
+    constexpr bool LATE_B = QW >= 16;
+    v4i ra[PF][8], rb[LATE_B ? 1 : PF][NB];
     auto issue_a = [&](int slot, const uint64_t* ap, int s) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) ra[slot][j] = load16(ap + 64 * s + 8 * j);
@@ -2044,19 +2049,20 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
         issue_a(slot, ap, s);
         issue_b(slot, qb, s);
     };
-    // 16 queries per wave: the MFMAs read the B operands from the ring slot
-    // and the slot is refilled after them (no copy of B, and the A slot's
-    // registers hold the planes meanwhile), which keeps the wave within 512
-    // registers; narrower waves copy B and refill the whole slot first
-    constexpr bool LATE_B = QW >= 16;
 
     Unit cur = find(4u * sh.worker + wave);
     if (cur.u >= n_units) return;
     const uint64_t* a_cur;
     uint32_t qb_cur[NB];
     lane_ptrs(cur, a_cur, qb_cur);
+    if constexpr (LATE_B) {
+        issue_b(0, qb_cur, 0);
 #pragma unroll
-    for (int d = 0; d < PF; ++d) issue(d, a_cur, qb_cur, d);
+        for (int d = 0; d < PF; ++d) issue_a(d, a_cur, d);
+    } else {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) issue(d, a_cur, qb_cur, d);
+    }
     for (;;) {
         const Unit nxt = find(cur.u + ustride);
         const bool has_next = nxt.u < n_units;
@@ -2097,9 +2103,13 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
 #pragma unroll
                     for (int a = 0; a < 8; ++a)
                         acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                            pl[a], rb[slot][jb], acc[a][jb], 0, 0, 0);
+                            pl[a], rb[0][jb], acc[a][jb], 0, 0, 0);
+                // the next step's B: this unit's step s + 1, or the next
+                // unit's step 0 (the last unit re-loads its final step)
+                const bool b_in = s + 1 < KS;
+                issue_b(0, b_in ? qb_cur : (has_next ? qb_nxt : qb_cur),
+                        b_in ? s + 1 : (has_next ? 0 : KS - 1));
                 issue_a(slot, ap, sn);
-                issue_b(slot, qbp, sn);
             } else {
                 v4i bq[NB];
 #pragma unroll
