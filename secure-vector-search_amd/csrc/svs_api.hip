@@ -916,7 +916,7 @@ template <int KE, int SRC>
 static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tiles,
                           const TileArgs& a, const int8_t* qplanes, const uint64_t* qc,
                           uint64_t* out) {
-    constexpr int PF16 = KE / 64 >= 4 ? 2 : 1;
+    constexpr int PF16 = KE == 256 ? 4 : 2;
     switch (kTileVariants[v].qg) {
         case 16:
             launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);

@@ -2531,13 +2531,16 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
             }
-            // Two-slot register ring over the K-steps: slot d's MFMAs run,
-            // then slot d is reloaded for the step two ahead (the tail
-            // re-loads the last step, in bounds), so one step's loads are
-            // always in flight behind the current step's MFMAs. The outer
-            // loop stays rolled so the compiler keeps that order.
-            v4i ra[2][RTW][4], rb[2][1 + NCT];
-            auto issue = [&](int slot, int kc) {
+            // Two-slot register ring of the row operands over the K-steps:
+            // slot d's MFMAs run, then slot d is reloaded for the step two
+            // ahead (the tail re-loads the last step, in bounds), so one
+            // step's loads are always in flight behind the current step's
+            // MFMAs. The query operands (in L2) take one slot, refilled with
+            // the next step's right after the MFMAs read it and issued ahead
+            // of the row refill. The outer loop stays rolled so the compiler
+            // keeps that order.
+            v4i ra[2][RTW][4], rb[1 + NCT];
+            auto issue_a = [&](int slot, int kc) {
                 // past the nonzero plaintext the plane loads repeat the last
                 // step's addresses (cache hits, results unused)
                 const int kp = kc < kw ? kc : (kw > 0 ? kw - 1 : 0);
@@ -2548,18 +2551,21 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                         ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kp);
                     ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
                 }
-                rb[slot][0] = load16b(bt + 64 * kc);
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) rb[slot][1 + ct] = load16b(bv[ct] + 64 * kc);
             };
-            issue(0, 0);
-            issue(1, 1);
+            auto issue_b = [&](int kc) {
+                rb[0] = load16b(bt + 64 * kc);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) rb[1 + ct] = load16b(bv[ct] + 64 * kc);
+            };
+            issue_b(0);
+            issue_a(0, 0);
+            issue_a(1, 1);
 #pragma unroll 1
             for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
                 for (int slot = 0; slot < 2; ++slot) {
                     if (kc0 + slot < kw) {
-                        const v4i b_t = rb[slot][0] & tmask;
+                        const v4i b_t = rb[0] & tmask;
 #pragma unroll
                         for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
@@ -2569,13 +2575,14 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                     }
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) {
-                        const v4i b_v = rb[slot][1 + ct] & vmask[ct];
+                        const v4i b_v = rb[1 + ct] & vmask[ct];
 #pragma unroll
                         for (int rt = 0; rt < RTW; ++rt)
                             asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                                 ra[slot][rt][3], b_v, asv[rt][ct], 0, 0, 0);
                     }
-                    issue(slot, min(kc0 + slot + 2, KC - 1));
+                    issue_b(min(kc0 + slot + 1, KC - 1));
+                    issue_a(slot, min(kc0 + slot + 2, KC - 1));
                 }
             }
             // The epilogue's inputs for this lane's (row, query) pairs, loaded
