@@ -948,11 +948,11 @@ static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& 
     }
 }
 
-template <int QG, int RTW>
+template <int QG>
 static void launch_decode_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
                             const int8_t* t_dense, const int8_t* vplanes, const uint64_t* mt_q,
                             const uint64_t* r, const uint64_t* dense, int64_t* scores) {
-    hipLaunchKernelGGL((k_decode_tiles<QG, RTW>), dim3(blocks), dim3(256), 0, st, a.lists,
+    hipLaunchKernelGGL((k_decode_tiles<QG>), dim3(blocks), dim3(256), 0, st, a.lists,
                        a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes, mt_q, r,
                        dense, scores);
 }
@@ -964,9 +964,9 @@ static void launch_decode_tiles(int v, hipStream_t st, uint32_t blocks, const Ti
                                 const uint64_t* mt_q, const uint64_t* r, const uint64_t* dense,
                                 int64_t* scores) {
     switch (kTileVariants[v].qg) {
-        case 16: launch_decode_v<16, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
-        case 4: launch_decode_v<4, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
-        default: launch_decode_v<8, PB_RT>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        case 16: launch_decode_v<16>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        case 4: launch_decode_v<4>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
+        default: launch_decode_v<8>(st, blocks, a, t_dense, vplanes, mt_q, r, dense, scores); break;
     }
 }
 

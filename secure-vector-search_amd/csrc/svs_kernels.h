@@ -2465,11 +2465,18 @@ k_prod_staged(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
 // exact int8 products over K = 1024:
 //   M T_j   = sum_a 256^a (s_a . T_j) + MP8_OFFSET sum T_j,
 //   S v_j   = sum_b 256^b (S . v'_{j,b}) + c sum S,
-// with A = the row's three M planes and S, B = T_j (one column per query,
-// columns 8..15 zero) and the eight v_enc byte planes of query j (two
-// queries per 16-column tile). Each wave owns PB_RT 16-row tiles; QG is the
-// tile's query group (the product's).
-template <int QG, int RTW>
+// with A = the row's three M planes and S, B = T_j (one column per query)
+// and the eight v_enc byte planes of query j (two queries per 16-column
+// tile). QG is the tile's query group (the product's); each wave owns
+// PB_RT 16-row tiles (64 of the tile's 256 rows).
+//
+// The query operands are the same for the block's four waves: at the start
+// of a tile the block stages all of them (16 K-steps x (1 + QG / 2) B tiles
+// x 1 KB, columns without a query zeroed) in LDS once, and the waves read
+// them from there, so the query planes cross L2 once per tile instead of
+// once per wave. The epilogue's staging area shares that LDS (after a block
+// barrier: every wave is past its K loop).
+template <int QG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
@@ -2477,11 +2484,20 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                const int8_t* __restrict__ t_dense, const int8_t* __restrict__ vplanes,
                const uint64_t* __restrict__ mt_qterm, const uint64_t* __restrict__ r,
                const uint64_t* __restrict__ dense, int64_t* __restrict__ scores) {
-    constexpr int KC = N / 64;  // 16 K-steps of 64 bytes
+    constexpr int KC = N / 64;   // 16 K-steps of 64 bytes
     constexpr int NCT = QG / 2;  // S v column tiles
+    constexpr int NBT = 1 + NCT; // B tiles per K-step: T, then the v planes
+    constexpr int RTW = PB_RT;
+    constexpr int BSM_BYTES = KC * NBT * WAVE * 16;
+    constexpr int DSM_BYTES = PB_WAVES * (3 + NCT) * 16 * 17 * 4;
+    constexpr int STAGE = KC * NBT * WAVE / 256;  // B entries per thread
+    static_assert(STAGE % 4 == 0, "staging in fours");
     static_assert(QG == 4 || QG == 8 || QG == 16, "query group");
-    static_assert(PB_RT % RTW == 0, "passes must tile the wave's rows");
-    __shared__ int32_t dsm[PB_WAVES][3 + NCT][16][17];
+    static_assert((KC * NBT) % PB_WAVES == 0, "the staging must tile the threads");
+    __shared__ __attribute__((aligned(16))) char smem[BSM_BYTES > DSM_BYTES ? BSM_BYTES : DSM_BYTES];
+    __shared__ uint32_t sq[QG];
+    v4i(*bsm)[NBT][WAVE] = reinterpret_cast<v4i(*)[NBT][WAVE]>(smem);
+    int32_t(*dsm)[3 + NCT][16][17] = reinterpret_cast<int32_t(*)[3 + NCT][16][17]>(smem);
     const int tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int grp = lane / 16, col = lane % 16;
@@ -2493,168 +2509,170 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
         const int gn = (int)tile.gn;
         // M T runs over the K-steps of the list's nonzero plaintext columns
         const int kw = (int)(L.mp_cols / 64);
-        // B sources; a column without a query reads query 0's data and is
-        // masked to zero (loads stay unconditional).
-        const uint32_t q0 = items[gitems[tile.g0]].q;
-        const bool t_ok = col < gn;
-        const int8_t* bt =
-            t_dense + (size_t)(t_ok ? items[gitems[tile.g0 + col]].q : q0) * N + 16 * grp;
-        const int tmask = t_ok ? -1 : 0;
-        const int8_t* bv[NCT];
-        int vmask[NCT];
+        // every wave is done with the previous tile's LDS (its epilogue)
+        __syncthreads();
+        if (tid < QG) sq[tid] = items[gitems[tile.g0 + min(tid, gn - 1)]].q;
+        __syncthreads();
+        // Stage the tile's query operands. Thread tid fills lane tid % 64 of
+        // B tiles (step, tile) = e / NBT, e % NBT for e = tid / 64 + 4 i; a
+        // column without a query (j >= gn) reads a present query's data and
+        // stores zeros (the loads stay unconditional and in bounds).
+#pragma unroll 1
+        for (int i0 = 0; i0 < STAGE; i0 += 4) {
+            v4i st[4];
 #pragma unroll
-        for (int ct = 0; ct < NCT; ++ct) {
-            const int j = 2 * ct + col / 8;
-            const bool ok = j < gn;
-            const uint32_t qj = ok ? items[gitems[tile.g0 + j]].q : q0;
-            bv[ct] = vplanes + ((size_t)qj * 8 + col % 8) * N + 16 * grp;
-            vmask[ct] = ok ? -1 : 0;
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + PB_WAVES * (i0 + i);
+                const int kc = e / NBT, bt = e % NBT;
+                const int j = bt == 0 ? col : 2 * (bt - 1) + col / 8;
+                const uint32_t q = sq[min(j, gn - 1)];
+                const int8_t* src = bt == 0 ? t_dense + (size_t)q * N
+                                            : vplanes + ((size_t)q * 8 + col % 8) * N;
+                const int mask = j < gn ? -1 : 0;
+                st[i] = load16b(src + 16 * grp + 64 * kc) & mask;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + PB_WAVES * (i0 + i);
+                bsm[e / NBT][e % NBT][lane] = st[i];
+            }
         }
-        // The wave's 64 rows in passes of RTW 16-row tiles (fewer live
-        // registers per pass for the wide query group).
+        __syncthreads();
+        const uint32_t prow0 = tile.row0 + 64 * wave;
+        const int8_t* amp[RTW];
+        const int8_t* as8[RTW];
+#pragma unroll
+        for (int rt = 0; rt < RTW; ++rt) {
+            const uint32_t row = min(prow0 + 16 * rt + col, m - 1);
+            amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
+            as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
+        }
+        v4i amt[RTW][3], asv[RTW][NCT];
+#pragma unroll
+        for (int rt = 0; rt < RTW; ++rt) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) amt[rt][a] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
+        }
+        // Two-slot register ring of the row operands over the K-steps: slot
+        // d's MFMAs run, then slot d is reloaded for the step two ahead (the
+        // tail re-loads the last step, in bounds), so one step's loads are
+        // always in flight behind the current step's MFMAs. The next step's
+        // query operands are read from LDS after the current step's MFMAs.
+        // The outer loop stays rolled so the compiler keeps that order.
+        v4i ra[2][RTW][4], rb[NBT];
+        auto issue_a = [&](int slot, int kc) {
+            // past the nonzero plaintext the plane loads repeat the last
+            // step's addresses (cache hits, results unused)
+            const int kp = kc < kw ? kc : (kw > 0 ? kw - 1 : 0);
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kp);
+                ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
+            }
+        };
+        auto read_b = [&](int kc) {
+#pragma unroll
+            for (int bt = 0; bt < NBT; ++bt) rb[bt] = bsm[kc][bt][lane];
+        };
+        read_b(0);
+        issue_a(0, 0);
+        issue_a(1, 1);
 #pragma unroll 1
-        for (int pass = 0; pass < PB_RT / RTW; ++pass) {
-            const uint32_t prow0 = tile.row0 + 64 * wave + 16 * RTW * pass;
-            const int8_t* amp[RTW];
-            const int8_t* as8[RTW];
+        for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
-            for (int rt = 0; rt < RTW; ++rt) {
-                const uint32_t row = min(prow0 + 16 * rt + col, m - 1);
-                amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
-                as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
-            }
-            v4i amt[RTW][3], asv[RTW][NCT];
+            for (int slot = 0; slot < 2; ++slot) {
+                if (kc0 + slot < kw) {
 #pragma unroll
-            for (int rt = 0; rt < RTW; ++rt) {
+                    for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) amt[rt][a] = v4i{0, 0, 0, 0};
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
-            }
-            // Two-slot register ring of the row operands over the K-steps:
-            // slot d's MFMAs run, then slot d is reloaded for the step two
-            // ahead (the tail re-loads the last step, in bounds), so one
-            // step's loads are always in flight behind the current step's
-            // MFMAs. The query operands (in L2) take one slot, refilled with
-            // the next step's right after the MFMAs read it and issued ahead
-            // of the row refill. The outer loop stays rolled so the compiler
-            // keeps that order.
-            v4i ra[2][RTW][4], rb[1 + NCT];
-            auto issue_a = [&](int slot, int kc) {
-                // past the nonzero plaintext the plane loads repeat the last
-                // step's addresses (cache hits, results unused)
-                const int kp = kc < kw ? kc : (kw > 0 ? kw - 1 : 0);
-#pragma unroll
-                for (int rt = 0; rt < RTW; ++rt) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-                        ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kp);
-                    ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
+                        for (int a = 0; a < 3; ++a)
+                            amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                ra[slot][rt][a], rb[0], amt[rt][a], 0, 0, 0);
                 }
-            };
-            auto issue_b = [&](int kc) {
-                rb[0] = load16b(bt + 64 * kc);
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) rb[1 + ct] = load16b(bv[ct] + 64 * kc);
-            };
-            issue_b(0);
-            issue_a(0, 0);
-            issue_a(1, 1);
-#pragma unroll 1
-            for (int kc0 = 0; kc0 < KC; kc0 += 2) {
+                for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
-                for (int slot = 0; slot < 2; ++slot) {
-                    if (kc0 + slot < kw) {
-                        const v4i b_t = rb[0] & tmask;
+                    for (int rt = 0; rt < RTW; ++rt)
+                        asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            ra[slot][rt][3], rb[1 + ct], asv[rt][ct], 0, 0, 0);
+                read_b(min(kc0 + slot + 1, KC - 1));
+                issue_a(slot, min(kc0 + slot + 2, KC - 1));
+            }
+        }
+        // The epilogue's inputs for this lane's (row, query) pairs, loaded
+        // all at once after the K loop (whose ring registers are free by
+        // then): the candidate index, r, the dense term, the row and query
+        // terms.
+        constexpr int NH = QG / 4;
+        uint32_t eg[RTW][NH];
+        uint64_t er[RTW][NH], ed[RTW][NH], es[RTW][NH], eq[RTW][NH];
 #pragma unroll
-                        for (int rt = 0; rt < RTW; ++rt)
+        for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                            for (int a = 0; a < 3; ++a)
-                                amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                    ra[slot][rt][a], b_t, amt[rt][a], 0, 0, 0);
+            for (int h = 0; h < NH; ++h) {
+                const int pr = lane + 64 * h;
+                const int row = pr % 16, j = pr / 16;
+                const uint32_t grow = prow0 + 16 * rt + row;
+                const bool ok = j < gn && grow < m;
+                const uint32_t it = gitems[tile.g0 + (ok ? j : 0)];
+                const uint32_t g = cand_ptr[it] + (ok ? grow : 0);
+                eg[rt][h] = ok ? g : 0xFFFFFFFFu;
+                er[rt][h] = r[g];
+                ed[rt][h] = dense[g];
+                es[rt][h] = L.s_rowterm[ok ? grow : 0];
+                eq[rt][h] = mt_qterm[(size_t)items[it].q * (MT_BLOCKS + 1) + kw];
+            }
+        // the staged query operands are dead for every wave: the LDS becomes
+        // the epilogue's staging area
+        __syncthreads();
+        // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i, column
+        // col), then each lane finishes (row, query) pairs lane, lane + 64, ...
+#pragma unroll
+        for (int rt = 0; rt < RTW; ++rt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dsm[wave][a][4 * grp + i][col] = amt[rt][a][i];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct)
+                    dsm[wave][3 + ct][4 * grp + i][col] = asv[rt][ct][i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const int pr = lane + 64 * h;
+                const int row = pr % 16, j = pr / 16;
+                if (eg[rt][h] != 0xFFFFFFFFu) {
+                    const int64_t mt = (int64_t)dsm[wave][0][row][j] +
+                                       (int64_t)dsm[wave][1][row][j] * 256 +
+                                       (int64_t)dsm[wave][2][row][j] * 65536;
+                    // S v = sum_b 256^b D_b with |D_b| <= 128 * 1024: two
+                    // halves of four planes each fit in 64 bits, and the high
+                    // half enters as 2^32 times itself
+                    int64_t sv_lo = 0, sv_hi = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        sv_lo += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
+                                 (int64_t)(1ll << (8 * b));
+                        sv_hi += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + 4 + b] *
+                                 (int64_t)(1ll << (8 * b));
                     }
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) {
-                        const v4i b_v = rb[1 + ct] & vmask[ct];
-#pragma unroll
-                        for (int rt = 0; rt < RTW; ++rt)
-                            asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                ra[slot][rt][3], b_v, asv[rt][ct], 0, 0, 0);
-                    }
-                    issue_b(min(kc0 + slot + 1, KC - 1));
-                    issue_a(slot, min(kc0 + slot + 2, KC - 1));
+                    const uint64_t sv = add_mod(i64_to_fp(sv_lo), mul_pow2(i64_to_fp(sv_hi), 32));
+                    uint64_t corr = add_mod(i64_to_fp(mt), eq[rt][h]);
+                    corr = add_mod(corr, add_mod(sv, es[rt][h]));
+                    corr = add_mod(corr, ed[rt][h]);
+                    scores[eg[rt][h]] = fp_to_signed(sub_mod(er[rt][h], corr));
                 }
             }
-            // The epilogue's inputs for this lane's (row, query) pairs, loaded
-            // all at once after the K loop (whose ring registers are free by
-            // then): the candidate index, r, the dense term, the row and
-            // query terms.
-            constexpr int NH = QG / 4;
-            uint32_t eg[RTW][NH];
-            uint64_t er[RTW][NH], ed[RTW][NH], es[RTW][NH], eq[RTW][NH];
-#pragma unroll
-            for (int rt = 0; rt < RTW; ++rt)
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    const int pr = lane + 64 * h;
-                    const int row = pr % 16, j = pr / 16;
-                    const uint32_t grow = prow0 + 16 * rt + row;
-                    const bool ok = j < gn && grow < m;
-                    const uint32_t it = gitems[tile.g0 + (ok ? j : 0)];
-                    const uint32_t g = cand_ptr[it] + (ok ? grow : 0);
-                    eg[rt][h] = ok ? g : 0xFFFFFFFFu;
-                    er[rt][h] = r[g];
-                    ed[rt][h] = dense[g];
-                    es[rt][h] = L.s_rowterm[ok ? grow : 0];
-                    eq[rt][h] = mt_qterm[(size_t)items[it].q * (MT_BLOCKS + 1) + kw];
-                }
-            // Epilogue per row tile: accumulators -> LDS (rows 4 grp + i,
-            // column col), then each lane finishes (row, query) pairs lane,
-            // lane + 64, ...
-#pragma unroll
-            for (int rt = 0; rt < RTW; ++rt) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) dsm[wave][a][4 * grp + i][col] = amt[rt][a][i];
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct)
-                        dsm[wave][3 + ct][4 * grp + i][col] = asv[rt][ct][i];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    const int pr = lane + 64 * h;
-                    const int row = pr % 16, j = pr / 16;
-                    if (eg[rt][h] != 0xFFFFFFFFu) {
-                        const int64_t mt = (int64_t)dsm[wave][0][row][j] +
-                                           (int64_t)dsm[wave][1][row][j] * 256 +
-                                           (int64_t)dsm[wave][2][row][j] * 65536;
-                        // S v = sum_b 256^b D_b with |D_b| <= 128 * 1024: two
-                        // halves of four planes each fit in 64 bits, and the
-                        // high half enters as 2^32 times itself
-                        int64_t sv_lo = 0, sv_hi = 0;
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            sv_lo += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + b] *
-                                     (int64_t)(1ll << (8 * b));
-                            sv_hi += (int64_t)dsm[wave][3 + j / 2][row][(j % 2) * 8 + 4 + b] *
-                                     (int64_t)(1ll << (8 * b));
-                        }
-                        const uint64_t sv =
-                            add_mod(i64_to_fp(sv_lo), mul_pow2(i64_to_fp(sv_hi), 32));
-                        uint64_t corr = add_mod(i64_to_fp(mt), eq[rt][h]);
-                        corr = add_mod(corr, add_mod(sv, es[rt][h]));
-                        corr = add_mod(corr, ed[rt][h]);
-                        scores[eg[rt][h]] = fp_to_signed(sub_mod(er[rt][h], corr));
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
