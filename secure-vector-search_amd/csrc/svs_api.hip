@@ -169,6 +169,7 @@ extern "C" int svs_ctx_create(int device, uint64_t budget_bytes, svs_ctx** out) 
     if (const char* g = getenv("SVS_PROD_SPLIT")) ctx->prod_split = atoi(g) ? 1 : 0;
     if (const char* g = getenv("SVS_PROD_SPLIT_PF")) ctx->prod_split_pf = atoi(g);
     if (const char* g = getenv("SVS_DENSE_PLANES")) ctx->dense_planes = atoi(g) ? 1 : 0;
+    if (const char* g = getenv("SVS_PROD_LDSB")) ctx->prod_ldsb = atoi(g) ? 1 : 0;
     // graphs off for the whole process (profilers that trace kernels have
     // crashed on the host inside graph capture; the kernels are the same)
     if (const char* g = getenv("SVS_GRAPHS")) ctx->graphs = atoi(g) ? 1 : 0;
@@ -919,7 +920,18 @@ static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tile
     constexpr int PF16 = KE == 256 ? 4 : 2;
     switch (kTileVariants[v].qg) {
         case 16:
-            launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);
+            if (ctx->prod_ldsb) {
+                // whole tiles per block, query planes staged in LDS
+                const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>(std::max<uint64_t>(cap_tiles, 1),
+                                          (uint64_t)ctx->n_cu * ctx->tile_grid));
+                hipLaunchKernelGGL((k_prod_planes_lb<KE, SRC, PF16>), dim3(blocks), dim3(256), 0,
+                                   st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr,
+                                   qplanes, qc, out);
+            } else {
+                launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc,
+                                                      out);
+            }
             break;
         case 4:
             launch_planes_v<KE, SRC, 4, 4, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc, out);
@@ -1508,6 +1520,7 @@ int svs::run_graphed(svs_ctx* ctx, svs_slot& sl, std::vector<uint64_t> key,
     key.push_back((uint64_t)ctx->prod_split);
     key.push_back((uint64_t)ctx->prod_split_pf);
     key.push_back((uint64_t)ctx->dense_planes);
+    key.push_back((uint64_t)ctx->prod_ldsb);
     for (DevBuf* b : sl.slot_scratch()) {
         key.push_back((uint64_t)(uintptr_t)b->ptr);
         key.push_back(b->cap);

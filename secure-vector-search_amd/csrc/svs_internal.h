@@ -167,6 +167,7 @@ struct svs_ctx : svs_slot {
     // tile on k_prod_planes (0, the default: measured faster so far)
     int prod_split = 0;
     int prod_split_pf = 2;  // k_prod_staged's ring depth in steps: 2 or 4
+    int prod_ldsb = 1;      // 16-query plane tiles: query planes staged in LDS per tile
     // the batched dense decode terms on the plane kernel (1, the default:
     // 0.17 ms faster per 256-query batch) or the window tiles (0)
     int dense_planes = 1;

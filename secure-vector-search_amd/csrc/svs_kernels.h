@@ -2137,6 +2137,141 @@ k_prod_planes(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
     }
 }
 
+// The plane product for 16-query tiles with the query operands in LDS: a
+// block takes whole plan tiles (256 rows of one list x up to 16 of its
+// queries); per tile it stages the tile's query planes (16 queries x 8 KE
+// bytes, in the B-operand order of k_prod_planes: step, B tile, lane) once
+// in LDS, and its four waves stream the tile's 16-row units (wave w: units
+// w, w + 4, ...) with the A ring of k_prod_planes, reading B from LDS. The
+// query planes then cross L2 once per tile instead of once per 16-row unit.
+// Tiles of an XCD's share go to its blocks in turn (worker, worker +
+// workers, ...), so the query groups of one row block run side by side on
+// one XCD. Absent query columns (j >= gn) repeat the last query and are not
+// written; rows past the list's end read its last row and are not written.
+template <int KE, int SRC, int PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_prod_planes_lb(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+                 const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
+                 const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+                 const int8_t* __restrict__ qplanes, const uint64_t* __restrict__ qcorr,
+                 uint64_t* __restrict__ out) {
+    constexpr int QW = 16, NB = QW / 2;
+    constexpr int KS = KE / 64;
+    constexpr uint32_t QBYTES = 8 * KE;
+    constexpr int STAGE = KS * NB * WAVE / 256;  // B entries per thread
+    static_assert(KS % PF == 0 && PF >= 1, "the ring must tile the steps");
+    static_assert(STAGE % 4 == 0, "staging in fours");
+    __shared__ v4i bsm[KS][NB][WAVE];
+    __shared__ uint32_t sq[QW];
+    const int tid = threadIdx.x;
+    const int lane = tid % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+
+    v4i ra[PF][8], rb[NB];
+    auto issue_a = [&](int slot, const uint64_t* ap, int s) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ra[slot][j] = load16(ap + 64 * s + 8 * j);
+    };
+    auto read_b = [&](int s) {
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) rb[jb] = bsm[s][jb][lane];
+    };
+    for (uint32_t t = sh.lo + sh.worker; t < sh.hi; t += sh.workers) {
+        const MTile tile = tiles[t];
+        const uint32_t m = lists[tile.list].m;
+        const uint64_t* menc = SRC ? lists[tile.list].alh : lists[tile.list].menc;
+        const uint64_t* rowcorr = SRC ? lists[tile.list].alh_rowcorr : lists[tile.list].rowcorr;
+        const uint32_t gn = tile.gn;
+        // every wave is done with the previous tile's query planes
+        __syncthreads();
+        if (tid < QW) sq[tid] = items[gitems[tile.g0 + min((uint32_t)tid, gn - 1)]].q;
+        __syncthreads();
+        // Stage the tile's query planes: thread tid fills lane tid % 64 of
+        // B tiles (step, tile) = e / NB, e % NB for e = tid / 64 + 4 i.
+#pragma unroll 1
+        for (int i0 = 0; i0 < STAGE; i0 += 4) {
+            v4i st[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + 4 * (i0 + i);
+                const int s = e / NB, jb = e % NB;
+                const uint32_t q = sq[2 * jb + jj];
+                st[i] = load16b(qplanes + (size_t)q * QBYTES + (uint32_t)(g * 8 + bpl) * 16u +
+                                512u * s);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + 4 * (i0 + i);
+                bsm[e / NB][e % NB][lane] = st[i];
+            }
+        }
+        __syncthreads();
+        // this wave's 16-row units of the tile: wave, wave + 4, ...
+        const uint32_t rows = min(256u, m - tile.row0);
+        const uint32_t nsb = (rows + 15) / 16;
+        if ((uint32_t)wave >= nsb) continue;
+        auto row_ptr = [&](uint32_t sb) -> const uint64_t* {
+            const uint32_t row = min(tile.row0 + 16u * sb + (uint32_t)col, m - 1);
+            return menc + (size_t)row * KE + 2 * g;
+        };
+        uint32_t sb = (uint32_t)wave;
+        const uint64_t* a_cur = row_ptr(sb);
+        read_b(0);
+#pragma unroll
+        for (int d = 0; d < PF; ++d) issue_a(d, a_cur, d);
+        for (;;) {
+            const uint32_t sbn = sb + 4;
+            const bool has_next = sbn < nsb;
+            const uint64_t* a_nxt = has_next ? row_ptr(sbn) : a_cur;
+            // epilogue input, loaded ahead of the steps: this lane writes
+            // row wrow (after the plane reduction) for query 2 jb + jj
+            const uint32_t row0 = tile.row0 + 16u * sb;
+            const uint32_t wrow = row0 + 4u * g + (uint32_t)(bpl & 3);
+            const uint64_t rcorr = gload_u64(rowcorr + min(wrow, m - 1));
+            // keep this load here: sunk to its use it would be the youngest
+            // load at the epilogue, whose wait would then drain the ring
+            __builtin_amdgcn_sched_barrier(0);
+            v4i acc[8][NB];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) acc[a][jb] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int slot = s % PF;
+                v4i pl[8];
+                planes_of(ra[slot], pl);
+                // the slot's next contents: this unit's step s + PF, or the
+                // next unit's first steps (the last unit re-loads its final
+                // step)
+                const bool in_unit = s + PF < KS;
+                const uint64_t* ap = in_unit ? a_cur : (has_next ? a_nxt : a_cur);
+                const int sn = in_unit ? s + PF : (has_next ? s + PF - KS : KS - 1);
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb)
+#pragma unroll
+                    for (int a = 0; a < 8; ++a)
+                        acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(pl[a], rb[jb],
+                                                                           acc[a][jb], 0, 0, 0);
+                // the next step's query planes (step 0 again after the last:
+                // the next unit is in the same tile)
+                read_b(s + 1 < KS ? s + 1 : 0);
+                issue_a(slot, ap, sn);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            planes_epilogue<NB>(acc, jj, bpl, wrow, m, rcorr, tile, true, gitems, items,
+                                cand_ptr, qcorr, out);
+            if (!has_next) break;
+            sb = sbn;
+            a_cur = a_nxt;
+        }
+    }
+    // drain the ring's last re-loads before the block ends
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
 // Calls f(ic<i>{}) for i = 0 .. N - 1 (i usable as a constant inside f).
 template <int I>
 struct ic {
