@@ -933,7 +933,7 @@ def main():
     rows = prof["gemv_rows"]
     mulmods = rows * 1024
     qw = variant["query_group"] if variant["query_group"] in (4, 16) else 8
-    kname = f"k_prod_planes<{qw},{2 if qw == 16 else 4},1>"
+    kname = "k_prod_planes_lb<1024,0,2>" if qw == 16 else f"k_prod_planes<{qw},4,1>"
     traffic, traffic_src = measured_traffic(kname, args)
     # useful int8 work of the byte-plane product: 1024 elements x 8 x 8 limb
     # byte products per (row, query) pair, 2 ops per MAC, against 5.0 POP/s
