@@ -960,11 +960,11 @@ static void launch_gemm(int v, hipStream_t st, uint32_t blocks, const TileArgs& 
     }
 }
 
-template <int QG>
+template <int QG, int NS = 2>
 static void launch_decode_v(hipStream_t st, uint32_t blocks, const TileArgs& a,
                             const int8_t* t_dense, const int8_t* vplanes, const uint64_t* mt_q,
                             const uint64_t* r, const uint64_t* dense, int64_t* scores) {
-    hipLaunchKernelGGL((k_decode_tiles<QG>), dim3(blocks), dim3(256), 0, st, a.lists,
+    hipLaunchKernelGGL((k_decode_tiles<QG, NS>), dim3(blocks), dim3(256), 0, st, a.lists,
                        a.tiles, a.nt, a.gitems, a.items, a.cand_ptr, t_dense, vplanes, mt_q, r,
                        dense, scores);
 }

@@ -2611,7 +2611,7 @@ k_prod_staged(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles
 // them from there, so the query planes cross L2 once per tile instead of
 // once per wave. The epilogue's staging area shares that LDS (after a block
 // barrier: every wave is past its K loop).
-template <int QG>
+template <int QG, int NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
                const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
@@ -2663,7 +2663,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
                 const uint32_t q = sq[min(j, gn - 1)];
                 const int8_t* src = bt == 0 ? t_dense + (size_t)q * N
                                             : vplanes + ((size_t)q * 8 + col % 8) * N;
-                const int mask = j < gn ? -1 : 0;
+                // T's tile is zero past the nonzero plaintext (kc >= kw)
+                const int mask = j < gn && (bt != 0 || kc < kw) ? -1 : 0;
                 st[i] = load16b(src + 16 * grp + 64 * kc) & mask;
             }
 #pragma unroll
@@ -2690,13 +2691,16 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct) asv[rt][ct] = v4i{0, 0, 0, 0};
         }
-        // Two-slot register ring of the row operands over the K-steps: slot
-        // d's MFMAs run, then slot d is reloaded for the step two ahead (the
-        // tail re-loads the last step, in bounds), so one step's loads are
-        // always in flight behind the current step's MFMAs. The next step's
-        // query operands are read from LDS after the current step's MFMAs.
-        // The outer loop stays rolled so the compiler keeps that order.
-        v4i ra[2][RTW][4], rb[NBT];
+        // NS-slot register ring of the row operands over the K-steps: slot
+        // d's MFMAs run, then slot d is reloaded for the step NS ahead (the
+        // tail re-loads the last step, in bounds), so NS - 1 steps' loads
+        // are always in flight behind the current step's MFMAs. The next
+        // step's query operands are read from LDS after the current step's
+        // MFMAs. M T runs on every step: its B tile is zero past the list's
+        // nonzero plaintext columns, so the repeated plane loads there add
+        // nothing and no branch splits the steps. One step per scheduling
+        // region keeps the loads where they are issued.
+        v4i ra[NS][RTW][4], rb[NBT];
         auto issue_a = [&](int slot, int kc) {
             // past the nonzero plaintext the plane loads repeat the last
             // step's addresses (cache hits, results unused)
@@ -2714,29 +2718,27 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             for (int bt = 0; bt < NBT; ++bt) rb[bt] = bsm[kc][bt][lane];
         };
         read_b(0);
-        issue_a(0, 0);
-        issue_a(1, 1);
-#pragma unroll 1
-        for (int kc0 = 0; kc0 < KC; kc0 += 2) {
 #pragma unroll
-            for (int slot = 0; slot < 2; ++slot) {
-                if (kc0 + slot < kw) {
+        for (int d = 0; d < NS; ++d) issue_a(d, d);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int rt = 0; rt < RTW; ++rt)
+        for (int kc = 0; kc < KC; ++kc) {
+            const int slot = kc % NS;
 #pragma unroll
-                        for (int a = 0; a < 3; ++a)
-                            amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                ra[slot][rt][a], rb[0], amt[rt][a], 0, 0, 0);
-                }
+            for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct)
+                for (int a = 0; a < 3; ++a)
+                    amt[rt][a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[slot][rt][a], rb[0],
+                                                                       amt[rt][a], 0, 0, 0);
 #pragma unroll
-                    for (int rt = 0; rt < RTW; ++rt)
-                        asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                            ra[slot][rt][3], rb[1 + ct], asv[rt][ct], 0, 0, 0);
-                read_b(min(kc0 + slot + 1, KC - 1));
-                issue_a(slot, min(kc0 + slot + 2, KC - 1));
-            }
+            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                for (int rt = 0; rt < RTW; ++rt)
+                    asv[rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                        ra[slot][rt][3], rb[1 + ct], asv[rt][ct], 0, 0, 0);
+            read_b(min(kc + 1, KC - 1));
+            issue_a(slot, min(kc + NS, KC - 1));
+            __builtin_amdgcn_sched_barrier(0);
         }
         // The epilogue's inputs for this lane's (row, query) pairs, loaded
         // all at once after the K loop (whose ring registers are free by
