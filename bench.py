@@ -4,7 +4,8 @@ against the plain per-query GPU path and the reference CPU path.
 
 Workload (defaults): N = 1,000,000 synthetic d = 768 unit vectors from a
 seeded 1024-component Gaussian mixture with Zipf(1.1) component weights;
-nlist = 1024 IVF lists (the mixture components); nprobe = 32; k = 10;
+nlist = 1024 IVF lists from a seeded k-means of the corpus (`--lists
+mixture`: the mixture components); nprobe = 32; k = 10;
 Sec128 BN encryption (n = 1024, n1 = 128, p = 2^61 - 1).
 
 `value` is the served throughput at the latency SLO: raw f32 queries arrive
@@ -726,7 +727,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="force a block-tile variant (skips the autotuner)")
-    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,38000,40000,42000,44000,48000",
+    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,40000,44000,48000,52000,56000,60000",
                     help="offered rates of the served (open-loop Poisson) levels; '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)

@@ -379,8 +379,10 @@ static int list_alloc(svs_index* idx, uint32_t list, uint32_t m, bool mp32, bool
     if (rc == SVS_OK) rc = take((void**)&L.d.rowcorr, sizeof(uint64_t) * m);
     if (rc == SVS_OK) rc = take((void**)&L.d.alh, sizeof(uint64_t) * (size_t)m * 2 * N1);
     if (rc == SVS_OK) rc = take((void**)&L.d.alh_rowcorr, sizeof(uint64_t) * m);
-    if (rc == SVS_OK && planes) rc = take((void**)&L.d.mp8, (size_t)m * 3 * N);
-    if (rc == SVS_OK) rc = take((void**)&L.d.s8, (size_t)m * N);
+    // the decode tiles' planes: rows padded to 16 (their tiled layout)
+    const size_t m16 = ((size_t)m + 15) & ~(size_t)15;
+    if (rc == SVS_OK && planes) rc = take((void**)&L.d.mp8, m16 * 3 * N);
+    if (rc == SVS_OK) rc = take((void**)&L.d.s8, m16 * N);
     if (rc == SVS_OK) rc = take((void**)&L.d.s_rowterm, sizeof(uint64_t) * m);
     if (rc != SVS_OK) free_list(ctx, L);
     L.d.m = m;
@@ -415,7 +417,7 @@ static int list_upload_s_ids(svs_index* idx, uint32_t list, const uint32_t* rowp
     for (uint32_t i = 0; i < m; ++i) idv[i] = ids ? ids[i] : i;
     HIP_TRY(hipMemcpy((void*)L.d.ids, idv.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
     // dense int8 S and its row term for the MFMA decode
-    const uint64_t words = (uint64_t)m * N / 4;
+    const uint64_t words = (((uint64_t)m + 15) & ~15ull) * N / 4;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0,
                        ctx->stream, (uint32_t*)L.d.s8, (uint32_t)words, 0u);
     hipLaunchKernelGGL(k_make_s8, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, L.d.s_slice,

@@ -1760,8 +1760,21 @@ __device__ __forceinline__ uint64_t i64_to_fp(int64_t x) {
     return x >= 0 ? u64_mod_p((uint64_t)x) : sub_mod(0, u64_mod_p(0ull - (uint64_t)x));
 }
 
+// The decode tiles' int8 operand layout (mp8 with PL = 3 planes, s8 with
+// PL = 1): 16-row blocks, per block 16 K-steps x PL planes of 1 KB, each
+// 1 KB the MFMA A operand of one 64-element step of the block's 16 rows as
+// a wave loads it: lane l = 16 g + r holds elements 64 kc + 16 g .. + 15 of
+// row 16 b + r. A wave's load of one (step, plane) is then 1 KB contiguous
+// (whole cache lines) instead of 64 bytes from each of 16 rows. Rows are
+// padded to a multiple of 16.
+__device__ __forceinline__ size_t tiled_off(uint32_t row, uint32_t k, int pl, int plane) {
+    const uint32_t b = row / 16, r = row % 16, kc = k / 64, g = (k % 64) / 16, e = k % 16;
+    return (((size_t)b * (N / 64) + kc) * pl + plane) * 1024 + (g * 16 + r) * 16 + e;
+}
+
 // M_plain rows (canonical, row-major m x 1024) -> the 24-bit byte planes of
-// mp8. The caller has checked that every signed lift is in (-2^23, 2^23).
+// mp8 (tiled as above). The caller has checked that every signed lift is in
+// (-2^23, 2^23).
 __global__ void k_make_mp8(const uint64_t* __restrict__ mp, uint32_t m, int8_t* __restrict__ mp8) {
     const uint64_t total = (uint64_t)m * N;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -1769,27 +1782,26 @@ __global__ void k_make_mp8(const uint64_t* __restrict__ mp, uint32_t m, int8_t* 
         const uint64_t x = mp[e];
         const int64_t sl = x <= P / 2 ? (int64_t)x : -(int64_t)(P - x);
         const uint32_t y = (uint32_t)(sl + MP8_LIMIT);
-        int8_t* row = mp8 + (e / N) * 3 * N + (e % N);
-        row[0] = (int8_t)((y & 0xFF) ^ 0x80);
-        row[N] = (int8_t)(((y >> 8) & 0xFF) ^ 0x80);
-        row[2 * N] = (int8_t)(((y >> 16) & 0xFF) ^ 0x80);
+        const uint32_t row = (uint32_t)(e / N), k = (uint32_t)(e % N);
+        mp8[tiled_off(row, k, 3, 0)] = (int8_t)((y & 0xFF) ^ 0x80);
+        mp8[tiled_off(row, k, 3, 1)] = (int8_t)(((y >> 8) & 0xFF) ^ 0x80);
+        mp8[tiled_off(row, k, 3, 2)] = (int8_t)(((y >> 16) & 0xFF) ^ 0x80);
     }
 }
 
-// S (sliced ELL) -> dense int8 rows (pre-zeroed) and the row term
-// c * sum_k S[row][k] of S v_enc's byte-offset identity.
+// S (sliced ELL) -> dense int8 (tiled as above, pre-zeroed) and the row
+// term c * sum_k S[row][k] of S v_enc's byte-offset identity.
 __global__ void k_make_s8(const uint32_t* __restrict__ s_slice, const uint16_t* __restrict__ s_ell,
                           uint32_t m, int8_t* __restrict__ s8, uint64_t* __restrict__ rowterm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    int8_t* row = s8 + (size_t)i * N;
     const uint32_t slice = i / SLICE, lane = i % SLICE;
     int64_t sum = 0;
     for (uint32_t e = s_slice[slice] + lane; e < s_slice[slice + 1]; e += SLICE) {
         const uint16_t se = s_ell[e];
         if (se == S_PAD) break;
         const int v = (se & 0x8000) ? -1 : 1;
-        row[se & 0x3FF] = (int8_t)v;
+        s8[tiled_off(i, se & 0x3FF, 1, 0)] = (int8_t)v;
         sum += v;
     }
     rowterm[i] = mul_mod(LIMB_OFFSET_C, i64_to_fp(sum));
@@ -2675,13 +2687,16 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
         }
         __syncthreads();
         const uint32_t prow0 = tile.row0 + 64 * wave;
+        // the wave's four 16-row blocks in the tiled planes (a block past
+        // the list's last one reads the last: in bounds, not written)
+        const uint32_t nb16 = (m + 15) / 16;
         const int8_t* amp[RTW];
         const int8_t* as8[RTW];
 #pragma unroll
         for (int rt = 0; rt < RTW; ++rt) {
-            const uint32_t row = min(prow0 + 16 * rt + col, m - 1);
-            amp[rt] = L.mp8 + (size_t)row * 3 * N + 16 * grp;
-            as8[rt] = L.s8 + (size_t)row * N + 16 * grp;
+            const uint32_t blk = min(prow0 / 16 + rt, nb16 - 1);
+            amp[rt] = L.mp8 + (size_t)blk * KC * 3 * 1024 + 16 * lane;
+            as8[rt] = L.s8 + (size_t)blk * KC * 1024 + 16 * lane;
         }
         v4i amt[RTW][3], asv[RTW][NCT];
 #pragma unroll
@@ -2709,8 +2724,8 @@ k_decode_tiles(const ListDev* __restrict__ lists, const MTile* __restrict__ tile
             for (int rt = 0; rt < RTW; ++rt) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
-                    ra[slot][rt][a] = load16b(amp[rt] + a * N + 64 * kp);
-                ra[slot][rt][3] = load16b(as8[rt] + 64 * kc);
+                    ra[slot][rt][a] = load16b(amp[rt] + (3 * kp + a) * 1024);
+                ra[slot][rt][3] = load16b(as8[rt] + kc * 1024);
             }
         };
         auto read_b = [&](int kc) {
