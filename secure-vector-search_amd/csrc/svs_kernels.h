@@ -243,11 +243,18 @@ k_verify_rows(const Item* __restrict__ items, uint32_t n_items,
     const uint32_t n_groups = (n_chunks + VR_GROUP - 1) / VR_GROUP;
     for (uint32_t gi = blockIdx.x * wpb + wave; gi < n_groups; gi += gridDim.x * wpb) {
         const uint32_t ch0 = gi * VR_GROUP;
+        // lane c < VR_GROUP finds the item of chunk ch0 + c (the searches
+        // run side by side); the other lanes take it by a broadcast
+        uint32_t my_it = 0;
+        if (lane < VR_GROUP && ch0 + lane < n_chunks)
+            my_it = find_seg(chunk_ptr, n_items, ch0 + lane);
+        const uint32_t it_mine =
+            __shfl(my_it, (int)min(lane / (3 * LAMBDA), (uint32_t)VR_GROUP - 1), WAVE);
         if (lane < VR_GROUP * 3 * LAMBDA) {
             const uint32_t c = lane / (3 * LAMBDA), t = (lane / 3) % LAMBDA, b = lane % 3;
             const uint32_t ch = ch0 + c;
             if (ch < n_chunks) {
-                const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+                const uint32_t it = it_mine;
                 if (item_verify[it]) {
                     const Item item = items[it];
                     const uint32_t row0 = (ch - chunk_ptr[it]) * CHUNK_ROWS;
@@ -267,7 +274,7 @@ k_verify_rows(const Item* __restrict__ items, uint32_t n_items,
         for (uint32_t c = 0; c < VR_GROUP; ++c) {
             const uint32_t ch = ch0 + c;
             if (ch >= n_chunks) break;
-            const uint32_t it = find_seg(chunk_ptr, n_items, ch);
+            const uint32_t it = __shfl(my_it, (int)c, WAVE);
             if (!item_verify[it]) {
                 if (lane < 2 * LAMBDA) partials[(size_t)ch * 2 * LAMBDA + lane] = 0;
                 continue;
