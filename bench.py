@@ -727,7 +727,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="force a block-tile variant (skips the autotuner)")
-    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,40000,44000,48000,52000,56000,60000",
+    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,40000,44000,48000,50000,52000,54000,56000,60000",
                     help="offered rates of the served (open-loop Poisson) levels; '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)
