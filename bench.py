@@ -727,7 +727,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="force a block-tile variant (skips the autotuner)")
-    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,40000,44000,48000,50000,52000,54000,56000,60000",
+    ap.add_argument("--serve-qps", default="20000,28000,32000,36000,40000,44000,48000,52000,54000,56000,58000,60000,62000,64000",
                     help="offered rates of the served (open-loop Poisson) levels; '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-batch", type=int, default=512)
@@ -827,9 +827,10 @@ def main():
     if args.tile_variant >= 0:
         ctx.set_tile_variant(args.tile_variant)
     elif not args.no_autotune:
-        # the timed batch size, plus the small-batch classes the server uses
+        # the timed batch size, plus the classes the server uses (batches
+        # past 256 queries form when arrivals outrun the device)
         tune = {}
-        for nb in sorted({args.batch, 16, 64}):
+        for nb in sorted({args.batch, 16, 64, min(512, v_q.shape[0])}):
             s0 = slice(0, nb)
             pp0 = np.arange(0, v_q[s0].shape[0] * args.nprobe + 1, args.nprobe, dtype=np.uint32)
             best, tms = index.autotune(v_q[s0], seeds[s0], pp0, probes[s0].reshape(-1),

@@ -157,7 +157,12 @@ struct svs_ctx : svs_slot {
     uint64_t fault_delta = 0;
     // chosen block-tile variant of the MFMA path per batch-size class
     // (n <= 16, <= 64, <= 256, larger)
-    int tile_variant[4] = {0, 0, 0, 0};
+    // tile variant per batch-size class (<= 16, <= 64, <= 256, more
+    // queries) until svs_autotune or svs_ctx_set_tile_variant sets one:
+    // 4-query tiles for the smallest batches, 16-query tiles (the plane
+    // product with the query planes in LDS) above, the fastest measured
+    // on the 1M-row k-means workload
+    int tile_variant[4] = {3, 4, 4, 4};
     // batched runs planned on the device (1) or on the host (0)
     int planner = 1;
     // persistent tile grids: resident blocks per CU times this
