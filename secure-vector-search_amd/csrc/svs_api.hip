@@ -1438,7 +1438,7 @@ int svs::run_view(svs_index* idx, svs_slot& sl, const PlanView& v, uint32_t k,
             ctx->launches += 2;
         } else {
             const uint32_t blocks = (uint32_t)std::min<uint64_t>(
-                (v.cap_cands + DC_THREADS - 1) / DC_THREADS, (uint64_t)ctx->n_cu * 32);
+                (v.cap_cands + DC_ROWS - 1) / DC_ROWS, (uint64_t)ctx->n_cu * 32);
             if (!dense) return fail(SVS_ERR_STATE, "decode without its dense terms");
             hipLaunchKernelGGL(k_decode_cands, dim3(blocks), dim3(DC_THREADS), 0, st, d_lists,
                                d_items, v.n_items, d_cand_ptr, (const uint64_t*)sl.r.ptr,
@@ -2449,7 +2449,7 @@ extern "C" int svs_decode(svs_index* idx, uint32_t list, const uint64_t* r, cons
                        dim3(256), 0, ctx->stream, idx->lists[list].d.alh, m,
                        (const uint64_t*)ctx->q_gl.ptr, (uint64_t*)ctx->dense.ptr);
     const uint32_t* pre = (const uint32_t*)ctx->prefixes.ptr;
-    hipLaunchKernelGGL(k_decode_cands, dim3((m + DC_THREADS - 1) / DC_THREADS), dim3(DC_THREADS), 0,
+    hipLaunchKernelGGL(k_decode_cands, dim3((m + DC_ROWS - 1) / DC_ROWS), dim3(DC_THREADS), 0,
                        ctx->stream,
                        (const ListDev*)idx->lists_dev, (const Item*)ctx->items.ptr, 1u, pre + 2,
                        (const uint64_t*)ctx->r.ptr, (const uint64_t*)ctx->q_venc.ptr,

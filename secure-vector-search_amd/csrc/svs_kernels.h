@@ -357,16 +357,23 @@ k_verify_finalize(const Item* __restrict__ items, uint32_t n_items,
 }
 
 // ================================================================== decode
-// Client decode (crypto.rs:901-940), one thread per candidate row:
+// Client decode (crypto.rs:901-940), DC_SPLIT threads per candidate row:
 //   Mv = r - (AL g + M T) - (H lt + S v_enc),  lt = L^T v_enc,
 // with the dense term AL g + H lt computed before (indexed like r).
 // Column-major client state keeps every access coalesced across a wave:
-// the gathered columns M_plain^T[t_e] and S sliced-ELL. S v_enc gathers 128 random entries of the query's v_enc per row, so the
-// block keeps the v_enc of its first candidate's query in LDS (a gather
-// from LDS instead of up to 64 cache lines per load); candidates of another
-// query in the same block read theirs from memory. Writes the signed score
-// (fp_to_signed, crypto.rs:128-135).
+// the gathered columns M_plain^T[t_e] and S sliced-ELL. The row's T entries
+// and S entries are dealt round-robin over its DC_SPLIT threads (eight of
+// each thread's in flight) and the partial sums combined with shuffles, so
+// a row's gathers are four short dependent chains rather than one long
+// one (the one-query decode has few rows per query and is bound by that
+// chain, not by bytes). S v_enc gathers 128 random entries of the query's
+// v_enc per row, so the block keeps the v_enc of its first candidate's
+// query in LDS (a gather from LDS instead of up to 64 cache lines per
+// load); candidates of another query in the same block read theirs from
+// memory. Writes the signed score (fp_to_signed, crypto.rs:128-135).
 constexpr int DC_THREADS = 256;
+constexpr int DC_SPLIT = 4;
+constexpr int DC_ROWS = DC_THREADS / DC_SPLIT;  // candidate rows per block pass
 __global__ void __launch_bounds__(DC_THREADS)
 k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items,
                uint32_t n_items, const uint32_t* __restrict__ cand_ptr,
@@ -376,8 +383,9 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
                uint64_t* __restrict__ mv_out) {
     __shared__ uint64_t v_lds[N];
     const uint32_t total = cand_ptr[n_items];
+    const uint32_t sub = threadIdx.x % DC_SPLIT;
     uint32_t q_lds = 0xFFFFFFFFu;  // block-uniform: the query held in v_lds
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    for (uint32_t base = blockIdx.x * DC_ROWS; base < total; base += gridDim.x * DC_ROWS) {
         const uint32_t q0 = items[find_seg(cand_ptr, n_items, base)].q;
         if (q0 != q_lds) {
             __syncthreads();  // earlier readers of v_lds are done
@@ -386,29 +394,33 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
             __syncthreads();
             q_lds = q0;
         }
-        const uint32_t g = base + threadIdx.x;
-        if (g >= total) continue;
+        // a thread past the last candidate works on the last one and
+        // writes nothing: every thread of a row takes part in its shuffles
+        const uint32_t gr = base + threadIdx.x / DC_SPLIT;
+        const bool live = gr < total;
+        const uint32_t g = live ? gr : total - 1;
         const uint32_t it = find_seg(cand_ptr, n_items, g);
         const Item item = items[it];
         const ListDev L = lists[item.list];
         const uint32_t i = g - cand_ptr[it];
         const uint32_t m = item.m;
         // M T: gathered columns of M_plain (sparse +-1 mask T of the query),
-        // eight columns in flight.
+        // entries sub, sub + DC_SPLIT, ... of T on this thread.
         uint64_t mt = 0;
         const uint16_t* tq = t_list + (size_t)item.q * N;
         const uint32_t tn = t_cnt[item.q];
 #pragma unroll 1
-        for (uint32_t e0 = 0; e0 < tn; e0 += 8) {
+        for (uint32_t e0 = sub; e0 < tn; e0 += 8 * DC_SPLIT) {
             uint64_t x[8];
             uint32_t neg = 0;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
+                const uint32_t e = e0 + j * DC_SPLIT;
                 x[j] = 0;
-                const uint16_t te = e0 + j < tn ? tq[e0 + j] : 0;
+                const uint16_t te = e < tn ? tq[e] : 0;
                 const uint32_t c = te & 0x3FF;
                 // M_plain is zero from column mp_cols on
-                if (e0 + j < tn && c < L.mp_cols) {
+                if (e < tn && c < L.mp_cols) {
                     if (L.mp_t32) {
                         const int32_t v = L.mp_t32[(size_t)c * m + i];
                         x[j] = v >= 0 ? (uint64_t)v : P - (uint64_t)(-(int64_t)v);
@@ -422,7 +434,8 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
             for (int j = 0; j < 8; ++j) mt = ((neg >> j) & 1) ? sub_mod(mt, x[j]) : add_mod(mt, x[j]);
         }
         // S v_enc from the row's sliced-ELL entries (a row's entries end at
-        // its first padding entry), eight entries in flight.
+        // its first padding entry, and every later one is padding too):
+        // entries sub, sub + DC_SPLIT, ... of the row on this thread.
         const bool own = item.q == q_lds;
         const uint64_t* vq = venc + (size_t)item.q * N;
         const uint32_t slice = i / SLICE, lane = i % SLICE;
@@ -430,11 +443,12 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
         uint64_t sv = 0;
         bool done = false;
 #pragma unroll 1
-        for (uint32_t e0 = s0 + lane; e0 < s1 && !done; e0 += 8 * SLICE) {
+        for (uint32_t e0 = s0 + lane + sub * SLICE; e0 < s1 && !done;
+             e0 += 8 * DC_SPLIT * SLICE) {
             uint16_t se[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const uint32_t e = e0 + j * SLICE;
+                const uint32_t e = e0 + j * DC_SPLIT * SLICE;
                 se[j] = e < s1 ? L.s_ell[e] : S_PAD;
             }
             uint64_t x[8];
@@ -447,10 +461,18 @@ k_decode_cands(const ListDev* __restrict__ lists, const Item* __restrict__ items
                 if (!done) sv = (se[j] & 0x8000) ? sub_mod(sv, x[j]) : add_mod(sv, x[j]);
             }
         }
-        const uint64_t corr = add_mod(add_mod(dense[g], mt), sv);
-        const uint64_t mv = sub_mod(r[g], corr);
-        if (scores) scores[g] = fp_to_signed(mv);
-        if (mv_out) mv_out[g] = mv;
+        // the row's DC_SPLIT partial sums (adjacent lanes of one wave)
+#pragma unroll
+        for (int off = 1; off < DC_SPLIT; off <<= 1) {
+            mt = add_mod(mt, shfl_xor_u64(mt, off));
+            sv = add_mod(sv, shfl_xor_u64(sv, off));
+        }
+        if (live && sub == 0) {
+            const uint64_t corr = add_mod(add_mod(dense[g], mt), sv);
+            const uint64_t mv = sub_mod(r[g], corr);
+            if (scores) scores[g] = fp_to_signed(mv);
+            if (mv_out) mv_out[g] = mv;
+        }
     }
 }
 
