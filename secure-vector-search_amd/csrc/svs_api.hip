@@ -923,13 +923,21 @@ static void launch_planes(svs_ctx* ctx, int v, hipStream_t st, uint64_t cap_tile
     switch (kTileVariants[v].qg) {
         case 16:
             if (ctx->prod_ldsb) {
-                // whole tiles per block, query planes staged in LDS
+                // whole tiles per block, query planes staged in LDS; the
+                // dense decode term (KE = 2 n1) with half-tile waves, two
+                // blocks per CU
+                constexpr uint32_t per_cu = KE == 2 * N1 ? 2 : 1;
                 const uint32_t blocks = (uint32_t)std::max<uint64_t>(
                     1, std::min<uint64_t>(std::max<uint64_t>(cap_tiles, 1),
-                                          (uint64_t)ctx->n_cu * ctx->tile_grid));
-                hipLaunchKernelGGL((k_prod_planes_lb<KE, SRC, PF16>), dim3(blocks), dim3(256), 0,
-                                   st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr,
-                                   qplanes, qc, out);
+                                          (uint64_t)ctx->n_cu * per_cu * ctx->tile_grid));
+                if constexpr (KE == 2 * N1)
+                    hipLaunchKernelGGL((k_prod_planes_lh<KE, SRC, 1>), dim3(blocks), dim3(256), 0,
+                                       st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr,
+                                       qplanes, qc, out);
+                else
+                    hipLaunchKernelGGL((k_prod_planes_lb<KE, SRC, PF16>), dim3(blocks), dim3(256), 0,
+                                       st, a.lists, a.tiles, a.nt, a.gitems, a.items, a.cand_ptr,
+                                       qplanes, qc, out);
             } else {
                 launch_planes_v<KE, SRC, 16, PF16, 1>(ctx, st, cap_tiles, a, nullptr, qplanes, qc,
                                                       out);

@@ -2313,6 +2313,132 @@ k_prod_planes_lb(const ListDev* __restrict__ lists, const MTile* __restrict__ ti
     __builtin_amdgcn_s_waitcnt(0);
 }
 
+// k_prod_planes_lb at two waves per SIMD: each wave takes 16 rows x one
+// half (8 queries) of a 16-query tile, so its accumulators are 128
+// registers and two blocks fit on a CU (32 KB of query planes each at
+// KE = 256), one wave's epilogue running beside the other's loads and
+// MFMAs. Units of a tile: its 16-row blocks x its halves (one half when
+// the tile holds at most 8 queries); wave w takes units w, w + 4, ..., so
+// the two halves of a row block run on neighbouring waves of the block.
+// Ring depth PF over the KE / 64 steps; MFMAs plane-major so that each
+// plane's registers free after its four MFMAs.
+template <int KE, int SRC, int PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_prod_planes_lh(const ListDev* __restrict__ lists, const MTile* __restrict__ tiles,
+                 const uint32_t* __restrict__ n_tiles_p, const uint32_t* __restrict__ gitems,
+                 const Item* __restrict__ items, const uint32_t* __restrict__ cand_ptr,
+                 const int8_t* __restrict__ qplanes, const uint64_t* __restrict__ qcorr,
+                 uint64_t* __restrict__ out) {
+    constexpr int QW = 16, NBT = QW / 2, NB = 4;
+    constexpr int KS = KE / 64;
+    constexpr uint32_t QBYTES = 8 * KE;
+    constexpr int STAGE = KS * NBT * WAVE / 256;
+    static_assert(KS % PF == 0 && PF >= 1, "the ring must tile the steps");
+    static_assert(STAGE % 4 == 0, "staging in fours");
+    __shared__ v4i bsm[KS][NBT][WAVE];
+    __shared__ uint32_t sq[QW];
+    const int tid = threadIdx.x;
+    const int lane = tid % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int g = lane / 16, col = lane % 16, jj = col / 8, bpl = col % 8;
+    const XcdShare sh = xcd_share(tiles, *n_tiles_p);
+
+    v4i ra[PF][8], rb[NB];
+    auto issue_a = [&](int slot, const uint64_t* ap, int s) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ra[slot][j] = load16(ap + 64 * s + 8 * j);
+    };
+    auto read_b = [&](int s, uint32_t half) {
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) rb[jb] = bsm[s][4 * half + jb][lane];
+    };
+    for (uint32_t t = sh.lo + sh.worker; t < sh.hi; t += sh.workers) {
+        const MTile tile = tiles[t];
+        const uint32_t m = lists[tile.list].m;
+        const uint64_t* menc = SRC ? lists[tile.list].alh : lists[tile.list].menc;
+        const uint64_t* rowcorr = SRC ? lists[tile.list].alh_rowcorr : lists[tile.list].rowcorr;
+        const uint32_t gn = tile.gn;
+        __syncthreads();
+        if (tid < QW) sq[tid] = items[gitems[tile.g0 + min((uint32_t)tid, gn - 1)]].q;
+        __syncthreads();
+#pragma unroll 1
+        for (int i0 = 0; i0 < STAGE; i0 += 4) {
+            v4i st[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + 4 * (i0 + i);
+                const int s = e / NBT, jb = e % NBT;
+                const uint32_t q = sq[2 * jb + jj];
+                st[i] = load16b(qplanes + (size_t)q * QBYTES + (uint32_t)(g * 8 + bpl) * 16u +
+                                512u * s);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = wave + 4 * (i0 + i);
+                bsm[e / NBT][e % NBT][lane] = st[i];
+            }
+        }
+        __syncthreads();
+        const uint32_t rows = min(256u, m - tile.row0);
+        const uint32_t nsb = (rows + 15) / 16;
+        const uint32_t nh = gn > 8 ? 2u : 1u;
+        const uint32_t nu = nsb * nh;
+        if ((uint32_t)wave >= nu) continue;
+        auto row_ptr = [&](uint32_t u) -> const uint64_t* {
+            const uint32_t row = min(tile.row0 + 16u * (u / nh) + (uint32_t)col, m - 1);
+            return menc + (size_t)row * KE + 2 * g;
+        };
+        uint32_t u = (uint32_t)wave;
+        const uint64_t* a_cur = row_ptr(u);
+        read_b(0, u % nh);
+#pragma unroll
+        for (int d = 0; d < PF; ++d) issue_a(d, a_cur, d);
+        for (;;) {
+            const uint32_t un = u + 4;
+            const bool has_next = un < nu;
+            const uint64_t* a_nxt = has_next ? row_ptr(un) : a_cur;
+            const uint32_t half = u % nh, half_n = has_next ? un % nh : half;
+            const uint32_t row0 = tile.row0 + 16u * (u / nh);
+            const uint32_t wrow = row0 + 4u * g + (uint32_t)(bpl & 3);
+            const uint64_t rcorr = gload_u64(rowcorr + min(wrow, m - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            v4i acc[8][NB];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) acc[a][jb] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int slot = s % PF;
+                v4i pl[8];
+                planes_of(ra[slot], pl);
+                const bool in_unit = s + PF < KS;
+                const uint64_t* ap = in_unit ? a_cur : (has_next ? a_nxt : a_cur);
+                const int sn = in_unit ? s + PF : (has_next ? s + PF - KS : KS - 1);
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int jb = 0; jb < NB; ++jb)
+                        acc[a][jb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(pl[a], rb[jb],
+                                                                           acc[a][jb], 0, 0, 0);
+                read_b(s + 1 < KS ? s + 1 : 0, s + 1 < KS ? half : half_n);
+                issue_a(slot, ap, sn);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // this half's queries as a tile of their own for the epilogue
+            MTile th = tile;
+            th.g0 = tile.g0 + 8u * half;
+            th.gn = gn - 8u * half;
+            planes_epilogue<NB>(acc, jj, bpl, wrow, m, rcorr, th, true, gitems, items, cand_ptr,
+                                qcorr, out);
+            if (!has_next) break;
+            u = un;
+            a_cur = a_nxt;
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
 // Calls f(ic<i>{}) for i = 0 .. N - 1 (i usable as a constant inside f).
 template <int I>
 struct ic {
